@@ -1,0 +1,254 @@
+"""blaze_amd — MI355X (gfx950) smoke/fire solver step behind Blaze's interface.
+
+Python mirror of the C ABI in ``include/blaze.h`` (ctypes over the in-tree
+``libblaze_amd.so``). It follows the reference's ``FluidSolver`` /
+``SceneSettings`` / ``Parser`` interface (asiroisvigneux/Blaze
+src/FluidSolver.h:38-110, src/SceneSettings.h:32-71, src/Parser.h:41-211):
+
+* :func:`parse_scene` reads a ``.blz`` file into :class:`Settings`;
+* :class:`Solver` owns the device state: ``add_source`` (addSource),
+  ``step`` (step), ``field`` / ``set_field`` (state access for tests and
+  consumers such as the renderer, which reads ``T`` like getTempGrid).
+
+Every compute call runs HIP kernels; there is no CPU fallback. Importing
+this package raises if the shared library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libblaze_amd.so")
+
+FIELD_T, FIELD_U, FIELD_V, FIELD_W, FIELD_DENSITY, FIELD_P, FIELD_RHS = range(7)
+FIELDS = {"T": FIELD_T, "U": FIELD_U, "V": FIELD_V, "W": FIELD_W, "density": FIELD_DENSITY,
+          "P": FIELD_P, "rhs": FIELD_RHS}
+(STAGE_COOLDOWN, STAGE_DRAG, STAGE_VORTICITY, STAGE_BUOYANCY, STAGE_WIND, STAGE_TURBULENCE,
+ STAGE_DIVERGENCE, STAGE_PRESSURE, STAGE_PROJECT_UPDATE, STAGE_ADVECT) = range(10)
+NUM_STAGES = 10
+STAGE_NAMES = ["cooldown", "drag", "vorticity", "buoyancy", "wind", "turbulence", "divergence",
+               "pressure", "project_update", "advect"]
+
+
+class Settings(C.Structure):
+    """C mirror of the solver subset of SceneSettings (blz_settings)."""
+
+    _fields_ = [
+        ("grid_res", C.c_int * 3), ("dx", C.c_float), ("dt", C.c_float),
+        ("domain_min", C.c_float * 3), ("domain_max", C.c_float * 3),
+        ("closed_bounds", C.c_int * 6), ("max_iter_solve", C.c_int),
+        ("density", C.c_float), ("gravity", C.c_float), ("buoyancy", C.c_float),
+        ("cooling_rate", C.c_float), ("vorticity_conf", C.c_float), ("drag", C.c_float),
+        ("wind_dir", C.c_int), ("wind_amp", C.c_float), ("wind_speed", C.c_float),
+        ("wind_turb_amp", C.c_float), ("wind_turb_scale", C.c_float),
+        ("turbulence_amp", C.c_float), ("turbulence_scale", C.c_float),
+        ("turb_mask_temp_ramp", C.c_float * 2), ("turb_mask_vel_ramp", C.c_float * 2),
+        ("source_animated", C.c_int), ("source_range", C.c_int * 2),
+        ("source_max_particle_count", C.c_int),
+        ("noise_seed", C.c_int), ("residual_tol", C.c_float), ("residual_check_every", C.c_int),
+        ("cfl_number", C.c_float), ("max_substeps", C.c_int), ("smoke_density", C.c_float),
+        ("warm_start", C.c_int),
+    ]
+
+    def copy(self) -> "Settings":
+        s = Settings()
+        C.pointer(s)[0] = self
+        return s
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("substeps", C.c_int), ("rbgs_iters", C.c_int), ("rbgs_iters_last", C.c_int),
+        ("residual", C.c_double), ("residual_initial", C.c_double),
+        ("warm_start_used", C.c_int), ("warm_start_fallbacks", C.c_int),
+        ("stage_ms", C.c_double * NUM_STAGES), ("source_ms", C.c_double), ("step_ms", C.c_double),
+    ]
+
+
+class ParticleGen(C.Structure):
+    _fields_ = [
+        ("count", C.c_int), ("seed", C.c_ulonglong), ("center", C.c_float * 3),
+        ("radius", C.c_float), ("pscale", C.c_float), ("temp", C.c_float),
+        ("vel", C.c_float * 3), ("radial_vel", C.c_float),
+    ]
+
+
+class BlazeError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"blaze_amd: {LIB_PATH} is missing; build it first (python -c "
+            "'import __graft_entry__ as g; g.build()' or make -C blaze_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "blz_abi_version": ([], C.c_int),
+        "blz_settings_default": ([P(Settings)], C.c_int),
+        "blz_parse_scene": ([C.c_char_p, P(Settings), C.c_char_p, C.c_int, C.c_char_p, C.c_int,
+                             P(ParticleGen), P(C.c_int)], C.c_int),
+        "blz_read_particles": ([C.c_char_p, P(C.c_float), C.c_int, P(C.c_int)], C.c_int),
+        "blz_count_particles": ([C.c_char_p, P(C.c_int)], C.c_int),
+        "blz_generate_particles": ([P(ParticleGen), P(C.c_float)], C.c_int),
+        "blz_create": ([P(Settings), P(vp)], C.c_int),
+        "blz_destroy": ([vp], C.c_int),
+        "blz_add_source": ([vp, P(C.c_float), C.c_int], C.c_int),
+        "blz_step": ([vp, C.c_float], C.c_int),
+        "blz_step_async": ([vp, C.c_float], C.c_int),
+        "blz_sync": ([vp], C.c_int),
+        "blz_run_stage": ([vp, C.c_int, C.c_float, C.c_float], C.c_int),
+        "blz_get_field": ([vp, C.c_int, P(vp), P(C.c_int)], C.c_int),
+        "blz_copy_field": ([vp, C.c_int, P(C.c_float)], C.c_int),
+        "blz_set_field": ([vp, C.c_int, P(C.c_float)], C.c_int),
+        "blz_get_stats": ([vp, P(Stats)], C.c_int),
+        "blz_set_timing": ([vp, C.c_int], C.c_int),
+        "blz_set_rbgs_variant": ([vp, C.c_int], C.c_int),
+        "blz_last_error": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib, sig
+
+
+_lib, ABI_SYMBOLS = _load()
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = _lib.blz_last_error().decode(errors="replace")
+        raise BlazeError(f"{what} failed ({rc}): {msg}")
+
+
+def default_settings() -> Settings:
+    s = Settings()
+    _check(_lib.blz_settings_default(C.byref(s)), "blz_settings_default")
+    return s
+
+
+def parse_scene(path: str):
+    """.blz -> (Settings, scene_dir, particle_file, ParticleGen or None)."""
+    s = Settings()
+    d = C.create_string_buffer(4096)
+    pf = C.create_string_buffer(4096)
+    g = ParticleGen()
+    has = C.c_int(0)
+    _check(_lib.blz_parse_scene(path.encode(), C.byref(s), d, 4096, pf, 4096, C.byref(g), C.byref(has)),
+           f"parse_scene({path})")
+    return s, d.value.decode(), pf.value.decode(), (g if has.value else None)
+
+
+def generate_particles(gen: ParticleGen) -> np.ndarray:
+    out = np.zeros((gen.count, 8), dtype=np.float32)
+    _check(_lib.blz_generate_particles(C.byref(gen), out.ctypes.data_as(C.POINTER(C.c_float))),
+           "generate_particles")
+    return out
+
+
+def read_particles(path: str) -> np.ndarray:
+    n = C.c_int(0)
+    _check(_lib.blz_count_particles(path.encode(), C.byref(n)), "count_particles")
+    out = np.zeros((n.value, 8), dtype=np.float32)
+    got = C.c_int(0)
+    _check(_lib.blz_read_particles(path.encode(), out.ctypes.data_as(C.POINTER(C.c_float)), n.value,
+                                   C.byref(got)), "read_particles")
+    return out[: got.value]
+
+
+def field_dims(s: Settings, which: int):
+    W, H, D = s.grid_res
+    return {FIELD_T: (W, H, D), FIELD_U: (W + 1, H, D), FIELD_V: (W, H + 1, D),
+            FIELD_W: (W, H, D + 1), FIELD_DENSITY: (W, H, D), FIELD_P: (W, H, D),
+            FIELD_RHS: (W, H, D)}[which]
+
+
+class Solver:
+    """One device solver (blz_solver). Arrays are returned as (D, H, W)-shaped
+    float32 views of the x-fastest layout."""
+
+    def __init__(self, settings: Settings):
+        self.settings = settings.copy()
+        self._h = C.c_void_p()
+        _check(_lib.blz_create(C.byref(self.settings), C.byref(self._h)), "blz_create")
+
+    def close(self):
+        if self._h:
+            _lib.blz_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_source(self, part8=None):
+        if part8 is None:
+            _check(_lib.blz_add_source(self._h, None, 0), "blz_add_source")
+            return
+        a = np.ascontiguousarray(part8, dtype=np.float32).reshape(-1, 8)
+        _check(_lib.blz_add_source(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[0]),
+               "blz_add_source")
+
+    def step(self, time: float):
+        _check(_lib.blz_step(self._h, C.c_float(time)), "blz_step")
+
+    def step_async(self, time: float):
+        _check(_lib.blz_step_async(self._h, C.c_float(time)), "blz_step_async")
+
+    def sync(self):
+        _check(_lib.blz_sync(self._h), "blz_sync")
+
+    def run_stage(self, stage: int, time: float, dt: float):
+        _check(_lib.blz_run_stage(self._h, stage, C.c_float(time), C.c_float(dt)), "blz_run_stage")
+
+    def field(self, which) -> np.ndarray:
+        which = FIELDS.get(which, which)
+        nx, ny, nz = field_dims(self.settings, which)
+        out = np.empty((nz, ny, nx), dtype=np.float32)
+        _check(_lib.blz_copy_field(self._h, which, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "blz_copy_field")
+        return out
+
+    def set_field(self, which, arr):
+        which = FIELDS.get(which, which)
+        nx, ny, nz = field_dims(self.settings, which)
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(nz, ny, nx)
+        _check(_lib.blz_set_field(self._h, which, a.ctypes.data_as(C.POINTER(C.c_float))),
+               "blz_set_field")
+
+    def device_field(self, which):
+        """(device pointer, (nx, ny, nz)) of a front buffer (getTempGrid for T)."""
+        which = FIELDS.get(which, which)
+        p = C.c_void_p()
+        dims = (C.c_int * 3)()
+        _check(_lib.blz_get_field(self._h, which, C.byref(p), dims), "blz_get_field")
+        return p.value, tuple(dims)
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(_lib.blz_get_stats(self._h, C.byref(st)), "blz_get_stats")
+        return st
+
+    def set_timing(self, on: bool):
+        _check(_lib.blz_set_timing(self._h, 1 if on else 0), "blz_set_timing")
+
+    def set_rbgs_variant(self, v: int):
+        _check(_lib.blz_set_rbgs_variant(self._h, v), "blz_set_rbgs_variant")
+
+
+def abi_version() -> int:
+    return _lib.blz_abi_version()

@@ -1,0 +1,443 @@
+// capi.hip — solver handle lifecycle, step orchestration and the C ABI.
+//
+// Reference: FluidSolver (asiroisvigneux/Blaze src/FluidSolver.cu:671-955).
+// The reference keeps device-heap Grid objects and synchronises after every
+// launch; here the host owns front/back pointers (a swap is a pointer swap),
+// all work of a step is enqueued on the handle's stream, and the host only
+// waits where it needs a value (residual checks, CFL substep count).
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "solver.h"
+
+namespace blz {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+void free_p2g_scratch(Solver& s);
+
+Solver::~Solver() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& F : f)
+        for (auto& b : F.buf)
+            if (b) (void)hipFree(b);
+    if (p.c[0]) (void)hipFree(p.c[0]);
+    if (rhs.c[0]) (void)hipFree(rhs.c[0]);
+    if (tmp) (void)hipFree(tmp);
+    if (red) (void)hipFree(red);
+    if (red_host) (void)hipHostFree(red_host);
+    if (part) (void)hipFree(part);
+    free_p2g_scratch(*this);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+static int validate(const blz_settings& s) {
+    for (int a = 0; a < 3; ++a)
+        if (s.grid_res[a] < 4 || s.grid_res[a] > 4096) {
+            set_error("grid_res must be in [4, 4096] on every axis");
+            return BLZ_E_INVALID;
+        }
+    if (s.grid_res[0] % 2) {
+        set_error("grid_res[0] (W) must be even (red/black split layout)");
+        return BLZ_E_INVALID;
+    }
+    const double cells = (double)(s.grid_res[0] + 1) * (s.grid_res[1] + 1) * (s.grid_res[2] + 1);
+    if (cells * 5.0 >= 4294967296.0 || cells >= 2147483648.0) {
+        set_error("grid too large for 32-bit cell indices");
+        return BLZ_E_INVALID;
+    }
+    if (!(s.dx > 0.0f) || !(s.dt > 0.0f) || !(s.density > 0.0f)) {
+        set_error("dx, dt and density must be positive");
+        return BLZ_E_INVALID;
+    }
+    if (s.max_iter_solve < 0) {
+        set_error("max_iter_solve must be >= 0");
+        return BLZ_E_INVALID;
+    }
+    return BLZ_OK;
+}
+
+int Solver::init(const blz_settings& st_) {
+    if (int e = validate(st_)) return e;
+    s = st_;
+    if (s.residual_check_every <= 0) s.residual_check_every = 10;
+    if (s.max_substeps <= 0) s.max_substeps = 8;
+    W = s.grid_res[0];
+    H = s.grid_res[1];
+    D = s.grid_res[2];
+    dx = s.dx;
+    smoke = s.smoke_density > 0.0f;
+    BLZ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // create_grids_kernel origins (src/FluidSolver.cu:36-43)
+    const int dims[F_N][3] = {{W, H, D}, {W + 1, H, D}, {W, H + 1, D}, {W, H, D + 1}, {W, H, D}};
+    const float org[F_N][3] = {{.5f, .5f, .5f}, {0.f, .5f, .5f}, {.5f, 0.f, .5f}, {.5f, .5f, 0.f}, {.5f, .5f, .5f}};
+    for (int g = 0; g < F_N; ++g) {
+        if (g == F_S && !smoke) continue;
+        Field& F = f[g];
+        F.nx = dims[g][0];
+        F.ny = dims[g][1];
+        F.nz = dims[g][2];
+        F.ox = org[g][0];
+        F.oy = org[g][1];
+        F.oz = org[g][2];
+        F.flip = (g == F_V);
+        for (int b = 0; b < 2; ++b) {
+            if (hipMalloc(&F.buf[b], F.count() * sizeof(float)) != hipSuccess) {
+                set_error("hipMalloc failed for a grid buffer");
+                return BLZ_E_NOMEM;
+            }
+            BLZ_HIP(hipMemset(F.buf[b], 0, F.count() * sizeof(float)));
+        }
+    }
+    const size_t N = (size_t)W * H * D;
+    float* pp = nullptr;
+    float* rr = nullptr;
+    if (hipMalloc(&pp, N * sizeof(float)) != hipSuccess || hipMalloc(&rr, N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess || hipMalloc(&red, 8 * sizeof(double)) != hipSuccess) {
+        if (pp) (void)hipFree(pp);
+        if (rr) (void)hipFree(rr);
+        set_error("hipMalloc failed for solver scratch");
+        return BLZ_E_NOMEM;
+    }
+    p.c[0] = pp;
+    p.c[1] = pp + N / 2;
+    rhs.c[0] = rr;
+    rhs.c[1] = rr + N / 2;
+    BLZ_HIP(hipMemset(pp, 0, N * sizeof(float))); // pressure zeroed once, :715-716
+    BLZ_HIP(hipMemset(rr, 0, N * sizeof(float)));
+    BLZ_HIP(hipMemset(red, 0, 8 * sizeof(double)));
+    BLZ_HIP(hipHostMalloc((void**)&red_host, 8 * sizeof(double), hipHostMallocDefault));
+    for (auto& e : ev) BLZ_HIP(hipEventCreate(&e));
+    std::memset(&st, 0, sizeof(st));
+    st.residual = -1.0;
+    st.residual_initial = -1.0;
+    return BLZ_OK;
+}
+
+int Solver::residual(float dt_, double* out) {
+    if (int e = launch_residual(*this, dt_, red)) return e;
+    BLZ_HIP(hipMemcpyAsync(red_host, red, sizeof(double), hipMemcpyDeviceToHost, stream));
+    BLZ_HIP(hipStreamSynchronize(stream));
+    *out = red_host[0];
+    return BLZ_OK;
+}
+
+int Solver::max_abs_velocity(float* out) {
+    if (int e = launch_max_abs_velocity(*this, red + 1)) return e;
+    BLZ_HIP(hipMemcpyAsync(red_host + 1, red + 1, sizeof(double), hipMemcpyDeviceToHost, stream));
+    BLZ_HIP(hipStreamSynchronize(stream));
+    *out = (float)red_host[1];
+    return BLZ_OK;
+}
+
+// gs_solve_kernel (src/FluidSolver.cu:517-599): a fixed sweep count, or
+// with residual_tol > 0 sweeps in chunks until the FP64 residual <= tol
+int Solver::project_solve(float dt_) {
+    const int cap = s.max_iter_solve;
+    if (!(s.residual_tol > 0.0f)) {
+        if (int e = launch_rbgs_sweeps(*this, dt_, cap)) return e;
+        st.rbgs_iters_last = cap;
+        st.residual = -1.0;
+        st.residual_initial = -1.0;
+        return BLZ_OK;
+    }
+    const double tol = (double)s.residual_tol;
+    double r = 0.0;
+    if (int e = residual(dt_, &r)) return e;
+    st.residual_initial = r;
+    int it = 0;
+    while (r > tol && it < cap) {
+        const int n = std::min(s.residual_check_every, cap - it);
+        if (int e = launch_rbgs_sweeps(*this, dt_, n)) return e;
+        it += n;
+        if (int e = residual(dt_, &r)) return e;
+    }
+    st.rbgs_iters_last = it;
+    st.residual = r;
+    return BLZ_OK;
+}
+
+int Solver::run_stage(int stage, float time, float dt_) {
+    hipEvent_t e0 = ev[2 * stage], e1 = ev[2 * stage + 1];
+    if (timing) BLZ_HIP(hipEventRecord(e0, stream));
+    int rc = BLZ_OK;
+    switch (stage) {
+    case BLZ_STAGE_COOLDOWN: rc = launch_cooldown(*this, dt_); break;
+    case BLZ_STAGE_DRAG: rc = launch_drag(*this, dt_); break;
+    case BLZ_STAGE_VORTICITY: rc = launch_vorticity(*this, dt_); break;
+    case BLZ_STAGE_BUOYANCY: rc = launch_buoyancy(*this, dt_); break;
+    case BLZ_STAGE_WIND: rc = launch_wind(*this, time, dt_); break;
+    case BLZ_STAGE_TURBULENCE: rc = launch_turbulence(*this, time, dt_); break;
+    case BLZ_STAGE_DIVERGENCE: rc = launch_divergence(*this); break;
+    case BLZ_STAGE_PRESSURE: rc = project_solve(dt_); break;
+    case BLZ_STAGE_PROJECT_UPDATE: rc = launch_pressure_update(*this, dt_); break;
+    case BLZ_STAGE_ADVECT: rc = launch_advect(*this, dt_); break;
+    default: set_error("unknown stage"); return BLZ_E_INVALID;
+    }
+    if (rc) return rc;
+    if (timing) {
+        BLZ_HIP(hipEventRecord(e1, stream));
+        BLZ_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        BLZ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        st.stage_ms[stage] += ms;
+    }
+    return BLZ_OK;
+}
+
+// FluidSolver::step, src/FluidSolver.cu:858-955 (stage gates :868-925)
+int Solver::substep(float time, float dt_) {
+    if (std::fabs(s.cooling_rate) > 0.0f)
+        if (int e = run_stage(BLZ_STAGE_COOLDOWN, time, dt_)) return e;
+    if (s.drag > 0.0f)
+        if (int e = run_stage(BLZ_STAGE_DRAG, time, dt_)) return e;
+    if (std::fabs(s.vorticity_conf) > 0.0f)
+        if (int e = run_stage(BLZ_STAGE_VORTICITY, time, dt_)) return e;
+    if (std::fabs(s.buoyancy) > 0.0f)
+        if (int e = run_stage(BLZ_STAGE_BUOYANCY, time, dt_)) return e;
+    if (s.wind_amp > 0.0f)
+        if (int e = run_stage(BLZ_STAGE_WIND, time, dt_)) return e;
+    if (s.turbulence_amp > 0.0f)
+        if (int e = run_stage(BLZ_STAGE_TURBULENCE, time, dt_)) return e;
+    if (int e = run_stage(BLZ_STAGE_DIVERGENCE, time, dt_)) return e;
+    if (int e = run_stage(BLZ_STAGE_PRESSURE, time, dt_)) return e;
+    st.rbgs_iters += st.rbgs_iters_last;
+    if (int e = run_stage(BLZ_STAGE_PROJECT_UPDATE, time, dt_)) return e;
+    if (int e = run_stage(BLZ_STAGE_ADVECT, time, dt_)) return e;
+    return BLZ_OK;
+}
+
+// One frame. With cfl_number > 0: n = ceil(max|u| dt / (cfl dx)) substeps
+// (clamped to [1, max_substeps]) of dt/n at times time + q*(dt/n).
+int Solver::step_enqueue(float time, bool sync) {
+    for (double& v : st.stage_ms) v = 0.0;
+    st.rbgs_iters = 0;
+    const float dtf = s.dt;
+    int n = 1;
+    if (s.cfl_number > 0.0f) {
+        float umax = 0.0f;
+        if (int e = max_abs_velocity(&umax)) return e;
+        const float q = std::ceil(umax * dtf / (s.cfl_number * dx));
+        n = !(q >= 1.0f) ? 1 : (q > (float)s.max_substeps ? s.max_substeps : (int)q);
+    }
+    const float dts = dtf / (float)n;
+    if (timing) BLZ_HIP(hipEventRecord(ev[2 * BLZ_NUM_STAGES], stream));
+    for (int q = 0; q < n; ++q)
+        if (int e = substep(time + (float)q * dts, dts)) return e;
+    st.substeps = n;
+    if (timing) {
+        BLZ_HIP(hipEventRecord(ev[2 * BLZ_NUM_STAGES + 1], stream));
+        BLZ_HIP(hipEventSynchronize(ev[2 * BLZ_NUM_STAGES + 1]));
+        float ms = 0.0f;
+        BLZ_HIP(hipEventElapsedTime(&ms, ev[2 * BLZ_NUM_STAGES], ev[2 * BLZ_NUM_STAGES + 1]));
+        st.step_ms = ms;
+    }
+    if (sync) BLZ_HIP(hipStreamSynchronize(stream));
+    return BLZ_OK;
+}
+
+int Solver::add_source(const float* part8, int n) {
+    if (part8) {
+        if (n < 0) {
+            set_error("particle count must be >= 0");
+            return BLZ_E_INVALID;
+        }
+        if (n > part_cap) {
+            if (part) BLZ_HIP(hipFree(part));
+            part = nullptr;
+            part_cap = 0;
+            if (hipMalloc(&part, (size_t)n * 8 * sizeof(float)) != hipSuccess) {
+                set_error("hipMalloc failed for particles");
+                return BLZ_E_NOMEM;
+            }
+            part_cap = n;
+        }
+        if (n > 0)
+            BLZ_HIP(hipMemcpyAsync(part, part8, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice, stream));
+        npart = n;
+    }
+    hipEvent_t e0 = ev[2 * BLZ_NUM_STAGES + 2], e1 = ev[2 * BLZ_NUM_STAGES + 3];
+    if (timing) BLZ_HIP(hipEventRecord(e0, stream));
+    if (int e = launch_p2g(*this)) return e;
+    if (timing) {
+        BLZ_HIP(hipEventRecord(e1, stream));
+        BLZ_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        BLZ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        st.source_ms = ms;
+    }
+    BLZ_HIP(hipStreamSynchronize(stream));
+    return BLZ_OK;
+}
+
+} // namespace blz
+
+using blz::Solver;
+
+struct blz_solver {
+    Solver s;
+};
+
+#define CHECK_HANDLE(h)                              \
+    do {                                             \
+        if (!(h)) {                                  \
+            blz::set_error("null solver handle");    \
+            return BLZ_E_INVALID;                    \
+        }                                            \
+    } while (0)
+
+extern "C" {
+
+const char* blz_last_error(void) { return blz::g_err.c_str(); }
+
+int blz_create(const blz_settings* s, blz_solver** out) {
+    if (!s || !out) {
+        blz::set_error("blz_create: null argument");
+        return BLZ_E_INVALID;
+    }
+    *out = nullptr;
+    blz_solver* h = new (std::nothrow) blz_solver();
+    if (!h) {
+        blz::set_error("out of host memory");
+        return BLZ_E_NOMEM;
+    }
+    if (int e = h->s.init(*s)) {
+        delete h;
+        return e;
+    }
+    *out = h;
+    return BLZ_OK;
+}
+
+int blz_destroy(blz_solver* h) {
+    delete h;
+    return BLZ_OK;
+}
+
+int blz_add_source(blz_solver* h, const float* part8, int count) {
+    CHECK_HANDLE(h);
+    return h->s.add_source(part8, count);
+}
+
+int blz_step(blz_solver* h, float time) {
+    CHECK_HANDLE(h);
+    return h->s.step_enqueue(time, true);
+}
+
+int blz_step_async(blz_solver* h, float time) {
+    CHECK_HANDLE(h);
+    return h->s.step_enqueue(time, false);
+}
+
+int blz_sync(blz_solver* h) {
+    CHECK_HANDLE(h);
+    BLZ_HIP(hipStreamSynchronize(h->s.stream));
+    return BLZ_OK;
+}
+
+int blz_run_stage(blz_solver* h, int stage, float time, float dt) {
+    CHECK_HANDLE(h);
+    if (stage < 0 || stage >= BLZ_NUM_STAGES) {
+        blz::set_error("stage out of range");
+        return BLZ_E_INVALID;
+    }
+    for (double& v : h->s.st.stage_ms) v = 0.0;
+    if (int e = h->s.run_stage(stage, time, dt)) return e;
+    if (stage == BLZ_STAGE_PRESSURE) h->s.st.rbgs_iters = h->s.st.rbgs_iters_last;
+    BLZ_HIP(hipStreamSynchronize(h->s.stream));
+    return BLZ_OK;
+}
+
+static int field_geom(Solver& s, int which, float** dev, size_t* count, int* dims) {
+    if (which == BLZ_FIELD_P || which == BLZ_FIELD_RHS) {
+        *dev = which == BLZ_FIELD_P ? s.p.c[0] : s.rhs.c[0];
+        *count = (size_t)s.W * s.H * s.D;
+        if (dims) { dims[0] = s.W; dims[1] = s.H; dims[2] = s.D; }
+        return BLZ_OK;
+    }
+    if (which < 0 || which > BLZ_FIELD_DENSITY || (which == BLZ_FIELD_DENSITY && !s.smoke)) {
+        blz::set_error("field not available");
+        return BLZ_E_INVALID;
+    }
+    const blz::Field& F = s.f[which];
+    *dev = F.front();
+    *count = F.count();
+    if (dims) { dims[0] = F.nx; dims[1] = F.ny; dims[2] = F.nz; }
+    return BLZ_OK;
+}
+
+int blz_get_field(blz_solver* h, int which, void** dev_ptr, int dims[3]) {
+    CHECK_HANDLE(h);
+    float* d = nullptr;
+    size_t n = 0;
+    if (int e = field_geom(h->s, which, &d, &n, dims)) return e;
+    if (dev_ptr) *dev_ptr = d;
+    return BLZ_OK;
+}
+
+int blz_copy_field(blz_solver* h, int which, float* host) {
+    CHECK_HANDLE(h);
+    if (!host) {
+        blz::set_error("null host buffer");
+        return BLZ_E_INVALID;
+    }
+    Solver& s = h->s;
+    float* d = nullptr;
+    size_t n = 0;
+    if (int e = field_geom(s, which, &d, &n, nullptr)) return e;
+    if (which == BLZ_FIELD_P || which == BLZ_FIELD_RHS) {
+        if (int e = blz::launch_p_to_natural(s, s.tmp, which == BLZ_FIELD_P ? s.p : s.rhs)) return e;
+        d = s.tmp;
+    }
+    BLZ_HIP(hipMemcpyAsync(host, d, n * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    return BLZ_OK;
+}
+
+int blz_set_field(blz_solver* h, int which, const float* host) {
+    CHECK_HANDLE(h);
+    if (!host) {
+        blz::set_error("null host buffer");
+        return BLZ_E_INVALID;
+    }
+    Solver& s = h->s;
+    float* d = nullptr;
+    size_t n = 0;
+    if (int e = field_geom(s, which, &d, &n, nullptr)) return e;
+    if (which == BLZ_FIELD_P || which == BLZ_FIELD_RHS) {
+        BLZ_HIP(hipMemcpyAsync(s.tmp, host, n * sizeof(float), hipMemcpyHostToDevice, s.stream));
+        if (int e = blz::launch_natural_to_rb(s, which == BLZ_FIELD_P ? s.p : s.rhs, s.tmp)) return e;
+    } else {
+        BLZ_HIP(hipMemcpyAsync(d, host, n * sizeof(float), hipMemcpyHostToDevice, s.stream));
+    }
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    return BLZ_OK;
+}
+
+int blz_get_stats(blz_solver* h, blz_stats* out) {
+    CHECK_HANDLE(h);
+    if (!out) {
+        blz::set_error("null stats");
+        return BLZ_E_INVALID;
+    }
+    *out = h->s.st;
+    return BLZ_OK;
+}
+
+int blz_set_timing(blz_solver* h, int on) {
+    CHECK_HANDLE(h);
+    h->s.timing = on != 0;
+    return BLZ_OK;
+}
+
+int blz_set_rbgs_variant(blz_solver* h, int variant) {
+    CHECK_HANDLE(h);
+    h->s.rbgs_variant = variant;
+    return BLZ_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,203 @@
+// dev_math.h — device-side sampling, noise and vector helpers of the solver.
+//
+// Every expression keeps the evaluation order of the reference source
+// (asiroisvigneux/Blaze src/Grid.h, src/FluidSolver.cu,
+// thirdparty/cuda-noise/cuda_noise.cuh) so that, built with
+// -ffp-contract=off and IEEE division/sqrt, results are bit-identical to
+// a serial CPU evaluation of the same formulas.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace blz {
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+// helper_math.h:1267-1269 (dot), :1314-1316 (length), :1443-1446 (cross)
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Read-only view of one staggered grid: dims, origin offset (cells), dx.
+// Index is x-fastest: i + j*nx + k*nx*ny (src/Grid.h:256-268).
+struct Gv {
+    const float* __restrict__ a;
+    int nx, ny, nz;
+    float ox, oy, oz;
+};
+
+__device__ __forceinline__ float ld(const Gv& g, int i, int j, int k) {
+    return g.a[i + j * g.nx + k * (g.nx * g.ny)];
+}
+
+// grid -> object position of cell (i,j,k): (i + o) * dx  (src/Grid.h:270-290)
+__device__ __forceinline__ f3 g2o(const Gv& g, float dx, int i, int j, int k) {
+    return mk3(((float)i + g.ox) * dx, ((float)j + g.oy) * dx, ((float)k + g.oz) * dx);
+}
+
+// trilinear_interpolate, src/Grid.h:94-108
+__device__ __forceinline__ float trilinear(const Gv& g, float dx, f3 p) {
+    const float gX = fmaxf(fminf(p.x / dx - g.ox, (float)g.nx - 1.001f), 0.0f);
+    const float gY = fmaxf(fminf(p.y / dx - g.oy, (float)g.ny - 1.001f), 0.0f);
+    const float gZ = fmaxf(fminf(p.z / dx - g.oz, (float)g.nz - 1.001f), 0.0f);
+    const int i = (int)gX, j = (int)gY, k = (int)gZ;
+    const float tx = gX - (float)i, ty = gY - (float)j, tz = gZ - (float)k;
+    const int sy = g.nx, sz = g.nx * g.ny;
+    const float* b = g.a + i + j * sy + k * sz;
+    const float c00 = b[0] * (1.0f - tx) + b[1] * tx;
+    const float c10 = b[sy] * (1.0f - tx) + b[sy + 1] * tx;
+    const float c01 = b[sz] * (1.0f - tx) + b[sz + 1] * tx;
+    const float c11 = b[sy + sz] * (1.0f - tx) + b[sy + sz + 1] * tx;
+    return (1.0f - tz) * ((1.0f - ty) * c00 + ty * c10) + tz * ((1.0f - ty) * c01 + ty * c11);
+}
+
+// cerp / cerpc, src/Grid.h:148-163 (cubic Lagrange on nodes -1,0,1,2)
+__device__ __forceinline__ float cerp(float q0, float q1, float q2, float q3, float s) {
+    const float w0 = (-1.0f / 3.0f) * s + 0.5f * s * s - (1.0f / 6.0f) * s * s * s;
+    const float w1 = 1.0f - s * s + 0.5f * (s * s * s - s);
+    const float w2 = s + 0.5f * (s * s - s * s * s);
+    const float w3 = (1.0f / 6.0f) * (s * s * s - s);
+    return w0 * q0 + w1 * q1 + w2 * q2 + w3 * q3;
+}
+
+// cubic weights once per axis; cerp(q, s) == cw_apply(cw(s), q) bit for bit
+struct CW {
+    float w0, w1, w2, w3;
+};
+__device__ __forceinline__ CW cw(float s) {
+    CW c;
+    c.w0 = (-1.0f / 3.0f) * s + 0.5f * s * s - (1.0f / 6.0f) * s * s * s;
+    c.w1 = 1.0f - s * s + 0.5f * (s * s * s - s);
+    c.w2 = s + 0.5f * (s * s - s * s * s);
+    c.w3 = (1.0f / 6.0f) * (s * s * s - s);
+    return c;
+}
+__device__ __forceinline__ float cw_apply(const CW& c, float q0, float q1, float q2, float q3) {
+    return c.w0 * q0 + c.w1 * q1 + c.w2 * q2 + c.w3 * q3;
+}
+
+// tricubic_interpolate, src/Grid.h:165-222: x cerps, then y, then z; with
+// clamp, every 1-D stage is clamped at 0 (cerpc).
+template <bool CLAMP>
+__device__ __forceinline__ float tricubic(const Gv& g, float dx, f3 p) {
+    const float gX = fmaxf(fminf(p.x / dx - g.ox, (float)g.nx - 1.001f), 0.0f);
+    const float gY = fmaxf(fminf(p.y / dx - g.oy, (float)g.ny - 1.001f), 0.0f);
+    const float gZ = fmaxf(fminf(p.z / dx - g.oz, (float)g.nz - 1.001f), 0.0f);
+    const int i1 = (int)gX, j1 = (int)gY, k1 = (int)gZ;
+    const int sy = g.nx, sz = g.nx * g.ny;
+    const int ix[4] = {max(i1 - 1, 0), i1, i1 + 1, min(i1 + 2, g.nx - 1)};
+    const int jy[4] = {max(j1 - 1, 0) * sy, j1 * sy, (j1 + 1) * sy, min(j1 + 2, g.ny - 1) * sy};
+    const int kz[4] = {max(k1 - 1, 0) * sz, k1 * sz, (k1 + 1) * sz, min(k1 + 2, g.nz - 1) * sz};
+    const CW cx = cw(gX - (float)i1), cy = cw(gY - (float)j1), cz = cw(gZ - (float)k1);
+    float rz[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float ry[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float* row = g.a + jy[b] + kz[c];
+            float v = cw_apply(cx, row[ix[0]], row[ix[1]], row[ix[2]], row[ix[3]]);
+            ry[b] = CLAMP ? fmaxf(v, 0.0f) : v;
+        }
+        float v = cw_apply(cy, ry[0], ry[1], ry[2], ry[3]);
+        rz[c] = CLAMP ? fmaxf(v, 0.0f) : v;
+    }
+    float v = cw_apply(cz, rz[0], rz[1], rz[2], rz[3]);
+    return CLAMP ? fmaxf(v, 0.0f) : v;
+}
+
+// fit, src/FluidSolver.cu:250-252
+__device__ __forceinline__ float fitf(float v, float minIn, float maxIn, float minOut, float maxOut) {
+    return fmaxf(fminf((v - minIn) / (maxIn - minIn), 1.0f), 0.0f) * (maxOut - minOut) + minOut;
+}
+
+// ---- Perlin noise, thirdparty/cuda-noise/cuda_noise.cuh ----
+
+// 32-bit integer mix, :46-56
+__device__ __forceinline__ uint32_t nhash(uint32_t s) {
+    s = (s + 0x7ed55d16u) + (s << 12);
+    s = (s ^ 0xc761c23cu) ^ (s >> 19);
+    s = (s + 0x165667b1u) + (s << 5);
+    s = (s + 0xd3a2646cu) ^ (s << 9);
+    s = (s + 0xfd7046c5u) + (s << 3);
+    s = (s ^ 0xb55a4f09u) ^ (s >> 16);
+    return s;
+}
+
+// saturating float -> uint32 (negative and NaN -> 0, >= 2^32 -> max)
+__device__ __forceinline__ uint32_t f2u_sat(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)v;
+}
+
+// randomIntGrid :120-123, float sum left to right
+__device__ __forceinline__ uint32_t lattice_hash(float x, float y, float z, float seed) {
+    float a = x * 1723.0f;
+    a = a + y * 93241.0f;
+    a = a + z * 149812.0f;
+    a = a + 3824.0f;
+    a = a + seed;
+    return nhash(f2u_sat(a));
+}
+
+// grad :179-201 (0xD = -y+z and 0xF = -y-z as in the source)
+__device__ __forceinline__ float ngrad(uint32_t h, float x, float y, float z) {
+    switch (h & 0xFu) {
+    case 0x0: return x + y;
+    case 0x1: return -x + y;
+    case 0x2: return x - y;
+    case 0x3: return -x - y;
+    case 0x4: return x + z;
+    case 0x5: return -x + z;
+    case 0x6: return x - z;
+    case 0x7: return -x - z;
+    case 0x8: return y + z;
+    case 0x9: return -y + z;
+    case 0xA: return y - z;
+    case 0xB: return -y - z;
+    case 0xC: return y + x;
+    case 0xD: return -y + z;
+    case 0xE: return y - x;
+    default: return -y - z;
+    }
+}
+
+__device__ __forceinline__ float nfade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+__device__ __forceinline__ float nlerp(float a, float b, float r) { return a * (1.0f - r) + b * r; }
+
+// perlinNoise :570-615
+__device__ __forceinline__ float perlin(f3 pos, float scale, int seed) {
+    const float fseed = (float)seed;
+    float px = pos.x * scale, py = pos.y * scale, pz = pos.z * scale;
+    const float ix = floorf(px), iy = floorf(py), iz = floorf(pz);
+    px -= ix;
+    py -= iy;
+    pz -= iz;
+    const float u = nfade(px), v = nfade(py), w = nfade(pz);
+    const float ix1 = ix + 1.0f, iy1 = iy + 1.0f, iz1 = iz + 1.0f;
+    const float i000 = ngrad(lattice_hash(ix, iy, iz, fseed), px, py, pz);
+    const float i100 = ngrad(lattice_hash(ix1, iy, iz, fseed), px - 1.0f, py, pz);
+    const float i010 = ngrad(lattice_hash(ix, iy1, iz, fseed), px, py - 1.0f, pz);
+    const float i110 = ngrad(lattice_hash(ix1, iy1, iz, fseed), px - 1.0f, py - 1.0f, pz);
+    const float i001 = ngrad(lattice_hash(ix, iy, iz1, fseed), px, py, pz - 1.0f);
+    const float i101 = ngrad(lattice_hash(ix1, iy, iz1, fseed), px - 1.0f, py, pz - 1.0f);
+    const float i011 = ngrad(lattice_hash(ix, iy1, iz1, fseed), px, py - 1.0f, pz - 1.0f);
+    const float i111 = ngrad(lattice_hash(ix1, iy1, iz1, fseed), px - 1.0f, py - 1.0f, pz - 1.0f);
+    const float x00 = nlerp(i000, i100, u);
+    const float x10 = nlerp(i010, i110, u);
+    const float x01 = nlerp(i001, i101, u);
+    const float x11 = nlerp(i011, i111, u);
+    const float y0 = nlerp(x00, x10, v);
+    const float y1 = nlerp(x01, x11, v);
+    return nlerp(y0, y1, w);
+}
+
+__device__ __forceinline__ bool abs_gt_1e5(float v) { return (double)fabsf(v) > 1e-5; }
+
+} // namespace blz

@@ -1,0 +1,179 @@
+/*
+ * blaze.h — C ABI of the MI355X smoke/fire solver step.
+ *
+ * Plain C: POD structs, pointers, sizes and int status codes. No HIP or
+ * torch types cross this boundary. Every call returns BLZ_OK (0) or a
+ * negative BLZ_E_* code and leaves a thread-local message readable with
+ * blz_last_error().
+ *
+ * Reference interface that this ABI replaces (asiroisvigneux/Blaze):
+ *   class FluidSolver            src/FluidSolver.h:38-110
+ *     FluidSolver(Timer*, SceneSettings*)   src/FluidSolver.cu:671-741  -> blz_create
+ *     ~FluidSolver()                        src/FluidSolver.cu:743-768  -> blz_destroy
+ *     addSource()                           src/FluidSolver.cu:770-821  -> blz_add_source
+ *     step()                                src/FluidSolver.cu:858-955  -> blz_step
+ *     getTempGrid()                         src/FluidSolver.h:108       -> blz_get_field(BLZ_FIELD_T)
+ *   struct SceneSettings (solver subset)    src/SceneSettings.h:32-71   -> blz_settings
+ *   Parser::sceneParse                      src/Parser.h:41-172         -> blz_parse_scene
+ *   Parser::sourceParticleParse             src/Parser.h:174-211        -> blz_read_particles
+ *
+ * Threading: one handle per host thread. A handle owns all of its device
+ * memory and one HIP stream; calls are synchronous from the host's view
+ * unless named *_async. Distinct handles may run concurrently.
+ */
+#ifndef BLAZE_H
+#define BLAZE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLZ_ABI_VERSION 1
+
+/* opaque solver handle (owns device memory and one HIP stream) */
+typedef struct blz_solver blz_solver;
+
+/* status codes */
+#define BLZ_OK 0
+#define BLZ_E_INVALID (-1)   /* bad argument or settings */
+#define BLZ_E_HIP (-2)       /* HIP runtime error */
+#define BLZ_E_NOMEM (-3)     /* device or host allocation failed */
+#define BLZ_E_IO (-4)        /* file missing or unreadable */
+#define BLZ_E_PARSE (-5)     /* malformed .blz / missing key */
+#define BLZ_E_NUMERIC (-6)   /* non-finite state detected */
+
+/* field selectors (front buffers unless stated) */
+#define BLZ_FIELD_T 0        /* temperature, W x H x D */
+#define BLZ_FIELD_U 1        /* x face velocity, (W+1) x H x D */
+#define BLZ_FIELD_V 2        /* y face velocity, W x (H+1) x D (grid y is flipped) */
+#define BLZ_FIELD_W 3        /* z face velocity, W x H x (D+1) */
+#define BLZ_FIELD_DENSITY 4  /* smoke density, W x H x D (only when enabled) */
+#define BLZ_FIELD_P 5        /* pressure, W x H x D, natural layout on copy */
+#define BLZ_FIELD_RHS 6      /* divergence right-hand side, W x H x D */
+#define BLZ_NUM_FIELDS 7
+
+/* single stages, for per-kernel parity harnesses (blz_run_stage) */
+#define BLZ_STAGE_COOLDOWN 0   /* temperature_cooldown_kernel  src/FluidSolver.cu:118-124 */
+#define BLZ_STAGE_DRAG 1       /* apply_drag_kernel            src/FluidSolver.cu:126-140 */
+#define BLZ_STAGE_VORTICITY 2  /* vorticity_confinement_kernel src/FluidSolver.cu:203-248 */
+#define BLZ_STAGE_BUOYANCY 3   /* add_buoyancy_kernel          src/FluidSolver.cu:142-156 */
+#define BLZ_STAGE_WIND 4       /* add_wind_kernel              src/FluidSolver.cu:337-390 */
+#define BLZ_STAGE_TURBULENCE 5 /* compute_turbulence + add_curl_noise src/FluidSolver.cu:271-335 */
+#define BLZ_STAGE_DIVERGENCE 6 /* compute_divergence_kernel    src/FluidSolver.cu:475-515 */
+#define BLZ_STAGE_PRESSURE 7   /* gs_solve_kernel              src/FluidSolver.cu:517-599 */
+#define BLZ_STAGE_PROJECT_UPDATE 8 /* pressure_gradient_update src/FluidSolver.cu:601-669 */
+#define BLZ_STAGE_ADVECT 9     /* advect_RK3_kernel x4 + swap  src/FluidSolver.cu:431-465,942-953 */
+#define BLZ_NUM_STAGES 10
+
+/*
+ * Solver subset of SceneSettings (src/SceneSettings.h:32-71) with explicit
+ * arrays in place of CUDA vector types, plus optional extensions that
+ * default to the reference behaviour (see blz_settings_default).
+ */
+typedef struct blz_settings {
+    int grid_res[3];          /* gridRes: W, H, D (cells) */
+    float dx;                 /* dx */
+    float dt;                 /* dt (per frame) */
+    float domain_min[3];      /* domainBboxMin (world) */
+    float domain_max[3];      /* domainBboxMax (world) */
+    int closed_bounds[6];     /* closedBounds: xMin,xMax,yMin,yMax,zMin,zMax */
+    int max_iter_solve;       /* maxIterSolve (cap when residual_tol > 0) */
+    float density;            /* density rho of the projection */
+    float gravity;            /* gravity (negative = world down) */
+    float buoyancy;
+    float cooling_rate;
+    float vorticity_conf;
+    float drag;
+    int wind_dir;             /* 0:X+ 1:X- 2:Z+ 3:Z- */
+    float wind_amp;
+    float wind_speed;
+    float wind_turb_amp;
+    float wind_turb_scale;
+    float turbulence_amp;
+    float turbulence_scale;
+    float turb_mask_temp_ramp[2];
+    float turb_mask_vel_ramp[2];
+    /* source description (src/SceneSettings.h:36-39) */
+    int source_animated;
+    int source_range[2];
+    int source_max_particle_count;
+    /* extensions (no reference counterpart; defaults reproduce the reference) */
+    int noise_seed;           /* turbulence Perlin seed, reference 123 */
+    float residual_tol;       /* >0: RBGS until ||div u||_inf <= tol (cap max_iter_solve) */
+    int residual_check_every; /* sweeps between residual checks (default 10) */
+    float cfl_number;         /* >0: substeps n = ceil(max|u| dt / (cfl dx)) */
+    int max_substeps;         /* cap on n (default 8) */
+    float smoke_density;      /* >0: enable smoke density grid, splat value */
+    int warm_start;           /* 0 plain RBGS, 1 learned warm start (see blz_set_warm_start_model) */
+} blz_settings;
+
+/* per-step statistics of the last blz_step / blz_run_stage call */
+typedef struct blz_stats {
+    int substeps;             /* CFL substeps taken in the last step */
+    int rbgs_iters;           /* RBGS sweeps in the last projection (sum over substeps) */
+    int rbgs_iters_last;      /* RBGS sweeps in the last substep's projection */
+    double residual;          /* ||div u||_inf proxy after the last projection (FP64), -1 if not computed */
+    double residual_initial;  /* same measure before the first sweep of the last projection */
+    int warm_start_used;      /* 1 when the learned prediction was kept */
+    int warm_start_fallbacks; /* number of rejected predictions so far */
+    double stage_ms[BLZ_NUM_STAGES]; /* device time per stage from HIP events (0 if timing off) */
+    double source_ms;         /* device time of the last blz_add_source */
+    double step_ms;           /* device time of the last blz_step */
+} blz_stats;
+
+/* synthetic particle set: uniform in a ball, deterministic for a seed */
+typedef struct blz_particle_gen {
+    int count;
+    unsigned long long seed;
+    float center[3];          /* world */
+    float radius;             /* world */
+    float pscale;             /* per-particle pscale (diameter of the splat) */
+    float temp;
+    float vel[3];             /* constant velocity */
+    float radial_vel;         /* + radial_vel * unit(pos - center) */
+} blz_particle_gen;
+
+/* ---- settings, scene files, particles (host only, no GPU needed) ---- */
+int blz_abi_version(void);
+int blz_settings_default(blz_settings* s);
+/* .blz JSON -> settings. scene_dir receives the directory of the file
+   (src/Parser.h:43-45); particle_file receives the "particleFile" value. */
+int blz_parse_scene(const char* path, blz_settings* out,
+                    char* scene_dir, int scene_dir_len,
+                    char* particle_file, int particle_file_len,
+                    blz_particle_gen* gen_out, int* has_gen);
+/* .part: 8 float32 per particle [px,py,pz,vx,vy,vz,pscale,temp], no header */
+int blz_read_particles(const char* path, float* part8, int max_count, int* count);
+int blz_count_particles(const char* path, int* count);
+int blz_generate_particles(const blz_particle_gen* g, float* part8);
+
+/* ---- solver ---- */
+int blz_create(const blz_settings* s, blz_solver** out);
+int blz_destroy(blz_solver* h);
+/* Particle-to-grid source (addSource). part8 != NULL uploads count particles
+   first; part8 == NULL re-splats the previously uploaded set. */
+int blz_add_source(blz_solver* h, const float* part8, int count);
+/* One frame (step): forces, projection, advection, with CFL substeps when
+   enabled. time is the frame's simulation time (Timer::time). */
+int blz_step(blz_solver* h, float time);
+/* Enqueue the same frame without the final stream synchronisation. */
+int blz_step_async(blz_solver* h, float time);
+int blz_sync(blz_solver* h);
+/* One stage with an explicit dt, for parity harnesses. */
+int blz_run_stage(blz_solver* h, int stage, float time, float dt);
+int blz_get_field(blz_solver* h, int which, void** dev_ptr, int dims[3]);
+int blz_copy_field(blz_solver* h, int which, float* host);
+int blz_set_field(blz_solver* h, int which, const float* host);
+int blz_get_stats(blz_solver* h, blz_stats* out);
+/* Enable per-stage HIP event timing (adds events around each stage). */
+int blz_set_timing(blz_solver* h, int on);
+/* Override the RBGS kernel variant (0 = auto). For benchmarks and tests. */
+int blz_set_rbgs_variant(blz_solver* h, int variant);
+
+const char* blz_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLAZE_H */
