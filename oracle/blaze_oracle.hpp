@@ -251,6 +251,7 @@ template <class R> struct Grid {
         const R radius = pscale / R(2);
         auto trunc = [](R v) -> int {
             /* (int) conversion; clamp only values far outside any grid */
+            if (!(v == v)) v = -R(1 << 30); /* NaN: empty footprint, as on the GPU */
             if (v > R(1 << 30)) v = R(1 << 30);
             if (v < -R(1 << 30)) v = -R(1 << 30);
             return (int)v;

@@ -1,0 +1,218 @@
+"""GPU parity: every solver stage and whole frames of the HIP path (through
+the C ABI) against the CPU oracle on the same seeded inputs.
+
+The bar is bit-exact FP32 equality: both sides evaluate the same formulas in
+the same order with IEEE division/sqrt and no FMA contraction, and every
+reference race (float atomics, inter-block Gauss-Seidel reads, boundary
+copies) is replaced by one fixed order on both sides.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+from helpers import SCENES, compare, dims, random_state, settings  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+B = pytest.importorskip("blaze_amd")
+O = pytest.importorskip("oracle")
+
+
+def pair(s, state):
+    g = B.Solver(s)
+    o = O.Oracle(s)
+    for w, a in state.items():
+        g.set_field(w, a)
+        o.set_field(w, a)
+    return g, o
+
+
+def test_field_roundtrip():
+    s = settings(B, (16, 12, 10), smoke_density=1.0)
+    st = random_state(s, 1, smoke=True)
+    g = B.Solver(s)
+    for w, a in st.items():
+        g.set_field(w, a)
+        compare(f"roundtrip {w}", g.field(w), a)
+
+
+STAGE_CASES = [
+    ("cooldown", B.STAGE_COOLDOWN, dict(cooling_rate=0.9)),
+    ("drag", B.STAGE_DRAG, dict(drag=0.5)),
+    ("vorticity", B.STAGE_VORTICITY, dict(vorticity_conf=0.5)),
+    ("buoyancy", B.STAGE_BUOYANCY, dict(buoyancy=0.7)),
+    ("wind_x", B.STAGE_WIND, dict(wind_dir=0, wind_amp=0.2, wind_speed=0.2, wind_turb_amp=7.5,
+                                  wind_turb_scale=0.125)),
+    ("wind_xneg", B.STAGE_WIND, dict(wind_dir=1, wind_amp=0.3, wind_speed=0.4, wind_turb_amp=2.0,
+                                     wind_turb_scale=0.5)),
+    ("wind_z", B.STAGE_WIND, dict(wind_dir=2, wind_amp=0.2, wind_speed=0.2, wind_turb_amp=7.5,
+                                  wind_turb_scale=0.125)),
+    ("wind_zneg", B.STAGE_WIND, dict(wind_dir=3, wind_amp=0.6, wind_speed=0.4, wind_turb_amp=3.75,
+                                     wind_turb_scale=0.625)),
+    ("turbulence", B.STAGE_TURBULENCE, dict(turbulence_amp=0.5, turbulence_scale=0.25, noise_seed=7,
+                                            turb_mask_temp_ramp=(0.0, 2.0), turb_mask_vel_ramp=(0.0, 3.0))),
+    ("turbulence_coarse", B.STAGE_TURBULENCE, dict(turbulence_amp=2.0, turbulence_scale=2.0,
+                                                   turb_mask_temp_ramp=(0.03, 0.14),
+                                                   turb_mask_vel_ramp=(0.0, 0.06))),
+    ("divergence_closed", B.STAGE_DIVERGENCE, dict(closed_bounds=(1, 1, 1, 1, 1, 1))),
+    ("divergence_mixed", B.STAGE_DIVERGENCE, dict(closed_bounds=(0, 1, 1, 0, 0, 1))),
+    ("pressure_fixed", B.STAGE_PRESSURE, dict(max_iter_solve=7)),
+    ("pressure_tol", B.STAGE_PRESSURE, dict(max_iter_solve=60, residual_tol=5.0, residual_check_every=7)),
+    ("update_closed", B.STAGE_PROJECT_UPDATE, dict(closed_bounds=(1, 1, 1, 1, 1, 1))),
+    ("update_open", B.STAGE_PROJECT_UPDATE, dict(closed_bounds=(0, 0, 0, 0, 0, 0))),
+    ("update_mixed", B.STAGE_PROJECT_UPDATE, dict(closed_bounds=(1, 0, 1, 0, 0, 1))),
+    ("advect", B.STAGE_ADVECT, dict()),
+    ("advect_smoke", B.STAGE_ADVECT, dict(smoke_density=1.0)),
+]
+
+
+@pytest.mark.parametrize("name,stage,kw", STAGE_CASES, ids=[c[0] for c in STAGE_CASES])
+@pytest.mark.parametrize("res", [(32, 24, 20), (16, 16, 16)], ids=["32x24x20", "16c"])
+def test_stage_bitwise(name, stage, kw, res):
+    print(f"[stage] {name} res={res}", flush=True)
+    s = settings(B, res, **kw)
+    st = random_state(s, seed=sum(name.encode()), vel_scale=0.7, smoke=s.smoke_density > 0)
+    g, o = pair(s, st)
+    time, dt = 1.375, 1.0 / 24.0
+    g.run_stage(stage, time, dt)
+    o.run_stage(stage, time, dt)
+    which = [0, 1, 2, 3, 5, 6] + ([4] if s.smoke_density > 0 else [])
+    for w in which:
+        compare(f"{name}:{['T', 'U', 'V', 'W', 'density', 'P', 'rhs'][w]}", g.field(w), o.field(w))
+    if stage == B.STAGE_PRESSURE:
+        gs, os_ = g.stats(), o.stats()
+        assert gs.rbgs_iters_last == os_["rbgs_iters_last"]
+        if s.residual_tol > 0:
+            assert gs.residual == os_["residual"]
+            assert gs.residual_initial == os_["residual_initial"]
+
+
+def random_particles(n, seed, lo=-0.1, hi=1.1):
+    rng = np.random.default_rng(seed)
+    p = np.zeros((n, 8), np.float32)
+    p[:, 0:3] = rng.uniform(lo, hi, (n, 3))
+    p[:, 3:6] = rng.standard_normal((n, 3))
+    p[:, 6] = rng.uniform(0.0, 0.15, n)      # some below dx/2 are skipped
+    p[:, 7] = rng.uniform(-0.5, 2.0, n)
+    p[::7, 3] = 1e-6                         # below the 1e-5 channel threshold
+    p[::11, 7] = 0.0
+    return p
+
+
+@pytest.mark.parametrize("smoke", [False, True])
+def test_p2g_bitwise(smoke):
+    s = settings(B, (32, 24, 20), smoke_density=0.8 if smoke else 0.0)
+    st = random_state(s, seed=3, smoke=smoke)
+    g, o = pair(s, st)
+    parts = random_particles(3000, 11)
+    g.add_source(parts)
+    o.add_source(parts)
+    for w in [0, 1, 2, 3] + ([4] if smoke else []):
+        compare(f"p2g:{w}", g.field(w), o.field(w))
+    # re-splat of the uploaded set on a changed state
+    st2 = random_state(s, seed=4, smoke=smoke)
+    for w, a in st2.items():
+        g.set_field(w, a)
+        o.set_field(w, a)
+    g.add_source(None)
+    o.add_source(None)
+    for w in [0, 1, 2, 3] + ([4] if smoke else []):
+        compare(f"p2g-resplat:{w}", g.field(w), o.field(w))
+
+
+def test_p2g_dense_hotspot():
+    """Many particles per cell (the contention case of the reference)."""
+    s = settings(B, (32, 32, 32))
+    g, o = pair(s, {})
+    gen = B.ParticleGen(count=50000, seed=5, radius=0.1, pscale=2.0 / 32, temp=5.0, radial_vel=2.0)
+    for a, v in enumerate((0.5, 0.3, 0.5)):
+        gen.center[a] = v
+    parts = B.generate_particles(gen)
+    g.add_source(parts)
+    o.add_source(parts)
+    for w in (0, 1, 2, 3):
+        compare(f"hotspot:{w}", g.field(w), o.field(w))
+
+
+def scene_settings(name, res=None, **kw):
+    s, _, _, gen = B.parse_scene(os.path.join(SCENES, name))
+    if res is not None:
+        W, H, D = res
+        s.grid_res[0], s.grid_res[1], s.grid_res[2] = W, H, D
+        s.dx = 1.0 / W
+        s.domain_max[0], s.domain_max[1], s.domain_max[2] = 1.0, H / W, D / W
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s, gen
+
+
+def run_frames(s, gen, frames, source_frames, check_every=1, fields=(0, 1, 2, 3, 4, 5)):
+    parts = B.generate_particles(gen)
+    g = B.Solver(s)
+    o = O.Oracle(s)
+    t = np.float32(0.0)
+    for it in range(1, frames + 1):
+        if it <= source_frames:
+            g.add_source(parts if it == 1 else None)
+            o.add_source(parts if it == 1 else None)
+        g.step(float(t))
+        o.step(float(t))
+        gs, os_ = g.stats(), o.stats()
+        assert gs.substeps == os_["substeps"], f"frame {it}: substeps {gs.substeps} vs {os_['substeps']}"
+        assert gs.rbgs_iters == os_["rbgs_iters"], f"frame {it}: iters {gs.rbgs_iters} vs {os_['rbgs_iters']}"
+        if it % check_every == 0 or it == frames:
+            for w in fields:
+                if w == 4 and s.smoke_density <= 0:
+                    continue
+                compare(f"frame {it} field {w}", g.field(w), o.field(w))
+        t = np.float32(t + np.float32(s.dt))
+    return g, o
+
+
+def test_frames_plume64_bitwise():
+    """64^3 plume (solid walls, open top, buoyancy, smoke density, CFL
+    substeps): 12 frames bit-exact."""
+    s, gen = scene_settings("plume64.blz")
+    print("[frames] plume64", flush=True)
+    run_frames(s, gen, 12, 12, check_every=3)
+
+
+def test_frames_plume128_features_bitwise():
+    """Every force of the 128^3 plume (vorticity, curl noise, wind, open x
+    sides and top) on a 48^3 grid, 8 frames bit-exact."""
+    s, gen = scene_settings("plume128.blz", res=(48, 48, 48))
+    gen.pscale = 0.75 / 48
+    gen.count = 20000
+    print("[frames] plume128 features @48^3", flush=True)
+    run_frames(s, gen, 8, 8, check_every=2)
+
+
+def test_frames_explosion_tol_bitwise():
+    """Explosion features (radial source once, cooling, vorticity, residual-
+    driven RBGS, CFL substeps, closed box) on a 40^3 grid."""
+    s, gen = scene_settings("explosion256.blz", res=(40, 40, 40), residual_tol=2e-3, max_iter_solve=120)
+    gen.count = 20000
+    gen.pscale = 0.6 / 40 * 2
+    print("[frames] explosion @40^3", flush=True)
+    run_frames(s, gen, 6, 1, check_every=2)
+
+
+def test_determinism_repeat():
+    """Two GPU runs of the same frames are bitwise identical."""
+    s, gen = scene_settings("plume128.blz", res=(32, 32, 32))
+    gen.count = 5000
+    outs = []
+    for _ in range(2):
+        parts = B.generate_particles(gen)
+        g = B.Solver(s)
+        t = np.float32(0.0)
+        for it in range(1, 5):
+            g.add_source(parts if it == 1 else None)
+            g.step(float(t))
+            t = np.float32(t + np.float32(s.dt))
+        outs.append([g.field(w) for w in (0, 1, 2, 3, 5)])
+    for a, b in zip(*outs):
+        compare("repeat", a, b)
