@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""bench.py — headline benchmark of the MI355X solver step.
+
+Workload: the 256^3 FP32 explosion (scenes/explosion256.blz): 1M particles
+sourced once, cooling, vorticity confinement, buoyancy, RBGS pressure solve
+to ||div u||_inf <= 1e-4 (cap 500 sweeps), CFL substeps. A bench "step" is one
+frame (FluidSolver::step); every CFL substep counts as a solver step in the
+voxel-steps/s metric. Warmup frames (including the sourcing frame) are
+untimed; the K timed frames are bracketed by stream synchronisations.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel
+(the RBGS colour pass; algorithmic 6 B/voxel per pass = 12 B/voxel per
+sweep, timed with HIP events on the solver stream) and a CPU baseline (the
+oracle on the host cores, bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+RBGS_BYTES_PER_VOXEL_PASS = 6.0   # 12 B/voxel per sweep (read p, read rhs, write p)
+# compulsory bytes per voxel of the non-solve part of a substep of this
+# workload: cooling 8, vorticity 24, buoyancy 12, divergence 16, update 28,
+# advection 40 (T, U, V, W, density)
+FIXED_BYTES_PER_VOXEL = 128.0
+RESIDUAL_BYTES_PER_VOXEL = 8.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scene", default=os.path.join(ROOT, "scenes", "explosion256.blz"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sweeps", type=int, default=6)
+    ap.add_argument("--rbgs-variant", type=int, default=0)
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg, quiet=False):
+    if not quiet:
+        print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(s, parts, substeps, iters_per_substep, checks_per_substep, n_sweeps, quiet):
+    """Time the oracle (all host threads; bitwise equal to serial) on one
+    substep of the same 256^3 workload, stage by stage, plus n_sweeps RBGS
+    sweeps, and scale to the GPU run's average sweeps and residual checks per
+    substep. Returns the CPU voxel-steps/s and a description."""
+    import oracle as O
+
+    s2 = s.copy()
+    s2.residual_tol = 0.0
+    s2.max_iter_solve = n_sweeps
+    o = O.Oracle(s2)
+    cores = O.max_threads()
+    t0 = time.perf_counter()
+    o.add_source(parts)
+    t_src = time.perf_counter() - t0
+    dt = float(s.dt) / max(1, substeps)
+    stages = []
+    if abs(s.cooling_rate) > 0:
+        stages.append(0)
+    if s.drag > 0:
+        stages.append(1)
+    if abs(s.vorticity_conf) > 0:
+        stages.append(2)
+    if abs(s.buoyancy) > 0:
+        stages.append(3)
+    if s.wind_amp > 0:
+        stages.append(4)
+    if s.turbulence_amp > 0:
+        stages.append(5)
+    stages += [6]
+    t_fixed = 0.0
+    per = {}
+    for st in stages:
+        t0 = time.perf_counter()
+        o.run_stage(st, 0.0, dt)
+        per[st] = time.perf_counter() - t0
+        t_fixed += per[st]
+    t0 = time.perf_counter()
+    o.run_stage(7, 0.0, dt)   # n_sweeps sweeps
+    t_sweeps = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.residual(dt)
+    t_res = time.perf_counter() - t0
+    for st in (8, 9):
+        t0 = time.perf_counter()
+        o.run_stage(st, 0.0, dt)
+        per[st] = time.perf_counter() - t0
+        t_fixed += per[st]
+    t_sweep = t_sweeps / n_sweeps
+    t_substep = t_fixed + iters_per_substep * t_sweep + checks_per_substep * t_res
+    W, H, D = s.grid_res
+    value = W * H * D / t_substep
+    log(f"[cpu] cores {cores} source {t_src:.2f}s fixed {t_fixed:.2f}s sweep {t_sweep * 1e3:.1f}ms "
+        f"residual {t_res * 1e3:.1f}ms -> substep {t_substep:.2f}s", quiet)
+    sample = (f"oracle FP32 on {W}x{H}x{D}: one substep's stages timed once (fixed part "
+              f"{t_fixed:.2f}s), {n_sweeps} RBGS sweeps ({t_sweep * 1e3:.1f} ms/sweep) and one residual "
+              f"check ({t_res * 1e3:.1f} ms), combined with the GPU run's averages of "
+              f"{iters_per_substep:.1f} sweeps and {checks_per_substep:.1f} checks per substep")
+    return {"value": value, "unit": "voxel-steps/s", "cores": cores, "kind": "port", "sample": sample}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the solver step is single-GPU; multi-GPU slab decomposition is design-only)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import blaze_amd as B
+
+    s, _, _, gen = B.parse_scene(a.scene)
+    W, H, D = s.grid_res
+    N3 = W * H * D
+    parts = B.generate_particles(gen)
+    g = B.Solver(s)
+    g.set_timing(True)
+    g.set_rbgs_variant(a.rbgs_variant)
+    t = np.float32(0.0)
+    frame = s.source_range[0]
+    sr_end = s.source_range[1]
+
+    def one_frame():
+        nonlocal t, frame
+        if frame <= sr_end:
+            g.add_source(parts if frame == s.source_range[0] else None)
+        g.step(float(t))
+        st = g.stats()
+        t = np.float32(t + np.float32(s.dt))
+        frame += 1
+        return st
+
+    tw = time.perf_counter()
+    for _ in range(a.warmup):
+        st = one_frame()
+        log(f"[warmup] frame {frame - 1}: substeps {st.substeps} sweeps {st.rbgs_iters} "
+            f"residual {st.residual:.3e} step {st.step_ms:.1f} ms", a.quiet)
+    g.sync()
+    log(f"[warmup] {time.perf_counter() - tw:.1f}s", a.quiet)
+
+    tot_sub = tot_iters = tot_checks = tot_launch = 0
+    tot_rbgs_ms = 0.0
+    stage_ms = [0.0] * B.NUM_STAGES
+    residuals = []
+    g.sync()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        st = one_frame()
+        tot_sub += st.substeps
+        tot_iters += st.rbgs_iters
+        tot_checks += st.residual_checks
+        tot_launch += st.rbgs_launches
+        tot_rbgs_ms += st.rbgs_kernel_ms
+        for i in range(B.NUM_STAGES):
+            stage_ms[i] += st.stage_ms[i]
+        residuals.append(st.residual)
+        log(f"[frame {frame - 1}] substeps {st.substeps} sweeps {st.rbgs_iters} checks "
+            f"{st.residual_checks} residual {st.residual:.3e} step {st.step_ms:.1f} ms", a.quiet)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+
+    value = N3 * tot_sub / elapsed
+    ms_per_step = elapsed * 1e3 / a.steps
+    avg_launch_s = (tot_rbgs_ms / 1e3) / max(1, tot_launch)
+    achieved = RBGS_BYTES_PER_VOXEL_PASS * N3 / avg_launch_s / 1e9 if tot_launch else 0.0
+    algo_bytes = N3 * (FIXED_BYTES_PER_VOXEL * tot_sub + 12.0 * tot_iters
+                       + RESIDUAL_BYTES_PER_VOXEL * tot_checks)
+    step_gbs = algo_bytes / elapsed / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "rbgs_pass_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        if tuple(tj.get("grid", ())) == (W, H, D):
+            traffic = tj.get("bytes_per_launch")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(s, parts, max(1, round(tot_sub / a.steps)), tot_iters / max(1, tot_sub),
+                           tot_checks / max(1, tot_sub), a.cpu_sweeps, a.quiet)
+
+    out = {
+        "metric": "voxel-steps/s at 256^3 (plain RBGS, ||div u||_inf <= 1e-4 cap 500)",
+        "value": value,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded particle ball, scenes/explosion256.blz)",
+        "config": {
+            "workload": f"explosion256: {W}x{H}x{D} FP32 MAC grid, {gen.count} particles r={gen.radius} "
+                        f"radial vel {gen.radial_vel} temp {gen.temp}, cooling {s.cooling_rate}, "
+                        f"vorticity {s.vorticity_conf}, RBGS to {s.residual_tol:g} (cap {s.max_iter_solve}), "
+                        f"CFL {s.cfl_number:g} (max {s.max_substeps} substeps), frames "
+                        f"{a.warmup + 1}..{a.warmup + a.steps}",
+            "grid": [W, H, D],
+            "substeps": tot_sub,
+            "rbgs_sweeps": tot_iters,
+            "residual_checks": tot_checks,
+            "final_residual": residuals[-1] if residuals else None,
+            "parallelism": "single GPU",
+        },
+        "hbm_gbs_algorithmic_step": step_gbs,
+        "stage_ms": {B.STAGE_NAMES[i]: stage_ms[i] for i in range(B.NUM_STAGES) if stage_ms[i] > 0},
+        "roofline": {
+            "kernel": "k_rbgs_pass (RBGS colour pass)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "avg_launch_us": avg_launch_s * 1e6,
+            "launches": tot_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,7 @@ class Stats(C.Structure):
         ("residual", C.c_double), ("residual_initial", C.c_double),
         ("warm_start_used", C.c_int), ("warm_start_fallbacks", C.c_int),
         ("stage_ms", C.c_double * NUM_STAGES), ("source_ms", C.c_double), ("step_ms", C.c_double),
+        ("rbgs_kernel_ms", C.c_double), ("rbgs_launches", C.c_int), ("residual_checks", C.c_int),
     ]
 
 
@@ -109,6 +110,7 @@ def _load():
         "blz_get_stats": ([vp, P(Stats)], C.c_int),
         "blz_set_timing": ([vp, C.c_int], C.c_int),
         "blz_set_rbgs_variant": ([vp, C.c_int], C.c_int),
+        "blz_selftest_division": ([C.c_int, P(C.c_ulonglong)], C.c_int),
         "blz_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -248,6 +250,14 @@ class Solver:
 
     def set_rbgs_variant(self, v: int):
         _check(_lib.blz_set_rbgs_variant(self._h, v), "blz_set_rbgs_variant")
+
+
+def selftest_division(divisor: int) -> int:
+    """Mismatches of the solver's device division by `divisor` (3..6) against
+    IEEE division over all 2^32 floats (expected 0)."""
+    n = C.c_ulonglong(0)
+    _check(_lib.blz_selftest_division(divisor, C.byref(n)), "blz_selftest_division")
+    return n.value
 
 
 def abi_version() -> int:

@@ -23,14 +23,13 @@ Solver::~Solver() {
     for (auto& F : f)
         for (auto& b : F.buf)
             if (b) (void)hipFree(b);
-    if (p.c[0]) (void)hipFree(p.c[0]);
-    if (rhs.c[0]) (void)hipFree(rhs.c[0]);
+    if (prb) (void)hipFree(prb);
     if (tmp) (void)hipFree(tmp);
     if (red) (void)hipFree(red);
     if (red_host) (void)hipHostFree(red_host);
     if (part) (void)hipFree(part);
     free_p2g_scratch(*this);
-    for (auto& e : ev)
+    for (auto& e : evpool)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -48,6 +47,11 @@ static int validate(const blz_settings& s) {
     const double cells = (double)(s.grid_res[0] + 1) * (s.grid_res[1] + 1) * (s.grid_res[2] + 1);
     if (cells * 5.0 >= 4294967296.0 || cells >= 2147483648.0) {
         set_error("grid too large for 32-bit cell indices");
+        return BLZ_E_INVALID;
+    }
+    const double padded = (double)(s.grid_res[0] / 2 + 2) * (s.grid_res[1] + 2) * ((s.grid_res[2] + 3) / 4 + 2) * 4;
+    if (16.0 * padded >= 4294967296.0) {
+        set_error("grid too large for 32-bit pressure plane offsets");
         return BLZ_E_INVALID;
     }
     if (!(s.dx > 0.0f) || !(s.dt > 0.0f) || !(s.density > 0.0f)) {
@@ -94,24 +98,27 @@ int Solver::init(const blz_settings& st_) {
         }
     }
     const size_t N = (size_t)W * H * D;
-    float* pp = nullptr;
-    float* rr = nullptr;
-    if (hipMalloc(&pp, N * sizeof(float)) != hipSuccess || hipMalloc(&rr, N * sizeof(float)) != hipSuccess ||
-        hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess || hipMalloc(&red, 8 * sizeof(double)) != hipSuccess) {
-        if (pp) (void)hipFree(pp);
-        if (rr) (void)hipFree(rr);
+    L.Wh = W / 2;
+    L.H = H;
+    L.D = D;
+    L.Wp = W / 2 + 2;
+    L.Hp = H + 2;
+    L.Dg = (D + 3) / 4;
+    Np = (size_t)L.Wp * L.Hp * (L.Dg + 2) * 4;
+    if (hipMalloc(&prb, 4 * Np * sizeof(float)) != hipSuccess || hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&red, (8 + RESIDUAL_BLOCKS) * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc failed for solver scratch");
         return BLZ_E_NOMEM;
     }
-    p.c[0] = pp;
-    p.c[1] = pp + N / 2;
-    rhs.c[0] = rr;
-    rhs.c[1] = rr + N / 2;
-    BLZ_HIP(hipMemset(pp, 0, N * sizeof(float))); // pressure zeroed once, :715-716
-    BLZ_HIP(hipMemset(rr, 0, N * sizeof(float)));
-    BLZ_HIP(hipMemset(red, 0, 8 * sizeof(double)));
+    p.c[0] = prb;
+    p.c[1] = prb + Np;
+    rhs.c[0] = prb + 2 * Np;
+    rhs.c[1] = prb + 3 * Np;
+    // pressure zeroed once (:715-716): +0 inside, -0 in the ghost shell
+    if (int e = launch_init_pressure(*this)) return e;
+    BLZ_HIP(hipMemset(red, 0, (8 + RESIDUAL_BLOCKS) * sizeof(double)));
+    BLZ_HIP(hipStreamSynchronize(stream));
     BLZ_HIP(hipHostMalloc((void**)&red_host, 8 * sizeof(double), hipHostMallocDefault));
-    for (auto& e : ev) BLZ_HIP(hipEventCreate(&e));
     std::memset(&st, 0, sizeof(st));
     st.residual = -1.0;
     st.residual_initial = -1.0;
@@ -119,6 +126,7 @@ int Solver::init(const blz_settings& st_) {
 }
 
 int Solver::residual(float dt_, double* out) {
+    st.residual_checks++;
     if (int e = launch_residual(*this, dt_, red)) return e;
     BLZ_HIP(hipMemcpyAsync(red_host, red, sizeof(double), hipMemcpyDeviceToHost, stream));
     BLZ_HIP(hipStreamSynchronize(stream));
@@ -134,12 +142,66 @@ int Solver::max_abs_velocity(float* out) {
     return BLZ_OK;
 }
 
+int Solver::span_begin(int* e0) {
+    *e0 = -1;
+    if (!timing) return BLZ_OK;
+    if (ev_used == evpool.size()) {
+        hipEvent_t e;
+        BLZ_HIP(hipEventCreate(&e));
+        evpool.push_back(e);
+    }
+    *e0 = (int)ev_used++;
+    BLZ_HIP(hipEventRecord(evpool[*e0], stream));
+    return BLZ_OK;
+}
+
+int Solver::span_end(int kind, int e0, int launches) {
+    if (!timing || e0 < 0) return BLZ_OK;
+    int e1 = -1;
+    if (int e = span_begin(&e1)) return e;
+    spans.push_back(Span{kind, e0, e1, launches});
+    return BLZ_OK;
+}
+
+// after a stream synchronisation: fold the recorded spans into the stats
+int Solver::resolve_spans() {
+    for (const Span& sp : spans) {
+        float ms = 0.0f;
+        BLZ_HIP(hipEventElapsedTime(&ms, evpool[sp.e0], evpool[sp.e1]));
+        if (sp.kind < BLZ_NUM_STAGES) st.stage_ms[sp.kind] += ms;
+        else if (sp.kind == SPAN_SOURCE) st.source_ms += ms;
+        else if (sp.kind == SPAN_STEP) st.step_ms += ms;
+        else if (sp.kind == SPAN_RBGS) {
+            st.rbgs_kernel_ms += ms;
+            st.rbgs_launches += sp.launches;
+        }
+    }
+    spans.clear();
+    ev_used = 0;
+    return BLZ_OK;
+}
+
+void clear_step_stats(blz_stats& st) {
+    for (double& v : st.stage_ms) v = 0.0;
+    st.step_ms = 0.0;
+    st.rbgs_kernel_ms = 0.0;
+    st.rbgs_launches = 0;
+    st.rbgs_iters = 0;
+    st.residual_checks = 0;
+}
+
 // gs_solve_kernel (src/FluidSolver.cu:517-599): a fixed sweep count, or
 // with residual_tol > 0 sweeps in chunks until the FP64 residual <= tol
 int Solver::project_solve(float dt_) {
     const int cap = s.max_iter_solve;
+    auto sweeps = [&](int n) -> int {
+        int e0 = -1;
+        if (int e = span_begin(&e0)) return e;
+        if (int e = launch_rbgs_sweeps(*this, dt_, n)) return e;
+        return span_end(SPAN_RBGS, e0, 2 * n);
+    };
     if (!(s.residual_tol > 0.0f)) {
-        if (int e = launch_rbgs_sweeps(*this, dt_, cap)) return e;
+        if (int e = sweeps(cap)) return e;
         st.rbgs_iters_last = cap;
         st.residual = -1.0;
         st.residual_initial = -1.0;
@@ -152,7 +214,7 @@ int Solver::project_solve(float dt_) {
     int it = 0;
     while (r > tol && it < cap) {
         const int n = std::min(s.residual_check_every, cap - it);
-        if (int e = launch_rbgs_sweeps(*this, dt_, n)) return e;
+        if (int e = sweeps(n)) return e;
         it += n;
         if (int e = residual(dt_, &r)) return e;
     }
@@ -162,8 +224,8 @@ int Solver::project_solve(float dt_) {
 }
 
 int Solver::run_stage(int stage, float time, float dt_) {
-    hipEvent_t e0 = ev[2 * stage], e1 = ev[2 * stage + 1];
-    if (timing) BLZ_HIP(hipEventRecord(e0, stream));
+    int e0 = -1;
+    if (int e = span_begin(&e0)) return e;
     int rc = BLZ_OK;
     switch (stage) {
     case BLZ_STAGE_COOLDOWN: rc = launch_cooldown(*this, dt_); break;
@@ -179,14 +241,7 @@ int Solver::run_stage(int stage, float time, float dt_) {
     default: set_error("unknown stage"); return BLZ_E_INVALID;
     }
     if (rc) return rc;
-    if (timing) {
-        BLZ_HIP(hipEventRecord(e1, stream));
-        BLZ_HIP(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        BLZ_HIP(hipEventElapsedTime(&ms, e0, e1));
-        st.stage_ms[stage] += ms;
-    }
-    return BLZ_OK;
+    return span_end(stage, e0, 0);
 }
 
 // FluidSolver::step, src/FluidSolver.cu:858-955 (stage gates :868-925)
@@ -214,8 +269,9 @@ int Solver::substep(float time, float dt_) {
 // One frame. With cfl_number > 0: n = ceil(max|u| dt / (cfl dx)) substeps
 // (clamped to [1, max_substeps]) of dt/n at times time + q*(dt/n).
 int Solver::step_enqueue(float time, bool sync) {
-    for (double& v : st.stage_ms) v = 0.0;
-    st.rbgs_iters = 0;
+    clear_step_stats(st);
+    spans.clear();
+    ev_used = 0;
     const float dtf = s.dt;
     int n = 1;
     if (s.cfl_number > 0.0f) {
@@ -225,18 +281,16 @@ int Solver::step_enqueue(float time, bool sync) {
         n = !(q >= 1.0f) ? 1 : (q > (float)s.max_substeps ? s.max_substeps : (int)q);
     }
     const float dts = dtf / (float)n;
-    if (timing) BLZ_HIP(hipEventRecord(ev[2 * BLZ_NUM_STAGES], stream));
+    int e0 = -1;
+    if (int e = span_begin(&e0)) return e;
     for (int q = 0; q < n; ++q)
         if (int e = substep(time + (float)q * dts, dts)) return e;
     st.substeps = n;
-    if (timing) {
-        BLZ_HIP(hipEventRecord(ev[2 * BLZ_NUM_STAGES + 1], stream));
-        BLZ_HIP(hipEventSynchronize(ev[2 * BLZ_NUM_STAGES + 1]));
-        float ms = 0.0f;
-        BLZ_HIP(hipEventElapsedTime(&ms, ev[2 * BLZ_NUM_STAGES], ev[2 * BLZ_NUM_STAGES + 1]));
-        st.step_ms = ms;
+    if (int e = span_end(SPAN_STEP, e0, 0)) return e;
+    if (sync) {
+        BLZ_HIP(hipStreamSynchronize(stream));
+        if (int e = resolve_spans()) return e;
     }
-    if (sync) BLZ_HIP(hipStreamSynchronize(stream));
     return BLZ_OK;
 }
 
@@ -260,17 +314,20 @@ int Solver::add_source(const float* part8, int n) {
             BLZ_HIP(hipMemcpyAsync(part, part8, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice, stream));
         npart = n;
     }
-    hipEvent_t e0 = ev[2 * BLZ_NUM_STAGES + 2], e1 = ev[2 * BLZ_NUM_STAGES + 3];
-    if (timing) BLZ_HIP(hipEventRecord(e0, stream));
+    // particle-to-grid synchronises internally (it sizes its scratch from a
+    // device count), so this span is resolved on its own
+    std::vector<Span> keep;
+    keep.swap(spans);
+    const size_t used = ev_used;
+    int e0 = -1;
+    if (int e = span_begin(&e0)) return e;
     if (int e = launch_p2g(*this)) return e;
-    if (timing) {
-        BLZ_HIP(hipEventRecord(e1, stream));
-        BLZ_HIP(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        BLZ_HIP(hipEventElapsedTime(&ms, e0, e1));
-        st.source_ms = ms;
-    }
+    if (int e = span_end(SPAN_SOURCE, e0, 0)) return e;
     BLZ_HIP(hipStreamSynchronize(stream));
+    st.source_ms = 0.0;
+    if (int e = resolve_spans()) return e;
+    spans.swap(keep);
+    ev_used = used;
     return BLZ_OK;
 }
 
@@ -336,7 +393,7 @@ int blz_step_async(blz_solver* h, float time) {
 int blz_sync(blz_solver* h) {
     CHECK_HANDLE(h);
     BLZ_HIP(hipStreamSynchronize(h->s.stream));
-    return BLZ_OK;
+    return h->s.resolve_spans();
 }
 
 int blz_run_stage(blz_solver* h, int stage, float time, float dt) {
@@ -345,15 +402,18 @@ int blz_run_stage(blz_solver* h, int stage, float time, float dt) {
         blz::set_error("stage out of range");
         return BLZ_E_INVALID;
     }
-    for (double& v : h->s.st.stage_ms) v = 0.0;
+    blz::clear_step_stats(h->s.st);
+    h->s.spans.clear();
+    h->s.ev_used = 0;
     if (int e = h->s.run_stage(stage, time, dt)) return e;
     if (stage == BLZ_STAGE_PRESSURE) h->s.st.rbgs_iters = h->s.st.rbgs_iters_last;
     BLZ_HIP(hipStreamSynchronize(h->s.stream));
-    return BLZ_OK;
+    return h->s.resolve_spans();
 }
 
 static int field_geom(Solver& s, int which, float** dev, size_t* count, int* dims) {
     if (which == BLZ_FIELD_P || which == BLZ_FIELD_RHS) {
+        // split padded layout (two planes of Np floats each)
         *dev = which == BLZ_FIELD_P ? s.p.c[0] : s.rhs.c[0];
         *count = (size_t)s.W * s.H * s.D;
         if (dims) { dims[0] = s.W; dims[1] = s.H; dims[2] = s.D; }
@@ -432,6 +492,26 @@ int blz_set_timing(blz_solver* h, int on) {
     CHECK_HANDLE(h);
     h->s.timing = on != 0;
     return BLZ_OK;
+}
+
+int blz_selftest_division(int divisor, unsigned long long* mismatches) {
+    if (divisor < 3 || divisor > 6 || !mismatches) {
+        blz::set_error("blz_selftest_division: divisor must be 3..6");
+        return BLZ_E_INVALID;
+    }
+    hipStream_t st = nullptr;
+    unsigned long long* d = nullptr;
+    BLZ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc = BLZ_OK;
+    if (hipMalloc(&d, sizeof(*d)) != hipSuccess) rc = BLZ_E_NOMEM;
+    if (!rc && hipMemsetAsync(d, 0, sizeof(*d), st) != hipSuccess) rc = BLZ_E_HIP;
+    if (!rc) rc = blz::selftest_division(st, divisor, d);
+    if (!rc && hipMemcpyAsync(mismatches, d, sizeof(*d), hipMemcpyDeviceToHost, st) != hipSuccess) rc = BLZ_E_HIP;
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = BLZ_E_HIP;
+    if (d) (void)hipFree(d);
+    (void)hipStreamDestroy(st);
+    if (rc) blz::set_error("blz_selftest_division failed");
+    return rc;
 }
 
 int blz_set_rbgs_variant(blz_solver* h, int variant) {

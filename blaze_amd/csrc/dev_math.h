@@ -200,4 +200,18 @@ __device__ __forceinline__ float perlin(f3 pos, float scale, int seed) {
 
 __device__ __forceinline__ bool abs_gt_1e5(float v) { return (double)fabsf(v) > 1e-5; }
 
+// a / d for d in {3, 4, 5, 6} with y = RN(1/d): q0 = a*y, r = a - d*q0
+// (exact by FMA), q = q0 + r*y. Checked against IEEE division for every
+// float: identical whenever 2^-125 <= |a| <= FLT_MAX (d = 3, 4, 5 also
+// below, except a = -0; d = 6 differs only where a/d is subnormal). Outside
+// that range the IEEE division is taken.
+__device__ __forceinline__ float div_small(float a, float d, float y) {
+    const float q0 = a * y;
+    const float r = fmaf(-q0, d, a);
+    float q = fmaf(r, y, q0);
+    const float m = fabsf(a);
+    if (!(m >= 0x1p-125f && m <= 3.40282347e38f)) q = a / d;
+    return q;
+}
+
 } // namespace blz

@@ -5,14 +5,13 @@
 //   compute_divergence_kernel :475-515, gs_solve_kernel :517-599,
 //   pressure_gradient_update_kernel :601-669, project() :823-856.
 //
-// Pressure and rhs are stored split by colour (red: (i+j) even, black: odd),
-// each colour plane (W/2) x H x D with element (i>>1) + j*(W/2) + k*(W/2)*H.
-// A sweep is a red pass then a black pass; in a pass every column of that
-// colour walks k = 0..D-1 updating in place, reading the other colour (fixed
-// during the pass) and its own k-1 value just computed. This equals the
-// reference's per-slab red/black order inside a thread block, without its
-// inter-block races, and every access of a pass is unit-stride in the split
-// layout.
+// Pressure and rhs are stored split by colour (red: (i+j) even, black: odd)
+// in padded planes (see RBL in solver.h). A sweep is a red pass then a black
+// pass; in a pass every column of that colour walks k = 0..D-1 updating in
+// place, reading the other colour (fixed during the pass) and its own k-1
+// value just computed. This equals the reference's per-slab red/black order
+// inside a thread block, without its inter-block races, and every access of
+// a pass is unit-stride across the wave.
 #include "solver.h"
 
 namespace blz {
@@ -21,15 +20,13 @@ namespace {
 
 constexpr unsigned BS = 256;
 
-__device__ __forceinline__ int rb_index(int i, int j, int k, int Wh, int H) { return (i >> 1) + j * Wh + k * Wh * H; }
-
-__device__ __forceinline__ float p_at(const RB& p, int i, int j, int k, int Wh, int H) {
-    return p.c[(i + j) & 1][rb_index(i, j, k, Wh, H)];
+__device__ __forceinline__ float p_at(const RB& p, const RBL& L, int i, int j, int k) {
+    return p.c[(i + j) & 1][L.at(i, j, k)];
 }
 
 // compute_divergence_kernel :475-515, written into the split layout
 __global__ void __launch_bounds__(BS) k_divergence(const float* __restrict__ U, const float* __restrict__ V,
-                                                   const float* __restrict__ Wf, RB rhs, int W, int H, int D,
+                                                   const float* __restrict__ Wf, RB rhs, RBL L, int W, int H, int D,
                                                    float sc, int4 cbA, int2 cbB) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= W * H * D) return;
@@ -57,71 +54,227 @@ __global__ void __launch_bounds__(BS) k_divergence(const float* __restrict__ U, 
     } else if (k == D - 1) {
         if (cbB.y) r += sc * w_hi;
     }
-    rhs.c[(i + j) & 1][rb_index(i, j, k, W >> 1, H)] = r;
+    rhs.c[(i + j) & 1][L.at(i, j, k)] = r;
 }
 
-// One colour pass. Thread per column (m, j) of the colour; software
-// pipelined PF slabs ahead (all loads of a pass are independent of the
-// values the pass writes, except the in-register k-1 value).
-template <int PF>
-__global__ void __launch_bounds__(BS) k_rbgs_pass(float* __restrict__ A, const float* __restrict__ O,
-                                                  const float* __restrict__ Rc, int W, int H, int D, int colour,
-                                                  float scale) {
+// XCD-aware block order: the dispatcher hands consecutive workgroups to the
+// 8 XCDs in turn, so logical block (b % 8) * (nb / 8) + b / 8 keeps runs of
+// neighbouring rows (which share their j +- 1 loads) on one XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
+struct DivTab {
+    float y3, y4, y5, y6; // RN(1/d)
+};
+
+__device__ __forceinline__ float rcp_of(int d, const DivTab& t) {
+    return d == 3 ? t.y3 : (d == 4 ? t.y4 : (d == 5 ? t.y5 : t.y6));
+}
+
+// One colour pass of the Gauss-Seidel sweep (gs_solve_kernel :526-598).
+// Thread per column of the colour, walking k = 0..D-1; per cell
+//   num = scale*rhs + p(i-1) + p(i+1) + p(j-1) + p(j+1) + p(k-1) + p(k+1)
+// in that order, p = num / (number of in-domain neighbours). Missing
+// neighbours read the -0 ghost shell (exact identity) and the k-1 term of
+// slab 0 is a -0 register, so there are no per-cell conditions.
+//
+// Memory: all four planes are one buffer resource; each lane keeps fixed
+// 32-bit byte offsets and the slab-group offset is a wave-uniform scalar. A
+// group of four slabs costs six 16-byte loads (rhs, four neighbours, own old
+// values) and one 16-byte store per lane, so PG groups (PG <= 8, within the
+// 63-deep memory counter) are in flight. The loop runs a multiple of PG
+// groups: groups past the end read and rewrite the ghost group with -0, and
+// slabs k >= D of the last group are stored as -0, so every group costs the
+// same memory operations. PG >= 2 (group g+1 is in flight while g is used).
+//
+// Division: FAST runs use q = q0 + r*y (q0 = num*y, r = num - d*q0 by FMA)
+// with the sign of num copied onto q, which equals IEEE num / d for every
+// finite num except nonzero |num| < 2^-125; if any lane of the wave meets
+// such a numerator (or a non-finite one) in a group, the run returns that
+// group before storing it and the pass resumes there with the EXACT run
+// (div_small, IEEE fallback per cell). Both runs read only values the pass
+// has not yet written, so the resumed run sees the same inputs.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+typedef int i4v __attribute__((ext_vector_type(4)));
+
+// Slab-group loads are issued as inline-asm buffer loads and consumed after
+// explicit s_waitcnt vmcnt(N) statements that take the destination registers
+// as in/out operands: uses cannot move above the wait, and the registers stay
+// allocated while a load is in flight. (Left to itself the compiler copies
+// the ring registers at the loop back-edge, which forces a full wait once per
+// iteration.) Every memory instruction of the loop is counted in N: six
+// loads and one store per group, nothing else (the kernel has no scratch).
+__device__ __forceinline__ f4v ld4_async(int voff, const i4v& rs, int soff) {
+    f4v r;
+    asm volatile("buffer_load_dwordx4 %0, %1, %2, %3 offen"
+                 : "=v"(r)
+                 : "v"(voff), "s"(rs), "s"(__builtin_amdgcn_readfirstlane(soff)));
+    return r;
+}
+__device__ __forceinline__ void st4_async(const f4v& v, int voff, const i4v& rs, int soff) {
+    asm volatile("buffer_store_dwordx4 %0, %1, %2, %3 offen"
+                 :
+                 : "v"(v), "v"(voff), "s"(rs), "s"(__builtin_amdgcn_readfirstlane(soff))
+                 : "memory");
+}
+template <int N>
+__device__ __forceinline__ void wait_group(f4v& a, f4v& b, f4v& c, f4v& d, f4v& e, f4v& f, f4v& g) {
+    asm volatile("s_waitcnt vmcnt(%7)" : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "+v"(e), "+v"(f), "+v"(g) : "n"(N));
+}
+__device__ __forceinline__ void hold6(f4v& a, f4v& b, f4v& c, f4v& d, f4v& e, f4v& f) {
+    asm volatile("s_waitcnt vmcnt(0)" : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "+v"(e), "+v"(f));
+}
+
+struct PassArgs {
+    i4v rs; // buffer resource: base, stride 0, size, raw dword access
+    int bC, bR, bL, bRt, bN, bS;
+    int D, Dg, gstride;
+    float scale, d_mid, d_edge, y_mid, y_edge;
+};
+
+// vmcnt bound before group u of a ring of PG groups (see column_run)
+template <int PG, int U>
+struct GroupWait {
+    static constexpr int N = (U < PG - 1) ? 6 * PG - 12 + U : 7 * (PG - 2);
+};
+
+template <int PG, int U, bool EXACT>
+__device__ __forceinline__ bool group_step(const PassArgs& a, f4v (&sl)[PG][6], int g, float& prev) {
+    constexpr int UN = (U + 1) % PG;
+    // slot U holds group g, slot UN group g+1. Ops issued after slot UN's
+    // last load: in the first pass over the ring 6*(PG-2-U) prologue loads
+    // plus 7 per finished group (= 6*PG - 12 + U); later, 7*(PG-2).
+    wait_group<GroupWait<PG, U>::N>(sl[U][0], sl[U][1], sl[U][2], sl[U][3], sl[U][4], sl[U][5], sl[UN][5]);
+    const int D = a.D;
+    const int k0 = 4 * g;
+    const float up[4] = {sl[U][5][1], sl[U][5][2], sl[U][5][3], sl[UN][5][0]};
+    float X[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        float x = a.scale * sl[U][0][s];
+        x = x + sl[U][1][s];
+        x = x + sl[U][2][s];
+        x = x + sl[U][3][s];
+        X[s] = x + sl[U][4][s];
+    }
+    f4v out;
+    unsigned bad = 0u;
+    float pv = prev;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = k0 + s;
+        float num = X[s] + pv;
+        num = num + up[s];
+        const bool edge = (k == 0) | (k == D - 1);
+        const float dd = edge ? a.d_edge : a.d_mid, yy = edge ? a.y_edge : a.y_mid;
+        float pn;
+        if (EXACT) {
+            pn = div_small(num, dd, yy);
+        } else {
+            const float q0 = num * yy;
+            const float rr = fmaf(-q0, dd, num);
+            pn = copysignf(fmaf(rr, yy, q0), num);
+            const float mag = fabsf(num);
+            const unsigned ok = (unsigned)(mag >= 0x1p-125f) & (unsigned)(mag <= 3.40282347e38f);
+            const unsigned zero = (unsigned)(mag == 0.0f);
+            bad |= (k < D) ? ((ok | zero) ^ 1u) : 0u;
+        }
+        out[s] = (k < D) ? pn : -0.0f; // slabs k >= D keep the -0 ghost value
+        pv = pn;
+    }
+    if (!EXACT && __builtin_amdgcn_ballot_w64(bad != 0u)) return false;
+    prev = pv;
+    st4_async(out, a.bC, a.rs, min(g, a.Dg) * a.gstride);
+    const int so = min(g + PG, a.Dg) * a.gstride; // group Dg is the ghost group
+    sl[U][0] = ld4_async(a.bR, a.rs, so);
+    sl[U][1] = ld4_async(a.bL, a.rs, so);
+    sl[U][2] = ld4_async(a.bRt, a.rs, so);
+    sl[U][3] = ld4_async(a.bN, a.rs, so);
+    sl[U][4] = ld4_async(a.bS, a.rs, so);
+    sl[U][5] = ld4_async(a.bC, a.rs, so);
+    return true;
+}
+
+template <int PG, bool EXACT, int U>
+__device__ __forceinline__ bool ring_steps(const PassArgs& a, f4v (&sl)[PG][6], int gb, float& prev, int* failed) {
+    if constexpr (U == PG) {
+        return true;
+    } else {
+        if (!group_step<PG, U, EXACT>(a, sl, gb + U, prev)) {
+            *failed = gb + U;
+            return false;
+        }
+        return ring_steps<PG, EXACT, U + 1>(a, sl, gb, prev, failed);
+    }
+}
+
+// Walk groups g_start.. of this thread's column. Returns the number of
+// groups (all done) or, for a fast run, the first group where some lane of
+// the wave needs the exact division (nothing of that group was stored).
+template <int PG, bool EXACT>
+__device__ __forceinline__ int column_run(const PassArgs& a, int g_start, float& prev) {
+    f4v sl[PG][6];
+#pragma unroll
+    for (int u = 0; u < PG; ++u) {
+        const int so = min(g_start + u, a.Dg) * a.gstride;
+        sl[u][0] = ld4_async(a.bR, a.rs, so);
+        sl[u][1] = ld4_async(a.bL, a.rs, so);
+        sl[u][2] = ld4_async(a.bRt, a.rs, so);
+        sl[u][3] = ld4_async(a.bN, a.rs, so);
+        sl[u][4] = ld4_async(a.bS, a.rs, so);
+        sl[u][5] = ld4_async(a.bC, a.rs, so);
+    }
+    int result = a.Dg;
+    for (int gb = g_start; gb < a.Dg; gb += PG) {
+        int failed = a.Dg;
+        if (!ring_steps<PG, EXACT, 0>(a, sl, gb, prev, &failed)) {
+            result = failed;
+            break;
+        }
+    }
+    // drain: no load may land after its registers are reused
+#pragma unroll
+    for (int u = 0; u < PG; ++u) hold6(sl[u][0], sl[u][1], sl[u][2], sl[u][3], sl[u][4], sl[u][5]);
+    return result;
+}
+
+template <int PG>
+__global__ void __launch_bounds__(64) k_rbgs_pass(float* __restrict__ base, unsigned nbytes, unsigned ownP,
+                                                  unsigned othP, unsigned ownR, int W, int H, int D, int Wp, int Hp,
+                                                  int colour, float scale, DivTab tab) {
     const int Wh = W >> 1;
-    const int t = blockIdx.x * BS + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
     if (t >= Wh * H) return;
     const int m = t % Wh, j = t / Wh;
     const int q = (j + colour) & 1;
     const int i = 2 * m + q;
-    const bool hL = i > 0, hR = i < W - 1, hN = j > 0, hS = j < H - 1;
-    const int plane = Wh * H;
-    const int oL = t + q - 1, oR = t + q, oN = t - Wh, oS = t + Wh;
-    const float cnt_xy = (float)((int)hL + (int)hR + (int)hN + (int)hS);
-
-    float r_rhs[PF], r_l[PF], r_r[PF], r_n[PF], r_s[PF], r_up[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-        const int k = u;
-        if (k < D) {
-            const int b = k * plane;
-            r_rhs[u] = Rc[t + b];
-            r_l[u] = hL ? O[oL + b] : 0.0f;
-            r_r[u] = hR ? O[oR + b] : 0.0f;
-            r_n[u] = hN ? O[oN + b] : 0.0f;
-            r_s[u] = hS ? O[oS + b] : 0.0f;
-            r_up[u] = (k < D - 1) ? A[t + b + plane] : 0.0f;
-        }
-    }
-    float prev = 0.0f;
-    for (int kb = 0; kb < D; kb += PF) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int k = kb + u;
-            if (k < D) {
-                float num = scale * r_rhs[u];
-                if (hL) num += r_l[u];
-                if (hR) num += r_r[u];
-                if (hN) num += r_n[u];
-                if (hS) num += r_s[u];
-                float den = cnt_xy;
-                if (k > 0) { num += prev; den += 1.0f; }
-                if (k < D - 1) { num += r_up[u]; den += 1.0f; }
-                const float pn = num / den;
-                A[t + k * plane] = pn;
-                prev = pn;
-                const int kn = k + PF;
-                if (kn < D) {
-                    const int b = kn * plane;
-                    r_rhs[u] = Rc[t + b];
-                    r_l[u] = hL ? O[oL + b] : 0.0f;
-                    r_r[u] = hR ? O[oR + b] : 0.0f;
-                    r_n[u] = hN ? O[oN + b] : 0.0f;
-                    r_s[u] = hS ? O[oS + b] : 0.0f;
-                    r_up[u] = (kn < D - 1) ? A[t + b + plane] : 0.0f;
-                }
-            }
-        }
-    }
+    const unsigned e0 = (unsigned)((Hp + (j + 1)) * Wp + (m + 1)); // element of group 0
+    const int cnt = (int)(i > 0) + (int)(i < W - 1) + (int)(j > 0) + (int)(j < H - 1);
+    PassArgs a;
+    const unsigned long long b64 = (unsigned long long)base;
+    a.rs[0] = (int)(unsigned)(b64 & 0xffffffffull);
+    a.rs[1] = (int)(unsigned)(b64 >> 32); // stride 0
+    a.rs[2] = (int)nbytes;
+    a.rs[3] = 0x00020000; // raw 32-bit access
+    a.bC = (int)(16u * e0 + ownP);
+    a.bR = (int)(16u * e0 + ownR);
+    a.bL = (int)(16u * (e0 + q - 1) + othP);
+    a.bRt = (int)(16u * (e0 + q) + othP);
+    a.bN = (int)(16u * (e0 - (unsigned)Wp) + othP);
+    a.bS = (int)(16u * (e0 + (unsigned)Wp) + othP);
+    a.D = D;
+    a.Dg = (D + 3) >> 2;
+    a.gstride = Hp * Wp * 16;
+    a.scale = scale;
+    a.d_mid = (float)(cnt + 2);
+    a.d_edge = (float)(cnt + 1);
+    a.y_mid = rcp_of(cnt + 2, tab);
+    a.y_edge = rcp_of(cnt + 1, tab);
+    float prev = -0.0f;
+    const int g = column_run<PG, false>(a, 0, prev);
+    if (g < a.Dg) column_run<PG, true>(a, g, prev);
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -130,54 +283,95 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// block max of non-negative doubles -> atomic max on the bit pattern
-__device__ __forceinline__ void block_max_to(double v, double* out) {
-    __shared__ double sm[BS / 64];
-    v = wave_max(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sm[wid] = v;
-    __syncthreads();
-    if (wid == 0) {
-        double w = lane < (int)(BS / 64) ? sm[lane] : 0.0;
-        w = wave_max(w);
-        if (lane == 0) atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(w));
-    }
+__device__ __forceinline__ double cell_res(double pc, double rc, double sum, double n, double c) {
+    double r = fabs(rc + c * (sum - n * pc));
+    return r != r ? __longlong_as_double(0x7ff0000000000000LL) : r;
 }
 
 // ||div u||_inf proxy: max over cells not adjacent to an open face of
-// |rhs + c (sum_nbr p - n p)|, c = dt/(rho dx^2), all in FP64; NaN -> +inf
-__global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, int W, int H, int D, double c, int4 cbA, int2 cbB,
-                                                 double* out) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    double r = 0.0;
-    if (t < W * H * D) {
-        const int i = t % W, j = (t / W) % H, k = t / (W * H);
-        const bool skip = (i == 0 && !cbA.x) || (i == W - 1 && !cbA.y) || (j == 0 && !cbA.w) ||
-                          (j == H - 1 && !cbA.z) || (k == 0 && !cbB.x) || (k == D - 1 && !cbB.y);
-        if (!skip) {
-            const int Wh = W >> 1;
+// |rhs + c (sum_nbr p - n p)|, c = dt/(rho dx^2), all in FP64; NaN -> +inf.
+// Thread per (m, j, k) handles the cell pair i = 2m, 2m+1 (loads unit-stride
+// across the wave). Grid-stride; one partial maximum per block, reduced by
+// k_max_partials.
+__global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int H, int D, double c, int4 cbA,
+                                                 int2 cbB, double* __restrict__ partial) {
+    const int Wh = W >> 1;
+    const int total = Wh * H * 4 * L.Dg;
+    double mx = 0.0;
+    for (int t = blockIdx.x * BS + threadIdx.x; t < total; t += gridDim.x * BS) {
+        // (k&3, m, j, k>>2) from fastest to slowest: 16 lanes cover 64
+        // contiguous bytes of a slab group row
+        const int comp = t & 3, r = t >> 2;
+        const int m = r % Wh, j = (r / Wh) % H, k = 4 * (r / (Wh * H)) + comp;
+        if (k >= D) continue;
+        const int c0 = j & 1, c1 = c0 ^ 1; // colours of i = 2m and 2m+1
+        const float* P0 = p.c[c0];
+        const float* P1 = p.c[c1];
+        const int i0 = 2 * m, i1 = 2 * m + 1;
+        const int x = L.at(i0, j, k);      // same slot in both colour planes
+        const int xl = L.at(i0 - 2, j, k); // m - 1
+        const int xr = L.at(i0 + 2, j, k); // m + 1
+        const int xn = L.at(i0, j - 1, k), xs = L.at(i0, j + 1, k);
+        const int xd = L.at(i0, j, k - 1), xu = L.at(i0, j, k + 1);
+        const float a0 = P0[x], a1 = P1[x]; // p(2m), p(2m+1)
+        const bool jn = j > 0, js = j < H - 1, kd = k > 0, ku = k < D - 1;
+        const bool skipz = (k == 0 && !cbB.x) || (k == D - 1 && !cbB.y);
+        const bool skipy = (j == 0 && !cbA.w) || (j == H - 1 && !cbA.z);
+        if (!(skipz || skipy || (i0 == 0 && !cbA.x))) { // cell i0 (colour c0)
             double sum = 0.0, n = 0.0;
-            if (i > 0) { sum += (double)p_at(p, i - 1, j, k, Wh, H); n += 1.0; }
-            if (i < W - 1) { sum += (double)p_at(p, i + 1, j, k, Wh, H); n += 1.0; }
-            if (j > 0) { sum += (double)p_at(p, i, j - 1, k, Wh, H); n += 1.0; }
-            if (j < H - 1) { sum += (double)p_at(p, i, j + 1, k, Wh, H); n += 1.0; }
-            if (k > 0) { sum += (double)p_at(p, i, j, k - 1, Wh, H); n += 1.0; }
-            if (k < D - 1) { sum += (double)p_at(p, i, j, k + 1, Wh, H); n += 1.0; }
-            const double pc = (double)p_at(p, i, j, k, Wh, H);
-            const double rc = (double)rhs.c[(i + j) & 1][rb_index(i, j, k, Wh, H)];
-            r = fabs(rc + c * (sum - n * pc));
-            if (r != r) r = __longlong_as_double(0x7ff0000000000000LL);
+            if (i0 > 0) { sum += (double)P1[xl]; n += 1.0; }
+            sum += (double)a1; n += 1.0; // i0 + 1 <= W - 1 always
+            if (jn) { sum += (double)P1[xn]; n += 1.0; }
+            if (js) { sum += (double)P1[xs]; n += 1.0; }
+            if (kd) { sum += (double)P0[xd]; n += 1.0; }
+            if (ku) { sum += (double)P0[xu]; n += 1.0; }
+            mx = fmax(mx, cell_res((double)a0, (double)rhs.c[c0][x], sum, n, c));
+        }
+        if (!(skipz || skipy || (i1 == W - 1 && !cbA.y))) { // cell i1 (colour c1)
+            double sum = 0.0, n = 0.0;
+            sum += (double)a0; n += 1.0; // i1 - 1 >= 0 always
+            if (i1 < W - 1) { sum += (double)P0[xr]; n += 1.0; }
+            if (jn) { sum += (double)P0[xn]; n += 1.0; }
+            if (js) { sum += (double)P0[xs]; n += 1.0; }
+            if (kd) { sum += (double)P1[xd]; n += 1.0; }
+            if (ku) { sum += (double)P1[xu]; n += 1.0; }
+            mx = fmax(mx, cell_res((double)a1, (double)rhs.c[c1][x], sum, n, c));
         }
     }
-    block_max_to(r, out);
+    __shared__ double sm[BS / 64];
+    mx = wave_max(mx);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sm[wid] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double w = sm[0];
+        for (int q = 1; q < (int)(BS / 64); ++q) w = fmax(w, sm[q]);
+        partial[blockIdx.x] = w;
+    }
+}
+
+// single block: max of n partial maxima -> out
+__global__ void __launch_bounds__(BS) k_max_partials(const double* __restrict__ partial, int n, double* out) {
+    double mx = 0.0;
+    for (int q = threadIdx.x; q < n; q += BS) mx = fmax(mx, partial[q]);
+    __shared__ double sm[BS / 64];
+    mx = wave_max(mx);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sm[wid] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double w = sm[0];
+        for (int q = 1; q < (int)(BS / 64); ++q) w = fmax(w, sm[q]);
+        *out = w;
+    }
 }
 
 // pressure_gradient_update_kernel :601-669 for the faces along AXIS,
 // out-of-place: interior faces subtract s*(p_hi - p_lo); boundary faces are
 // 0 when closed, else the updated value of the adjacent interior face.
 template <int AXIS>
-__global__ void __launch_bounds__(BS) k_pupdate(const float* __restrict__ in, float* __restrict__ out, RB p, int W,
-                                                int H, int D, float sc, int closedLo, int closedHi) {
+__global__ void __launch_bounds__(BS) k_pupdate(const float* __restrict__ in, float* __restrict__ out, RB p, RBL L,
+                                                int W, int H, int D, float sc, int closedLo, int closedHi) {
     const int fx = W + (AXIS == 0), fy = H + (AXIS == 1), fz = D + (AXIS == 2);
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= fx * fy * fz) return;
@@ -185,7 +379,6 @@ __global__ void __launch_bounds__(BS) k_pupdate(const float* __restrict__ in, fl
     const int f = AXIS == 0 ? i : (AXIS == 1 ? j : k);
     const int nA = AXIS == 0 ? W : (AXIS == 1 ? H : D);
     const int sF = AXIS == 0 ? 1 : (AXIS == 1 ? fx : fx * fy); // face stride along the axis
-    const int Wh = W >> 1;
     int g, fi = i, fj = j, fk = k; // face whose updated value is taken
     if (f == 0) {
         if (closedLo) { out[t] = 0.0f; return; }
@@ -198,25 +391,40 @@ __global__ void __launch_bounds__(BS) k_pupdate(const float* __restrict__ in, fl
     } else {
         g = t;
     }
-    // interior face (fi,fj,fk) lies between cells (f-1) and f along AXIS
-    const float p_hi = p_at(p, fi, fj, fk, Wh, H);
-    const float p_lo = AXIS == 0 ? p_at(p, fi - 1, fj, fk, Wh, H)
-                                 : (AXIS == 1 ? p_at(p, fi, fj - 1, fk, Wh, H) : p_at(p, fi, fj, fk - 1, Wh, H));
+    // interior face (fi,fj,fk) lies between cells f-1 and f along AXIS
+    const float p_hi = p_at(p, L, fi, fj, fk);
+    const float p_lo = AXIS == 0 ? p_at(p, L, fi - 1, fj, fk)
+                                 : (AXIS == 1 ? p_at(p, L, fi, fj - 1, fk) : p_at(p, L, fi, fj, fk - 1));
     out[t] = in[g] - sc * (p_hi - p_lo);
 }
 
-__global__ void __launch_bounds__(BS) k_rb_to_natural(RB p, float* __restrict__ out, int W, int H, int D) {
+__global__ void __launch_bounds__(BS) k_rb_to_natural(RB p, RBL L, float* __restrict__ out, int W, int H, int D) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= W * H * D) return;
     const int i = t % W, j = (t / W) % H, k = t / (W * H);
-    out[t] = p_at(p, i, j, k, W >> 1, H);
+    out[t] = p_at(p, L, i, j, k);
 }
 
-__global__ void __launch_bounds__(BS) k_natural_to_rb(RB p, const float* __restrict__ in, int W, int H, int D) {
+__global__ void __launch_bounds__(BS) k_natural_to_rb(RB p, RBL L, const float* __restrict__ in, int W, int H,
+                                                      int D) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= W * H * D) return;
     const int i = t % W, j = (t / W) % H, k = t / (W * H);
-    p.c[(i + j) & 1][rb_index(i, j, k, W >> 1, H)] = in[t];
+    p.c[(i + j) & 1][L.at(i, j, k)] = in[t];
+}
+
+// pressure planes: +0 inside, -0 in the ghost shell; rhs planes: 0
+__global__ void __launch_bounds__(BS) k_init_pressure(float* __restrict__ prb, size_t Np, RBL L) {
+    const size_t n = 4 * Np;
+    for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < n; t += (size_t)gridDim.x * BS) {
+        const size_t e = t % Np;
+        const int comp = (int)(e & 3);
+        const size_t v = e >> 2;
+        const int m = (int)(v % L.Wp), j = (int)((v / L.Wp) % L.Hp), g = (int)(v / ((size_t)L.Wp * L.Hp));
+        const int k = (g - 1) * 4 + comp;
+        const bool inside = m >= 1 && m <= L.Wh && j >= 1 && j <= L.H && g >= 1 && g <= L.Dg && k < L.D;
+        prb[t] = (t < 2 * Np && !inside) ? -0.0f : 0.0f;
+    }
 }
 
 // max |a| over an array (NaN counts as +inf), into a shared double slot
@@ -227,12 +435,35 @@ __global__ void __launch_bounds__(BS) k_maxabs(const float* __restrict__ a, size
         if (v != v) v = __longlong_as_double(0x7ff0000000000000LL);
         m = fmax(m, v);
     }
-    block_max_to(m, out);
+    __shared__ double sm[BS / 64];
+    m = wave_max(m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sm[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double w = sm[0];
+        for (int q = 1; q < (int)(BS / 64); ++q) w = fmax(w, sm[q]);
+        atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(w));
+    }
 }
 
 __global__ void k_fill_doubles(double* d, int n, double v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) d[t] = v;
+}
+
+// every float a: div_small(a, d, RN(1/d)) against IEEE a / d
+__global__ void __launch_bounds__(256) k_div_selftest(float d, float y, unsigned long long* bad) {
+    unsigned long long local = 0;
+    for (unsigned long long u = (unsigned long long)blockIdx.x * 256 + threadIdx.x; u < (1ull << 32);
+         u += (unsigned long long)gridDim.x * 256) {
+        const float a = __uint_as_float((unsigned)u);
+        const float ref = a / d;
+        const float got = div_small(a, d, y);
+        const bool same = __float_as_uint(ref) == __float_as_uint(got) || (ref != ref && got != got);
+        local += same ? 0 : 1;
+    }
+    if (local) atomicAdd(bad, local);
 }
 
 } // namespace
@@ -254,22 +485,46 @@ static inline int2 cb_b(const Solver& s) {
     return make_int2(c[4], c[5]);
 }
 
-int launch_divergence(Solver& s) {
-    const float sc = 1.0f / s.dx;
-    LAUNCH(k_divergence, (size_t)s.W * s.H * s.D, s.f[F_U].front(), s.f[F_V].front(), s.f[F_W].front(), s.rhs, s.W,
-           s.H, s.D, sc, cb_a(s), cb_b(s));
+int launch_init_pressure(Solver& s) {
+    const unsigned nb = (unsigned)std::min<size_t>(blocks_for(4 * s.Np, BS), 8192);
+    hipLaunchKernelGGL(k_init_pressure, dim3(nb), dim3(BS), 0, s.stream, s.prb, s.Np, s.L);
+    BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
 
-int launch_rbgs_sweeps(Solver& s, float dt, int sweeps) {
-    const float scale = (s.s.density * s.dx * s.dx) / dt; // Bridson p.75, :530
+int launch_divergence(Solver& s) {
+    const float sc = 1.0f / s.dx;
+    LAUNCH(k_divergence, (size_t)s.W * s.H * s.D, s.f[F_U].front(), s.f[F_V].front(), s.f[F_W].front(), s.rhs, s.L,
+           s.W, s.H, s.D, sc, cb_a(s), cb_b(s));
+    return BLZ_OK;
+}
+
+template <int PG>
+static int rbgs_passes(Solver& s, float scale, int sweeps) {
     const size_t cols = (size_t)(s.W / 2) * s.H;
+    const dim3 grid(blocks_for(cols, 64)), block(64);
+    const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
+    const unsigned plane_bytes = (unsigned)(s.Np * 4);
     for (int l = 0; l < sweeps; ++l) {
         for (int c = 0; c < 2; ++c) {
-            LAUNCH((k_rbgs_pass<8>), cols, s.p.c[c], s.p.c[c ^ 1], s.rhs.c[c], s.W, s.H, s.D, c, scale);
+            hipLaunchKernelGGL((k_rbgs_pass<PG>), grid, block, 0, s.stream, s.prb, 4 * plane_bytes, c * plane_bytes,
+                               (c ^ 1) * plane_bytes, (2 + c) * plane_bytes, s.W, s.H, s.D, s.L.Wp, s.L.Hp, c, scale,
+                               tab);
+            BLZ_HIP(hipGetLastError());
         }
     }
     return BLZ_OK;
+}
+
+// Gauss-Seidel sweeps (red pass, black pass each). variant: slab groups in
+// flight (2, 4 or 8; 0 = 8).
+int launch_rbgs_sweeps(Solver& s, float dt, int sweeps) {
+    const float scale = (s.s.density * s.dx * s.dx) / dt; // Bridson p.75, :530
+    switch (s.rbgs_variant) {
+    case 2: return rbgs_passes<2>(s, scale, sweeps);
+    case 4: return rbgs_passes<4>(s, scale, sweeps);
+    default: return rbgs_passes<8>(s, scale, sweeps);
+    }
 }
 
 int launch_clear_doubles(Solver& s, double* dev, int n) {
@@ -280,8 +535,14 @@ int launch_clear_doubles(Solver& s, double* dev, int n) {
 
 int launch_residual(Solver& s, float dt, double* dev_out) {
     const double c = (double)dt / ((double)s.s.density * (double)s.dx * (double)s.dx);
-    if (int e = launch_clear_doubles(s, dev_out, 1)) return e;
-    LAUNCH(k_residual, (size_t)s.W * s.H * s.D, s.p, s.rhs, s.W, s.H, s.D, c, cb_a(s), cb_b(s), dev_out);
+    const size_t pairs = (size_t)(s.W / 2) * s.H * 4 * s.L.Dg;
+    const int nb = (int)std::min<size_t>(blocks_for(pairs, BS), RESIDUAL_BLOCKS);
+    double* partial = s.red + 8;
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, c, cb_a(s),
+                       cb_b(s), partial);
+    BLZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(BS), 0, s.stream, partial, nb, dev_out);
+    BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
 
@@ -302,9 +563,9 @@ int launch_pressure_update(Solver& s, float dt) {
     Field& U = s.f[F_U];
     Field& V = s.f[F_V];
     Field& W = s.f[F_W];
-    LAUNCH(k_pupdate<0>, U.count(), U.front(), U.back(), s.p, s.W, s.H, s.D, sc, cb[0], cb[1]);
-    LAUNCH(k_pupdate<1>, V.count(), V.front(), V.back(), s.p, s.W, s.H, s.D, sc, cb[3], cb[2]);
-    LAUNCH(k_pupdate<2>, W.count(), W.front(), W.back(), s.p, s.W, s.H, s.D, sc, cb[4], cb[5]);
+    LAUNCH(k_pupdate<0>, U.count(), U.front(), U.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[0], cb[1]);
+    LAUNCH(k_pupdate<1>, V.count(), V.front(), V.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[3], cb[2]);
+    LAUNCH(k_pupdate<2>, W.count(), W.front(), W.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[4], cb[5]);
     U.swap();
     V.swap();
     W.swap();
@@ -312,12 +573,19 @@ int launch_pressure_update(Solver& s, float dt) {
 }
 
 int launch_p_to_natural(Solver& s, float* out, const RB& src) {
-    LAUNCH(k_rb_to_natural, (size_t)s.W * s.H * s.D, src, out, s.W, s.H, s.D);
+    LAUNCH(k_rb_to_natural, (size_t)s.W * s.H * s.D, src, s.L, out, s.W, s.H, s.D);
     return BLZ_OK;
 }
 
 int launch_natural_to_rb(Solver& s, RB& dst, const float* in) {
-    LAUNCH(k_natural_to_rb, (size_t)s.W * s.H * s.D, dst, in, s.W, s.H, s.D);
+    LAUNCH(k_natural_to_rb, (size_t)s.W * s.H * s.D, dst, s.L, in, s.W, s.H, s.D);
+    return BLZ_OK;
+}
+
+int selftest_division(hipStream_t st, int d, unsigned long long* dev_bad) {
+    const float fd = (float)d;
+    hipLaunchKernelGGL(k_div_selftest, dim3(4096), dim3(256), 0, st, fd, 1.0f / fd, dev_bad);
+    BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
 

@@ -36,13 +36,30 @@ struct Field {
     Gv view_back() const { return Gv{back(), nx, ny, nz, ox, oy, oz}; }
 };
 
-// Pressure and right-hand side live in a red/black split layout: colour
-// c = (i+j)&1, element (i>>1) + j*(W/2) + k*(W/2)*H of plane c.
+// Pressure and right-hand side live in a red/black split, padded layout
+// that interleaves four consecutive z-slabs per column: colour c = (i+j)&1,
+// m = i>>1, slab group g = k>>2; plane c stores float4 elements indexed
+// ((g+1)*Hp + (j+1))*Wp + (m+1) (Wp = W/2+2, Hp = H+2, Dg = ceil(D/4) groups
+// plus one ghost group on each side), component k&3. A wave reading one
+// group of 64 neighbouring columns reads 1 KiB contiguous, and one 16-byte
+// load per lane covers four Gauss-Seidel steps. The ghost shell (m, j or g
+// outside the grid, and slabs k >= D of the last group) of the pressure
+// planes holds -0.0f, the additive identity of IEEE arithmetic
+// (x + -0 == x for every x), so an update adds all neighbour slots
+// unconditionally.
+struct RBL {
+    int Wh, H, D, Wp, Hp, Dg;
+    __host__ __device__ int at(int i, int j, int k) const {
+        return ((((k >> 2) + 1) * Hp + (j + 1)) * Wp + ((i >> 1) + 1)) * 4 + (k & 3);
+    }
+};
 struct RB {
     float* c[2] = {nullptr, nullptr};
 };
 
 enum { F_T = 0, F_U, F_V, F_W, F_S, F_N };
+
+constexpr int RESIDUAL_BLOCKS = 1024;
 
 struct Solver {
     blz_settings s{};
@@ -51,9 +68,12 @@ struct Solver {
     bool smoke = false;
     hipStream_t stream = nullptr;
     Field f[F_N];
-    RB p, rhs;
+    RB p, rhs;              // four planes of one allocation: p red, p black, rhs red, rhs black
+    float* prb = nullptr;
+    RBL L{};
+    size_t Np = 0;          // floats per padded plane
     float* tmp = nullptr;     // 7 * W*H*D scratch for cell-centred intermediates
-    double* red = nullptr;    // reduction slots (device)
+    double* red = nullptr;    // reduction slots (device): 8 results + RESIDUAL_BLOCKS partials
     double* red_host = nullptr; // pinned mirror
     // particles
     float* part = nullptr;
@@ -61,11 +81,22 @@ struct Solver {
     // particle-to-grid scratch (grow-only)
     void* p2g_buf = nullptr;
     size_t p2g_bytes = 0;
-    // stats / timing
+    // stats / timing: spans of device work bracketed by pooled events,
+    // resolved after the stream has been synchronised (no extra waits)
     blz_stats st{};
     bool timing = false;
-    hipEvent_t ev[2 * (BLZ_NUM_STAGES + 2)] = {};
+    enum { SPAN_SOURCE = BLZ_NUM_STAGES, SPAN_STEP, SPAN_RBGS };
+    struct Span {
+        int kind, e0, e1, launches;
+    };
+    std::vector<hipEvent_t> evpool;
+    size_t ev_used = 0;
+    std::vector<Span> spans;
     int rbgs_variant = 0;
+
+    int span_begin(int* e0);
+    int span_end(int kind, int e0, int launches);
+    int resolve_spans();
 
     ~Solver();
     int init(const blz_settings& st);
@@ -95,6 +126,8 @@ int launch_max_abs_velocity(Solver& s, double* dev_out);
 int launch_clear_doubles(Solver& s, double* dev, int n);
 int launch_p_to_natural(Solver& s, float* out, const RB& src);
 int launch_natural_to_rb(Solver& s, RB& dst, const float* in);
+int launch_init_pressure(Solver& s);
+int selftest_division(hipStream_t st, int d, unsigned long long* dev_bad);
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 

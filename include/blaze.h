@@ -48,7 +48,7 @@ typedef struct blz_solver blz_solver;
 #define BLZ_FIELD_V 2        /* y face velocity, W x (H+1) x D (grid y is flipped) */
 #define BLZ_FIELD_W 3        /* z face velocity, W x H x (D+1) */
 #define BLZ_FIELD_DENSITY 4  /* smoke density, W x H x D (only when enabled) */
-#define BLZ_FIELD_P 5        /* pressure, W x H x D, natural layout on copy */
+#define BLZ_FIELD_P 5        /* pressure, W x H x D (natural layout on copy; device view is split by colour) */
 #define BLZ_FIELD_RHS 6      /* divergence right-hand side, W x H x D */
 #define BLZ_NUM_FIELDS 7
 
@@ -119,6 +119,9 @@ typedef struct blz_stats {
     double stage_ms[BLZ_NUM_STAGES]; /* device time per stage from HIP events (0 if timing off) */
     double source_ms;         /* device time of the last blz_add_source */
     double step_ms;           /* device time of the last blz_step */
+    double rbgs_kernel_ms;    /* device time of the RBGS sweep batches alone */
+    int rbgs_launches;        /* RBGS kernel launches behind rbgs_kernel_ms */
+    int residual_checks;      /* FP64 residual evaluations in the last step */
 } blz_stats;
 
 /* synthetic particle set: uniform in a ball, deterministic for a seed */
@@ -167,8 +170,12 @@ int blz_set_field(blz_solver* h, int which, const float* host);
 int blz_get_stats(blz_solver* h, blz_stats* out);
 /* Enable per-stage HIP event timing (adds events around each stage). */
 int blz_set_timing(blz_solver* h, int on);
-/* Override the RBGS kernel variant (0 = auto). For benchmarks and tests. */
+/* RBGS kernel variant: four-slab groups in flight per thread (2, 4, 8; 0 = 8). */
 int blz_set_rbgs_variant(blz_solver* h, int variant);
+/* Self-test of the pressure solve's division by a neighbour count d in
+   3..6: runs the solver's device division on every 32-bit float and counts
+   results that differ from IEEE a / d (expected 0). */
+int blz_selftest_division(int divisor, unsigned long long* mismatches);
 
 const char* blz_last_error(void);
 

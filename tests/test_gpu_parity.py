@@ -90,6 +90,42 @@ def test_stage_bitwise(name, stage, kw, res):
             assert gs.residual_initial == os_["residual_initial"]
 
 
+@pytest.mark.parametrize("variant", [2, 4, 8])
+@pytest.mark.parametrize("res", [(32, 24, 20), (64, 8, 40), (16, 10, 21), (32, 16, 64)],
+                         ids=["32x24x20", "64x8x40", "16x10x21", "32x16x64"])
+def test_rbgs_variants_bitwise(variant, res):
+    """Every RBGS kernel variant gives the oracle's sweeps bit for bit."""
+    s = settings(B, res, max_iter_solve=9)
+    st = random_state(s, seed=variant)
+    g, o = pair(s, st)
+    g.set_rbgs_variant(variant)
+    g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    compare(f"rbgs v{variant}", g.field(B.FIELD_P), o.field(B.FIELD_P))
+
+
+@pytest.mark.parametrize("d", [3, 4, 5, 6])
+def test_division_exhaustive(d):
+    """The pressure solve's device division by a neighbour count equals IEEE
+    division for all 2^32 float inputs."""
+    assert B.selftest_division(d) == 0
+
+
+def test_residual_matches_oracle():
+    """The FP64 residual (max over cells, open-face cells skipped) equals the
+    oracle's value exactly, for several boundary combinations."""
+    for cb in [(1, 1, 1, 1, 1, 1), (0, 1, 1, 0, 1, 0), (0, 0, 0, 0, 0, 0)]:
+        s = settings(B, (32, 24, 20), closed_bounds=cb, max_iter_solve=10, residual_tol=1e-30,
+                     residual_check_every=10)
+        st = random_state(s, seed=sum(cb))
+        g, o = pair(s, st)
+        g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+        o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+        gs, os_ = g.stats(), o.stats()
+        assert gs.residual_initial == os_["residual_initial"], cb
+        assert gs.residual == os_["residual"], cb
+
+
 def random_particles(n, seed, lo=-0.1, hi=1.1):
     rng = np.random.default_rng(seed)
     p = np.zeros((n, 8), np.float32)
