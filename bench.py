@@ -40,6 +40,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=6)
     ap.add_argument("--rbgs-variant", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a device copy of 256 MiB (reference kernel for HBM counter scaling)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run RBGS sweeps (for hardware counter runs)")
+    ap.add_argument("--probe-sweeps", type=int, default=20)
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -47,6 +52,44 @@ def parse():
 def log(msg, quiet=False):
     if not quiet:
         print(msg, file=sys.stderr, flush=True)
+
+
+def calib_copy(quiet, mib=256):
+    """Device-to-device copy of a known size through the HIP runtime (the
+    reference kernel for scaling the HBM byte counters)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[calib] copied 3 x {mib} MiB device to device", quiet)
+
+
+def traffic_probe(a):
+    """RBGS sweeps alone on the scene's grid (random smooth state), for
+    hardware counter runs: no particle sort, no host synchronisation inside
+    the sweeps."""
+    import numpy as np
+
+    import blaze_amd as B
+
+    s, _, _, _ = B.parse_scene(a.scene)
+    s.residual_tol = 0.0
+    s.max_iter_solve = a.probe_sweeps
+    W, H, D = s.grid_res
+    g = B.Solver(s)
+    g.set_rbgs_variant(a.rbgs_variant)
+    rng = np.random.default_rng(0)
+    g.set_field(B.FIELD_RHS, rng.standard_normal((D, H, W)).astype(np.float32))
+    g.run_stage(B.STAGE_PRESSURE, 0.0, float(s.dt))
+    log(f"[probe] {a.probe_sweeps} sweeps on {W}x{H}x{D} done", a.quiet)
 
 
 def cpu_baseline(s, parts, substeps, iters_per_substep, checks_per_substep, n_sweeps, quiet):
@@ -120,6 +163,12 @@ def main():
     import numpy as np
 
     import blaze_amd as B
+
+    if a.calib_copy:
+        calib_copy(a.quiet)
+    if a.traffic_probe:
+        traffic_probe(a)
+        return
 
     s, _, _, gen = B.parse_scene(a.scene)
     W, H, D = s.grid_res
