@@ -290,52 +290,81 @@ __device__ __forceinline__ double cell_res(double pc, double rc, double sum, dou
 
 // ||div u||_inf proxy: max over cells not adjacent to an open face of
 // |rhs + c (sum_nbr p - n p)|, c = dt/(rho dx^2), all in FP64; NaN -> +inf.
-// Thread per (m, j, k) handles the cell pair i = 2m, 2m+1 (loads unit-stride
-// across the wave). Grid-stride; one partial maximum per block, reduced by
-// k_max_partials.
+// The neighbour sum runs in the order i-1, i+1, j-1, j+1, k-1, k+1 over all
+// six slots: a missing neighbour is a -0 ghost, the identity of the sum, so
+// it equals summing the existing neighbours only; n counts those.
+// Thread per (pair column m, row j, segment of RES_SEG slab groups): per
+// group, ten coalesced 16-byte loads cover eight cells (the red/black pair
+// for four slabs); z neighbours come from the previous and next group of the
+// thread's own column. Grid-stride; one partial maximum per block, reduced
+// by k_max_partials.
+constexpr int RES_SEG = 4;
+
 __global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int H, int D, double c, int4 cbA,
                                                  int2 cbB, double* __restrict__ partial) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     const int Wh = W >> 1;
-    const int total = Wh * H * 4 * L.Dg;
+    const int nseg = (L.Dg + RES_SEG - 1) / RES_SEG;
+    const int total = Wh * H * nseg;
+    const int gs = L.Wp * L.Hp; // float4 elements per group
+    const bool openL = !cbA.x, openR = !cbA.y, openN = !cbA.w, openS = !cbA.z, openD = !cbB.x, openU = !cbB.y;
     double mx = 0.0;
     for (int t = blockIdx.x * BS + threadIdx.x; t < total; t += gridDim.x * BS) {
-        // (k&3, m, j, k>>2) from fastest to slowest: 16 lanes cover 64
-        // contiguous bytes of a slab group row
-        const int comp = t & 3, r = t >> 2;
-        const int m = r % Wh, j = (r / Wh) % H, k = 4 * (r / (Wh * H)) + comp;
-        if (k >= D) continue;
-        const int c0 = j & 1, c1 = c0 ^ 1; // colours of i = 2m and 2m+1
-        const float* P0 = p.c[c0];
-        const float* P1 = p.c[c1];
+        const int m = t % Wh, j = (t / Wh) % H, seg = t / (Wh * H);
+        const int c0 = j & 1, c1 = c0 ^ 1;   // colours of i0 = 2m and i1 = 2m+1
+        const f4* P0 = (const f4*)p.c[c0];
+        const f4* P1 = (const f4*)p.c[c1];
+        const f4* R0 = (const f4*)rhs.c[c0];
+        const f4* R1 = (const f4*)rhs.c[c1];
+        const int e0 = (j + 1) * L.Wp + (m + 1) + gs; // element of group 0
         const int i0 = 2 * m, i1 = 2 * m + 1;
-        const int x = L.at(i0, j, k);      // same slot in both colour planes
-        const int xl = L.at(i0 - 2, j, k); // m - 1
-        const int xr = L.at(i0 + 2, j, k); // m + 1
-        const int xn = L.at(i0, j - 1, k), xs = L.at(i0, j + 1, k);
-        const int xd = L.at(i0, j, k - 1), xu = L.at(i0, j, k + 1);
-        const float a0 = P0[x], a1 = P1[x]; // p(2m), p(2m+1)
-        const bool jn = j > 0, js = j < H - 1, kd = k > 0, ku = k < D - 1;
-        const bool skipz = (k == 0 && !cbB.x) || (k == D - 1 && !cbB.y);
-        const bool skipy = (j == 0 && !cbA.w) || (j == H - 1 && !cbA.z);
-        if (!(skipz || skipy || (i0 == 0 && !cbA.x))) { // cell i0 (colour c0)
-            double sum = 0.0, n = 0.0;
-            if (i0 > 0) { sum += (double)P1[xl]; n += 1.0; }
-            sum += (double)a1; n += 1.0; // i0 + 1 <= W - 1 always
-            if (jn) { sum += (double)P1[xn]; n += 1.0; }
-            if (js) { sum += (double)P1[xs]; n += 1.0; }
-            if (kd) { sum += (double)P0[xd]; n += 1.0; }
-            if (ku) { sum += (double)P0[xu]; n += 1.0; }
-            mx = fmax(mx, cell_res((double)a0, (double)rhs.c[c0][x], sum, n, c));
-        }
-        if (!(skipz || skipy || (i1 == W - 1 && !cbA.y))) { // cell i1 (colour c1)
-            double sum = 0.0, n = 0.0;
-            sum += (double)a0; n += 1.0; // i1 - 1 >= 0 always
-            if (i1 < W - 1) { sum += (double)P0[xr]; n += 1.0; }
-            if (jn) { sum += (double)P0[xn]; n += 1.0; }
-            if (js) { sum += (double)P0[xs]; n += 1.0; }
-            if (kd) { sum += (double)P1[xd]; n += 1.0; }
-            if (ku) { sum += (double)P1[xu]; n += 1.0; }
-            mx = fmax(mx, cell_res((double)a1, (double)rhs.c[c1][x], sum, n, c));
+        const double nxy0 = (double)((int)(i0 > 0) + 1 + (int)(j > 0) + (int)(j < H - 1));
+        const double nxy1 = (double)(1 + (int)(i1 < W - 1) + (int)(j > 0) + (int)(j < H - 1));
+        const bool skipxy0 = (i0 == 0 && openL) || (j == 0 && openN) || (j == H - 1 && openS);
+        const bool skipxy1 = (i1 == W - 1 && openR) || (j == 0 && openN) || (j == H - 1 && openS);
+        const int g0 = seg * RES_SEG, g1 = min(g0 + RES_SEG, L.Dg);
+        f4 prv0 = P0[e0 + (g0 - 1) * gs], prv1 = P1[e0 + (g0 - 1) * gs];
+        f4 cur0 = P0[e0 + g0 * gs], cur1 = P1[e0 + g0 * gs];
+        for (int g = g0; g < g1; ++g) {
+            const int e = e0 + g * gs;
+            const f4 nxt0 = P0[e + gs], nxt1 = P1[e + gs]; // group g+1 (ghost group after the last)
+            const f4 l1 = P1[e - 1], r0 = P0[e + 1];
+            const f4 n1 = P1[e - L.Wp], s1 = P1[e + L.Wp];
+            const f4 n0 = P0[e - L.Wp], s0 = P0[e + L.Wp];
+            const f4 q0 = R0[e], q1 = R1[e];
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl) {
+                const int k = 4 * g + sl;
+                if (k >= D) break;
+                const float dn0 = sl > 0 ? cur0[sl - 1] : prv0[3], up0 = sl < 3 ? cur0[sl + 1] : nxt0[0];
+                const float dn1 = sl > 0 ? cur1[sl - 1] : prv1[3], up1 = sl < 3 ? cur1[sl + 1] : nxt1[0];
+                const double nz = (double)((int)(k > 0) + (int)(k < D - 1));
+                const bool skipz = (k == 0 && openD) || (k == D - 1 && openU);
+                if (!(skipz || skipxy0)) {
+                    double sum = 0.0;
+                    sum += (double)l1[sl];
+                    sum += (double)cur1[sl];
+                    sum += (double)n1[sl];
+                    sum += (double)s1[sl];
+                    sum += (double)dn0;
+                    sum += (double)up0;
+                    mx = fmax(mx, cell_res((double)cur0[sl], (double)q0[sl], sum, nxy0 + nz, c));
+                }
+                if (!(skipz || skipxy1)) {
+                    double sum = 0.0;
+                    sum += (double)cur0[sl];
+                    sum += (double)r0[sl];
+                    sum += (double)n0[sl];
+                    sum += (double)s0[sl];
+                    sum += (double)dn1;
+                    sum += (double)up1;
+                    mx = fmax(mx, cell_res((double)cur1[sl], (double)q1[sl], sum, nxy1 + nz, c));
+                }
+            }
+            prv0 = cur0;
+            prv1 = cur1;
+            cur0 = nxt0;
+            cur1 = nxt1;
         }
     }
     __shared__ double sm[BS / 64];
@@ -535,8 +564,8 @@ int launch_clear_doubles(Solver& s, double* dev, int n) {
 
 int launch_residual(Solver& s, float dt, double* dev_out) {
     const double c = (double)dt / ((double)s.s.density * (double)s.dx * (double)s.dx);
-    const size_t pairs = (size_t)(s.W / 2) * s.H * 4 * s.L.Dg;
-    const int nb = (int)std::min<size_t>(blocks_for(pairs, BS), RESIDUAL_BLOCKS);
+    const size_t items = (size_t)(s.W / 2) * s.H * ((s.L.Dg + RES_SEG - 1) / RES_SEG);
+    const int nb = (int)std::min<size_t>(blocks_for(items, BS), RESIDUAL_BLOCKS);
     double* partial = s.red + 8;
     hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, c, cb_a(s),
                        cb_b(s), partial);
