@@ -4,7 +4,9 @@
 // The reference keeps device-heap Grid objects and synchronises after every
 // launch; here the host owns front/back pointers (a swap is a pointer swap),
 // all work of a step is enqueued on the handle's stream, and the host only
-// waits where it needs a value (residual checks, CFL substep count).
+// waits where it needs a value (CFL substep count) or to stay a few sweep
+// batches ahead of a residual-driven solve, whose convergence test runs on
+// the device.
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -27,6 +29,12 @@ Solver::~Solver() {
     if (tmp) (void)hipFree(tmp);
     if (red) (void)hipFree(red);
     if (red_host) (void)hipHostFree(red_host);
+    if (ctl) (void)hipFree(ctl);
+    if (ctl_host) (void)hipHostFree(ctl_host);
+    if (zero_flag) (void)hipFree(zero_flag);
+    if (done_host) (void)hipHostFree((void*)done_host);
+    for (hipEvent_t e : batch_ev)
+        if (e) (void)hipEventDestroy(e);
     if (part) (void)hipFree(part);
     free_p2g_scratch(*this);
     for (auto& e : evpool)
@@ -60,6 +68,10 @@ static int validate(const blz_settings& s) {
     }
     if (s.max_iter_solve < 0) {
         set_error("max_iter_solve must be >= 0");
+        return BLZ_E_INVALID;
+    }
+    if (s.max_substeps > MAX_SOLVES) {
+        set_error("max_substeps must be <= 64");
         return BLZ_E_INVALID;
     }
     return BLZ_OK;
@@ -119,18 +131,24 @@ int Solver::init(const blz_settings& st_) {
     BLZ_HIP(hipMemset(red, 0, (8 + RESIDUAL_BLOCKS) * sizeof(double)));
     BLZ_HIP(hipStreamSynchronize(stream));
     BLZ_HIP(hipHostMalloc((void**)&red_host, 8 * sizeof(double), hipHostMallocDefault));
+    if (hipMalloc(&ctl, MAX_SOLVES * sizeof(SolveCtl)) != hipSuccess ||
+        hipMalloc(&zero_flag, sizeof(int)) != hipSuccess) {
+        set_error("hipMalloc failed for solver control blocks");
+        return BLZ_E_NOMEM;
+    }
+    BLZ_HIP(hipMemset(ctl, 0, MAX_SOLVES * sizeof(SolveCtl)));
+    BLZ_HIP(hipMemset(zero_flag, 0, sizeof(int)));
+    BLZ_HIP(hipHostMalloc((void**)&ctl_host, MAX_SOLVES * sizeof(SolveCtl), hipHostMallocDefault));
+    void* dh = nullptr;
+    BLZ_HIP(hipHostMalloc(&dh, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    done_host = (volatile int*)dh;
+    *done_host = 0;
+    BLZ_HIP(hipHostGetDevicePointer((void**)&done_host_dev, dh, 0));
+    for (hipEvent_t& e : batch_ev) BLZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    BLZ_HIP(hipDeviceSynchronize());
     std::memset(&st, 0, sizeof(st));
     st.residual = -1.0;
     st.residual_initial = -1.0;
-    return BLZ_OK;
-}
-
-int Solver::residual(float dt_, double* out) {
-    st.residual_checks++;
-    if (int e = launch_residual(*this, dt_, red)) return e;
-    BLZ_HIP(hipMemcpyAsync(red_host, red, sizeof(double), hipMemcpyDeviceToHost, stream));
-    BLZ_HIP(hipStreamSynchronize(stream));
-    *out = red_host[0];
     return BLZ_OK;
 }
 
@@ -155,11 +173,11 @@ int Solver::span_begin(int* e0) {
     return BLZ_OK;
 }
 
-int Solver::span_end(int kind, int e0, int launches) {
+int Solver::span_end(int kind, int e0, int launches, int solve, int batch) {
     if (!timing || e0 < 0) return BLZ_OK;
     int e1 = -1;
     if (int e = span_begin(&e1)) return e;
-    spans.push_back(Span{kind, e0, e1, launches});
+    spans.push_back(Span{kind, e0, e1, launches, solve, batch});
     return BLZ_OK;
 }
 
@@ -172,6 +190,10 @@ int Solver::resolve_spans() {
         else if (sp.kind == SPAN_SOURCE) st.source_ms += ms;
         else if (sp.kind == SPAN_STEP) st.step_ms += ms;
         else if (sp.kind == SPAN_RBGS) {
+            // a batch enqueued after its solve had converged ran as no-ops
+            if (sp.solve >= 0 && sp.solve < (int)solves.size() && solves[sp.solve] >= 0 &&
+                sp.batch >= ctl_host[sp.solve].checks - 1)
+                continue;
             st.rbgs_kernel_ms += ms;
             st.rbgs_launches += sp.launches;
         }
@@ -179,6 +201,35 @@ int Solver::resolve_spans() {
     spans.clear();
     ev_used = 0;
     return BLZ_OK;
+}
+
+// copy the control blocks of this step's residual-driven solves to the host
+int Solver::enqueue_solve_stats() {
+    if (!solves.empty())
+        BLZ_HIP(hipMemcpyAsync(ctl_host, ctl, solves.size() * sizeof(SolveCtl), hipMemcpyDeviceToHost, stream));
+    return BLZ_OK;
+}
+
+int Solver::finalize() {
+    for (size_t q = 0; q < solves.size(); ++q) {
+        int it = solves[q];
+        double r = -1.0, r0 = -1.0;
+        if (it < 0) {
+            it = -it - 1;
+        } else {
+            it = ctl_host[q].iters;
+            r = ctl_host[q].res;
+            r0 = ctl_host[q].res0;
+            st.residual_checks += ctl_host[q].checks;
+        }
+        st.rbgs_iters += it;
+        st.rbgs_iters_last = it;
+        st.residual = r;
+        st.residual_initial = r0;
+    }
+    const int e = resolve_spans();
+    solves.clear();
+    return e;
 }
 
 void clear_step_stats(blz_stats& st) {
@@ -191,35 +242,49 @@ void clear_step_stats(blz_stats& st) {
 }
 
 // gs_solve_kernel (src/FluidSolver.cu:517-599): a fixed sweep count, or
-// with residual_tol > 0 sweeps in chunks until the FP64 residual <= tol
+// with residual_tol > 0 the loop
+//     r = residual; it = 0
+//     while (r > tol && it < cap) { n = min(every, cap - it); n sweeps; it += n; r = residual }
+// run on the device: every batch of sweeps is followed by a residual check
+// that updates the solve's control block, and once it is done the remaining
+// kernels return at entry. The host enqueues batches while the solve is not
+// known to have converged, at most SOLVE_LOOKAHEAD past the last batch it
+// has seen finish. Statistics: solves[q] records the solve, ctl_host[q]
+// its result after the step.
 int Solver::project_solve(float dt_) {
     const int cap = s.max_iter_solve;
-    auto sweeps = [&](int n) -> int {
+    const int q = (int)solves.size();
+    if (q >= MAX_SOLVES) {
+        set_error("too many pressure solves enqueued without synchronisation");
+        return BLZ_E_INVALID;
+    }
+    auto sweeps = [&](int n, const int* stop, int batch) -> int {
         int e0 = -1;
         if (int e = span_begin(&e0)) return e;
-        if (int e = launch_rbgs_sweeps(*this, dt_, n)) return e;
-        return span_end(SPAN_RBGS, e0, 2 * n);
+        if (int e = launch_rbgs_sweeps(*this, dt_, n, stop)) return e;
+        return span_end(SPAN_RBGS, e0, 2 * n, q, batch);
     };
     if (!(s.residual_tol > 0.0f)) {
-        if (int e = sweeps(cap)) return e;
-        st.rbgs_iters_last = cap;
-        st.residual = -1.0;
-        st.residual_initial = -1.0;
-        return BLZ_OK;
+        solves.push_back(-cap - 1);
+        return sweeps(cap, zero_flag, 0);
     }
+    solves.push_back(cap);
     const double tol = (double)s.residual_tol;
-    double r = 0.0;
-    if (int e = residual(dt_, &r)) return e;
-    st.residual_initial = r;
-    int it = 0;
-    while (r > tol && it < cap) {
-        const int n = std::min(s.residual_check_every, cap - it);
-        if (int e = sweeps(n)) return e;
+    const int every = s.residual_check_every;
+    const int seq = ++solve_seq;
+    SolveCtl* c = ctl + q;
+    if (int e = launch_ctl_reset(*this, c)) return e;
+    if (int e = launch_residual_check(*this, dt_, c, done_host_dev, seq, tol, cap, 0, 1)) return e;
+    int b = 0;
+    for (int it = 0; it < cap; ++b) {
+        if (*done_host == seq) break;
+        const int n = std::min(every, cap - it);
+        if (int e = sweeps(n, &c->done, b)) return e;
         it += n;
-        if (int e = residual(dt_, &r)) return e;
+        if (int e = launch_residual_check(*this, dt_, c, done_host_dev, seq, tol, cap, n, 0)) return e;
+        BLZ_HIP(hipEventRecord(batch_ev[b % (SOLVE_LOOKAHEAD + 1)], stream));
+        if (b >= SOLVE_LOOKAHEAD) BLZ_HIP(hipEventSynchronize(batch_ev[(b - SOLVE_LOOKAHEAD) % (SOLVE_LOOKAHEAD + 1)]));
     }
-    st.rbgs_iters_last = it;
-    st.residual = r;
     return BLZ_OK;
 }
 
@@ -260,7 +325,6 @@ int Solver::substep(float time, float dt_) {
         if (int e = run_stage(BLZ_STAGE_TURBULENCE, time, dt_)) return e;
     if (int e = run_stage(BLZ_STAGE_DIVERGENCE, time, dt_)) return e;
     if (int e = run_stage(BLZ_STAGE_PRESSURE, time, dt_)) return e;
-    st.rbgs_iters += st.rbgs_iters_last;
     if (int e = run_stage(BLZ_STAGE_PROJECT_UPDATE, time, dt_)) return e;
     if (int e = run_stage(BLZ_STAGE_ADVECT, time, dt_)) return e;
     return BLZ_OK;
@@ -271,6 +335,7 @@ int Solver::substep(float time, float dt_) {
 int Solver::step_enqueue(float time, bool sync) {
     clear_step_stats(st);
     spans.clear();
+    solves.clear();
     ev_used = 0;
     const float dtf = s.dt;
     int n = 1;
@@ -287,9 +352,10 @@ int Solver::step_enqueue(float time, bool sync) {
         if (int e = substep(time + (float)q * dts, dts)) return e;
     st.substeps = n;
     if (int e = span_end(SPAN_STEP, e0, 0)) return e;
+    if (int e = enqueue_solve_stats()) return e;
     if (sync) {
         BLZ_HIP(hipStreamSynchronize(stream));
-        if (int e = resolve_spans()) return e;
+        if (int e = finalize()) return e;
     }
     return BLZ_OK;
 }
@@ -393,7 +459,7 @@ int blz_step_async(blz_solver* h, float time) {
 int blz_sync(blz_solver* h) {
     CHECK_HANDLE(h);
     BLZ_HIP(hipStreamSynchronize(h->s.stream));
-    return h->s.resolve_spans();
+    return h->s.finalize();
 }
 
 int blz_run_stage(blz_solver* h, int stage, float time, float dt) {
@@ -404,11 +470,12 @@ int blz_run_stage(blz_solver* h, int stage, float time, float dt) {
     }
     blz::clear_step_stats(h->s.st);
     h->s.spans.clear();
+    h->s.solves.clear();
     h->s.ev_used = 0;
     if (int e = h->s.run_stage(stage, time, dt)) return e;
-    if (stage == BLZ_STAGE_PRESSURE) h->s.st.rbgs_iters = h->s.st.rbgs_iters_last;
+    if (int e = h->s.enqueue_solve_stats()) return e;
     BLZ_HIP(hipStreamSynchronize(h->s.stream));
-    return h->s.resolve_spans();
+    return h->s.finalize();
 }
 
 static int field_geom(Solver& s, int which, float** dev, size_t* count, int* dims) {

@@ -243,7 +243,8 @@ __device__ __forceinline__ int column_run(const PassArgs& a, int g_start, float&
 template <int PG>
 __global__ void __launch_bounds__(64) k_rbgs_pass(float* __restrict__ base, unsigned nbytes, unsigned ownP,
                                                   unsigned othP, unsigned ownR, int W, int H, int D, int Wp, int Hp,
-                                                  int colour, float scale, DivTab tab) {
+                                                  int colour, float scale, DivTab tab, const int* __restrict__ stop) {
+    if (*stop) return; // the solve has converged
     const int Wh = W >> 1;
     const int t = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
     if (t >= Wh * H) return;
@@ -297,11 +298,12 @@ __device__ __forceinline__ double cell_res(double pc, double rc, double sum, dou
 // group, ten coalesced 16-byte loads cover eight cells (the red/black pair
 // for four slabs); z neighbours come from the previous and next group of the
 // thread's own column. Grid-stride; one partial maximum per block, reduced
-// by k_max_partials.
+// by k_res_final. `stop` is the solve's done flag: a converged solve skips.
 constexpr int RES_SEG = 4;
 
 __global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int H, int D, double c, int4 cbA,
-                                                 int2 cbB, double* __restrict__ partial) {
+                                                 int2 cbB, double* __restrict__ partial, const int* __restrict__ stop) {
+    if (*stop) return;
     typedef float f4 __attribute__((ext_vector_type(4)));
     const int Wh = W >> 1;
     const int nseg = (L.Dg + RES_SEG - 1) / RES_SEG;
@@ -379,8 +381,13 @@ __global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int
     }
 }
 
-// single block: max of n partial maxima -> out
-__global__ void __launch_bounds__(BS) k_max_partials(const double* __restrict__ partial, int n, double* out) {
+// single block: the residual is the max of the partial maxima; advance the
+// solve's control block as the loop `while (r > tol && it < cap)` does
+// On completion the solve's sequence number goes to host_done (mapped host
+// memory), which the host polls to stop enqueuing sweeps.
+__global__ void __launch_bounds__(BS) k_res_final(const double* __restrict__ partial, int n, SolveCtl* c,
+                                                  int* host_done, int seq, double tol, int cap, int add, int first) {
+    if (c->done) return;
     double mx = 0.0;
     for (int q = threadIdx.x; q < n; q += BS) mx = fmax(mx, partial[q]);
     __shared__ double sm[BS / 64];
@@ -389,10 +396,30 @@ __global__ void __launch_bounds__(BS) k_max_partials(const double* __restrict__ 
     if (lane == 0) sm[wid] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double w = sm[0];
-        for (int q = 1; q < (int)(BS / 64); ++q) w = fmax(w, sm[q]);
-        *out = w;
+        double r = sm[0];
+        for (int q = 1; q < (int)(BS / 64); ++q) r = fmax(r, sm[q]);
+        if (first) {
+            c->res0 = r;
+            c->iters = 0;
+        } else {
+            c->iters += add;
+        }
+        c->res = r;
+        c->checks += 1;
+        if (!(r > tol) || c->iters >= cap) {
+            c->done = 1;
+            __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __threadfence_system();
+        }
     }
+}
+
+__global__ void k_ctl_reset(SolveCtl* c) {
+    c->done = 0;
+    c->iters = 0;
+    c->checks = 0;
+    c->res = -1.0;
+    c->res0 = -1.0;
 }
 
 // pressure_gradient_update_kernel :601-669 for the faces along AXIS,
@@ -529,7 +556,7 @@ int launch_divergence(Solver& s) {
 }
 
 template <int PG>
-static int rbgs_passes(Solver& s, float scale, int sweeps) {
+static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop) {
     const size_t cols = (size_t)(s.W / 2) * s.H;
     const dim3 grid(blocks_for(cols, 64)), block(64);
     const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
@@ -538,7 +565,7 @@ static int rbgs_passes(Solver& s, float scale, int sweeps) {
         for (int c = 0; c < 2; ++c) {
             hipLaunchKernelGGL((k_rbgs_pass<PG>), grid, block, 0, s.stream, s.prb, 4 * plane_bytes, c * plane_bytes,
                                (c ^ 1) * plane_bytes, (2 + c) * plane_bytes, s.W, s.H, s.D, s.L.Wp, s.L.Hp, c, scale,
-                               tab);
+                               tab, stop);
             BLZ_HIP(hipGetLastError());
         }
     }
@@ -547,13 +574,19 @@ static int rbgs_passes(Solver& s, float scale, int sweeps) {
 
 // Gauss-Seidel sweeps (red pass, black pass each). variant: slab groups in
 // flight (2, 4 or 8; 0 = 8).
-int launch_rbgs_sweeps(Solver& s, float dt, int sweeps) {
+int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop) {
     const float scale = (s.s.density * s.dx * s.dx) / dt; // Bridson p.75, :530
     switch (s.rbgs_variant) {
-    case 2: return rbgs_passes<2>(s, scale, sweeps);
-    case 4: return rbgs_passes<4>(s, scale, sweeps);
-    default: return rbgs_passes<8>(s, scale, sweeps);
+    case 2: return rbgs_passes<2>(s, scale, sweeps, stop);
+    case 4: return rbgs_passes<4>(s, scale, sweeps, stop);
+    default: return rbgs_passes<8>(s, scale, sweeps, stop);
     }
+}
+
+int launch_ctl_reset(Solver& s, SolveCtl* c) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s.stream, c);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
 }
 
 int launch_clear_doubles(Solver& s, double* dev, int n) {
@@ -562,15 +595,17 @@ int launch_clear_doubles(Solver& s, double* dev, int n) {
     return BLZ_OK;
 }
 
-int launch_residual(Solver& s, float dt, double* dev_out) {
-    const double c = (double)dt / ((double)s.s.density * (double)s.dx * (double)s.dx);
+int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int add,
+                          int first) {
+    const double cc = (double)dt / ((double)s.s.density * (double)s.dx * (double)s.dx);
     const size_t items = (size_t)(s.W / 2) * s.H * ((s.L.Dg + RES_SEG - 1) / RES_SEG);
     const int nb = (int)std::min<size_t>(blocks_for(items, BS), RESIDUAL_BLOCKS);
     double* partial = s.red + 8;
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, c, cb_a(s),
-                       cb_b(s), partial);
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, cc, cb_a(s),
+                       cb_b(s), partial, &c->done);
     BLZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(BS), 0, s.stream, partial, nb, dev_out);
+    hipLaunchKernelGGL(k_res_final, dim3(1), dim3(BS), 0, s.stream, partial, nb, c, host_done, seq, tol, cap, add,
+                       first);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }

@@ -59,6 +59,20 @@ struct RB {
 
 enum { F_T = 0, F_U, F_V, F_W, F_S, F_N };
 
+// Per-solve control block of the residual-driven Gauss-Seidel loop, kept on
+// the device: the residual kernel updates it and the sweep and residual
+// kernels of the same solve do nothing once `done` is set, so the host
+// enqueues sweeps without waiting for residual values.
+struct SolveCtl {
+    int done;   // 1 once the residual <= tol or the sweep cap is reached
+    int iters;  // sweeps counted so far
+    int checks; // residual evaluations so far
+    int pad;
+    double res, res0; // last and first residual
+};
+constexpr int MAX_SOLVES = 64;     // pressure solves per step (bounds max_substeps)
+constexpr int SOLVE_LOOKAHEAD = 2; // sweep batches the host may enqueue past the last one it has seen finish
+
 constexpr int RESIDUAL_BLOCKS = 1024;
 
 struct Solver {
@@ -75,6 +89,14 @@ struct Solver {
     float* tmp = nullptr;     // 7 * W*H*D scratch for cell-centred intermediates
     double* red = nullptr;    // reduction slots (device): 8 results + RESIDUAL_BLOCKS partials
     double* red_host = nullptr; // pinned mirror
+    SolveCtl* ctl = nullptr;      // MAX_SOLVES control blocks (device)
+    SolveCtl* ctl_host = nullptr; // pinned copy, filled at the end of a step
+    int* zero_flag = nullptr;     // device int that stays 0: stop flag of fixed-count solves
+    volatile int* done_host = nullptr; // mapped pinned int: sequence number of the last converged solve
+    int* done_host_dev = nullptr;      // its device address
+    int solve_seq = 0;
+    std::vector<int> solves; // solves enqueued since the last finalize: sweep cap, or -1 (fixed count)
+    hipEvent_t batch_ev[SOLVE_LOOKAHEAD + 1] = {};
     // particles
     float* part = nullptr;
     int npart = 0, part_cap = 0;
@@ -88,6 +110,7 @@ struct Solver {
     enum { SPAN_SOURCE = BLZ_NUM_STAGES, SPAN_STEP, SPAN_RBGS };
     struct Span {
         int kind, e0, e1, launches;
+        int solve, batch; // RBGS batches: owning solve and batch index
     };
     std::vector<hipEvent_t> evpool;
     size_t ev_used = 0;
@@ -95,8 +118,9 @@ struct Solver {
     int rbgs_variant = 0;
 
     int span_begin(int* e0);
-    int span_end(int kind, int e0, int launches);
+    int span_end(int kind, int e0, int launches, int solve = -1, int batch = -1);
     int resolve_spans();
+    int finalize(); // after a stream synchronisation: solve statistics and spans
 
     ~Solver();
     int init(const blz_settings& st);
@@ -105,7 +129,7 @@ struct Solver {
     int substep(float time, float dt);
     int run_stage(int stage, float time, float dt);
     int project_solve(float dt);
-    int residual(float dt, double* out);
+    int enqueue_solve_stats();
     int max_abs_velocity(float* out);
 };
 
@@ -117,8 +141,10 @@ int launch_vorticity(Solver& s, float dt);
 int launch_wind(Solver& s, float time, float dt);
 int launch_turbulence(Solver& s, float time, float dt);
 int launch_divergence(Solver& s);
-int launch_rbgs_sweeps(Solver& s, float dt, int sweeps);
-int launch_residual(Solver& s, float dt, double* dev_out);
+int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop);
+int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int add,
+                          int first);
+int launch_ctl_reset(Solver& s, SolveCtl* c);
 int launch_pressure_update(Solver& s, float dt);
 int launch_advect(Solver& s, float dt);
 int launch_p2g(Solver& s);
