@@ -9,8 +9,9 @@ voxel-steps/s metric. Warmup frames (including the sourcing frame) are
 untimed; the K timed frames are bracketed by stream synchronisations.
 
 Prints one JSON line with the metric, the roofline of the dominant kernel
-(the RBGS colour pass; algorithmic 6 B/voxel per pass = 12 B/voxel per
-sweep, timed with HIP events on the solver stream) and a CPU baseline (the
+(the RBGS sweep kernel; algorithmic 12 B/voxel per sweep: read p, read rhs,
+write p -- one launch of the fused sweep, or two colour passes of 6 B/voxel
+each -- timed with HIP events on the solver stream) and a CPU baseline (the
 oracle on the host cores, bounded sample of the same workload).
 """
 import argparse
@@ -23,7 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-RBGS_BYTES_PER_VOXEL_PASS = 6.0   # 12 B/voxel per sweep (read p, read rhs, write p)
+RBGS_BYTES_PER_VOXEL_SWEEP = 12.0  # read p, read rhs, write p
+RBGS_KERNELS = {1: ("k_rbgs_pass", "RBGS colour pass, two per sweep", "rbgs_pass_traffic.json"),
+                2: ("k_rbgs_fused", "RBGS fused sweep, one per sweep", "rbgs_fused_traffic.json")}
 # compulsory bytes per voxel of the non-solve part of a substep of this
 # workload: cooling 8, vorticity 24, buoyancy 12, divergence 16, update 28,
 # advection 40 (T, U, V, W, density)
@@ -199,7 +202,8 @@ def main():
     g.sync()
     log(f"[warmup] {time.perf_counter() - tw:.1f}s", a.quiet)
 
-    tot_sub = tot_iters = tot_checks = tot_launch = 0
+    tot_sub = tot_iters = tot_checks = tot_launch = tot_timed_sweeps = 0
+    kernel_kind = 0
     tot_rbgs_ms = 0.0
     stage_ms = [0.0] * B.NUM_STAGES
     residuals = []
@@ -211,7 +215,9 @@ def main():
         tot_iters += st.rbgs_iters
         tot_checks += st.residual_checks
         tot_launch += st.rbgs_launches
+        tot_timed_sweeps += st.rbgs_sweeps_timed
         tot_rbgs_ms += st.rbgs_kernel_ms
+        kernel_kind = st.rbgs_kernel or kernel_kind
         for i in range(B.NUM_STAGES):
             stage_ms[i] += st.stage_ms[i]
         residuals.append(st.residual)
@@ -222,15 +228,19 @@ def main():
 
     value = N3 * tot_sub / elapsed
     ms_per_step = elapsed * 1e3 / a.steps
+    # algorithmic bytes of the timed sweeps over their device time; per
+    # launch that is 12 B/voxel (fused sweep) or 6 B/voxel (colour pass)
     avg_launch_s = (tot_rbgs_ms / 1e3) / max(1, tot_launch)
-    achieved = RBGS_BYTES_PER_VOXEL_PASS * N3 / avg_launch_s / 1e9 if tot_launch else 0.0
+    achieved = (RBGS_BYTES_PER_VOXEL_SWEEP * N3 * tot_timed_sweeps / (tot_rbgs_ms / 1e3) / 1e9
+                if tot_rbgs_ms > 0 else 0.0)
+    kname, kdesc, kfile = RBGS_KERNELS.get(kernel_kind, ("?", "?", ""))
     algo_bytes = N3 * (FIXED_BYTES_PER_VOXEL * tot_sub + 12.0 * tot_iters
                        + RESIDUAL_BYTES_PER_VOXEL * tot_checks)
     step_gbs = algo_bytes / elapsed / 1e9
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "rbgs_pass_traffic.json")
-    if os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", kfile) if kfile else ""
+    if tpath and os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
         if tuple(tj.get("grid", ())) == (W, H, D):
@@ -270,7 +280,7 @@ def main():
         "hbm_gbs_algorithmic_step": step_gbs,
         "stage_ms": {B.STAGE_NAMES[i]: stage_ms[i] for i in range(B.NUM_STAGES) if stage_ms[i] > 0},
         "roofline": {
-            "kernel": "k_rbgs_pass (RBGS colour pass)",
+            "kernel": f"{kname} ({kdesc})",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -66,6 +66,7 @@ class Stats(C.Structure):
         ("warm_start_used", C.c_int), ("warm_start_fallbacks", C.c_int),
         ("stage_ms", C.c_double * NUM_STAGES), ("source_ms", C.c_double), ("step_ms", C.c_double),
         ("rbgs_kernel_ms", C.c_double), ("rbgs_launches", C.c_int), ("residual_checks", C.c_int),
+        ("rbgs_sweeps_timed", C.c_int), ("rbgs_kernel", C.c_int),
     ]
 
 
@@ -112,6 +113,17 @@ def _load():
         "blz_set_rbgs_variant": ([vp, C.c_int], C.c_int),
         "blz_selftest_division": ([C.c_int, P(C.c_ulonglong)], C.c_int),
         "blz_last_error": ([], C.c_char_p),
+        "blz_net_create": ([C.c_ulonglong, P(vp)], C.c_int),
+        "blz_net_destroy": ([vp], C.c_int),
+        "blz_net_num_params": ([vp, P(C.c_longlong)], C.c_int),
+        "blz_net_get_params": ([vp, P(C.c_float)], C.c_int),
+        "blz_net_set_params": ([vp, P(C.c_float)], C.c_int),
+        "blz_net_get_grads": ([vp, P(C.c_float)], C.c_int),
+        "blz_net_forward": ([vp, P(C.c_float), P(C.c_int), P(C.c_float)], C.c_int),
+        "blz_net_train_step": ([vp, P(C.c_float), P(C.c_float), P(C.c_int), C.c_float, C.c_int,
+                                P(C.c_double)], C.c_int),
+        "blz_set_warm_start": ([vp, vp], C.c_int),
+        "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -250,6 +262,85 @@ class Solver:
 
     def set_rbgs_variant(self, v: int):
         _check(_lib.blz_set_rbgs_variant(self._h, v), "blz_set_rbgs_variant")
+
+    def set_warm_start(self, net):
+        """Attach a Net (None detaches); used when settings.warm_start != 0
+        and residual_tol > 0."""
+        self._net = net
+        _check(_lib.blz_set_warm_start(self._h, net._h if net is not None else None), "blz_set_warm_start")
+
+    def warm_start_dims(self):
+        nd, cd = (C.c_int * 3)(), (C.c_int * 3)()
+        _check(_lib.blz_warm_start_input(self._h, C.c_float(1.0), None, nd, cd), "blz_warm_start_input")
+        return tuple(nd), tuple(cd)
+
+    def warm_start_input(self, dt: float) -> np.ndarray:
+        """(2, Dn, Hn, Wn) float32: normalised coarse residual and solid mask."""
+        (wn, hn, dn), _ = self.warm_start_dims()
+        out = np.empty((2, dn, hn, wn), np.float32)
+        _check(_lib.blz_warm_start_input(self._h, C.c_float(dt), out.ctypes.data_as(C.POINTER(C.c_float)),
+                                         None, None), "blz_warm_start_input")
+        return out
+
+
+class Net:
+    """The warm-start U-Net (blz_net): parameters live on the device; inputs
+    are (B, 2, D, H, W) float32 arrays (residual, mask), outputs (B, D, H, W)."""
+
+    def __init__(self, seed: int = 0):
+        self._h = C.c_void_p()
+        _check(_lib.blz_net_create(C.c_ulonglong(seed), C.byref(self._h)), "blz_net_create")
+        n = C.c_longlong()
+        _check(_lib.blz_net_num_params(self._h, C.byref(n)), "blz_net_num_params")
+        self.num_params = n.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.blz_net_destroy(h)
+            self._h = None
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.num_params, np.float32)
+        _check(_lib.blz_net_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float))), "blz_net_get_params")
+        return out
+
+    def set_params(self, p: np.ndarray):
+        a = np.ascontiguousarray(p, np.float32)
+        assert a.size == self.num_params
+        _check(_lib.blz_net_set_params(self._h, a.ctypes.data_as(C.POINTER(C.c_float))), "blz_net_set_params")
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.num_params, np.float32)
+        _check(_lib.blz_net_get_grads(self._h, out.ctypes.data_as(C.POINTER(C.c_float))), "blz_net_get_grads")
+        return out
+
+    @staticmethod
+    def _prep(x):
+        x = np.ascontiguousarray(x, np.float32)
+        assert x.ndim == 5 and x.shape[1] == 2, "input must be (B, 2, D, H, W)"
+        B, _, D, H, W = x.shape
+        # channel-major per batch -> two planes of B*D*H*W values
+        planes = np.ascontiguousarray(x.transpose(1, 0, 2, 3, 4))
+        dims = (C.c_int * 4)(B, D, H, W)
+        return planes, dims, (B, D, H, W)
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        planes, dims, shp = self._prep(x)
+        out = np.empty(shp, np.float32)
+        _check(_lib.blz_net_forward(self._h, planes.ctypes.data_as(C.POINTER(C.c_float)), dims,
+                                    out.ctypes.data_as(C.POINTER(C.c_float))), "blz_net_forward")
+        return out
+
+    def train_step(self, x: np.ndarray, target: np.ndarray, lr: float = 1e-3, update: bool = True) -> float:
+        planes, dims, shp = self._prep(x)
+        t = np.ascontiguousarray(target, np.float32)
+        assert t.shape == shp
+        loss = C.c_double()
+        _check(_lib.blz_net_train_step(self._h, planes.ctypes.data_as(C.POINTER(C.c_float)),
+                                       t.ctypes.data_as(C.POINTER(C.c_float)), dims, C.c_float(lr),
+                                       1 if update else 0, C.byref(loss)), "blz_net_train_step")
+        return loss.value
 
 
 def selftest_division(divisor: int) -> int:

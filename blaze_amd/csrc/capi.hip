@@ -31,6 +31,9 @@ Solver::~Solver() {
     if (red_host) (void)hipHostFree(red_host);
     if (ctl) (void)hipFree(ctl);
     if (ctl_host) (void)hipHostFree(ctl_host);
+    if (ws_net) delete ws_net;
+    if (ws_buf) (void)hipFree(ws_buf);
+    if (ws_backup) (void)hipFree(ws_backup);
     if (zero_flag) (void)hipFree(zero_flag);
     if (done_host) (void)hipHostFree((void*)done_host);
     for (hipEvent_t e : batch_ev)
@@ -117,7 +120,12 @@ int Solver::init(const blz_settings& st_) {
     L.Hp = H + 2;
     L.Dg = (D + 3) / 4;
     Np = (size_t)L.Wp * L.Hp * (L.Dg + 2) * 4;
-    if (hipMalloc(&prb, 4 * Np * sizeof(float)) != hipSuccess || hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess ||
+    // six planes when their byte offsets (plus a tail that absorbs the fused
+    // sweep's reads two rows past the last group) fit 32 bits, else four
+    // (fused sweeps unavailable)
+    const size_t tail = (size_t)2 * L.Wp * L.Hp * 4;
+    prb_floats = (6 * Np + tail) * 4 < ((size_t)1 << 32) && W <= 4096 ? 6 * Np + tail : 4 * Np;
+    if (hipMalloc(&prb, prb_floats * sizeof(float)) != hipSuccess || hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess ||
         hipMalloc(&red, (8 + RESIDUAL_BLOCKS) * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc failed for solver scratch");
         return BLZ_E_NOMEM;
@@ -173,11 +181,11 @@ int Solver::span_begin(int* e0) {
     return BLZ_OK;
 }
 
-int Solver::span_end(int kind, int e0, int launches, int solve, int batch) {
+int Solver::span_end(int kind, int e0, int launches, int solve, int batch, int sweeps) {
     if (!timing || e0 < 0) return BLZ_OK;
     int e1 = -1;
     if (int e = span_begin(&e1)) return e;
-    spans.push_back(Span{kind, e0, e1, launches, solve, batch});
+    spans.push_back(Span{kind, e0, e1, launches, solve, batch, sweeps});
     return BLZ_OK;
 }
 
@@ -196,6 +204,7 @@ int Solver::resolve_spans() {
                 continue;
             st.rbgs_kernel_ms += ms;
             st.rbgs_launches += sp.launches;
+            st.rbgs_sweeps_timed += sp.sweeps;
         }
     }
     spans.clear();
@@ -226,6 +235,10 @@ int Solver::finalize() {
         st.rbgs_iters_last = it;
         st.residual = r;
         st.residual_initial = r0;
+        if (solves[q] >= 0) {
+            st.warm_start_used = ctl_host[q].ws == 1 ? 1 : 0;
+            if (ctl_host[q].ws == 2) st.warm_start_fallbacks += 1;
+        }
     }
     const int e = resolve_spans();
     solves.clear();
@@ -237,6 +250,7 @@ void clear_step_stats(blz_stats& st) {
     st.step_ms = 0.0;
     st.rbgs_kernel_ms = 0.0;
     st.rbgs_launches = 0;
+    st.rbgs_sweeps_timed = 0;
     st.rbgs_iters = 0;
     st.residual_checks = 0;
 }
@@ -259,10 +273,11 @@ int Solver::project_solve(float dt_) {
         return BLZ_E_INVALID;
     }
     auto sweeps = [&](int n, const int* stop, int batch) -> int {
-        int e0 = -1;
+        int e0 = -1, launches = 0, kind = 0;
         if (int e = span_begin(&e0)) return e;
-        if (int e = launch_rbgs_sweeps(*this, dt_, n, stop)) return e;
-        return span_end(SPAN_RBGS, e0, 2 * n, q, batch);
+        if (int e = launch_rbgs_sweeps(*this, dt_, n, stop, &launches, &kind)) return e;
+        st.rbgs_kernel = kind;
+        return span_end(SPAN_RBGS, e0, launches, q, batch, n);
     };
     if (!(s.residual_tol > 0.0f)) {
         solves.push_back(-cap - 1);
@@ -275,6 +290,8 @@ int Solver::project_solve(float dt_) {
     SolveCtl* c = ctl + q;
     if (int e = launch_ctl_reset(*this, c)) return e;
     if (int e = launch_residual_check(*this, dt_, c, done_host_dev, seq, tol, cap, 0, 1)) return e;
+    if (ws_net && s.warm_start)
+        if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap)) return e;
     int b = 0;
     for (int it = 0; it < cap; ++b) {
         if (*done_host == seq) break;
@@ -579,6 +596,37 @@ int blz_selftest_division(int divisor, unsigned long long* mismatches) {
     (void)hipStreamDestroy(st);
     if (rc) blz::set_error("blz_selftest_division failed");
     return rc;
+}
+
+int blz_set_warm_start(blz_solver* h, blz_net* net) {
+    CHECK_HANDLE(h);
+    BLZ_HIP(hipStreamSynchronize(h->s.stream));
+    return blz::warm_start_attach(h->s, net ? &net->n : nullptr);
+}
+
+int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]) {
+    CHECK_HANDLE(h);
+    Solver& s = h->s;
+    if (!s.ws_net) {
+        blz::set_error("no warm-start network attached (blz_set_warm_start)");
+        return BLZ_E_INVALID;
+    }
+    if (net_dims) {
+        net_dims[0] = s.Wn;
+        net_dims[1] = s.Hn;
+        net_dims[2] = s.Dn;
+    }
+    if (coarse_dims) {
+        coarse_dims[0] = s.Wc;
+        coarse_dims[1] = s.Hc;
+        coarse_dims[2] = s.Dc;
+    }
+    if (!in2) return BLZ_OK;
+    if (!(dt > 0.0f)) {
+        blz::set_error("dt must be positive");
+        return BLZ_E_INVALID;
+    }
+    return blz::warm_start_input(s, dt, in2);
 }
 
 int blz_set_rbgs_variant(blz_solver* h, int variant) {

@@ -278,6 +278,319 @@ __global__ void __launch_bounds__(64) k_rbgs_pass(float* __restrict__ base, unsi
     if (g < a.Dg) column_run<PG, true>(a, g, prev);
 }
 
+// ---- fused sweep ----------------------------------------------------------
+// One whole sweep (the red pass and the black pass) per launch, reading the
+// old pressure from one plane set (src) and writing the new one to the other
+// (dst), at 12 bytes per cell instead of the 16 of two in-place passes.
+//
+// A wave covers 64 consecutive cells i0..i0+63 of row j, lane = cell column,
+// and walks slab groups g = 0..Dg-1. Per slab each lane evaluates three
+// update chains, each with the reference operation order of its cell:
+//   red lane   c1 = its own red cell;  c2 = the red cell just outside the
+//              wave that the wave's edge black lane needs (lane 63 when lane
+//              0 is black: (i0-1, j); lane 0 when lane 63 is black:
+//              (i0+64, j)), otherwise a copy of c1;  c3 unused.
+//   black lane c1 = red (i, j-1);  c2 = red (i, j+1);  c3 = its own black
+//              cell, from the new reds (i-1, j), (i+1, j) of the neighbour
+//              lanes (ds_bpermute; wave edges take the other edge's c2),
+//              c1, c2, its own new k-1 and old k+1 values.
+// Red values of a neighbouring row are evaluated again by the lanes that
+// need them, from the same old black values in the same order, so every
+// value equals the two-pass sweep's (bitwise); only a lane's own cell is
+// stored. The old values stay untouched in src, so no lane can see a new
+// value early, whatever the schedule. Lanes with i >= W write -0 into the
+// ghost pair of their row (keeping it -0).
+//
+// Loads: 14 per group (6 per red-type chain: rhs, own old value for k+1,
+// x-1, x+1, y-1, y+1 black neighbours; the black chain's rhs and own old
+// value), one store; PG groups in flight (vmcnt bound per slot below).
+// Division: q0 = num*y, r = fma(-q0, d, num), q = fma(r, y, q0) equals IEEE
+// num / d for num = +0 and 2^-125 <= |num| <= FLT_MAX (exhaustively checked,
+// blz_selftest_division); any other numerator of a chain whose value is used
+// sends the wave to the exact run from that group (ping-pong: the inputs
+// are still unmodified).
+constexpr int FNL = 14; // loads per group
+constexpr int FNU = 3;  // of which the k+1 sources, issued first
+enum { F_PR1 = 0, F_PR2, F_PB3, F_RR1, F_XL1, F_XR1, F_YN1, F_YS1, F_RR2, F_XL2, F_XR2, F_YN2, F_YS2, F_RB3 };
+
+template <int PG, int U, bool FIRST>
+struct FusedWait {
+    static constexpr int N = FIRST && U < PG - 1 ? (FNL - FNU) + (PG - 2 - U) * FNL + U * (1 + FNL)
+                                                 : (FNL - FNU) + (PG - 2) * (FNL + 1);
+};
+
+struct FusedArgs {
+    i4v rs;
+    int vo[FNL];        // per-lane byte offsets of each load in group 0 (plane excluded)
+    int vst;            // per-lane byte offset of the store (plane included)
+    unsigned pl[FNL];   // plane byte offset of each load
+    int D, Dg, gstride;
+    float scale;
+    float dm[3], ym[3], de[3], ye[3]; // denominators/reciprocals per chain, mid and k-edge
+    int lane;
+    unsigned active, isRed;
+    unsigned need1, need2, need3;        // chain results that are used (0/1)
+    unsigned okL, okR, okN, okS;         // neighbour cells of the black update inside the grid
+};
+
+template <int PG>
+__device__ __forceinline__ void fused_load(const FusedArgs& a, f4v (&sl)[FNL], int g) {
+    const int so = min(g, a.Dg) * a.gstride;
+#pragma unroll
+    for (int q = 0; q < FNL; ++q) sl[q] = ld4_async(a.vo[q], a.rs, (int)(a.pl[q] + (unsigned)so));
+}
+
+// wait for a slot's loads and the next slot's k+1 sources (first FNU loads);
+// the registers are in/out operands of the wait itself
+template <int N>
+__device__ __forceinline__ void fused_wait(f4v (&cur)[FNL], f4v (&nxt)[FNL]) {
+    asm volatile("s_waitcnt vmcnt(%17)"
+                 : "+v"(cur[0]), "+v"(cur[1]), "+v"(cur[2]), "+v"(cur[3]), "+v"(cur[4]), "+v"(cur[5]), "+v"(cur[6]),
+                   "+v"(cur[7]), "+v"(cur[8]), "+v"(cur[9]), "+v"(cur[10]), "+v"(cur[11]), "+v"(cur[12]),
+                   "+v"(cur[13]), "+v"(nxt[0]), "+v"(nxt[1]), "+v"(nxt[2])
+                 : "n"(N));
+}
+
+__device__ __forceinline__ float fdiv_fast(float num, float d, float y) {
+    const float q0 = num * y;
+    const float r = fmaf(-q0, d, num);
+    return fmaf(r, y, q0);
+}
+__device__ __forceinline__ unsigned fdiv_unsafe(float num) {
+    const float mag = fabsf(num);
+    const unsigned ok = (unsigned)(mag >= 0x1p-125f) & (unsigned)(mag <= 3.40282347e38f);
+    const unsigned pz = (unsigned)(__float_as_uint(num) == 0u);
+    return (ok | pz) ^ 1u;
+}
+
+__device__ __forceinline__ float lane_pull(float v, int src_lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+}
+
+template <int PG, int U, bool EXACT, bool FIRST>
+__device__ __forceinline__ bool fused_group(const FusedArgs& a, f4v (&sl)[PG][FNL], int g, float (&prev)[3]) {
+    constexpr int UN = (U + 1) % PG;
+    fused_wait<FusedWait<PG, U, FIRST>::N>(sl[U], sl[UN]);
+    const int D = a.D;
+    const int k0 = 4 * g;
+    const f4v* c = sl[U];
+    f4v out;
+    unsigned bad = 0u;
+    float p1 = prev[0], p2 = prev[1], p3 = prev[2];
+    const int lm = (a.lane + 63) & 63, lp = (a.lane + 1) & 63;
+    // phase 1: the two red-type chains over the group's four slabs
+    float r1[4], r2[4], n1[4], n2[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = k0 + s;
+        const bool edge = (k == 0) | (k == D - 1);
+        const float up1 = s < 3 ? c[F_PR1][s + 1] : sl[UN][F_PR1][0];
+        const float up2 = s < 3 ? c[F_PR2][s + 1] : sl[UN][F_PR2][0];
+        float x = a.scale * c[F_RR1][s];
+        x = x + c[F_XL1][s];
+        x = x + c[F_XR1][s];
+        x = x + c[F_YN1][s];
+        x = x + c[F_YS1][s];
+        n1[s] = x + p1;
+        n1[s] = n1[s] + up1;
+        x = a.scale * c[F_RR2][s];
+        x = x + c[F_XL2][s];
+        x = x + c[F_XR2][s];
+        x = x + c[F_YN2][s];
+        x = x + c[F_YS2][s];
+        n2[s] = x + p2;
+        n2[s] = n2[s] + up2;
+        if (EXACT) {
+            r1[s] = div_small(n1[s], edge ? a.de[0] : a.dm[0], edge ? a.ye[0] : a.ym[0]);
+            r2[s] = div_small(n2[s], edge ? a.de[1] : a.dm[1], edge ? a.ye[1] : a.ym[1]);
+        } else {
+            r1[s] = fdiv_fast(n1[s], edge ? a.de[0] : a.dm[0], edge ? a.ye[0] : a.ym[0]);
+            r2[s] = fdiv_fast(n2[s], edge ? a.de[1] : a.dm[1], edge ? a.ye[1] : a.ym[1]);
+        }
+        p1 = r1[s];
+        p2 = r2[s];
+    }
+    // phase 2: new reds beside the black cell: lane -+ 1's chain 1, across
+    // the wave edge the opposite edge lane's chain 2
+    float Lv[4], Rv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        Lv[s] = lane_pull(a.lane == 63 ? r2[s] : r1[s], lm);
+        Rv[s] = lane_pull(a.lane == 0 ? r2[s] : r1[s], lp);
+    }
+    // one wait for all eight exchanges instead of one per slab
+    asm volatile("" : "+v"(Lv[0]), "+v"(Lv[1]), "+v"(Lv[2]), "+v"(Lv[3]), "+v"(Rv[0]), "+v"(Rv[1]), "+v"(Rv[2]),
+                 "+v"(Rv[3]));
+    // phase 3: the black chain
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = k0 + s;
+        const bool edge = (k == 0) | (k == D - 1);
+        const float up3 = s < 3 ? c[F_PB3][s + 1] : sl[UN][F_PB3][0];
+        const float vL = a.okL ? Lv[s] : -0.0f, vR = a.okR ? Rv[s] : -0.0f;
+        const float vN = a.okN ? r1[s] : -0.0f, vS = a.okS ? r2[s] : -0.0f;
+        float x = a.scale * c[F_RB3][s];
+        x = x + vL;
+        x = x + vR;
+        x = x + vN;
+        x = x + vS;
+        float n3 = x + p3;
+        n3 = n3 + up3;
+        float r3;
+        if (EXACT) {
+            r3 = div_small(n3, edge ? a.de[2] : a.dm[2], edge ? a.ye[2] : a.ym[2]);
+        } else {
+            r3 = fdiv_fast(n3, edge ? a.de[2] : a.dm[2], edge ? a.ye[2] : a.ym[2]);
+            const unsigned b = (a.need1 & fdiv_unsafe(n1[s])) | (a.need2 & fdiv_unsafe(n2[s])) |
+                               (a.need3 & fdiv_unsafe(n3));
+            bad |= (k < D) ? b : 0u;
+        }
+        const float mine = a.isRed ? r1[s] : r3;
+        out[s] = (k < D && a.active) ? mine : -0.0f;
+        p3 = r3;
+    }
+    if (!EXACT && __builtin_amdgcn_ballot_w64(bad != 0u)) return false;
+    prev[0] = p1;
+    prev[1] = p2;
+    prev[2] = p3;
+    st4_async(out, a.vst, a.rs, min(g, a.Dg) * a.gstride);
+    fused_load<PG>(a, sl[U], g + PG);
+    return true;
+}
+
+template <int PG, bool EXACT, bool FIRST, int U>
+__device__ __forceinline__ bool fused_ring(const FusedArgs& a, f4v (&sl)[PG][FNL], int gb, float (&prev)[3],
+                                           int* failed) {
+    if constexpr (U == PG) {
+        return true;
+    } else {
+        if (!fused_group<PG, U, EXACT, FIRST>(a, sl, gb + U, prev)) {
+            *failed = gb + U;
+            return false;
+        }
+        return fused_ring<PG, EXACT, FIRST, U + 1>(a, sl, gb, prev, failed);
+    }
+}
+
+__device__ __forceinline__ void hold_slot(f4v (&r)[FNL]) {
+    asm volatile("s_waitcnt vmcnt(0)"
+                 : "+v"(r[0]), "+v"(r[1]), "+v"(r[2]), "+v"(r[3]), "+v"(r[4]), "+v"(r[5]), "+v"(r[6]), "+v"(r[7]),
+                   "+v"(r[8]), "+v"(r[9]), "+v"(r[10]), "+v"(r[11]), "+v"(r[12]), "+v"(r[13]));
+}
+
+// Walk groups g_start.. of the wave. Returns Dg, or for a fast run the first
+// group where an exact division is needed (that group was not stored).
+template <int PG, bool EXACT>
+__device__ __forceinline__ int fused_run(const FusedArgs& a, int g_start, float (&prev)[3]) {
+    f4v sl[PG][FNL];
+#pragma unroll
+    for (int u = 0; u < PG; ++u) fused_load<PG>(a, sl[u], g_start + u);
+    int result = a.Dg;
+    int failed = a.Dg;
+    bool ok = fused_ring<PG, EXACT, true, 0>(a, sl, g_start, prev, &failed);
+    for (int gb = g_start + PG; ok && gb < a.Dg; gb += PG) ok = fused_ring<PG, EXACT, false, 0>(a, sl, gb, prev, &failed);
+    if (!ok) result = failed;
+#pragma unroll
+    for (int u = 0; u < PG; ++u) hold_slot(sl[u]);
+    return result;
+}
+
+__device__ __forceinline__ int cell_cnt(int i, int j, int W, int H) {
+    return (int)(i > 0) + (int)(i < W - 1) + (int)(j > 0) + (int)(j < H - 1);
+}
+
+template <int PG>
+__global__ void __launch_bounds__(64) k_rbgs_fused(float* __restrict__ base, unsigned nbytes, unsigned srcR,
+                                                   unsigned srcB, unsigned rhsR, unsigned rhsB, unsigned dstR,
+                                                   unsigned dstB, int W, int H, int D, int Wp, int Hp, float scale,
+                                                   DivTab tab, const int* __restrict__ stop) {
+    if (*stop) return; // the solve has converged
+    const int nwr = (W + 63) >> 6; // waves per row
+    const int wv = xcd_block(blockIdx.x, gridDim.x);
+    const int j = wv / nwr;
+    const int i0 = (wv - j * nwr) * 64;
+    const int lane = threadIdx.x;
+    const int i = i0 + lane;
+    const bool active = i < W;
+    const bool red = ((i + j) & 1) == 0;
+    // byte offset of cell (ii, jj) in padded group 1 of a plane
+    auto off = [&](int ii, int jj) -> int { return 16 * (((Hp + (jj + 1)) * Wp) + ((ii >> 1) + 1)); };
+    FusedArgs a;
+    const unsigned long long b64 = (unsigned long long)base;
+    a.rs[0] = (int)(unsigned)(b64 & 0xffffffffull);
+    a.rs[1] = (int)(unsigned)(b64 >> 32);
+    a.rs[2] = (int)nbytes;
+    a.rs[3] = 0x00020000;
+    a.D = D;
+    a.Dg = (D + 3) >> 2;
+    a.gstride = Hp * Wp * 16;
+    a.scale = scale;
+    a.lane = lane;
+    // chain cells
+    const bool haloL = red && lane == 63; // lane 0 black: needs red (i0-1, j)
+    const bool haloR = red && lane == 0;  // lane 63 black: needs red (i0+64, j)
+    const int ic = active ? i : W - 1;    // address column of an inactive lane
+    int c1i, c1j, c2i, c2j;
+    if (red) {
+        c1i = ic;
+        c1j = j;
+        // a halo cell outside the grid is not used: read the lane's own
+        c2i = haloL && i0 >= 1 ? i0 - 1 : (haloR && i0 + 64 < W ? i0 + 64 : ic);
+        c2j = j;
+    } else {
+        c1i = ic;
+        c1j = j - 1;
+        c2i = ic;
+        c2j = j + 1;
+    }
+    const int ci[2] = {c1i, c2i}, cj[2] = {c1j, c2j};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int ii = ci[q], jj = cj[q];
+        const int base_slot = q == 0 ? F_RR1 : F_RR2;
+        a.vo[q == 0 ? F_PR1 : F_PR2] = off(ii, jj);
+        a.pl[q == 0 ? F_PR1 : F_PR2] = srcR;
+        a.vo[base_slot] = off(ii, jj);
+        a.pl[base_slot] = rhsR;
+        a.vo[base_slot + 1] = off(ii - 1, jj); // x-1
+        a.vo[base_slot + 2] = off(ii + 1, jj); // x+1
+        a.vo[base_slot + 3] = off(ii, jj - 1); // y-1
+        a.vo[base_slot + 4] = off(ii, jj + 1); // y+1
+        for (int t = 1; t <= 4; ++t) a.pl[base_slot + t] = srcB;
+        const int cnt = cell_cnt(ii, jj, W, H);
+        a.dm[q] = (float)(cnt + 2);
+        a.de[q] = (float)(cnt + 1);
+        a.ym[q] = rcp_of(cnt + 2, tab);
+        a.ye[q] = rcp_of(cnt + 1, tab);
+    }
+    a.vo[F_PB3] = off(ic, j);
+    a.pl[F_PB3] = srcB;
+    a.vo[F_RB3] = off(ic, j);
+    a.pl[F_RB3] = rhsB;
+    {
+        const int cnt = cell_cnt(ic, j, W, H);
+        a.dm[2] = (float)(cnt + 2);
+        a.de[2] = (float)(cnt + 1);
+        a.ym[2] = rcp_of(cnt + 2, tab);
+        a.ye[2] = rcp_of(cnt + 1, tab);
+    }
+    // store: own cell in its colour's dst plane; inactive lanes write -0
+    // into the row's ghost pair
+    a.vst = (int)((red ? dstR : dstB) + (unsigned)(active ? off(i, j) : off(W, j)));
+    a.isRed = red ? 1u : 0u;
+    a.active = active ? 1u : 0u;
+    a.need1 = active && (red || j > 0) ? 1u : 0u;
+    a.need2 = (red ? ((haloL && i0 >= 1) || (haloR && i0 + 64 < W)) : (active && j < H - 1)) ? 1u : 0u;
+    a.need3 = active && !red ? 1u : 0u;
+    a.okL = i >= 1 ? 1u : 0u;
+    a.okR = i + 1 < W ? 1u : 0u;
+    a.okN = j > 0 ? 1u : 0u;
+    a.okS = j < H - 1 ? 1u : 0u;
+    float prev[3] = {-0.0f, -0.0f, -0.0f};
+    const int g = fused_run<PG, false>(a, 0, prev);
+    if (g < a.Dg) fused_run<PG, true>(a, g, prev);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -418,8 +731,10 @@ __global__ void k_ctl_reset(SolveCtl* c) {
     c->done = 0;
     c->iters = 0;
     c->checks = 0;
+    c->ws = 0;
     c->res = -1.0;
     c->res0 = -1.0;
+    c->res_ws = -1.0;
 }
 
 // pressure_gradient_update_kernel :601-669 for the faces along AXIS,
@@ -470,16 +785,19 @@ __global__ void __launch_bounds__(BS) k_natural_to_rb(RB p, RBL L, const float* 
 }
 
 // pressure planes: +0 inside, -0 in the ghost shell; rhs planes: 0
-__global__ void __launch_bounds__(BS) k_init_pressure(float* __restrict__ prb, size_t Np, RBL L) {
-    const size_t n = 4 * Np;
+// planes 0, 1 (home pressure set) and 4, 5 (spare set): +0 inside, -0 in
+// the ghost shell; rhs planes 2, 3 and the tail: +0
+__global__ void __launch_bounds__(BS) k_init_pressure(float* __restrict__ prb, size_t Np, size_t n, RBL L) {
     for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < n; t += (size_t)gridDim.x * BS) {
+        const size_t plane = t / Np;
         const size_t e = t % Np;
         const int comp = (int)(e & 3);
         const size_t v = e >> 2;
         const int m = (int)(v % L.Wp), j = (int)((v / L.Wp) % L.Hp), g = (int)(v / ((size_t)L.Wp * L.Hp));
         const int k = (g - 1) * 4 + comp;
         const bool inside = m >= 1 && m <= L.Wh && j >= 1 && j <= L.H && g >= 1 && g <= L.Dg && k < L.D;
-        prb[t] = (t < 2 * Np && !inside) ? -0.0f : 0.0f;
+        const bool pset = plane == 0 || plane == 1 || plane == 4 || plane == 5;
+        prb[t] = (pset && !inside) ? -0.0f : 0.0f;
     }
 }
 
@@ -542,8 +860,8 @@ static inline int2 cb_b(const Solver& s) {
 }
 
 int launch_init_pressure(Solver& s) {
-    const unsigned nb = (unsigned)std::min<size_t>(blocks_for(4 * s.Np, BS), 8192);
-    hipLaunchKernelGGL(k_init_pressure, dim3(nb), dim3(BS), 0, s.stream, s.prb, s.Np, s.L);
+    const unsigned nb = (unsigned)std::min<size_t>(blocks_for(s.prb_floats, BS), 8192);
+    hipLaunchKernelGGL(k_init_pressure, dim3(nb), dim3(BS), 0, s.stream, s.prb, s.Np, s.prb_floats, s.L);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
@@ -572,11 +890,48 @@ static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop) {
     return BLZ_OK;
 }
 
-// Gauss-Seidel sweeps (red pass, black pass each). variant: slab groups in
-// flight (2, 4 or 8; 0 = 8).
-int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop) {
+// Pairs of fused sweeps: home set -> spare set -> home set, so the pressure
+// is in the home planes after every pair (and at every residual check).
+template <int PG>
+static int fused_pairs(Solver& s, float scale, int pairs, const int* stop) {
+    const int nwr = (s.W + 63) / 64;
+    const dim3 grid((unsigned)(nwr * s.H)), block(64);
+    const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
+    const unsigned pb = (unsigned)(s.Np * 4);
+    const unsigned nbytes = (unsigned)(s.prb_floats * 4);
+    for (int l = 0; l < pairs; ++l) {
+        for (int h = 0; h < 2; ++h) {
+            const unsigned src = h == 0 ? 0u : 4u * pb, dst = h == 0 ? 4u * pb : 0u;
+            hipLaunchKernelGGL((k_rbgs_fused<PG>), grid, block, 0, s.stream, s.prb, nbytes, src, src + pb, 2u * pb,
+                               3u * pb, dst, dst + pb, s.W, s.H, s.D, s.L.Wp, s.L.Hp, scale, tab, stop);
+            BLZ_HIP(hipGetLastError());
+        }
+    }
+    return BLZ_OK;
+}
+
+bool fused_available(const Solver& s) { return s.prb_floats >= 6 * s.Np; }
+
+// Gauss-Seidel sweeps. variant: 2/4/8 = two in-place colour passes per
+// sweep with that many slab groups in flight; 102/103 = fused sweeps (2 or
+// 3 groups in flight) in pairs, an odd last sweep as two colour passes;
+// 0 = the default (fused, 3 groups, where the spare planes exist).
+int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* launches, int* kind) {
     const float scale = (s.s.density * s.dx * s.dx) / dt; // Bridson p.75, :530
-    switch (s.rbgs_variant) {
+    int v = s.rbgs_variant;
+    if (v == 0) v = fused_available(s) ? 103 : 8;
+    if (v >= 100 && !fused_available(s)) v = 8;
+    if (v >= 100) {
+        const int pairs = sweeps / 2;
+        *launches = 2 * pairs + 2 * (sweeps & 1);
+        *kind = 2;
+        int e = v == 102 ? fused_pairs<2>(s, scale, pairs, stop) : fused_pairs<3>(s, scale, pairs, stop);
+        if (e) return e;
+        return (sweeps & 1) ? rbgs_passes<8>(s, scale, 1, stop) : BLZ_OK;
+    }
+    *launches = 2 * sweeps;
+    *kind = 1;
+    switch (v) {
     case 2: return rbgs_passes<2>(s, scale, sweeps, stop);
     case 4: return rbgs_passes<4>(s, scale, sweeps, stop);
     default: return rbgs_passes<8>(s, scale, sweeps, stop);
@@ -608,6 +963,16 @@ int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int 
                        first);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
+}
+
+int launch_residual_ws(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap) {
+    const double cc = (double)dt / ((double)s.s.density * (double)s.dx * (double)s.dx);
+    const size_t items = (size_t)(s.W / 2) * s.H * ((s.L.Dg + RES_SEG - 1) / RES_SEG);
+    const int nb = (int)std::min<size_t>(blocks_for(items, BS), RESIDUAL_BLOCKS);
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, cc, cb_a(s),
+                       cb_b(s), s.red + 8, &c->done);
+    BLZ_HIP(hipGetLastError());
+    return launch_residual_ws_final(s, c, host_done, seq, tol, cap, nb);
 }
 
 int launch_max_abs_velocity(Solver& s, double* dev_out) {

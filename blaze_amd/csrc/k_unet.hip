@@ -1,0 +1,731 @@
+// k_unet.hip — kernels and host driver of the warm-start U-Net (unet.h):
+// 3x3x3 convolutions as implicit GEMMs on the BF16 MFMA units (FP32
+// accumulation), their weight and input gradients, pooling, upsampling, the
+// 1x1 head with the mean-squared-error loss, and Adam. No reference
+// counterpart (the reference has no learned component).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <random>
+#include <vector>
+
+#include "unet.h"
+
+namespace blz {
+
+namespace {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4m __attribute__((ext_vector_type(4)));
+
+// BF16 <-> FP32: round to nearest even, NaN kept quiet
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ bf16 f2bf(float f) {
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)((u >> 16) | 0x40u);
+    return (bf16)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+struct Geo {
+    int B, D, H, W;
+};
+
+__device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, int& z, int& y, int& x) {
+    x = (int)(v % g.W);
+    long long t = v / g.W;
+    y = (int)(t % g.H);
+    t /= g.H;
+    z = (int)(t % g.D);
+    b = (int)(t / g.D);
+}
+
+// 3x3x3 convolution, zero padding: Y[v][co] = sum_tap sum_ci X[v+tap][ci] W[tap][co][ci].
+// A wave computes 16 consecutive voxels x all NT*16 output channels with
+// v_mfma_f32_16x16x16_bf16: A = 16 voxels x 16 input channels (lane l:
+// voxel l%16, channels 4*(l/16)..+3), B = 16 input x 16 output channels
+// (lane l: output channel l%16, input channels 4*(l/16)..+3), D = 16
+// voxels x 16 output channels (lane l: output channel l%16, voxels
+// 4*(l/16)..+3). Optional epilogue: ReLU, or a product with (M > 0).
+template <int NT>
+__global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, int Cx, int xoff, int Cin,
+                                                   const bf16* __restrict__ Wt, bf16* __restrict__ Y, int Cy, int yoff,
+                                                   const bf16* __restrict__ Mk, int Cm, int moff, int relu, Geo g,
+                                                   long long nvox) {
+    const int lane = threadIdx.x & 63;
+    const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (v0 >= nvox) return;
+    const int li = lane & 15, kb = lane >> 4;
+    const long long va = v0 + li;
+    int b, z, y, x;
+    vox_coords(va < nvox ? va : nvox - 1, g, b, z, y, x);
+    const bool vok = va < nvox;
+    f4m acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f4m{0.f, 0.f, 0.f, 0.f};
+    const int nchunk = Cin >> 4;
+    const long long HW = (long long)g.H * g.W;
+    for (int tap = 0; tap < 27; ++tap) {
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int zz = z + dz, yy = y + dy, xx = x + dx;
+        const bool inb = vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+        const long long vv = va + dz * HW + dy * g.W + dx;
+        const bf16* xp = X + (inb ? vv : 0) * Cx + xoff + kb * 4;
+        const bf16* wp = Wt + ((size_t)tap * (NT * 16) + li) * Cin + kb * 4;
+        for (int c = 0; c < nchunk; ++c) {
+            s4v a = {0, 0, 0, 0};
+            if (inb) a = *(const s4v*)(xp + c * 16);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const s4v bw = *(const s4v*)(wp + (size_t)t * 16 * Cin + c * 16);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[t], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long long vo = v0 + kb * 4 + q;
+            if (vo >= nvox) continue;
+            float r = acc[t][q];
+            const int co = t * 16 + li;
+            if (relu) r = fmaxf(r, 0.0f);
+            if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : 0.0f;
+            Y[vo * Cy + yoff + co] = f2bf(r);
+        }
+    }
+}
+
+// weight gradient: dW[tap][co][ci] += sum_v G[v][co] X[v+tap][ci]. A wave owns
+// one 16x16 (co, ci) tile and a run of voxels; per 16 voxels the G fragment
+// (A = 16 co x 16 voxels) is loaded once and used for all 27 taps (B = 16
+// voxels x 16 ci, shifted by the tap). FP32 partial tiles are added with
+// atomics.
+__global__ void __launch_bounds__(256) k_conv3_dw(const bf16* __restrict__ X, int Cx, int xoff, int Cin,
+                                                  const bf16* __restrict__ Gd, int Cg, int goff, int Cout,
+                                                  float* __restrict__ dW, Geo g, long long nvox, long long run) {
+    const int lane = threadIdx.x & 63;
+    const long long wv = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nct = Cout >> 4, ncit = Cin >> 4;
+    const int tiles = nct * ncit;
+    const long long chunk = wv / tiles;
+    const int tile = (int)(wv % tiles);
+    const int cot = tile / ncit, cit = tile % ncit;
+    const long long vbeg = chunk * run;
+    if (vbeg >= nvox) return;
+    const long long vend = min(nvox, vbeg + run);
+    const int li = lane & 15, kb = lane >> 4;
+    const long long HW = (long long)g.H * g.W;
+    f4m acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = f4m{0.f, 0.f, 0.f, 0.f};
+    for (long long vb = vbeg; vb < vend; vb += 16) {
+        s4v a;
+        int bq[4], zq[4], yq[4], xq[4];
+        bool okq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long long v = vb + kb * 4 + q;
+            okq[q] = v < vend;
+            vox_coords(okq[q] ? v : vbeg, g, bq[q], zq[q], yq[q], xq[q]);
+            a[q] = okq[q] ? (short)Gd[v * Cg + goff + cot * 16 + li] : (short)0;
+        }
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            s4v bx;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int zz = zq[q] + dz, yy = yq[q] + dy, xx = xq[q] + dx;
+                const bool inb = okq[q] && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+                const long long vv = vb + kb * 4 + q + dz * HW + dy * g.W + dx;
+                bx[q] = inb ? (short)X[vv * Cx + xoff + cit * 16 + li] : (short)0;
+            }
+            acc[tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx, acc[tap], 0, 0, 0);
+        }
+    }
+    // D[i][j]: i = co (4*kb + q), j = ci (li)
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            atomicAdd(&dW[((size_t)tap * Cout + cot * 16 + kb * 4 + q) * Cin + cit * 16 + li], acc[tap][q]);
+}
+
+// Wt[tap'][ci][co] = W[26 - tap'][co][ci]: the input gradient is the forward
+// convolution of the output gradient with these weights
+__global__ void k_wflip(const bf16* __restrict__ W, bf16* __restrict__ Wt, int Cout, int Cin) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = 27 * Cout * Cin;
+    if (t >= n) return;
+    const int ci = t % Cin, co = (t / Cin) % Cout, tap = t / (Cin * Cout);
+    Wt[((size_t)(26 - tap) * Cin + ci) * Cout + co] = W[t];
+}
+
+__device__ __forceinline__ long long parent_of(long long v, const Geo& fine, int& b, int& z, int& y, int& x) {
+    vox_coords(v, fine, b, z, y, x);
+    const int D2 = fine.D / 2, H2 = fine.H / 2, W2 = fine.W / 2;
+    return (((long long)b * D2 + z / 2) * H2 + y / 2) * W2 + x / 2;
+}
+
+// 2x2x2 average pooling (coarse voxel per thread-channel)
+__global__ void k_pool_fwd(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd, int doff,
+                           int C, Geo fine, long long ncoarse) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncoarse * C) return;
+    const int c = (int)(t % C);
+    const long long vc = t / C;
+    const Geo cg{fine.B, fine.D / 2, fine.H / 2, fine.W / 2};
+    int b, z, y, x;
+    vox_coords(vc, cg, b, z, y, x);
+    float s = 0.0f;
+    for (int q = 0; q < 8; ++q) {
+        const long long vf = (((long long)b * fine.D + 2 * z + (q >> 2)) * fine.H + 2 * y + ((q >> 1) & 1)) * fine.W +
+                             2 * x + (q & 1);
+        s += bf2f(S[vf * Cs + soff + c]);
+    }
+    Dd[vc * Cd + doff + c] = f2bf(s * 0.125f);
+}
+
+// pooling backward added to the gradient of the pooled activation S, then
+// ReLU mask of S: gS = (gS + gP[parent] / 8) * (S > 0)
+__global__ void k_pool_bwd(const bf16* __restrict__ gP, int Cp, int poff, bf16* __restrict__ gS, int Cgs, int gsoff,
+                           const bf16* __restrict__ S, int Cs, int soff, int C, Geo fine, long long nfine) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfine * C) return;
+    const int c = (int)(t % C);
+    const long long v = t / C;
+    int b, z, y, x;
+    const long long vp = parent_of(v, fine, b, z, y, x);
+    const float r = bf2f(gS[v * Cgs + gsoff + c]) + bf2f(gP[vp * Cp + poff + c]) * 0.125f;
+    gS[v * Cgs + gsoff + c] = f2bf(bf2f(S[v * Cs + soff + c]) > 0.0f ? r : 0.0f);
+}
+
+// nearest 2x upsampling of a coarse view into a fine view
+__global__ void k_up_fwd(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd, int doff, int C,
+                         Geo fine, long long nfine) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfine * C) return;
+    const int c = (int)(t % C);
+    const long long v = t / C;
+    int b, z, y, x;
+    const long long vp = parent_of(v, fine, b, z, y, x);
+    Dd[v * Cd + doff + c] = S[vp * Cs + soff + c];
+}
+
+// upsampling backward: gB[parent] = sum of the 8 children's gradients, then
+// ReLU mask of the coarse activation B
+__global__ void k_up_bwd(const bf16* __restrict__ gU, int Cu, int uoff, bf16* __restrict__ gB, int Cb, int boff,
+                         const bf16* __restrict__ Bm, int Cbm, int bmoff, int C, Geo fine, long long ncoarse) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncoarse * C) return;
+    const int c = (int)(t % C);
+    const long long vc = t / C;
+    const Geo cg{fine.B, fine.D / 2, fine.H / 2, fine.W / 2};
+    int b, z, y, x;
+    vox_coords(vc, cg, b, z, y, x);
+    float s = 0.0f;
+    for (int q = 0; q < 8; ++q) {
+        const long long vf = (((long long)b * fine.D + 2 * z + (q >> 2)) * fine.H + 2 * y + ((q >> 1) & 1)) * fine.W +
+                             2 * x + (q & 1);
+        s += bf2f(gU[vf * Cu + uoff + c]);
+    }
+    gB[vc * Cb + boff + c] = f2bf(bf2f(Bm[vc * Cbm + bmoff + c]) > 0.0f ? s : 0.0f);
+}
+
+// 1x1 head: out[v] = sum_c X[v][c] w[c] (FP32)
+__global__ void k_head_fwd(const bf16* __restrict__ X, const float* __restrict__ w, float* __restrict__ out,
+                           long long n) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NET_HEAD; ++c) s = fmaf(bf2f(X[v * NET_HEAD + c]), bf2f(f2bf(w[c])), s);
+    out[v] = s;
+}
+
+// loss = sum (out - target)^2 / n; gout = 2 (out - target) / n
+__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ out, const float* __restrict__ tg,
+                                             float* __restrict__ gout, double* loss, long long n) {
+    double s = 0.0;
+    const float inv = 1.0f / (float)n;
+    for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        const float d = out[v] - tg[v];
+        gout[v] = 2.0f * d * inv;
+        s += (double)d * (double)d;
+    }
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ double sm[4];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (sm[0] + sm[1] + sm[2] + sm[3]) / (double)n);
+}
+
+// head backward: gX[v][c] = gout[v] w[c] (ReLU mask of X), gw[c] += sum_v gout[v] X[v][c]
+__global__ void __launch_bounds__(256) k_head_bwd(const float* __restrict__ gout, const bf16* __restrict__ X,
+                                                  const float* __restrict__ w, bf16* __restrict__ gX,
+                                                  float* __restrict__ gw, long long n) {
+    float acc[NET_HEAD];
+#pragma unroll
+    for (int c = 0; c < NET_HEAD; ++c) acc[c] = 0.0f;
+    for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        const float go = gout[v];
+#pragma unroll
+        for (int c = 0; c < NET_HEAD; ++c) {
+            const float xv = bf2f(X[v * NET_HEAD + c]);
+            acc[c] = fmaf(go, xv, acc[c]);
+            gX[v * NET_HEAD + c] = f2bf(xv > 0.0f ? go * bf2f(f2bf(w[c])) : 0.0f);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NET_HEAD; ++c) {
+        float s = acc[c];
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&gw[c], s);
+    }
+}
+
+__global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
+                       float* __restrict__ V, size_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gg = G[t];
+    const float m = b1 * M[t] + (1.0f - b1) * gg;
+    const float v = b2 * V[t] + (1.0f - b2) * gg * gg;
+    M[t] = m;
+    V[t] = v;
+    P[t] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ P, bf16* __restrict__ Wb, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) Wb[t] = f2bf(P[t]);
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, size_t n, float v) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ void k_fill_bf16(bf16* __restrict__ p, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+__global__ void k_fill_d(double* p) { *p = 0.0; }
+
+// x0[v] = (res[v], mask[v], 0 ...) in BF16
+__global__ void k_pack_input(const float* __restrict__ res, const float* __restrict__ mask, bf16* __restrict__ x0,
+                             long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * NET_IN_CH) return;
+    const int c = (int)(t % NET_IN_CH);
+    const long long v = t / NET_IN_CH;
+    const float val = c == 0 ? res[v] : (c == 1 ? mask[v] : 0.0f);
+    x0[t] = f2bf(val);
+}
+
+inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+} // namespace
+
+#define NET_HIP(x)                                                   \
+    do {                                                             \
+        hipError_t e_ = (x);                                         \
+        if (e_ != hipSuccess) {                                      \
+            blz::set_error(std::string("HIP: ") + hipGetErrorString(e_)); \
+            return BLZ_E_HIP;                                        \
+        }                                                            \
+    } while (0)
+
+static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const NetDims& d, bool transposed) {
+    const int cin = transposed ? kConv[layer].cout : kConv[layer].cin;
+    const int cout = transposed ? kConv[layer].cin : kConv[layer].cout;
+    const bf16* w = (transposed ? n.Wt : n.Wb) + n.off[layer];
+    const Geo g{d.B, d.D, d.H, d.W};
+    const long long nv = d.n();
+    const unsigned grid = nblk((nv + 15) / 16, 4);
+#define CONV_CASE(NT)                                                                                              \
+    case NT:                                                                                                       \
+        hipLaunchKernelGGL(k_conv3_fwd<NT>, dim3(grid), dim3(256), 0, n.stream, x.p, x.C, x.off, cin, w, y.p, y.C, \
+                           y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv);                                  \
+        break;
+    switch (cout / 16) {
+        CONV_CASE(1)
+        CONV_CASE(2)
+        CONV_CASE(3)
+        CONV_CASE(4)
+        CONV_CASE(6)
+    default: set_error("unsupported channel count"); return BLZ_E_INVALID;
+    }
+#undef CONV_CASE
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+static int conv_dw(Net& n, int layer, TV x, TV gy, const NetDims& d) {
+    const int cin = kConv[layer].cin, cout = kConv[layer].cout;
+    const Geo g{d.B, d.D, d.H, d.W};
+    const long long nv = d.n();
+    const int tiles = (cout / 16) * (cin / 16);
+    // voxel run per wave: at least 512, at most 32 runs per tile
+    long long run = std::max<long long>(512, (nv + 31) / 32);
+    run = (run + 15) / 16 * 16;
+    const long long chunks = (nv + run - 1) / run;
+    const long long waves = chunks * tiles;
+    hipLaunchKernelGGL(k_conv3_dw, dim3(nblk(waves, 4)), dim3(256), 0, n.stream, x.p, x.C, x.off, cin, gy.p, gy.C,
+                       gy.off, cout, n.G + n.off[layer], g, nv, run);
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+Net::~Net() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (!view) {
+        for (void* p : {(void*)P, (void*)G, (void*)M, (void*)V, (void*)Wb, (void*)Wt, (void*)loss})
+            if (p) (void)hipFree(p);
+        if (loss_host) (void)hipHostFree(loss_host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    if (act) (void)hipFree(act);
+}
+
+int Net::init_view(const Net& src, hipStream_t st) {
+    view = true;
+    stream = st;
+    for (int l = 0; l <= L_NCONV; ++l) off[l] = src.off[l];
+    nparam = src.nparam;
+    P = src.P;
+    Wb = src.Wb;
+    Wt = src.Wt;
+    return BLZ_OK;
+}
+
+int Net::init(unsigned long long seed) {
+    NET_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    size_t o = 0;
+    for (int l = 0; l < L_NCONV; ++l) {
+        off[l] = o;
+        o += (size_t)27 * kConv[l].cout * kConv[l].cin;
+    }
+    off[L_NCONV] = o;
+    nparam = o + NET_HEAD;
+    if (hipMalloc(&P, nparam * 4) != hipSuccess || hipMalloc(&G, nparam * 4) != hipSuccess ||
+        hipMalloc(&M, nparam * 4) != hipSuccess || hipMalloc(&V, nparam * 4) != hipSuccess ||
+        hipMalloc(&Wb, nparam * 2) != hipSuccess || hipMalloc(&Wt, nparam * 2) != hipSuccess ||
+        hipMalloc(&loss, sizeof(double)) != hipSuccess) {
+        set_error("hipMalloc failed for network parameters");
+        return BLZ_E_NOMEM;
+    }
+    NET_HIP(hipHostMalloc((void**)&loss_host, sizeof(double), hipHostMallocDefault));
+    // He-normal initialisation from a seeded generator (fan-in = 27 x used
+    // input channels; the padded input channels of the first layer are 0)
+    std::vector<float> h(nparam, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < L_NCONV; ++l) {
+        const int cin = kConv[l].cin, cout = kConv[l].cout;
+        const int used = l == L_E1A ? 2 : cin;
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (27.0f * (float)used)));
+        for (int tap = 0; tap < 27; ++tap)
+            for (int co = 0; co < cout; ++co)
+                for (int ci = 0; ci < cin; ++ci)
+                    h[off[l] + ((size_t)tap * cout + co) * cin + ci] = ci < used ? nd(rng) : 0.0f;
+    }
+    std::normal_distribution<float> nh(0.0f, std::sqrt(1.0f / NET_HEAD));
+    for (int c = 0; c < NET_HEAD; ++c) h[off[L_NCONV] + c] = nh(rng);
+    NET_HIP(hipMemcpy(P, h.data(), nparam * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblk((long long)nparam, 256)), dim3(256), 0, stream, M, nparam, 0.0f);
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblk((long long)nparam, 256)), dim3(256), 0, stream, V, nparam, 0.0f);
+    NET_HIP(hipGetLastError());
+    adam_t = 0;
+    return refresh_weights();
+}
+
+int Net::refresh_weights() {
+    hipLaunchKernelGGL(k_to_bf16, dim3(nblk((long long)nparam, 256)), dim3(256), 0, stream, P, Wb, nparam);
+    for (int l = 0; l < L_NCONV; ++l) {
+        const int n = 27 * kConv[l].cout * kConv[l].cin;
+        hipLaunchKernelGGL(k_wflip, dim3(nblk(n, 256)), dim3(256), 0, stream, Wb + off[l], Wt + off[l],
+                           kConv[l].cout, kConv[l].cin);
+    }
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int Net::ensure_dims(const NetDims& d) {
+    if (d.D % 4 || d.H % 4 || d.W % 4 || d.B < 1 || d.D < 4 || d.H < 4 || d.W < 4) {
+        set_error("network grid must be a multiple of 4 (>= 4) on every axis");
+        return BLZ_E_INVALID;
+    }
+    if (d.B == dims.B && d.D == dims.D && d.H == dims.H && d.W == dims.W) return BLZ_OK;
+    const long long n1 = d.n(), n2 = d.half().n(), n3 = d.half().half().n();
+    // (buffer, elements) in BF16, then FP32 arrays
+    struct Item {
+        bf16** p;
+        long long n;
+    };
+    const Item items[] = {{&x0, n1 * 16},   {&a1, n1 * 16},     {&cat1, n1 * 48},  {&p1, n2 * 16},    {&a2, n2 * 32},
+                          {&cat2, n2 * 96}, {&p2, n3 * 32},     {&a3, n3 * 64},    {&b3, n3 * 64},    {&a4, n2 * 32},
+                          {&b4, n2 * 32},   {&a5, n1 * 16},     {&b5, n1 * 16},    {&g_b5, n1 * 16},  {&g_a5, n1 * 16},
+                          {&g_cat1, n1 * 48}, {&g_b4, n2 * 32}, {&g_a4, n2 * 32},  {&g_cat2, n2 * 96}, {&g_b3, n3 * 64},
+                          {&g_a3, n3 * 64}, {&g_p2, n3 * 32},   {&g_a2, n2 * 32},  {&g_p1, n2 * 16},  {&g_a1, n1 * 16}};
+    size_t bytes = 0;
+    for (const Item& it : items) bytes += ((size_t)it.n * 2 + 255) / 256 * 256;
+    const size_t fbytes = ((size_t)n1 * 4 + 255) / 256 * 256;
+    bytes += 3 * fbytes;
+    if (bytes > act_bytes) {
+        if (act) NET_HIP(hipFree(act));
+        act = nullptr;
+        act_bytes = 0;
+        if (hipMalloc(&act, bytes) != hipSuccess) {
+            set_error("hipMalloc failed for network activations");
+            return BLZ_E_NOMEM;
+        }
+        act_bytes = bytes;
+    }
+    char* c = (char*)act;
+    for (const Item& it : items) {
+        *it.p = (bf16*)c;
+        c += ((size_t)it.n * 2 + 255) / 256 * 256;
+    }
+    out = (float*)c;
+    c += fbytes;
+    target = (float*)c;
+    c += fbytes;
+    gout = (float*)c;
+    // zero everything once: padded input channels stay 0
+    hipLaunchKernelGGL(k_fill_bf16, dim3(nblk((long long)(bytes / 2), 256)), dim3(256), 0, stream, (bf16*)act,
+                       bytes / 2);
+    NET_HIP(hipGetLastError());
+    dims = d;
+    return BLZ_OK;
+}
+
+int Net::forward() {
+    const NetDims d1 = dims, d2 = d1.half(), d3 = d2.half();
+    const TV none{nullptr, 0, 0};
+    const Geo g1{d1.B, d1.D, d1.H, d1.W}, g2{d2.B, d2.D, d2.H, d2.W};
+    const long long n1 = d1.n(), n2 = d2.n(), n3 = d3.n();
+    int e;
+    if ((e = conv_fwd(*this, L_E1A, TV{x0, 16, 0}, TV{a1, 16, 0}, none, true, d1, false))) return e;
+    if ((e = conv_fwd(*this, L_E1B, TV{a1, 16, 0}, TV{cat1, 48, 32}, none, true, d1, false))) return e;
+    hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(n2 * 16, 256)), dim3(256), 0, stream, cat1, 48, 32, p1, 16, 0, 16, g1, n2);
+    if ((e = conv_fwd(*this, L_E2A, TV{p1, 16, 0}, TV{a2, 32, 0}, none, true, d2, false))) return e;
+    if ((e = conv_fwd(*this, L_E2B, TV{a2, 32, 0}, TV{cat2, 96, 64}, none, true, d2, false))) return e;
+    hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(n3 * 32, 256)), dim3(256), 0, stream, cat2, 96, 64, p2, 32, 0, 32, g2, n3);
+    if ((e = conv_fwd(*this, L_E3A, TV{p2, 32, 0}, TV{a3, 64, 0}, none, true, d3, false))) return e;
+    if ((e = conv_fwd(*this, L_E3B, TV{a3, 64, 0}, TV{b3, 64, 0}, none, true, d3, false))) return e;
+    hipLaunchKernelGGL(k_up_fwd, dim3(nblk(n2 * 64, 256)), dim3(256), 0, stream, b3, 64, 0, cat2, 96, 0, 64, g2, n2);
+    if ((e = conv_fwd(*this, L_D2A, TV{cat2, 96, 0}, TV{a4, 32, 0}, none, true, d2, false))) return e;
+    if ((e = conv_fwd(*this, L_D2B, TV{a4, 32, 0}, TV{b4, 32, 0}, none, true, d2, false))) return e;
+    hipLaunchKernelGGL(k_up_fwd, dim3(nblk(n1 * 32, 256)), dim3(256), 0, stream, b4, 32, 0, cat1, 48, 0, 32, g1, n1);
+    if ((e = conv_fwd(*this, L_D1A, TV{cat1, 48, 0}, TV{a5, 16, 0}, none, true, d1, false))) return e;
+    if ((e = conv_fwd(*this, L_D1B, TV{a5, 16, 0}, TV{b5, 16, 0}, none, true, d1, false))) return e;
+    hipLaunchKernelGGL(k_head_fwd, dim3(nblk(n1, 256)), dim3(256), 0, stream, b5, P + off[L_NCONV], out, n1);
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int Net::loss_and_grad() {
+    const long long n1 = dims.n();
+    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(1), 0, stream, loss);
+    hipLaunchKernelGGL(k_mse, dim3((unsigned)std::min<long long>(nblk(n1, 256), 1024)), dim3(256), 0, stream, out,
+                       target, gout, loss, n1);
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int Net::backward() {
+    const NetDims d1 = dims, d2 = d1.half(), d3 = d2.half();
+    const Geo g1{d1.B, d1.D, d1.H, d1.W}, g2{d2.B, d2.D, d2.H, d2.W};
+    const long long n1 = d1.n(), n2 = d2.n(), n3 = d3.n();
+    const TV none{nullptr, 0, 0};
+    int e;
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblk((long long)nparam, 256)), dim3(256), 0, stream, G, nparam, 0.0f);
+    // head (ReLU mask of b5 folded in)
+    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)std::min<long long>(nblk(n1, 256), 512)), dim3(256), 0, stream,
+                       gout, b5, P + off[L_NCONV], g_b5, G + off[L_NCONV], n1);
+    NET_HIP(hipGetLastError());
+    // d1b: a5 -> b5
+    if ((e = conv_dw(*this, L_D1B, TV{a5, 16, 0}, TV{g_b5, 16, 0}, d1))) return e;
+    if ((e = conv_fwd(*this, L_D1B, TV{g_b5, 16, 0}, TV{g_a5, 16, 0}, TV{a5, 16, 0}, false, d1, true))) return e;
+    // d1a: cat1 -> a5; input gradient over all 48 channels (unmasked)
+    if ((e = conv_dw(*this, L_D1A, TV{cat1, 48, 0}, TV{g_a5, 16, 0}, d1))) return e;
+    if ((e = conv_fwd(*this, L_D1A, TV{g_a5, 16, 0}, TV{g_cat1, 48, 0}, none, false, d1, true))) return e;
+    // upsampled b4 (channels 0..31 of cat1) -> b4, ReLU mask of b4
+    hipLaunchKernelGGL(k_up_bwd, dim3(nblk(n2 * 32, 256)), dim3(256), 0, stream, g_cat1, 48, 0, g_b4, 32, 0, b4, 32,
+                       0, 32, g1, n2);
+    // d2b: a4 -> b4
+    if ((e = conv_dw(*this, L_D2B, TV{a4, 32, 0}, TV{g_b4, 32, 0}, d2))) return e;
+    if ((e = conv_fwd(*this, L_D2B, TV{g_b4, 32, 0}, TV{g_a4, 32, 0}, TV{a4, 32, 0}, false, d2, true))) return e;
+    // d2a: cat2 -> a4
+    if ((e = conv_dw(*this, L_D2A, TV{cat2, 96, 0}, TV{g_a4, 32, 0}, d2))) return e;
+    if ((e = conv_fwd(*this, L_D2A, TV{g_a4, 32, 0}, TV{g_cat2, 96, 0}, none, false, d2, true))) return e;
+    // upsampled b3 (channels 0..63 of cat2) -> b3, ReLU mask of b3
+    hipLaunchKernelGGL(k_up_bwd, dim3(nblk(n3 * 64, 256)), dim3(256), 0, stream, g_cat2, 96, 0, g_b3, 64, 0, b3, 64,
+                       0, 64, g2, n3);
+    // e3b: a3 -> b3
+    if ((e = conv_dw(*this, L_E3B, TV{a3, 64, 0}, TV{g_b3, 64, 0}, d3))) return e;
+    if ((e = conv_fwd(*this, L_E3B, TV{g_b3, 64, 0}, TV{g_a3, 64, 0}, TV{a3, 64, 0}, false, d3, true))) return e;
+    // e3a: p2 -> a3
+    if ((e = conv_dw(*this, L_E3A, TV{p2, 32, 0}, TV{g_a3, 64, 0}, d3))) return e;
+    if ((e = conv_fwd(*this, L_E3A, TV{g_a3, 64, 0}, TV{g_p2, 32, 0}, none, false, d3, true))) return e;
+    // pool of s2 (channels 64..95 of cat2): add into s2's gradient, ReLU mask of s2
+    hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(n2 * 32, 256)), dim3(256), 0, stream, g_p2, 32, 0, g_cat2, 96, 64, cat2,
+                       96, 64, 32, g2, n2);
+    // e2b: a2 -> s2
+    if ((e = conv_dw(*this, L_E2B, TV{a2, 32, 0}, TV{g_cat2, 96, 64}, d2))) return e;
+    if ((e = conv_fwd(*this, L_E2B, TV{g_cat2, 96, 64}, TV{g_a2, 32, 0}, TV{a2, 32, 0}, false, d2, true))) return e;
+    // e2a: p1 -> a2
+    if ((e = conv_dw(*this, L_E2A, TV{p1, 16, 0}, TV{g_a2, 32, 0}, d2))) return e;
+    if ((e = conv_fwd(*this, L_E2A, TV{g_a2, 32, 0}, TV{g_p1, 16, 0}, none, false, d2, true))) return e;
+    // pool of s1 (channels 32..47 of cat1)
+    hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(n1 * 16, 256)), dim3(256), 0, stream, g_p1, 16, 0, g_cat1, 48, 32, cat1,
+                       48, 32, 16, g1, n1);
+    // e1b: a1 -> s1
+    if ((e = conv_dw(*this, L_E1B, TV{a1, 16, 0}, TV{g_cat1, 48, 32}, d1))) return e;
+    if ((e = conv_fwd(*this, L_E1B, TV{g_cat1, 48, 32}, TV{g_a1, 16, 0}, TV{a1, 16, 0}, false, d1, true))) return e;
+    // e1a: x0 -> a1 (no input gradient)
+    if ((e = conv_dw(*this, L_E1A, TV{x0, 16, 0}, TV{g_a1, 16, 0}, d1))) return e;
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int Net::adam(float lr, float b1, float b2, float eps) {
+    ++adam_t;
+    const float c1 = 1.0f - std::pow(b1, (float)adam_t), c2 = 1.0f - std::pow(b2, (float)adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(nblk((long long)nparam, 256)), dim3(256), 0, stream, P, G, M, V, nparam, lr, b1,
+                       b2, eps, c1, c2);
+    NET_HIP(hipGetLastError());
+    return refresh_weights();
+}
+
+int net_pack_input(Net& n, const float* res, const float* mask, hipStream_t st) {
+    const long long n1 = n.dims.n();
+    hipLaunchKernelGGL(k_pack_input, dim3(nblk(n1 * NET_IN_CH, 256)), dim3(256), 0, st, res, mask, n.x0, n1);
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+} // namespace blz
+
+// ---- C ABI of the network (include/blaze.h) ----
+using blz::Net;
+
+extern "C" {
+
+int blz_net_create(unsigned long long seed, blz_net** out) {
+    if (!out) {
+        blz::set_error("blz_net_create: null argument");
+        return BLZ_E_INVALID;
+    }
+    *out = nullptr;
+    blz_net* h = new (std::nothrow) blz_net();
+    if (!h) {
+        blz::set_error("out of host memory");
+        return BLZ_E_NOMEM;
+    }
+    if (int e = h->n.init(seed)) {
+        delete h;
+        return e;
+    }
+    *out = h;
+    return BLZ_OK;
+}
+
+int blz_net_destroy(blz_net* h) {
+    delete h;
+    return BLZ_OK;
+}
+
+int blz_net_num_params(blz_net* h, long long* n) {
+    if (!h || !n) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    *n = (long long)h->n.nparam;
+    return BLZ_OK;
+}
+
+int blz_net_get_params(blz_net* h, float* host) {
+    if (!h || !host) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    NET_HIP(hipStreamSynchronize(h->n.stream));
+    NET_HIP(hipMemcpy(host, h->n.P, h->n.nparam * 4, hipMemcpyDeviceToHost));
+    return BLZ_OK;
+}
+
+int blz_net_set_params(blz_net* h, const float* host) {
+    if (!h || !host) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    NET_HIP(hipStreamSynchronize(h->n.stream));
+    NET_HIP(hipMemcpy(h->n.P, host, h->n.nparam * 4, hipMemcpyHostToDevice));
+    if (int e = h->n.refresh_weights()) return e;
+    NET_HIP(hipStreamSynchronize(h->n.stream));
+    return BLZ_OK;
+}
+
+int blz_net_get_grads(blz_net* h, float* host) {
+    if (!h || !host) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    NET_HIP(hipStreamSynchronize(h->n.stream));
+    NET_HIP(hipMemcpy(host, h->n.G, h->n.nparam * 4, hipMemcpyDeviceToHost));
+    return BLZ_OK;
+}
+
+static int net_load(Net& n, const float* in2, const float* target, const int dims[4]) {
+    if (!in2 || !dims) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    const blz::NetDims d{dims[0], dims[1], dims[2], dims[3]};
+    if (int e = n.ensure_dims(d)) return e;
+    const long long nv = d.n();
+    // staging: input channels (2 x nv) into out/gout, then packed into x0
+    NET_HIP(hipMemcpyAsync(n.out, in2, nv * 4, hipMemcpyHostToDevice, n.stream));
+    NET_HIP(hipMemcpyAsync(n.gout, in2 + nv, nv * 4, hipMemcpyHostToDevice, n.stream));
+    if (int e = blz::net_pack_input(n, n.out, n.gout, n.stream)) return e;
+    if (target) NET_HIP(hipMemcpyAsync(n.target, target, nv * 4, hipMemcpyHostToDevice, n.stream));
+    return BLZ_OK;
+}
+
+int blz_net_forward(blz_net* h, const float* in2, const int dims[4], float* out) {
+    if (!h || !out) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    Net& n = h->n;
+    if (int e = net_load(n, in2, nullptr, dims)) return e;
+    if (int e = n.forward()) return e;
+    NET_HIP(hipMemcpyAsync(out, n.out, n.dims.n() * 4, hipMemcpyDeviceToHost, n.stream));
+    NET_HIP(hipStreamSynchronize(n.stream));
+    return BLZ_OK;
+}
+
+int blz_net_train_step(blz_net* h, const float* in2, const float* target, const int dims[4], float lr, int update,
+                       double* loss) {
+    if (!h || !target) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    Net& n = h->n;
+    if (int e = net_load(n, in2, target, dims)) return e;
+    if (int e = n.forward()) return e;
+    if (int e = n.loss_and_grad()) return e;
+    if (int e = n.backward()) return e;
+    if (update)
+        if (int e = n.adam(lr, 0.9f, 0.999f, 1e-8f)) return e;
+    NET_HIP(hipMemcpyAsync(n.loss_host, n.loss, sizeof(double), hipMemcpyDeviceToHost, n.stream));
+    NET_HIP(hipStreamSynchronize(n.stream));
+    if (loss) *loss = *n.loss_host;
+    return BLZ_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,263 @@
+// k_warm.hip — learned warm start of the pressure solve (no reference
+// counterpart: gs_solve_kernel, src/FluidSolver.cu:517-599, always starts
+// from the previous pressure).
+//
+// Before the sweeps of a residual-driven solve:
+//  1. restrict the residual the previous pressure leaves,
+//       r = scale*rhs + sum_nbr p - n p        (scale = rho dx^2 / dt)
+//     to a 4x coarser grid (sum over each 4x4x4 block / 4, which is the
+//     right-hand side of the coarse Poisson problem for a smooth error), with
+//     a solid mask (1 where a block touches a domain wall or lies outside);
+//  2. normalise by s = max |coarse r| and run the U-Net (bias-free, so its
+//     output scales with s);
+//  3. save p, add s * trilinear(prediction) to p;
+//  4. evaluate the FP64 residual: the prediction is kept only if it is lower
+//     than before, otherwise p is restored (RBGS stays the judge and the
+//     fallback). The sweeps then run to the same tolerance as without it.
+// All decisions are taken on the device (no host round trip).
+#include "solver.h"
+
+namespace blz {
+
+namespace {
+
+constexpr unsigned BS = 256;
+
+__device__ __forceinline__ bf16 f2bf_w(float f) {
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)((u >> 16) | 0x40u);
+    return (bf16)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float pget(const RB& p, const RBL& L, int i, int j, int k) {
+    return p.c[(i + j) & 1][L.at(i, j, k)];
+}
+
+// thread per network cell (I, J, K)
+__global__ void __launch_bounds__(BS) k_ws_restrict(RB p, RB rhs, RBL L, int W, int H, int D, float scale, int Wc,
+                                                    int Hc, int Dc, int Wn, int Hn, int Dn, float* __restrict__ res,
+                                                    float* __restrict__ mask, unsigned* __restrict__ amax,
+                                                    const int* __restrict__ stop) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= Wn * Hn * Dn || *stop) return;
+    const int I = t % Wn, J = (t / Wn) % Hn, K = t / (Wn * Hn);
+    if (I >= Wc || J >= Hc || K >= Dc) {
+        res[t] = 0.0f;
+        mask[t] = 1.0f;
+        return;
+    }
+    float sum = 0.0f;
+    bool wall = false;
+    for (int k = 4 * K; k < min(4 * K + 4, D); ++k)
+        for (int j = 4 * J; j < min(4 * J + 4, H); ++j)
+            for (int i = 4 * I; i < min(4 * I + 4, W); ++i) {
+                const float pc = pget(p, L, i, j, k);
+                float r = scale * pget(rhs, L, i, j, k);
+                float n = 0.0f;
+                if (i > 0) { r += pget(p, L, i - 1, j, k); n += 1.0f; }
+                if (i < W - 1) { r += pget(p, L, i + 1, j, k); n += 1.0f; }
+                if (j > 0) { r += pget(p, L, i, j - 1, k); n += 1.0f; }
+                if (j < H - 1) { r += pget(p, L, i, j + 1, k); n += 1.0f; }
+                if (k > 0) { r += pget(p, L, i, j, k - 1); n += 1.0f; }
+                if (k < D - 1) { r += pget(p, L, i, j, k + 1); n += 1.0f; }
+                sum += r - n * pc;
+                wall |= n < 6.0f;
+            }
+    const float rc = 0.25f * sum;
+    res[t] = rc;
+    mask[t] = wall ? 1.0f : 0.0f;
+    // |x| bit patterns order like the values (NaN above +inf)
+    atomicMax(amax, __float_as_uint(rc) & 0x7fffffffu);
+}
+
+// network input: (r / s, mask) in BF16, 16 channels
+__global__ void k_ws_pack(const float* __restrict__ res, const float* __restrict__ mask,
+                          const unsigned* __restrict__ amax, bf16* __restrict__ x0, int n) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n * NET_IN_CH) return;
+    const int c = t % NET_IN_CH, v = t / NET_IN_CH;
+    const float s = __uint_as_float(*amax);
+    float val = 0.0f;
+    if (c == 0) val = s > 0.0f ? res[v] / s : 0.0f;
+    if (c == 1) val = mask[v];
+    x0[t] = f2bf_w(val);
+}
+
+// FP32 normalised input for data generation: (r / s, mask)
+__global__ void k_ws_export(const float* __restrict__ res, const float* __restrict__ mask,
+                            const unsigned* __restrict__ amax, float* __restrict__ out2, int n) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const float s = __uint_as_float(*amax);
+    out2[t] = s > 0.0f ? res[t] / s : 0.0f;
+    out2[n + t] = mask[t];
+}
+
+// p += s * trilinear(prediction) at every fine cell (natural order thread)
+__global__ void __launch_bounds__(BS) k_ws_prolong(RB p, RBL L, int W, int H, int D, const float* __restrict__ pred,
+                                                   int Wc, int Hc, int Dc, int Wn, int Hn,
+                                                   const unsigned* __restrict__ amax, const SolveCtl* __restrict__ c) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= W * H * D || c->done) return;
+    const float s = __uint_as_float(*amax);
+    if (!(s > 0.0f) || !(s <= 3.4e38f)) return;
+    const int i = t % W, j = (t / W) % H, k = t / (W * H);
+    auto axis = [](int q, int n, int& a0, int& a1, float& w) {
+        float f = ((float)q + 0.5f) * 0.25f - 0.5f;
+        f = fminf(fmaxf(f, 0.0f), (float)(n - 1));
+        a0 = (int)f;
+        a1 = min(a0 + 1, n - 1);
+        w = f - (float)a0;
+    };
+    int x0, x1, y0, y1, z0, z1;
+    float wx, wy, wz;
+    axis(i, Wc, x0, x1, wx);
+    axis(j, Hc, y0, y1, wy);
+    axis(k, Dc, z0, z1, wz);
+    auto P = [&](int x, int y, int z) { return pred[((size_t)z * Hn + y) * Wn + x]; };
+    const float c00 = P(x0, y0, z0) * (1.0f - wx) + P(x1, y0, z0) * wx;
+    const float c10 = P(x0, y1, z0) * (1.0f - wx) + P(x1, y1, z0) * wx;
+    const float c01 = P(x0, y0, z1) * (1.0f - wx) + P(x1, y0, z1) * wx;
+    const float c11 = P(x0, y1, z1) * (1.0f - wx) + P(x1, y1, z1) * wx;
+    const float c0 = c00 * (1.0f - wy) + c10 * wy, c1 = c01 * (1.0f - wy) + c11 * wy;
+    const float corr = s * (c0 * (1.0f - wz) + c1 * wz);
+    float& dst = p.c[(i + j) & 1][L.at(i, j, k)];
+    dst = dst + corr;
+}
+
+// residual after the correction: keep it if lower, else mark for restore
+__global__ void __launch_bounds__(BS) k_res_final_ws(const double* __restrict__ partial, int n, SolveCtl* c,
+                                                     int* host_done, int seq, double tol, int cap) {
+    if (c->done) return;
+    double mx = 0.0;
+    for (int q = threadIdx.x; q < n; q += BS) mx = fmax(mx, partial[q]);
+    for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    __shared__ double sm[BS / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sm[0];
+        for (int q = 1; q < (int)(BS / 64); ++q) r = fmax(r, sm[q]);
+        c->checks += 1;
+        c->res_ws = r;
+        if (r < c->res) {
+            c->res = r;
+            c->ws = 1;
+        } else {
+            c->ws = 2;
+        }
+        if (!(c->res > tol) || c->iters >= cap) {
+            c->done = 1;
+            __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __threadfence_system();
+        }
+    }
+}
+
+// restore the saved pressure when the prediction was rejected
+__global__ void __launch_bounds__(BS) k_ws_restore(const SolveCtl* __restrict__ c, const float* __restrict__ backup,
+                                                   float* __restrict__ p, size_t n) {
+    if (c->ws != 2) return;
+    for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < n; t += (size_t)gridDim.x * BS) p[t] = backup[t];
+}
+
+__global__ void k_ws_zero(unsigned* a) { *a = 0u; }
+
+inline unsigned nb(size_t n) { return (unsigned)((n + BS - 1) / BS); }
+
+} // namespace
+
+int warm_start_attach(Solver& s, const Net* net) {
+    if (s.ws_net) {
+        delete s.ws_net;
+        s.ws_net = nullptr;
+    }
+    s.ws_src = net;
+    if (!net) return BLZ_OK;
+    s.Wc = (s.W + 3) / 4;
+    s.Hc = (s.H + 3) / 4;
+    s.Dc = (s.D + 3) / 4;
+    s.Wn = std::max(4, (s.Wc + 3) / 4 * 4);
+    s.Hn = std::max(4, (s.Hc + 3) / 4 * 4);
+    s.Dn = std::max(4, (s.Dc + 3) / 4 * 4);
+    const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
+    if (!s.ws_buf && hipMalloc(&s.ws_buf, (2 * nn + 64) * sizeof(float)) != hipSuccess) {
+        set_error("hipMalloc failed for warm-start buffers");
+        return BLZ_E_NOMEM;
+    }
+    if (!fused_available(s) && !s.ws_backup && hipMalloc(&s.ws_backup, 2 * s.Np * sizeof(float)) != hipSuccess) {
+        set_error("hipMalloc failed for the warm-start pressure backup");
+        return BLZ_E_NOMEM;
+    }
+    Net* v = new (std::nothrow) Net();
+    if (!v) {
+        set_error("out of host memory");
+        return BLZ_E_NOMEM;
+    }
+    v->init_view(*net, s.stream);
+    if (int e = v->ensure_dims(NetDims{1, s.Dn, s.Hn, s.Wn})) {
+        delete v;
+        return e;
+    }
+    s.ws_net = v;
+    return BLZ_OK;
+}
+
+static int ws_restrict(Solver& s, float dt, const int* stop) {
+    const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
+    float* res = s.ws_buf;
+    float* mask = s.ws_buf + nn;
+    unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
+    hipLaunchKernelGGL(k_ws_zero, dim3(1), dim3(1), 0, s.stream, amax);
+    hipLaunchKernelGGL(k_ws_restrict, dim3(nb(nn)), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, s.Wc,
+                       s.Hc, s.Dc, s.Wn, s.Hn, s.Dn, res, mask, amax, stop);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap) {
+    if (!s.ws_net) return BLZ_OK;
+    const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
+    float* res = s.ws_buf;
+    float* mask = s.ws_buf + nn;
+    unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
+    if (int e = ws_restrict(s, dt, &c->done)) return e;
+    hipLaunchKernelGGL(k_ws_pack, dim3(nb(nn * NET_IN_CH)), dim3(BS), 0, s.stream, res, mask, amax, s.ws_net->x0,
+                       (int)nn);
+    BLZ_HIP(hipGetLastError());
+    if (int e = s.ws_net->forward()) return e;
+    // save the pressure planes, then add the prediction
+    float* backup = s.ws_backup ? s.ws_backup : s.prb + 4 * s.Np;
+    BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
+    hipLaunchKernelGGL(k_ws_prolong, dim3(nb((size_t)s.W * s.H * s.D)), dim3(BS), 0, s.stream, s.p, s.L, s.W, s.H,
+                       s.D, s.ws_net->out, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, amax, c);
+    BLZ_HIP(hipGetLastError());
+    if (int e = launch_residual_ws(s, dt, c, host_done, seq, tol, cap)) return e;
+    hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
+                       backup, s.prb, 2 * s.Np);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int warm_start_input(Solver& s, float dt, float* host_in2) {
+    const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
+    if (int e = ws_restrict(s, dt, s.zero_flag)) return e;
+    float* out2 = (float*)s.tmp;
+    hipLaunchKernelGGL(k_ws_export, dim3(nb(nn)), dim3(BS), 0, s.stream, s.ws_buf, s.ws_buf + nn,
+                       (const unsigned*)(s.ws_buf + 2 * nn), out2, (int)nn);
+    BLZ_HIP(hipGetLastError());
+    BLZ_HIP(hipMemcpyAsync(host_in2, out2, 2 * nn * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    return BLZ_OK;
+}
+
+// the residual evaluation of the corrected pressure (partials of k_residual)
+int launch_residual_ws_final(Solver& s, SolveCtl* c, int* host_done, int seq, double tol, int cap, int nb_partial) {
+    hipLaunchKernelGGL(k_res_final_ws, dim3(1), dim3(BS), 0, s.stream, s.red + 8, nb_partial, c, host_done, seq, tol,
+                       cap);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+} // namespace blz

@@ -9,6 +9,7 @@
 
 #include "common.h"
 #include "dev_math.h"
+#include "unet.h"
 
 namespace blz {
 
@@ -67,8 +68,10 @@ struct SolveCtl {
     int done;   // 1 once the residual <= tol or the sweep cap is reached
     int iters;  // sweeps counted so far
     int checks; // residual evaluations so far
-    int pad;
+    int ws;     // learned warm start: 0 not applied, 1 kept, 2 rejected (pressure restored)
     double res, res0; // last and first residual
+    double res_ws;    // residual right after the learned correction
+    double pad;
 };
 constexpr int MAX_SOLVES = 64;     // pressure solves per step (bounds max_substeps)
 constexpr int SOLVE_LOOKAHEAD = 2; // sweep batches the host may enqueue past the last one it has seen finish
@@ -82,8 +85,10 @@ struct Solver {
     bool smoke = false;
     hipStream_t stream = nullptr;
     Field f[F_N];
-    RB p, rhs;              // four planes of one allocation: p red, p black, rhs red, rhs black
+    RB p, rhs;              // planes of one allocation: p red, p black, rhs red, rhs black,
+                            // spare p red, spare p black (fused sweeps), then a tail of slack
     float* prb = nullptr;
+    size_t prb_floats = 0;  // floats in prb
     RBL L{};
     size_t Np = 0;          // floats per padded plane
     float* tmp = nullptr;     // 7 * W*H*D scratch for cell-centred intermediates
@@ -110,15 +115,22 @@ struct Solver {
     enum { SPAN_SOURCE = BLZ_NUM_STAGES, SPAN_STEP, SPAN_RBGS };
     struct Span {
         int kind, e0, e1, launches;
-        int solve, batch; // RBGS batches: owning solve and batch index
+        int solve, batch, sweeps; // RBGS batches: owning solve, batch index, sweeps
     };
     std::vector<hipEvent_t> evpool;
     size_t ev_used = 0;
     std::vector<Span> spans;
     int rbgs_variant = 0;
+    // learned warm start (attached network; activations of its own)
+    const Net* ws_src = nullptr;
+    Net* ws_net = nullptr;     // view: shares ws_src's weights, owns activations
+    int Wc = 0, Hc = 0, Dc = 0; // coarse grid (4x), real cells
+    int Wn = 0, Hn = 0, Dn = 0; // network grid (coarse rounded up to multiples of 4)
+    float* ws_buf = nullptr;   // coarse residual, mask (Wn*Hn*Dn each), scale bits
+    float* ws_backup = nullptr; // pressure backup when there are no spare planes
 
     int span_begin(int* e0);
-    int span_end(int kind, int e0, int launches, int solve = -1, int batch = -1);
+    int span_end(int kind, int e0, int launches, int solve = -1, int batch = -1, int sweeps = 0);
     int resolve_spans();
     int finalize(); // after a stream synchronisation: solve statistics and spans
 
@@ -141,10 +153,18 @@ int launch_vorticity(Solver& s, float dt);
 int launch_wind(Solver& s, float time, float dt);
 int launch_turbulence(Solver& s, float time, float dt);
 int launch_divergence(Solver& s);
-int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop);
+// enqueues `sweeps` sweeps; *launches = kernels launched, *kind = 1 (two
+// colour passes per sweep) or 2 (fused sweeps)
+int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* launches, int* kind);
+bool fused_available(const Solver& s);
 int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int add,
                           int first);
 int launch_ctl_reset(Solver& s, SolveCtl* c);
+int launch_residual_ws(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
+int launch_residual_ws_final(Solver& s, SolveCtl* c, int* host_done, int seq, double tol, int cap, int nb_partial);
+int warm_start_attach(Solver& s, const Net* net);
+int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
+int warm_start_input(Solver& s, float dt, float* host_in2);
 int launch_pressure_update(Solver& s, float dt);
 int launch_advect(Solver& s, float dt);
 int launch_p2g(Solver& s);

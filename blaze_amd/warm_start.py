@@ -1,0 +1,157 @@
+"""Training of the learned warm start on data the solver generates.
+
+No reference counterpart. A pair is taken at a pressure solve of a running
+scene: the network input is the solver's own coarse input (the residual the
+previous pressure leaves, restricted 4x and normalised, and the solid
+mask: ``Solver.warm_start_input``); the target is the coarse Poisson
+correction for that input, n E - sum_nbr E = r (the same Gauss-Seidel
+solver on the coarse grid, run for many sweeps from zero, mean removed).
+Training runs on the GPU through the library's own forward, backward and
+Adam kernels (``Net.train_step``).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from . import (FIELD_P, FIELD_RHS, STAGE_ADVECT, STAGE_BUOYANCY, STAGE_COOLDOWN, STAGE_DIVERGENCE, STAGE_DRAG,
+               STAGE_PRESSURE, STAGE_PROJECT_UPDATE, STAGE_TURBULENCE, STAGE_VORTICITY, STAGE_WIND, Net, Solver,
+               default_settings, generate_particles, parse_scene)
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(PKG, "warm_start_weights.npy")
+
+
+def _force_stages(s):
+    st = []
+    if abs(s.cooling_rate) > 0:
+        st.append(STAGE_COOLDOWN)
+    if s.drag > 0:
+        st.append(STAGE_DRAG)
+    if abs(s.vorticity_conf) > 0:
+        st.append(STAGE_VORTICITY)
+    if abs(s.buoyancy) > 0:
+        st.append(STAGE_BUOYANCY)
+    if s.wind_amp > 0:
+        st.append(STAGE_WIND)
+    if s.turbulence_amp > 0:
+        st.append(STAGE_TURBULENCE)
+    return st
+
+
+def coarse_target(x, coarse_dims, sweeps):
+    """Solve n E - sum_nbr E = x[0] on the coarse grid (Neumann, same
+    Gauss-Seidel solver, zero start) and return E padded to the network
+    grid with its mean over the real cells removed."""
+    wc, hc, dc = coarse_dims
+    _, dn, hn, wn = x.shape
+    cs = default_settings()
+    cs.grid_res[0], cs.grid_res[1], cs.grid_res[2] = wc, hc, dc
+    cs.dx, cs.dt, cs.density = 1.0, 1.0, 1.0
+    cs.max_iter_solve = sweeps
+    cs.residual_tol = 0.0
+    g = Solver(cs)
+    g.set_field(FIELD_RHS, np.ascontiguousarray(x[0, :dc, :hc, :wc]))
+    g.set_field(FIELD_P, np.zeros((dc, hc, wc), np.float32))
+    g.run_stage(STAGE_PRESSURE, 0.0, 1.0)
+    e = g.field(FIELD_P).astype(np.float64)
+    e -= e.mean()
+    out = np.zeros((dn, hn, wn), np.float32)
+    out[:dc, :hc, :wc] = e
+    return out
+
+
+def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, log=None):
+    """Run `scene` (a .blz path) for `frames` frames stage by stage; at each
+    `every`-th frame record (input, target). `res` overrides the grid
+    (downsampled runs keep the domain and scale dx)."""
+    s, _, _, gen = parse_scene(scene)
+    if res is not None:
+        f = s.grid_res[0] / res[0]
+        for a in range(3):
+            s.grid_res[a] = res[a]
+        s.dx = s.dx * f
+        if gen is not None:
+            gen.pscale = gen.pscale * f
+    s.cfl_number = 0.0
+    s.warm_start = 0
+    parts = generate_particles(gen)
+    g = Solver(s)
+    net = seed_net or Net(seed=0)
+    g.set_warm_start(net)
+    _, cdims = g.warm_start_dims()
+    dt = float(s.dt)
+    pairs = []
+    t = np.float32(0.0)
+    for fr in range(frames):
+        frame = s.source_range[0] + fr
+        if frame <= s.source_range[1]:
+            g.add_source(parts if (s.source_animated == 0 or fr == 0) else None)
+        for st in _force_stages(s):
+            g.run_stage(st, float(t), dt)
+        g.run_stage(STAGE_DIVERGENCE, float(t), dt)
+        if fr % every == 0:
+            x = g.warm_start_input(dt)
+            if np.isfinite(x).all() and np.abs(x[0]).max() > 0:
+                pairs.append((x, coarse_target(x, cdims, sweeps)))
+        g.run_stage(STAGE_PRESSURE, float(t), dt)
+        g.run_stage(STAGE_PROJECT_UPDATE, float(t), dt)
+        g.run_stage(STAGE_ADVECT, float(t), dt)
+        t = np.float32(t + np.float32(dt))
+        if log and fr % 20 == 0:
+            log(f"[data] {os.path.basename(scene)} res {tuple(s.grid_res)} frame {fr}: {len(pairs)} pairs")
+    g.set_warm_start(None)
+    return pairs
+
+
+def augment(pairs):
+    """Mirror images along x and z (the Poisson operator and the walls'
+    treatment are symmetric under them)."""
+    out = []
+    for x, y in pairs:
+        out.append((x, y))
+        out.append((np.ascontiguousarray(x[:, :, :, ::-1]), np.ascontiguousarray(y[:, :, ::-1])))
+        out.append((np.ascontiguousarray(x[:, ::-1]), np.ascontiguousarray(y[::-1])))
+    return out
+
+
+def train(net, pairs, steps=1500, batch=4, lr=2e-3, seed=0, log=None):
+    """Adam on mini-batches of equal-size pairs; cosine learning-rate decay.
+    Returns the loss history."""
+    rng = np.random.default_rng(seed)
+    groups = {}
+    for x, y in pairs:
+        groups.setdefault(x.shape, []).append((x, y))
+    keys = sorted(groups, key=lambda k: -np.prod(k))
+    weights = np.array([len(groups[k]) * np.prod(k[1:]) for k in keys], np.float64)
+    weights /= weights.sum()
+    hist = []
+    for it in range(steps):
+        k = keys[rng.choice(len(keys), p=weights)]
+        grp = groups[k]
+        idx = rng.choice(len(grp), size=min(batch, len(grp)), replace=False)
+        xb = np.stack([grp[i][0] for i in idx])
+        yb = np.stack([grp[i][1] for i in idx])
+        cur = lr * 0.5 * (1.0 + np.cos(np.pi * it / steps))
+        hist.append(net.train_step(xb, yb, lr=cur))
+        if log and (it % 100 == 0 or it == steps - 1):
+            log(f"[train] step {it} loss {np.mean(hist[-100:]):.4e} lr {cur:.2e}")
+    return hist
+
+
+def relative_error(net, pairs):
+    """Mean relative L2 error of the prediction over pairs."""
+    errs = []
+    for x, y in pairs:
+        p = net.forward(x[None])[0].astype(np.float64)
+        p -= p.mean()
+        errs.append(np.linalg.norm(p - y) / (np.linalg.norm(y) + 1e-30))
+    return float(np.mean(errs))
+
+
+def load_net(path=WEIGHTS, seed=0):
+    net = Net(seed=seed)
+    if os.path.exists(path):
+        net.set_params(np.load(path, allow_pickle=False))
+    return net

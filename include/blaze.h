@@ -122,6 +122,8 @@ typedef struct blz_stats {
     double rbgs_kernel_ms;    /* device time of the RBGS sweep batches alone */
     int rbgs_launches;        /* RBGS kernel launches behind rbgs_kernel_ms */
     int residual_checks;      /* FP64 residual evaluations in the last step */
+    int rbgs_sweeps_timed;    /* RBGS sweeps behind rbgs_kernel_ms */
+    int rbgs_kernel;          /* kernel of the last sweeps: 1 two colour passes, 2 fused sweep */
 } blz_stats;
 
 /* synthetic particle set: uniform in a ball, deterministic for a seed */
@@ -170,8 +172,40 @@ int blz_set_field(blz_solver* h, int which, const float* host);
 int blz_get_stats(blz_solver* h, blz_stats* out);
 /* Enable per-stage HIP event timing (adds events around each stage). */
 int blz_set_timing(blz_solver* h, int on);
-/* RBGS kernel variant: four-slab groups in flight per thread (2, 4, 8; 0 = 8). */
+/* RBGS kernel variant (no reference counterpart): 2, 4, 8 = two in-place
+   colour passes per sweep with that many four-slab groups in flight per
+   thread; 102, 103 = fused whole-sweep kernel (2 or 3 groups in flight),
+   sweeps in pairs through a spare plane set; 0 = default (103 where the
+   spare planes exist, else 8). */
 int blz_set_rbgs_variant(blz_solver* h, int variant);
+
+/* ---- learned warm start (no reference counterpart; the reference solve
+   always starts from the previous pressure, src/FluidSolver.cu:517-599) ----
+   A 3-level 3D U-Net (16/32/64 channels, BF16 weights, FP32 accumulation)
+   on a 4x coarser grid maps (residual of the previous pressure, solid mask)
+   to a pressure correction. Network grids are (batch, D, H, W) with D, H,
+   W multiples of 4; inputs are two FP32 channels (residual, mask), each
+   batch*D*H*W values, x fastest. */
+typedef struct blz_net blz_net;
+int blz_net_create(unsigned long long seed, blz_net** out); /* He-normal init from seed */
+int blz_net_destroy(blz_net* net);
+int blz_net_num_params(blz_net* net, long long* n);
+int blz_net_get_params(blz_net* net, float* host);       /* FP32 master weights */
+int blz_net_set_params(blz_net* net, const float* host);
+int blz_net_get_grads(blz_net* net, float* host);        /* gradient of the last training step */
+int blz_net_forward(blz_net* net, const float* in2, const int dims[4], float* out);
+/* One step of mean-squared-error training: forward, backward, and (update
+   != 0) an Adam step with learning rate lr; *loss = loss before the step. */
+int blz_net_train_step(blz_net* net, const float* in2, const float* target, const int dims[4], float lr,
+                       int update, double* loss);
+/* Attach a network (NULL detaches). With warm_start != 0 and residual_tol
+   > 0, each solve applies the correction before its sweeps and keeps it only
+   if the FP64 residual drops (else the pressure is restored). The network
+   must outlive the attachment. */
+int blz_set_warm_start(blz_solver* h, blz_net* net);
+/* The network input for the current pressure and rhs (for generating
+   training data): in2 = 2 x net_dims values; dims reported in (W, H, D). */
+int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]);
 /* Self-test of the pressure solve's division by a neighbour count d in
    3..6: runs the solver's device division on every 32-bit float and counts
    results that differ from IEEE a / d (expected 0). */

@@ -90,18 +90,48 @@ def test_stage_bitwise(name, stage, kw, res):
             assert gs.residual_initial == os_["residual_initial"]
 
 
-@pytest.mark.parametrize("variant", [2, 4, 8])
-@pytest.mark.parametrize("res", [(32, 24, 20), (64, 8, 40), (16, 10, 21), (32, 16, 64)],
-                         ids=["32x24x20", "64x8x40", "16x10x21", "32x16x64"])
-def test_rbgs_variants_bitwise(variant, res):
-    """Every RBGS kernel variant gives the oracle's sweeps bit for bit."""
-    s = settings(B, res, max_iter_solve=9)
-    st = random_state(s, seed=variant)
+RBGS_SHAPES = [(32, 24, 20), (64, 8, 40), (16, 10, 21), (32, 16, 64), (130, 6, 9), (128, 5, 8), (66, 4, 5)]
+
+
+@pytest.mark.parametrize("variant", [2, 4, 8, 102, 103])
+@pytest.mark.parametrize("res", RBGS_SHAPES, ids=["x".join(map(str, r)) for r in RBGS_SHAPES])
+@pytest.mark.parametrize("iters", [9, 8])
+def test_rbgs_variants_bitwise(variant, res, iters):
+    """Every RBGS kernel variant gives the oracle's sweeps bit for bit: the
+    two-pass in-place kernels (2/4/8 slab groups in flight) and the fused
+    ping-pong sweeps (102/103; an odd count ends with a two-pass sweep).
+    Widths 130, 128 and 66 put the fused kernel's row over several waves,
+    with a partly filled last wave."""
+    print(f"[rbgs] v{variant} res={res} iters={iters}", flush=True)
+    s = settings(B, res, max_iter_solve=iters)
+    st = random_state(s, seed=variant + iters)
     g, o = pair(s, st)
     g.set_rbgs_variant(variant)
     g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
     o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
     compare(f"rbgs v{variant}", g.field(B.FIELD_P), o.field(B.FIELD_P))
+
+
+@pytest.mark.parametrize("variant", [8, 103])
+@pytest.mark.parametrize("kind", ["tiny", "negzero"])
+def test_rbgs_exact_division_fallback(variant, kind):
+    """Numerators outside the fast division's exact range (|num| < 2^-125,
+    and -0 for the fused kernel) take the exact path: still bit for bit."""
+    print(f"[rbgs-exact] v{variant} {kind}", flush=True)
+    s = settings(B, (64, 12, 10), max_iter_solve=6)
+    st = random_state(s, seed=11)
+    if kind == "tiny":
+        st[B.FIELD_P] = (st[B.FIELD_P] * np.float32(1e-38)).astype(np.float32)
+        st[B.FIELD_RHS] = (st[B.FIELD_RHS] * np.float32(3e-38)).astype(np.float32)
+    else:
+        st[B.FIELD_P] = np.full_like(st[B.FIELD_P], -0.0)
+        st[B.FIELD_RHS] = np.full_like(st[B.FIELD_RHS], -0.0)
+        st[B.FIELD_RHS][3, 5, 7] = np.float32(1e-3)
+    g, o = pair(s, st)
+    g.set_rbgs_variant(variant)
+    g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    compare(f"rbgs-exact v{variant} {kind}", g.field(B.FIELD_P), o.field(B.FIELD_P))
 
 
 @pytest.mark.parametrize("d", [3, 4, 5, 6])
