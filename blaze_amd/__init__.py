@@ -70,6 +70,14 @@ class Stats(C.Structure):
     ]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("scenes", C.c_int), ("variant", C.c_int), ("graph_nodes", C.c_int), ("pad_", C.c_int),
+        ("voxels", C.c_longlong), ("capture_ms", C.c_double), ("step_ms", C.c_double),
+        ("tune_variant", C.c_int * 6), ("tune_ms", C.c_double * 6),
+    ]
+
+
 class ParticleGen(C.Structure):
     _fields_ = [
         ("count", C.c_int), ("seed", C.c_ulonglong), ("center", C.c_float * 3),
@@ -124,6 +132,14 @@ def _load():
                                 P(C.c_double)], C.c_int),
         "blz_set_warm_start": ([vp, vp], C.c_int),
         "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
+        "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
+        "blz_batch_create": ([P(Settings), C.c_int, P(vp)], C.c_int),
+        "blz_batch_destroy": ([vp], C.c_int),
+        "blz_batch_solver": ([vp, C.c_int, P(vp)], C.c_int),
+        "blz_batch_set_warm_start": ([vp, vp], C.c_int),
+        "blz_batch_autotune": ([vp, C.c_int, P(C.c_int)], C.c_int),
+        "blz_batch_step": ([vp, C.c_float], C.c_int),
+        "blz_batch_get_stats": ([vp, P(BatchStats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -263,6 +279,15 @@ class Solver:
     def set_rbgs_variant(self, v: int):
         _check(_lib.blz_set_rbgs_variant(self._h, v), "blz_set_rbgs_variant")
 
+    def autotune(self, sweeps: int = 20):
+        """Time every RBGS variant on this grid and keep the fastest; returns
+        (variant, {variant: ms per sweep})."""
+        v = C.c_int()
+        ms = (C.c_float * 9)()
+        _check(_lib.blz_autotune_rbgs(self._h, sweeps, C.byref(v), ms), "blz_autotune_rbgs")
+        cand = [2, 4, 8, 102, 103, 202, 203, 402, 403]
+        return v.value, {c: ms[i] for i, c in enumerate(cand) if ms[i] > 0}
+
     def set_warm_start(self, net):
         """Attach a Net (None detaches); used when settings.warm_start != 0
         and residual_tol > 0."""
@@ -281,6 +306,58 @@ class Solver:
         _check(_lib.blz_warm_start_input(self._h, C.c_float(dt), out.ctypes.data_as(C.POINTER(C.c_float)),
                                          None, None), "blz_warm_start_input")
         return out
+
+
+class _Borrowed(Solver):
+    """A scene handle owned by a Batch (not destroyed here)."""
+
+    def __init__(self, h, settings, owner):
+        self._h = h
+        self.settings = settings.copy()
+        self._owner = owner
+
+    def __del__(self):
+        pass
+
+
+class Batch:
+    """Many scenes stepped together (blz_batch): one captured graph replay per
+    step, all scenes' sweeps and residual checks in single launches."""
+
+    def __init__(self, settings_list):
+        arr = (Settings * len(settings_list))(*[x.copy() for x in settings_list])
+        self._h = C.c_void_p()
+        _check(_lib.blz_batch_create(arr, len(settings_list), C.byref(self._h)), "blz_batch_create")
+        self.scenes = []
+        for i, st in enumerate(settings_list):
+            h = C.c_void_p()
+            _check(_lib.blz_batch_solver(self._h, i, C.byref(h)), "blz_batch_solver")
+            self.scenes.append(_Borrowed(h, st, self))
+        self._net = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.blz_batch_destroy(h)
+            self._h = None
+
+    def set_warm_start(self, net):
+        self._net = net
+        _check(_lib.blz_batch_set_warm_start(self._h, net._h if net is not None else None),
+               "blz_batch_set_warm_start")
+
+    def autotune(self, sweeps: int = 20) -> int:
+        v = C.c_int()
+        _check(_lib.blz_batch_autotune(self._h, sweeps, C.byref(v)), "blz_batch_autotune")
+        return v.value
+
+    def step(self, time: float):
+        _check(_lib.blz_batch_step(self._h, C.c_float(time)), "blz_batch_step")
+
+    def stats(self) -> BatchStats:
+        st = BatchStats()
+        _check(_lib.blz_batch_get_stats(self._h, C.byref(st)), "blz_batch_get_stats")
+        return st
 
 
 class Net:

@@ -1,0 +1,456 @@
+// batch.hip — many independent scenes of uneven grids stepped together: one
+// step of every scene is one replay of a captured HIP graph, and the
+// pressure sweeps and residual checks of all scenes run as single launches
+// over a table of grids (k_rbgs_fused_multi, k_residual_multi), so small
+// grids share the GPU instead of being launch-bound one by one. The sweep
+// kernel's tile parameters (slab groups in flight, waves per block) come
+// from an autotuner that times each candidate on the batch itself.
+//
+// No reference counterpart (the reference runs one scene per process,
+// src/main.cu:72-117). Each scene computes exactly what blz_step computes
+// for it alone with the same settings (the same kernels in the same order).
+#include <algorithm>
+#include <chrono>
+#include <new>
+#include <vector>
+
+#include "solver.h"
+
+namespace blz {
+
+int launch_fused_multi(hipStream_t st, int variant, const FusedScene* tab, int n, int blocks);
+int launch_residual_multi(hipStream_t st, const ResScene* tab, int n, int blocks, int add, int first);
+int fused_waves(const Solver& s);
+int residual_blocks(const Solver& s);
+
+struct Batch {
+    std::vector<blz_solver*> sc;
+    hipStream_t root = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int next = 0; // graph of the next replay
+    int field_cur[2][1024][F_N] = {};
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    float* off_host = nullptr; // 2 per scene (pinned)
+    float* off_dev = nullptr;
+    FusedScene* fs = nullptr;  // 2n entries: home -> spare, then spare -> home
+    ResScene* rs = nullptr;    // n entries
+    std::vector<FusedScene> fs_h;
+    std::vector<ResScene> rs_h;
+    int fs_blocks = 0, rs_blocks = 0;
+    int every = 10, nbatch = 0;
+    int variant = 103;
+    int nodes = 0;
+    double capture_ms = 0.0, step_ms = 0.0;
+    double tune_ms[6] = {0, 0, 0, 0, 0, 0};
+    const Net* net = nullptr;
+
+    ~Batch();
+    int init(const blz_settings* st, int n);
+    int build_tables(int v);
+    int capture(int which);
+    int enqueue_step(hipStream_t st);
+    int step(float time);
+    int autotune(int sweeps, int* best);
+};
+
+static const int kVariants[6] = {102, 103, 202, 203, 402, 403};
+static int wpb_of(int v) { return v / 100; }
+
+Batch::~Batch() {
+    if (root) (void)hipStreamSynchronize(root);
+    for (auto& e : exec)
+        if (e) (void)hipGraphExecDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipEvent_t e : join)
+        if (e) (void)hipEventDestroy(e);
+    if (off_host) (void)hipHostFree(off_host);
+    if (off_dev) (void)hipFree(off_dev);
+    if (fs) (void)hipFree(fs);
+    if (rs) (void)hipFree(rs);
+    for (blz_solver* h : sc) delete h;
+    if (root) (void)hipStreamDestroy(root);
+}
+
+int Batch::init(const blz_settings* st, int n) {
+    if (n < 1 || n > 1024) {
+        set_error("batch size must be in [1, 1024]");
+        return BLZ_E_INVALID;
+    }
+    every = st[0].residual_check_every > 0 ? st[0].residual_check_every : 10;
+    for (int q = 0; q < n; ++q) {
+        const blz_settings& s = st[q];
+        const int ev = s.residual_check_every > 0 ? s.residual_check_every : 10;
+        if (!(s.residual_tol > 0.0f) || s.cfl_number > 0.0f || ev != every || (every & 1) ||
+            s.max_iter_solve % every != 0) {
+            set_error("batched scenes need residual_tol > 0, cfl_number 0, one even residual_check_every "
+                      "dividing max_iter_solve");
+            return BLZ_E_INVALID;
+        }
+    }
+    BLZ_HIP(hipStreamCreateWithFlags(&root, hipStreamNonBlocking));
+    BLZ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    BLZ_HIP(hipHostMalloc((void**)&off_host, 2 * n * sizeof(float), hipHostMallocDefault));
+    if (hipMalloc(&off_dev, 2 * n * sizeof(float)) != hipSuccess || hipMalloc(&fs, 2 * n * sizeof(FusedScene)) != hipSuccess ||
+        hipMalloc(&rs, n * sizeof(ResScene)) != hipSuccess) {
+        set_error("hipMalloc failed for batch tables");
+        return BLZ_E_NOMEM;
+    }
+    for (int q = 0; q < n; ++q) {
+        blz_solver* h = new (std::nothrow) blz_solver();
+        if (!h) {
+            set_error("out of host memory");
+            return BLZ_E_NOMEM;
+        }
+        sc.push_back(h);
+        if (int e = h->s.init(st[q])) return e;
+        if (!fused_available(h->s)) {
+            set_error("batched scene too large for the fused sweep planes");
+            return BLZ_E_INVALID;
+        }
+        h->s.dev_off = off_dev + 2 * q;
+        hipEvent_t e;
+        BLZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        join.push_back(e);
+        nbatch = std::max(nbatch, h->s.s.max_iter_solve / every);
+    }
+    return build_tables(variant);
+}
+
+int Batch::build_tables(int v) {
+    const int n = (int)sc.size();
+    const int wpb = wpb_of(v);
+    fs_h.assign(2 * n, FusedScene{});
+    rs_h.assign(n, ResScene{});
+    int fb = 0, rb = 0;
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        const unsigned pb = (unsigned)(s.Np * 4);
+        const float dt = s.s.dt;
+        const int nw = fused_waves(s);
+        for (int h = 0; h < 2; ++h) {
+            FusedScene& e = fs_h[h * n + q];
+            const unsigned src = h == 0 ? 0u : 4u * pb, dst = h == 0 ? 4u * pb : 0u;
+            e.base = s.prb;
+            e.nbytes = (unsigned)(s.prb_floats * 4);
+            e.srcR = src;
+            e.srcB = src + pb;
+            e.rhsR = 2u * pb;
+            e.rhsB = 3u * pb;
+            e.dstR = dst;
+            e.dstB = dst + pb;
+            e.W = s.W;
+            e.H = s.H;
+            e.D = s.D;
+            e.Wp = s.L.Wp;
+            e.Hp = s.L.Hp;
+            e.scale = (s.s.density * s.dx * s.dx) / dt; // gs_solve_kernel :530
+            e.stop = &s.ctl[0].done;
+            e.first_block = fb;
+            e.nwaves = nw;
+        }
+        fb += (nw + wpb - 1) / wpb;
+        ResScene& r = rs_h[q];
+        r.p = s.p;
+        r.rhs = s.rhs;
+        r.L = s.L;
+        r.W = s.W;
+        r.H = s.H;
+        r.D = s.D;
+        r.c = (double)dt / ((double)s.s.density * (double)s.dx * (double)s.dx);
+        for (int a = 0; a < 6; ++a) r.cb[a] = s.s.closed_bounds[a];
+        r.partial = s.red + 8;
+        r.ctl = s.ctl;
+        r.first_block = rb;
+        r.nblocks = residual_blocks(s);
+        r.tol = (double)s.s.residual_tol;
+        r.cap = s.s.max_iter_solve;
+        r.seq = 0;
+        r.host_done = s.done_host_dev;
+        rb += r.nblocks;
+    }
+    fs_blocks = fb;
+    rs_blocks = rb;
+    BLZ_HIP(hipMemcpy(fs, fs_h.data(), 2 * n * sizeof(FusedScene), hipMemcpyHostToDevice));
+    BLZ_HIP(hipMemcpy(rs, rs_h.data(), n * sizeof(ResScene), hipMemcpyHostToDevice));
+    variant = v;
+    return BLZ_OK;
+}
+
+// one step of every scene: per-scene stages on the scenes' streams, the
+// solves of all scenes on `st`
+int Batch::enqueue_step(hipStream_t st) {
+    const int n = (int)sc.size();
+    BLZ_HIP(hipMemcpyAsync(off_dev, off_host, 2 * n * sizeof(float), hipMemcpyHostToDevice, st));
+    BLZ_HIP(hipEventRecord(fork, st));
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
+        const float dt = s.s.dt;
+        // FluidSolver::step stages before the solve (src/FluidSolver.cu:868-925)
+        if (std::fabs(s.s.cooling_rate) > 0.0f)
+            if (int e = s.run_stage(BLZ_STAGE_COOLDOWN, 0.0f, dt)) return e;
+        if (s.s.drag > 0.0f)
+            if (int e = s.run_stage(BLZ_STAGE_DRAG, 0.0f, dt)) return e;
+        if (std::fabs(s.s.vorticity_conf) > 0.0f)
+            if (int e = s.run_stage(BLZ_STAGE_VORTICITY, 0.0f, dt)) return e;
+        if (std::fabs(s.s.buoyancy) > 0.0f)
+            if (int e = s.run_stage(BLZ_STAGE_BUOYANCY, 0.0f, dt)) return e;
+        if (s.s.wind_amp > 0.0f)
+            if (int e = s.run_stage(BLZ_STAGE_WIND, 0.0f, dt)) return e;
+        if (s.s.turbulence_amp > 0.0f)
+            if (int e = s.run_stage(BLZ_STAGE_TURBULENCE, 0.0f, dt)) return e;
+        if (int e = s.run_stage(BLZ_STAGE_DIVERGENCE, 0.0f, dt)) return e;
+        SolveCtl* c = s.ctl;
+        const double tol = (double)s.s.residual_tol;
+        const int cap = s.s.max_iter_solve;
+        if (int e = launch_ctl_reset(s, c)) return e;
+        if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, tol, cap, 0, 1)) return e;
+        if (s.ws_net && s.s.warm_start)
+            if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap)) return e;
+        BLZ_HIP(hipEventRecord(join[q], s.stream));
+        BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+    }
+    // every scene's sweeps and checks; a converged scene's blocks return at entry
+    for (int b = 0; b < nbatch; ++b) {
+        for (int l = 0; l < every / 2; ++l) {
+            if (int e = launch_fused_multi(st, variant, fs, n, fs_blocks)) return e;
+            if (int e = launch_fused_multi(st, variant, fs + n, n, fs_blocks)) return e;
+        }
+        if (int e = launch_residual_multi(st, rs, n, rs_blocks, every, 0)) return e;
+    }
+    BLZ_HIP(hipEventRecord(fork, st));
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
+        const float dt = s.s.dt;
+        if (int e = s.run_stage(BLZ_STAGE_PROJECT_UPDATE, 0.0f, dt)) return e;
+        if (int e = s.run_stage(BLZ_STAGE_ADVECT, 0.0f, dt)) return e;
+        BLZ_HIP(hipMemcpyAsync(s.ctl_host, s.ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s.stream));
+        BLZ_HIP(hipEventRecord(join[q], s.stream));
+        BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+    }
+    return BLZ_OK;
+}
+
+// Capture graph `which`. The fields' front/back buffers alternate within a
+// step, so the pointers baked into a capture hold for one parity of steps:
+// graph 0 starts from the state at capture time, graph 1 from the state
+// graph 0 leaves; replays alternate.
+int Batch::capture(int which) {
+    const int n = (int)sc.size();
+    for (int q = 0; q < n; ++q) {
+        BLZ_HIP(hipStreamSynchronize(sc[q]->s.stream));
+        sc[q]->s.timing = false;
+        for (int g = 0; g < F_N; ++g) field_cur[which][q][g] = sc[q]->s.f[g].cur;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    hipGraph_t graph = nullptr;
+    BLZ_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
+    int e = enqueue_step(root);
+    hipError_t ce = hipStreamEndCapture(root, &graph);
+    if (e) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return e;
+    }
+    BLZ_HIP(ce);
+    size_t nn = 0;
+    BLZ_HIP(hipGraphGetNodes(graph, nullptr, &nn));
+    nodes = (int)nn;
+    if (exec[which]) BLZ_HIP(hipGraphExecDestroy(exec[which]));
+    exec[which] = nullptr;
+    hipError_t ie = hipGraphInstantiate(&exec[which], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BLZ_HIP(ie);
+    capture_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return BLZ_OK;
+}
+
+int Batch::step(float time) {
+    const int n = (int)sc.size();
+    if (!exec[0]) {
+        // capture both parities from the current state; the enqueue swapped
+        // the host pointers once per captured step
+        if (int e = capture(0)) return e;
+        if (int e = capture(1)) return e;
+        for (int q = 0; q < n; ++q)
+            for (int g = 0; g < F_N; ++g) sc[q]->s.f[g].cur = field_cur[0][q][g];
+        next = 0;
+    }
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        noise_offsets(s, time, off_host + 2 * q);
+        BLZ_HIP(hipStreamSynchronize(s.stream)); // sourcing or field access before the step
+        clear_step_stats(s.st);
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    BLZ_HIP(hipGraphLaunch(exec[next], root));
+    BLZ_HIP(hipStreamSynchronize(root));
+    step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    next ^= 1;
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        for (int g = 0; g < F_N; ++g) s.f[g].cur = field_cur[next][q][g];
+        s.solves.assign(1, s.s.max_iter_solve);
+        s.spans.clear();
+        s.ev_used = 0;
+        s.st.substeps = 1;
+        if (int e = s.finalize()) return e;
+    }
+    return BLZ_OK;
+}
+
+// time `sweeps` sweeps of every candidate on the batch (pressure restored
+// afterwards), keep the fastest
+int Batch::autotune(int sweeps, int* best) {
+    const int n = (int)sc.size();
+    sweeps = std::max(2, sweeps & ~1);
+    for (int q = 0; q < n; ++q) BLZ_HIP(hipStreamSynchronize(sc[q]->s.stream));
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        BLZ_HIP(hipMemcpyAsync(s.tmp, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, root));
+        BLZ_HIP(hipMemsetAsync(s.ctl, 0, sizeof(SolveCtl), root));
+    }
+    hipEvent_t e0, e1;
+    BLZ_HIP(hipEventCreate(&e0));
+    BLZ_HIP(hipEventCreate(&e1));
+    int bestv = variant;
+    double bestms = 1e30;
+    int rc = BLZ_OK;
+    for (int c = 0; c < 6 && !rc; ++c) {
+        const int v = kVariants[c];
+        if ((rc = build_tables(v))) break;
+        // one untimed pair, then the timed sweeps
+        if ((rc = launch_fused_multi(root, v, fs, n, fs_blocks)) || (rc = launch_fused_multi(root, v, fs + n, n, fs_blocks)))
+            break;
+        (void)hipEventRecord(e0, root);
+        for (int l = 0; l < sweeps / 2 && !rc; ++l) {
+            rc = launch_fused_multi(root, v, fs, n, fs_blocks);
+            if (!rc) rc = launch_fused_multi(root, v, fs + n, n, fs_blocks);
+        }
+        (void)hipEventRecord(e1, root);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = BLZ_E_HIP;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        tune_ms[c] = ms / (float)sweeps;
+        if (ms < bestms) {
+            bestms = ms;
+            bestv = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        (void)hipMemcpyAsync(s.prb, s.tmp, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, root);
+    }
+    BLZ_HIP(hipStreamSynchronize(root));
+    if (rc) return rc;
+    if (int e = build_tables(bestv)) return e;
+    for (auto& x : exec)
+        if (x) {
+            BLZ_HIP(hipGraphExecDestroy(x));
+            x = nullptr;
+        }
+    if (best) *best = bestv;
+    return BLZ_OK;
+}
+
+} // namespace blz
+
+using blz::Batch;
+
+struct blz_batch {
+    Batch b;
+};
+
+extern "C" {
+
+int blz_batch_create(const blz_settings* settings, int n, blz_batch** out) {
+    if (!settings || !out) {
+        blz::set_error("blz_batch_create: null argument");
+        return BLZ_E_INVALID;
+    }
+    *out = nullptr;
+    blz_batch* h = new (std::nothrow) blz_batch();
+    if (!h) {
+        blz::set_error("out of host memory");
+        return BLZ_E_NOMEM;
+    }
+    if (int e = h->b.init(settings, n)) {
+        delete h;
+        return e;
+    }
+    *out = h;
+    return BLZ_OK;
+}
+
+int blz_batch_destroy(blz_batch* b) {
+    delete b;
+    return BLZ_OK;
+}
+
+int blz_batch_solver(blz_batch* b, int i, blz_solver** out) {
+    if (!b || !out || i < 0 || i >= (int)b->b.sc.size()) {
+        blz::set_error("blz_batch_solver: bad argument");
+        return BLZ_E_INVALID;
+    }
+    *out = b->b.sc[i];
+    return BLZ_OK;
+}
+
+int blz_batch_set_warm_start(blz_batch* b, blz_net* net) {
+    if (!b) {
+        blz::set_error("null batch");
+        return BLZ_E_INVALID;
+    }
+    for (blz_solver* h : b->b.sc) {
+        BLZ_HIP(hipStreamSynchronize(h->s.stream));
+        if (int e = blz::warm_start_attach(h->s, net ? &net->n : nullptr)) return e;
+    }
+    for (auto& x : b->b.exec)
+        if (x) {
+            BLZ_HIP(hipGraphExecDestroy(x));
+            x = nullptr;
+        }
+    return BLZ_OK;
+}
+
+int blz_batch_autotune(blz_batch* b, int sweeps, int* variant) {
+    if (!b) {
+        blz::set_error("null batch");
+        return BLZ_E_INVALID;
+    }
+    return b->b.autotune(sweeps, variant);
+}
+
+int blz_batch_step(blz_batch* b, float time) {
+    if (!b) {
+        blz::set_error("null batch");
+        return BLZ_E_INVALID;
+    }
+    return b->b.step(time);
+}
+
+int blz_batch_get_stats(blz_batch* b, blz_batch_stats* out) {
+    if (!b || !out) {
+        blz::set_error("null argument");
+        return BLZ_E_INVALID;
+    }
+    Batch& B = b->b;
+    out->scenes = (int)B.sc.size();
+    out->variant = B.variant;
+    out->graph_nodes = B.nodes;
+    out->capture_ms = B.capture_ms;
+    out->step_ms = B.step_ms;
+    long long vox = 0;
+    for (blz_solver* h : B.sc) vox += (long long)h->s.W * h->s.H * h->s.D;
+    out->voxels = vox;
+    for (int c = 0; c < 6; ++c) {
+        out->tune_variant[c] = blz::kVariants[c];
+        out->tune_ms[c] = B.tune_ms[c];
+    }
+    return BLZ_OK;
+}
+
+} // extern "C"

@@ -7,6 +7,7 @@
 // waits where it needs a value (CFL substep count) or to stay a few sweep
 // batches ahead of a residual-driven solve, whose convergence test runs on
 // the device.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -418,9 +419,6 @@ int Solver::add_source(const float* part8, int n) {
 
 using blz::Solver;
 
-struct blz_solver {
-    Solver s;
-};
 
 #define CHECK_HANDLE(h)                              \
     do {                                             \
@@ -629,8 +627,59 @@ int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], i
     return blz::warm_start_input(s, dt, in2);
 }
 
+int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep) {
+    CHECK_HANDLE(h);
+    Solver& s = h->s;
+    static const int cand[] = {2, 4, 8, 102, 103, 202, 203, 402, 403};
+    sweeps = std::max(2, sweeps & ~1);
+    const float dt = s.s.dt;
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    BLZ_HIP(hipMemcpyAsync(s.tmp, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
+    hipEvent_t e0, e1;
+    BLZ_HIP(hipEventCreate(&e0));
+    BLZ_HIP(hipEventCreate(&e1));
+    const int keep = s.rbgs_variant;
+    int best = keep, rc = BLZ_OK, k = 0;
+    float bestms = 1e30f;
+    for (int v : cand) {
+        if (v >= 100 && !blz::fused_available(s)) continue;
+        s.rbgs_variant = v;
+        int launches = 0, kind = 0;
+        rc = blz::launch_rbgs_sweeps(s, dt, 2, s.zero_flag, &launches, &kind);
+        if (!rc) rc = hipEventRecord(e0, s.stream) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
+        if (!rc) rc = blz::launch_rbgs_sweeps(s, dt, sweeps, s.zero_flag, &launches, &kind);
+        if (!rc) rc = hipEventRecord(e1, s.stream) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
+        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
+        if (rc) break;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)sweeps;
+        if (ms_per_sweep) ms_per_sweep[k] = ms;
+        ++k;
+        if (ms < bestms) {
+            bestms = ms;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipMemcpyAsync(s.prb, s.tmp, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream);
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    s.rbgs_variant = rc ? keep : best;
+    if (rc) {
+        blz::set_error("autotune failed");
+        return rc;
+    }
+    if (variant) *variant = best;
+    return BLZ_OK;
+}
+
 int blz_set_rbgs_variant(blz_solver* h, int variant) {
     CHECK_HANDLE(h);
+    if (!blz::rbgs_variant_valid(variant)) {
+        blz::set_error("unknown RBGS variant");
+        return BLZ_E_INVALID;
+    }
     h->s.rbgs_variant = variant;
     return BLZ_OK;
 }

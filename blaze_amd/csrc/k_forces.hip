@@ -134,7 +134,10 @@ __global__ void __launch_bounds__(BS) k_face_gather(float* __restrict__ face, co
 // add_wind_kernel :337-390, restricted to the border slabs it touches
 __global__ void __launch_bounds__(BS) k_wind(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
                                              int nW, int nH, int nD, float dx, int xwind, int4 slabs, int nslab,
-                                             float offset, float freq, float tamp, float amp, float dirv, float dt) {
+                                             float offset, float freq, float tamp, float amp, float dirv, float dt,
+                                             const float* __restrict__ offp) {
+    // offp: the offset from device memory (captured steps), else `offset`
+    if (offp) offset = *offp;
     // slab axis is x (xwind) or z; the other two axes span [0, n+1)
     const int a = xwind ? nH + 1 : nW + 1; // fastest of the remaining axes
     const int b = xwind ? nD + 1 : nH + 1;
@@ -167,7 +170,8 @@ __global__ void __launch_bounds__(BS) k_wind(float* __restrict__ U, float* __res
 template <int AXIS>
 __global__ void __launch_bounds__(BS) k_turb_noise(Gv T, Gv U, Gv V, Gv W, Gv G, float* __restrict__ out, float dx,
                                                    float offset, float freq, int seed, float amp, float tr0, float tr1,
-                                                   float vr0, float vr1) {
+                                                   float vr0, float vr1, const float* __restrict__ offp) {
+    if (offp) offset = *offp;
     const int n = G.nx * G.ny * G.nz;
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
@@ -265,6 +269,18 @@ int launch_vorticity(Solver& s, float dt) {
     return gather3(s, true);
 }
 
+// host offsets of the wind and turbulence noise at `time` (the values a
+// captured step reads from device memory)
+void noise_offsets(const Solver& s, float time, float out[2]) {
+    const float sx = s.s.domain_max[0] - s.s.domain_min[0];
+    const float sy = s.s.domain_max[1] - s.s.domain_min[1];
+    const float sz = s.s.domain_max[2] - s.s.domain_min[2];
+    const float avgDom = (sx + sy + sz) / 3.0f;
+    out[0] = fmodf(time * avgDom * s.s.wind_speed, 25000.0f);
+    const float avgGrid = (float)(s.W + s.H + s.D) / 3.0f;
+    out[1] = fmodf(time * avgGrid, 25000.0f);
+}
+
 int launch_wind(Solver& s, float time, float dt) {
     const bool xwind = s.s.wind_dir <= 1;
     const float freq = 1.0f / s.s.wind_turb_scale;
@@ -282,7 +298,7 @@ int launch_wind(Solver& s, float time, float dt) {
     const size_t b = xwind ? (size_t)s.D + 1 : (size_t)s.H + 1;
     LAUNCH(k_wind, a * b * ns, s.f[F_U].front(), s.f[F_V].front(), s.f[F_W].front(), s.W, s.H, s.D, s.dx,
            xwind ? 1 : 0, make_int4(sl[0], sl[1], sl[2], sl[3]), ns, offset, freq, s.s.wind_turb_amp, s.s.wind_amp,
-           dirv, dt);
+           dirv, dt, s.dev_off ? s.dev_off : nullptr);
     return BLZ_OK;
 }
 
@@ -295,11 +311,11 @@ int launch_turbulence(Solver& s, float time, float dt) {
     const float* vr = s.s.turb_mask_vel_ramp;
     const Gv T = s.f[F_T].view(), U = s.f[F_U].view(), V = s.f[F_V].view(), W = s.f[F_W].view();
     LAUNCH(k_turb_noise<0>, s.f[F_U].count(), T, U, V, W, U, s.f[F_U].back(), s.dx, offset, freq, s.s.noise_seed, amp,
-           tr[0], tr[1], vr[0], vr[1]);
+           tr[0], tr[1], vr[0], vr[1], s.dev_off ? s.dev_off + 1 : nullptr);
     LAUNCH(k_turb_noise<1>, s.f[F_V].count(), T, U, V, W, V, s.f[F_V].back(), s.dx, offset, freq, s.s.noise_seed, amp,
-           tr[0], tr[1], vr[0], vr[1]);
+           tr[0], tr[1], vr[0], vr[1], s.dev_off ? s.dev_off + 1 : nullptr);
     LAUNCH(k_turb_noise<2>, s.f[F_W].count(), T, U, V, W, W, s.f[F_W].back(), s.dx, offset, freq, s.s.noise_seed, amp,
-           tr[0], tr[1], vr[0], vr[1]);
+           tr[0], tr[1], vr[0], vr[1], s.dev_off ? s.dev_off + 1 : nullptr);
     const float eps = 1e-20f * (1.0f / (s.dx * dt));
     LAUNCH(k_curl, (size_t)s.W * s.H * s.D, s.f[F_U].back(), s.f[F_V].back(), s.f[F_W].back(), s.tmp, s.W, s.H, s.D,
            s.dx, dt, eps);

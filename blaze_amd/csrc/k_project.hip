@@ -499,17 +499,15 @@ __device__ __forceinline__ int cell_cnt(int i, int j, int W, int H) {
     return (int)(i > 0) + (int)(i < W - 1) + (int)(j > 0) + (int)(j < H - 1);
 }
 
+// one wave (block) wv of a fused sweep of one grid
 template <int PG>
-__global__ void __launch_bounds__(64) k_rbgs_fused(float* __restrict__ base, unsigned nbytes, unsigned srcR,
-                                                   unsigned srcB, unsigned rhsR, unsigned rhsB, unsigned dstR,
-                                                   unsigned dstB, int W, int H, int D, int Wp, int Hp, float scale,
-                                                   DivTab tab, const int* __restrict__ stop) {
-    if (*stop) return; // the solve has converged
+__device__ __forceinline__ void fused_wave(float* __restrict__ base, unsigned nbytes, unsigned srcR, unsigned srcB,
+                                           unsigned rhsR, unsigned rhsB, unsigned dstR, unsigned dstB, int W, int H,
+                                           int D, int Wp, int Hp, float scale, const DivTab& tab, int wv) {
     const int nwr = (W + 63) >> 6; // waves per row
-    const int wv = xcd_block(blockIdx.x, gridDim.x);
     const int j = wv / nwr;
     const int i0 = (wv - j * nwr) * 64;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int i = i0 + lane;
     const bool active = i < W;
     const bool red = ((i + j) & 1) == 0;
@@ -591,6 +589,39 @@ __global__ void __launch_bounds__(64) k_rbgs_fused(float* __restrict__ base, uns
     if (g < a.Dg) fused_run<PG, true>(a, g, prev);
 }
 
+// WPB waves per block (independent waves; no block-level synchronisation)
+template <int PG, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_rbgs_fused(float* __restrict__ base, unsigned nbytes, unsigned srcR,
+                                                         unsigned srcB, unsigned rhsR, unsigned rhsB, unsigned dstR,
+                                                         unsigned dstB, int W, int H, int D, int Wp, int Hp,
+                                                         float scale, DivTab tab, const int* __restrict__ stop,
+                                                         int nwaves) {
+    if (*stop) return; // the solve has converged
+    const int wv = xcd_block(blockIdx.x, gridDim.x) * WPB + (int)(threadIdx.x >> 6);
+    if (wv >= nwaves) return;
+    fused_wave<PG>(base, nbytes, srcR, srcB, rhsR, rhsB, dstR, dstB, W, H, D, Wp, Hp, scale, tab, wv);
+}
+
+// the same sweep over several grids in one launch: block b belongs to the
+// last table entry whose first_block <= b (binary search, block-uniform)
+template <int PG, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_rbgs_fused_multi(const FusedScene* __restrict__ sc, int n,
+                                                               DivTab tab) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sc[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const FusedScene& e = sc[lo];
+    if (*e.stop) return;
+    const int wv = (b - e.first_block) * WPB + (int)(threadIdx.x >> 6);
+    if (wv >= e.nwaves) return;
+    fused_wave<PG>(e.base, e.nbytes, e.srcR, e.srcB, e.rhsR, e.rhsB, e.dstR, e.dstB, e.W, e.H, e.D, e.Wp, e.Hp,
+                   e.scale, tab, wv);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -614,9 +645,10 @@ __device__ __forceinline__ double cell_res(double pc, double rc, double sum, dou
 // by k_res_final. `stop` is the solve's done flag: a converged solve skips.
 constexpr int RES_SEG = 4;
 
-__global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int H, int D, double c, int4 cbA,
-                                                 int2 cbB, double* __restrict__ partial, const int* __restrict__ stop) {
-    if (*stop) return;
+// block blk of nblk blocks of the residual of one grid
+__device__ __forceinline__ void residual_block(const RB& p, const RB& rhs, const RBL& L, int W, int H, int D,
+                                               double c, int4 cbA, int2 cbB, double* __restrict__ partial, int blk,
+                                               int nblk) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const int Wh = W >> 1;
     const int nseg = (L.Dg + RES_SEG - 1) / RES_SEG;
@@ -624,7 +656,7 @@ __global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int
     const int gs = L.Wp * L.Hp; // float4 elements per group
     const bool openL = !cbA.x, openR = !cbA.y, openN = !cbA.w, openS = !cbA.z, openD = !cbB.x, openU = !cbB.y;
     double mx = 0.0;
-    for (int t = blockIdx.x * BS + threadIdx.x; t < total; t += gridDim.x * BS) {
+    for (int t = blk * BS + threadIdx.x; t < total; t += nblk * BS) {
         const int m = t % Wh, j = (t / Wh) % H, seg = t / (Wh * H);
         const int c0 = j & 1, c1 = c0 ^ 1;   // colours of i0 = 2m and i1 = 2m+1
         const f4* P0 = (const f4*)p.c[c0];
@@ -690,16 +722,37 @@ __global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int
     if (threadIdx.x == 0) {
         double w = sm[0];
         for (int q = 1; q < (int)(BS / 64); ++q) w = fmax(w, sm[q]);
-        partial[blockIdx.x] = w;
+        partial[blk] = w;
     }
+}
+
+__global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int H, int D, double c, int4 cbA,
+                                                 int2 cbB, double* __restrict__ partial, const int* __restrict__ stop) {
+    if (*stop) return;
+    residual_block(p, rhs, L, W, H, D, c, cbA, cbB, partial, blockIdx.x, gridDim.x);
+}
+
+// the residual of several grids in one launch (table as for the fused sweep)
+__global__ void __launch_bounds__(BS) k_residual_multi(const ResScene* __restrict__ sc, int n) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sc[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const ResScene& e = sc[lo];
+    if (e.ctl->done) return;
+    residual_block(e.p, e.rhs, e.L, e.W, e.H, e.D, e.c, make_int4(e.cb[0], e.cb[1], e.cb[2], e.cb[3]),
+                   make_int2(e.cb[4], e.cb[5]), e.partial, b - e.first_block, e.nblocks);
 }
 
 // single block: the residual is the max of the partial maxima; advance the
 // solve's control block as the loop `while (r > tol && it < cap)` does
 // On completion the solve's sequence number goes to host_done (mapped host
 // memory), which the host polls to stop enqueuing sweeps.
-__global__ void __launch_bounds__(BS) k_res_final(const double* __restrict__ partial, int n, SolveCtl* c,
-                                                  int* host_done, int seq, double tol, int cap, int add, int first) {
+__device__ __forceinline__ void res_final_block(const double* __restrict__ partial, int n, SolveCtl* c,
+                                                int* host_done, int seq, double tol, int cap, int add, int first) {
     if (c->done) return;
     double mx = 0.0;
     for (int q = threadIdx.x; q < n; q += BS) mx = fmax(mx, partial[q]);
@@ -725,6 +778,17 @@ __global__ void __launch_bounds__(BS) k_res_final(const double* __restrict__ par
             __threadfence_system();
         }
     }
+}
+
+__global__ void __launch_bounds__(BS) k_res_final(const double* __restrict__ partial, int n, SolveCtl* c,
+                                                  int* host_done, int seq, double tol, int cap, int add, int first) {
+    res_final_block(partial, n, c, host_done, seq, tol, cap, add, first);
+}
+
+// block q finishes grid q of a multi-grid residual launch
+__global__ void __launch_bounds__(BS) k_res_final_multi(const ResScene* __restrict__ sc, int add, int first) {
+    const ResScene& e = sc[blockIdx.x];
+    res_final_block(e.partial, e.nblocks, e.ctl, e.host_done, e.seq, e.tol, e.cap, add, first);
 }
 
 __global__ void k_ctl_reset(SolveCtl* c) {
@@ -892,18 +956,20 @@ static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop) {
 
 // Pairs of fused sweeps: home set -> spare set -> home set, so the pressure
 // is in the home planes after every pair (and at every residual check).
-template <int PG>
+template <int PG, int WPB>
 static int fused_pairs(Solver& s, float scale, int pairs, const int* stop) {
     const int nwr = (s.W + 63) / 64;
-    const dim3 grid((unsigned)(nwr * s.H)), block(64);
+    const int nwaves = nwr * s.H;
+    const dim3 grid((unsigned)((nwaves + WPB - 1) / WPB)), block(64 * WPB);
     const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
     const unsigned pb = (unsigned)(s.Np * 4);
     const unsigned nbytes = (unsigned)(s.prb_floats * 4);
     for (int l = 0; l < pairs; ++l) {
         for (int h = 0; h < 2; ++h) {
             const unsigned src = h == 0 ? 0u : 4u * pb, dst = h == 0 ? 4u * pb : 0u;
-            hipLaunchKernelGGL((k_rbgs_fused<PG>), grid, block, 0, s.stream, s.prb, nbytes, src, src + pb, 2u * pb,
-                               3u * pb, dst, dst + pb, s.W, s.H, s.D, s.L.Wp, s.L.Hp, scale, tab, stop);
+            hipLaunchKernelGGL((k_rbgs_fused<PG, WPB>), grid, block, 0, s.stream, s.prb, nbytes, src, src + pb,
+                               2u * pb, 3u * pb, dst, dst + pb, s.W, s.H, s.D, s.L.Wp, s.L.Hp, scale, tab, stop,
+                               nwaves);
             BLZ_HIP(hipGetLastError());
         }
     }
@@ -913,9 +979,26 @@ static int fused_pairs(Solver& s, float scale, int pairs, const int* stop) {
 bool fused_available(const Solver& s) { return s.prb_floats >= 6 * s.Np; }
 
 // Gauss-Seidel sweeps. variant: 2/4/8 = two in-place colour passes per
-// sweep with that many slab groups in flight; 102/103 = fused sweeps (2 or
-// 3 groups in flight) in pairs, an odd last sweep as two colour passes;
-// 0 = the default (fused, 3 groups, where the spare planes exist).
+// sweep with that many slab groups in flight; 100*w + g (w = 1, 2, 4 waves
+// per block, g = 2, 3 groups in flight) = fused sweeps in pairs, an odd
+// last sweep as two colour passes; 0 = the default (103 where the spare
+// planes exist, else 8).
+bool rbgs_variant_valid(int v) {
+    return v == 0 || v == 2 || v == 4 || v == 8 || v == 102 || v == 103 || v == 202 || v == 203 || v == 402 ||
+           v == 403;
+}
+
+static int fused_dispatch(Solver& s, int v, float scale, int pairs, const int* stop) {
+    switch (v) {
+    case 102: return fused_pairs<2, 1>(s, scale, pairs, stop);
+    case 202: return fused_pairs<2, 2>(s, scale, pairs, stop);
+    case 203: return fused_pairs<3, 2>(s, scale, pairs, stop);
+    case 402: return fused_pairs<2, 4>(s, scale, pairs, stop);
+    case 403: return fused_pairs<3, 4>(s, scale, pairs, stop);
+    default: return fused_pairs<3, 1>(s, scale, pairs, stop);
+    }
+}
+
 int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* launches, int* kind) {
     const float scale = (s.s.density * s.dx * s.dx) / dt; // Bridson p.75, :530
     int v = s.rbgs_variant;
@@ -925,8 +1008,7 @@ int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* la
         const int pairs = sweeps / 2;
         *launches = 2 * pairs + 2 * (sweeps & 1);
         *kind = 2;
-        int e = v == 102 ? fused_pairs<2>(s, scale, pairs, stop) : fused_pairs<3>(s, scale, pairs, stop);
-        if (e) return e;
+        if (int e = fused_dispatch(s, v, scale, pairs, stop)) return e;
         return (sweeps & 1) ? rbgs_passes<8>(s, scale, 1, stop) : BLZ_OK;
     }
     *launches = 2 * sweeps;
@@ -936,6 +1018,40 @@ int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* la
     case 4: return rbgs_passes<4>(s, scale, sweeps, stop);
     default: return rbgs_passes<8>(s, scale, sweeps, stop);
     }
+}
+
+int fused_waves(const Solver& s) { return ((s.W + 63) / 64) * s.H; }
+
+int residual_blocks(const Solver& s) {
+    const size_t items = (size_t)(s.W / 2) * s.H * ((s.L.Dg + RES_SEG - 1) / RES_SEG);
+    return (int)std::min<size_t>(blocks_for(items, BS), RESIDUAL_BLOCKS);
+}
+
+template <int PG, int WPB>
+static int fused_multi(hipStream_t st, const FusedScene* tab, int n, int blocks) {
+    const DivTab dt{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
+    hipLaunchKernelGGL((k_rbgs_fused_multi<PG, WPB>), dim3((unsigned)blocks), dim3(64 * WPB), 0, st, tab, n, dt);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int launch_fused_multi(hipStream_t st, int variant, const FusedScene* tab, int n, int blocks) {
+    switch (variant) {
+    case 102: return fused_multi<2, 1>(st, tab, n, blocks);
+    case 202: return fused_multi<2, 2>(st, tab, n, blocks);
+    case 203: return fused_multi<3, 2>(st, tab, n, blocks);
+    case 402: return fused_multi<2, 4>(st, tab, n, blocks);
+    case 403: return fused_multi<3, 4>(st, tab, n, blocks);
+    default: return fused_multi<3, 1>(st, tab, n, blocks);
+    }
+}
+
+int launch_residual_multi(hipStream_t st, const ResScene* tab, int n, int blocks, int add, int first) {
+    hipLaunchKernelGGL(k_residual_multi, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    BLZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_res_final_multi, dim3((unsigned)n), dim3(BS), 0, st, tab, add, first);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
 }
 
 int launch_ctl_reset(Solver& s, SolveCtl* c) {

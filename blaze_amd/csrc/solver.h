@@ -78,6 +78,31 @@ constexpr int SOLVE_LOOKAHEAD = 2; // sweep batches the host may enqueue past th
 
 constexpr int RESIDUAL_BLOCKS = 1024;
 
+// one grid of a multi-grid fused sweep launch (byte offsets into base)
+struct FusedScene {
+    float* base;
+    unsigned nbytes, srcR, srcB, rhsR, rhsB, dstR, dstB;
+    int W, H, D, Wp, Hp;
+    float scale;
+    const int* stop;
+    int first_block, nwaves;
+};
+
+// one grid of a multi-grid residual launch
+struct ResScene {
+    RB p, rhs;
+    RBL L;
+    int W, H, D;
+    double c;
+    int cb[6];
+    double* partial;
+    SolveCtl* ctl;
+    int first_block, nblocks;
+    double tol;
+    int cap, seq;
+    int* host_done;
+};
+
 struct Solver {
     blz_settings s{};
     int W = 0, H = 0, D = 0;
@@ -128,6 +153,9 @@ struct Solver {
     int Wn = 0, Hn = 0, Dn = 0; // network grid (coarse rounded up to multiples of 4)
     float* ws_buf = nullptr;   // coarse residual, mask (Wn*Hn*Dn each), scale bits
     float* ws_backup = nullptr; // pressure backup when there are no spare planes
+    // captured (batched) steps: noise offsets of wind and turbulence read
+    // from device memory instead of kernel arguments
+    const float* dev_off = nullptr;
 
     int span_begin(int* e0);
     int span_end(int kind, int e0, int launches, int solve = -1, int batch = -1, int sweeps = 0);
@@ -151,12 +179,15 @@ int launch_drag(Solver& s, float dt);
 int launch_buoyancy(Solver& s, float dt);
 int launch_vorticity(Solver& s, float dt);
 int launch_wind(Solver& s, float time, float dt);
+void noise_offsets(const Solver& s, float time, float out[2]);
 int launch_turbulence(Solver& s, float time, float dt);
 int launch_divergence(Solver& s);
 // enqueues `sweeps` sweeps; *launches = kernels launched, *kind = 1 (two
 // colour passes per sweep) or 2 (fused sweeps)
 int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* launches, int* kind);
 bool fused_available(const Solver& s);
+bool rbgs_variant_valid(int v);
+void clear_step_stats(blz_stats& st);
 int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int add,
                           int first);
 int launch_ctl_reset(Solver& s, SolveCtl* c);
@@ -178,3 +209,7 @@ int selftest_division(hipStream_t st, int d, unsigned long long* dev_bad);
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 } // namespace blz
+
+struct blz_solver {
+    blz::Solver s;
+};

@@ -155,3 +155,47 @@ def load_net(path=WEIGHTS, seed=0):
     if os.path.exists(path):
         net.set_params(np.load(path, allow_pickle=False))
     return net
+
+
+def main(argv=None):
+    """Generate pairs from the plume scenes (native and 2x-downsampled
+    grids), train, report held-out error, save the FP32 weights."""
+    import argparse
+    import time
+
+    root = os.path.dirname(PKG)
+    ap = argparse.ArgumentParser(description="train the warm-start U-Net on solver-generated pairs")
+    ap.add_argument("--frames", type=int, default=120)
+    ap.add_argument("--every", type=int, default=2)
+    ap.add_argument("--sweeps", type=int, default=2000, help="coarse Gauss-Seidel sweeps per target")
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=WEIGHTS)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    t0 = time.time()
+    scenes = [(os.path.join(root, "scenes", "plume64.blz"), None), (os.path.join(root, "scenes", "plume64.blz"),
+                                                                     (32, 32, 32)),
+              (os.path.join(root, "scenes", "plume128.blz"), None), (os.path.join(root, "scenes", "plume128.blz"),
+                                                                      (64, 64, 64))]
+    pairs = []
+    for path, res in scenes:
+        pairs += scene_pairs(path, a.frames, res=res, sweeps=a.sweeps, every=a.every, log=log)
+    log(f"[data] {len(pairs)} pairs in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(a.seed)
+    perm = rng.permutation(len(pairs))
+    n_test = max(1, len(pairs) // 10)
+    test = [pairs[i] for i in perm[:n_test]]
+    tr = augment([pairs[i] for i in perm[n_test:]])
+    net = Net(seed=a.seed)
+    log(f"[train] untrained held-out relative error {relative_error(net, test):.3f}")
+    train(net, tr, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+    log(f"[train] held-out relative error {relative_error(net, test):.3f} ({time.time() - t0:.1f}s)")
+    np.save(a.out, net.params(), allow_pickle=False)
+    log(f"[train] saved {a.out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -178,6 +178,12 @@ int blz_set_timing(blz_solver* h, int on);
    sweeps in pairs through a spare plane set; 0 = default (103 where the
    spare planes exist, else 8). */
 int blz_set_rbgs_variant(blz_solver* h, int variant);
+/* Autotuner (no reference counterpart): time `sweeps` sweeps of every
+   variant above on this grid (pressure restored afterwards) and keep the
+   fastest; ms_per_sweep (optional, 9 entries) receives each time in the
+   order 2, 4, 8, 102, 103, 202, 203, 402, 403 (fused ones skipped when the
+   spare planes do not exist). */
+int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep);
 
 /* ---- learned warm start (no reference counterpart; the reference solve
    always starts from the previous pressure, src/FluidSolver.cu:517-599) ----
@@ -206,6 +212,39 @@ int blz_set_warm_start(blz_solver* h, blz_net* net);
 /* The network input for the current pressure and rhs (for generating
    training data): in2 = 2 x net_dims values; dims reported in (W, H, D). */
 int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]);
+
+/* ---- batched scenes (no reference counterpart; the reference steps one
+   scene per process, src/main.cu:72-117) ----
+   n independent scenes of any grid sizes; one blz_batch_step advances every
+   scene by one step as a single replay of a captured HIP graph, with the
+   pressure sweeps and residual checks of all scenes in single launches.
+   Each scene computes what blz_step computes for it alone. Scenes need
+   residual_tol > 0, cfl_number 0 and one even residual_check_every that
+   divides their max_iter_solve. Sourcing (blz_add_source on the scene
+   handles) happens between steps. */
+typedef struct blz_batch blz_batch;
+typedef struct blz_batch_stats {
+    int scenes;
+    int variant;              /* fused sweep variant in use (blz_set_rbgs_variant numbering) */
+    int graph_nodes;          /* nodes of one captured step */
+    int pad_;
+    long long voxels;         /* cells over all scenes */
+    double capture_ms;        /* host time spent capturing and instantiating */
+    double step_ms;           /* wall time of the last blz_batch_step replay */
+    int tune_variant[6];      /* autotuner candidates ... */
+    double tune_ms[6];        /* ... and their time per sweep over the batch (0 if not tuned) */
+} blz_batch_stats;
+int blz_batch_create(const blz_settings* settings, int n, blz_batch** out);
+int blz_batch_destroy(blz_batch* b);
+/* Borrowed handle of scene i (sourcing, fields, stats); do not destroy. */
+int blz_batch_solver(blz_batch* b, int i, blz_solver** out);
+int blz_batch_set_warm_start(blz_batch* b, blz_net* net);
+/* Time `sweeps` sweeps of each candidate tile configuration (slab groups in
+   flight x waves per block) over the whole batch, keep the fastest; the
+   pressure fields are restored afterwards. */
+int blz_batch_autotune(blz_batch* b, int sweeps, int* variant);
+int blz_batch_step(blz_batch* b, float time);
+int blz_batch_get_stats(blz_batch* b, blz_batch_stats* out);
 /* Self-test of the pressure solve's division by a neighbour count d in
    3..6: runs the solver's device division on every 32-bit float and counts
    results that differ from IEEE a / d (expected 0). */

@@ -1,0 +1,98 @@
+"""GPU tests of batched scenes (blaze_amd/csrc/batch.hip): every scene of a
+batch, stepped by captured-graph replays with multi-grid sweep and residual
+launches, equals the same scene stepped alone (and hence the oracle) bit for
+bit, before and after the autotuner changes the tile configuration, over
+enough steps to replay both captured graphs."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+from helpers import compare, settings  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+B = pytest.importorskip("blaze_amd")
+
+SHAPES = [(32, 24, 16), (48, 40, 32), (64, 32, 24), (40, 16, 20)]
+
+
+def scene(res, seed):
+    rng = np.random.default_rng(seed)
+    s = settings(B, res, max_iter_solve=40, residual_tol=float(rng.uniform(1e-4, 5e-4)), residual_check_every=10,
+                 buoyancy=float(rng.uniform(0.5, 2.0)), vorticity_conf=0.3, cooling_rate=0.5,
+                 closed_bounds=(1, 1, 1, 0, 1, 1), wind_amp=0.2, wind_speed=0.2, wind_turb_amp=2.0,
+                 wind_turb_scale=0.25)
+    n = 4000
+    W, H, D = res
+    c = np.array([0.5 * W, 0.3 * H, 0.5 * D]) * s.dx
+    r = float(rng.uniform(0.03, 0.1)) * W * s.dx
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = c + d * (r * rng.random((n, 1)) ** (1 / 3))
+    part = np.zeros((n, 8), np.float32)
+    part[:, :3] = pos
+    part[:, 6] = 2.5 * s.dx
+    part[:, 7] = 1.0
+    return s, part
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_equals_single_scenes(tune):
+    print(f"[batch] tune={tune}", flush=True)
+    cases = [scene(r, i) for i, r in enumerate(SHAPES)]
+    b = B.Batch([c[0] for c in cases])
+    singles = [B.Solver(c[0]) for c in cases]
+    for i, (s, part) in enumerate(cases):
+        b.scenes[i].add_source(part)
+        singles[i].add_source(part)
+    if tune:
+        v = b.autotune(10)
+        st = b.stats()
+        print(f"[batch] autotuned variant {v}: " + ", ".join(
+            f"{st.tune_variant[k]}={st.tune_ms[k]:.4f}ms" for k in range(6)), flush=True)
+    t = np.float32(0.0)
+    for step in range(3):
+        b.step(float(t))
+        for g in singles:
+            g.step(float(t))
+        t = np.float32(t + np.float32(cases[0][0].dt))
+        for i in range(len(cases)):
+            gs, ss = b.scenes[i].stats(), singles[i].stats()
+            assert gs.rbgs_iters == ss.rbgs_iters, (step, i)
+            assert gs.residual == ss.residual, (step, i)
+    for i in range(len(cases)):
+        for w in (B.FIELD_T, B.FIELD_U, B.FIELD_V, B.FIELD_W, B.FIELD_P):
+            compare(f"batch scene {i} field {w}", b.scenes[i].field(w), singles[i].field(w))
+    st = b.stats()
+    print(f"[batch] nodes {st.graph_nodes} capture {st.capture_ms:.1f} ms step {st.step_ms:.2f} ms", flush=True)
+    assert st.graph_nodes > 0
+
+
+def test_batch_rejects_unsupported_settings():
+    s, _ = scene((32, 24, 16), 0)
+    s.cfl_number = 1.0
+    with pytest.raises(B.BlazeError, match="cfl_number"):
+        B.Batch([s])
+
+
+def test_single_grid_autotune_keeps_results():
+    """The autotuner leaves the pressure as it found it and picks a variant
+    whose sweeps are still bit-exact."""
+    O = pytest.importorskip("oracle")
+    s = settings(B, (64, 40, 24), max_iter_solve=12)
+    rng = np.random.default_rng(9)
+    rhs = rng.standard_normal((24, 40, 64)).astype(np.float32)
+    p = rng.standard_normal((24, 40, 64)).astype(np.float32)
+    g, o = B.Solver(s), O.Oracle(s)
+    for x in (g, o):
+        x.set_field(B.FIELD_RHS, rhs)
+        x.set_field(B.FIELD_P, p)
+    v, times = g.autotune(10)
+    print(f"[autotune] best {v}: {times}", flush=True)
+    compare("autotune keeps p", g.field(B.FIELD_P), p)
+    g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    compare(f"autotuned v{v}", g.field(B.FIELD_P), o.field(B.FIELD_P))
