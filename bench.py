@@ -49,6 +49,12 @@ def parse():
                     help="only run RBGS sweeps (for hardware counter runs)")
     ap.add_argument("--probe-sweeps", type=int, default=20)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--warm-every", type=int, default=1,
+                    help="warm_start setting of that leg (1: before the sweeps, N >= 2: also every N sweeps)")
+    ap.add_argument("--batched", action="store_true",
+                    help="instead: 64 scenes of 32^3..128^3 in one captured-graph step (warm start on)")
+    ap.add_argument("--scenes", type=int, default=64)
     return ap.parse_args()
 
 
@@ -173,16 +179,44 @@ def main():
         traffic_probe(a)
         return
 
+    if a.batched:
+        batched(a)
+        return
     s, _, _, gen = B.parse_scene(a.scene)
     W, H, D = s.grid_res
     N3 = W * H * D
     parts = B.generate_particles(gen)
+    plain = run_leg(a, s, parts, None)
+    warm = None
+    if not a.no_warm_start:
+        import blaze_amd.warm_start as WS
+
+        if os.path.exists(WS.WEIGHTS):
+            warm = run_leg(a, s, parts, WS.load_net())
+    report(a, s, gen, parts, plain, warm)
+
+
+def run_leg(a, s, parts, net):
+    """Warmup frames, then a.steps timed frames of the scene; net != None:
+    with the learned warm start (same tolerance and cap)."""
+    import numpy as np
+
+    import blaze_amd as B
+
+    W, H, D = s.grid_res
+    N3 = W * H * D
+    s = s.copy()
+    if net is not None:
+        s.warm_start = a.warm_every
     g = B.Solver(s)
     g.set_timing(True)
     g.set_rbgs_variant(a.rbgs_variant)
+    if net is not None:
+        g.set_warm_start(net)
     t = np.float32(0.0)
     frame = s.source_range[0]
     sr_end = s.source_range[1]
+    tag = "warm" if net is not None else "plain"
 
     def one_frame():
         nonlocal t, frame
@@ -197,10 +231,12 @@ def main():
     tw = time.perf_counter()
     for _ in range(a.warmup):
         st = one_frame()
-        log(f"[warmup] frame {frame - 1}: substeps {st.substeps} sweeps {st.rbgs_iters} "
+        log(f"[{tag} warmup] frame {frame - 1}: substeps {st.substeps} sweeps {st.rbgs_iters} "
             f"residual {st.residual:.3e} step {st.step_ms:.1f} ms", a.quiet)
     g.sync()
-    log(f"[warmup] {time.perf_counter() - tw:.1f}s", a.quiet)
+    log(f"[{tag} warmup] {time.perf_counter() - tw:.1f}s", a.quiet)
+    kept = rejected0 = 0
+    rejected0 = g.stats().warm_start_fallbacks
 
     tot_sub = tot_iters = tot_checks = tot_launch = tot_timed_sweeps = 0
     kernel_kind = 0
@@ -221,10 +257,25 @@ def main():
         for i in range(B.NUM_STAGES):
             stage_ms[i] += st.stage_ms[i]
         residuals.append(st.residual)
-        log(f"[frame {frame - 1}] substeps {st.substeps} sweeps {st.rbgs_iters} checks "
+        kept += st.warm_start_used
+        log(f"[{tag} frame {frame - 1}] substeps {st.substeps} sweeps {st.rbgs_iters} checks "
             f"{st.residual_checks} residual {st.residual:.3e} step {st.step_ms:.1f} ms", a.quiet)
     g.sync()
     elapsed = time.perf_counter() - t0
+    return dict(elapsed=elapsed, tot_sub=tot_sub, tot_iters=tot_iters, tot_checks=tot_checks, tot_launch=tot_launch,
+                tot_timed_sweeps=tot_timed_sweeps, tot_rbgs_ms=tot_rbgs_ms, kernel_kind=kernel_kind,
+                stage_ms=stage_ms, residuals=residuals, kept=kept,
+                rejected=g.stats().warm_start_fallbacks - rejected0, variant=g.stats().rbgs_kernel)
+
+
+def report(a, s, gen, parts, L, warm):
+    import blaze_amd as B
+
+    W, H, D = s.grid_res
+    N3 = W * H * D
+    elapsed, tot_sub, tot_iters, tot_checks = L["elapsed"], L["tot_sub"], L["tot_iters"], L["tot_checks"]
+    tot_launch, tot_timed_sweeps, tot_rbgs_ms = L["tot_launch"], L["tot_timed_sweeps"], L["tot_rbgs_ms"]
+    kernel_kind, stage_ms, residuals = L["kernel_kind"], L["stage_ms"], L["residuals"]
 
     value = N3 * tot_sub / elapsed
     ms_per_step = elapsed * 1e3 / a.steps
@@ -252,7 +303,7 @@ def main():
                            tot_checks / max(1, tot_sub), a.cpu_sweeps, a.quiet)
 
     out = {
-        "metric": "voxel-steps/s at 256^3 (plain RBGS, ||div u||_inf <= 1e-4 cap 500)",
+        "metric": "voxel-steps/s at 256\u00b3, plain RBGS vs warm-start at equal \u2016\u2207\u00b7u\u2016\u221e; HBM GB/s",
         "value": value,
         "unit": "voxel-steps/s",
         "n_gpus": 1,
@@ -291,6 +342,102 @@ def main():
             "launches": tot_launch,
         },
         "cpu_baseline": cpu,
+    }
+    if warm is not None:
+        wv = N3 * warm["tot_sub"] / warm["elapsed"]
+        out["warm_start"] = {
+            "value": wv,
+            "unit": "voxel-steps/s",
+            "ratio_vs_plain": wv / value,
+            "ms_per_step": warm["elapsed"] * 1e3 / a.steps,
+            "rbgs_sweeps": warm["tot_iters"],
+            "final_residual": warm["residuals"][-1] if warm["residuals"] else None,
+            "plain_final_residual": residuals[-1] if residuals else None,
+            "mean_final_residual_ratio": (sum(warm["residuals"]) / max(1e-300, sum(residuals))
+                                          if residuals and warm["residuals"] else None),
+            "corrections_kept": warm["kept"],
+            "corrections_rejected": warm["rejected"],
+            "setting": f"warm_start={a.warm_every}, weights blaze_amd/warm_start_weights.npy",
+        }
+    print(json.dumps(out), flush=True)
+
+
+def batched(a):
+    """64 independent scenes, grids N^3 with N uniform in [32, 128] (rounded
+    to multiples of 8, seed 5), source radius U[0.03, 0.1], buoyancy
+    U[0.5, 2.0], learned warm start on; one captured-graph step for all,
+    autotuned sweep tiles."""
+    import numpy as np
+
+    import blaze_amd as B
+    import blaze_amd.warm_start as WS
+
+    rng = np.random.default_rng(5)
+    sets, parts = [], []
+    for q in range(a.scenes):
+        n = int(np.clip(round(rng.uniform(32, 128) / 8) * 8, 32, 128))
+        s = B.default_settings()
+        for ax in range(3):
+            s.grid_res[ax] = n
+            s.domain_min[ax] = 0.0
+            s.domain_max[ax] = 1.0
+        s.dx = 1.0 / n
+        s.dt = 1.0 / 24.0
+        s.density = 1.0
+        s.gravity = -9.8
+        s.buoyancy = float(rng.uniform(0.5, 2.0))
+        for ax, v in enumerate((1, 1, 1, 0, 1, 1)):
+            s.closed_bounds[ax] = v
+        s.residual_tol = 1e-4
+        s.max_iter_solve = 100
+        s.residual_check_every = 10
+        s.cfl_number = 0.0
+        s.warm_start = 1
+        g = B.ParticleGen()
+        g.count = int(20000 * (n / 64) ** 3)
+        g.seed = 1 + q
+        g.center[0], g.center[1], g.center[2] = 0.5, 0.1, 0.5
+        g.radius = float(rng.uniform(0.03, 0.1))
+        g.pscale = 2.0 / n
+        g.temp = 1.0
+        sets.append(s)
+        parts.append(B.generate_particles(g))
+    b = B.Batch(sets)
+    net = WS.load_net()
+    b.set_warm_start(net)
+    for q in range(a.scenes):
+        b.scenes[q].add_source(parts[q])
+    v = b.autotune(10)
+    t = np.float32(0.0)
+    for _ in range(a.warmup):
+        b.step(float(t))
+        t = np.float32(t + np.float32(1.0 / 24.0))
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        b.step(float(t))
+        t = np.float32(t + np.float32(1.0 / 24.0))
+    elapsed = time.perf_counter() - t0
+    st = b.stats()
+    vox = st.voxels
+    res = [b.scenes[q].stats().residual for q in range(a.scenes)]
+    out = {
+        "metric": "voxel-steps/s, 64 batched scenes of 32^3..128^3 (one captured-graph step, warm start on)",
+        "value": vox * a.steps / elapsed,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded particle balls)",
+        "config": {"workload": f"{a.scenes} scenes, N uniform in [32,128] step 8 (seed 5), RBGS to 1e-4 cap 100",
+                   "voxels": vox, "parallelism": "single GPU"},
+        "batch": {"autotuned_variant": v, "graph_nodes": st.graph_nodes, "capture_ms": st.capture_ms,
+                  "tune_ms_per_sweep": {st.tune_variant[k]: st.tune_ms[k] for k in range(6)},
+                  "max_final_residual": max(res)},
     }
     print(json.dumps(out), flush=True)
 

@@ -133,6 +133,7 @@ def _load():
         "blz_set_warm_start": ([vp, vp], C.c_int),
         "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
         "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
+        "blz_warm_start_apply": ([vp, C.c_float, P(C.c_float), P(C.c_double), P(C.c_double), P(C.c_int)], C.c_int),
         "blz_batch_create": ([P(Settings), C.c_int, P(vp)], C.c_int),
         "blz_batch_destroy": ([vp], C.c_int),
         "blz_batch_solver": ([vp, C.c_int, P(vp)], C.c_int),
@@ -298,6 +299,15 @@ class Solver:
         nd, cd = (C.c_int * 3)(), (C.c_int * 3)()
         _check(_lib.blz_warm_start_input(self._h, C.c_float(1.0), None, nd, cd), "blz_warm_start_input")
         return tuple(nd), tuple(cd)
+
+    def warm_start_apply(self, dt: float, pred: np.ndarray):
+        """Apply a given coarse prediction (Dn, Hn, Wn) as the learned
+        correction would be; returns (residual before, after, kept)."""
+        a = np.ascontiguousarray(pred, np.float32)
+        b, f, k = C.c_double(), C.c_double(), C.c_int()
+        _check(_lib.blz_warm_start_apply(self._h, C.c_float(dt), a.ctypes.data_as(C.POINTER(C.c_float)),
+                                         C.byref(b), C.byref(f), C.byref(k)), "blz_warm_start_apply")
+        return b.value, f.value, bool(k.value)
 
     def warm_start_input(self, dt: float) -> np.ndarray:
         """(2, Dn, Hn, Wn) float32: normalised coarse residual and solid mask."""

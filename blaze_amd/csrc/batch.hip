@@ -218,6 +218,25 @@ int Batch::enqueue_step(hipStream_t st) {
             if (int e = launch_fused_multi(st, variant, fs + n, n, fs_blocks)) return e;
         }
         if (int e = launch_residual_multi(st, rs, n, rs_blocks, every, 0)) return e;
+        // warm_start = N >= 2: each scene's learned correction every N sweeps
+        const int it = (b + 1) * every;
+        bool any = false;
+        for (int q = 0; q < n; ++q) {
+            const Solver& s = sc[q]->s;
+            any |= s.ws_net && s.s.warm_start >= 2 && it % s.s.warm_start == 0 && it < s.s.max_iter_solve;
+        }
+        if (!any) continue;
+        BLZ_HIP(hipEventRecord(fork, st));
+        for (int q = 0; q < n; ++q) {
+            Solver& s = sc[q]->s;
+            BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
+            if (s.ws_net && s.s.warm_start >= 2 && it % s.s.warm_start == 0 && it < s.s.max_iter_solve)
+                if (int e = warm_start_apply(s, s.s.dt, s.ctl, s.done_host_dev, 0, (double)s.s.residual_tol,
+                                             s.s.max_iter_solve))
+                    return e;
+            BLZ_HIP(hipEventRecord(join[q], s.stream));
+            BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+        }
     }
     BLZ_HIP(hipEventRecord(fork, st));
     for (int q = 0; q < n; ++q) {

@@ -237,8 +237,8 @@ int Solver::finalize() {
         st.residual = r;
         st.residual_initial = r0;
         if (solves[q] >= 0) {
-            st.warm_start_used = ctl_host[q].ws == 1 ? 1 : 0;
-            if (ctl_host[q].ws == 2) st.warm_start_fallbacks += 1;
+            st.warm_start_used = ctl_host[q].ws_kept;
+            st.warm_start_fallbacks += ctl_host[q].ws_rejected;
         }
     }
     const int e = resolve_spans();
@@ -256,6 +256,54 @@ void clear_step_stats(blz_stats& st) {
     st.residual_checks = 0;
 }
 
+// Time `sweeps` sweeps of every RBGS variant on the current pressure and
+// rhs (the pressure is restored afterwards; all variants give the same
+// bits); *best = the fastest. Synchronises the stream.
+int Solver::autotune(int sweeps, int* best, float* ms_per_sweep) {
+    static const int cand[] = {2, 4, 8, 102, 103, 202, 203, 402, 403};
+    sweeps = std::max(2, sweeps & ~1);
+    const float dt = s.dt;
+    BLZ_HIP(hipStreamSynchronize(stream));
+    BLZ_HIP(hipMemcpyAsync(tmp, prb, 2 * Np * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    hipEvent_t e0, e1;
+    BLZ_HIP(hipEventCreate(&e0));
+    BLZ_HIP(hipEventCreate(&e1));
+    const int keep = rbgs_variant;
+    int bestv = 8, rc = BLZ_OK, k = 0;
+    float bestms = 1e30f;
+    for (int v : cand) {
+        if (v >= 100 && !fused_available(*this)) continue;
+        rbgs_variant = v;
+        int launches = 0, kind = 0;
+        rc = launch_rbgs_sweeps(*this, dt, 2, zero_flag, &launches, &kind);
+        if (!rc) rc = hipEventRecord(e0, stream) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
+        if (!rc) rc = launch_rbgs_sweeps(*this, dt, sweeps, zero_flag, &launches, &kind);
+        if (!rc) rc = hipEventRecord(e1, stream) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
+        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
+        if (rc) break;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)sweeps;
+        if (ms_per_sweep) ms_per_sweep[k] = ms;
+        ++k;
+        if (ms < bestms) {
+            bestms = ms;
+            bestv = v;
+        }
+    }
+    rbgs_variant = keep;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipMemcpyAsync(prb, tmp, 2 * Np * sizeof(float), hipMemcpyDeviceToDevice, stream);
+    BLZ_HIP(hipStreamSynchronize(stream));
+    if (rc) {
+        set_error("RBGS autotune failed");
+        return rc;
+    }
+    *best = bestv;
+    return BLZ_OK;
+}
+
 // gs_solve_kernel (src/FluidSolver.cu:517-599): a fixed sweep count, or
 // with residual_tol > 0 the loop
 //     r = residual; it = 0
@@ -268,6 +316,9 @@ void clear_step_stats(blz_stats& st) {
 // its result after the step.
 int Solver::project_solve(float dt_) {
     const int cap = s.max_iter_solve;
+    // variant 0: the autotuner's choice for this grid, made at the first solve
+    if (rbgs_variant == 0 && rbgs_auto == 0 && cap > 0)
+        if (int e = autotune(10, &rbgs_auto, nullptr)) return e;
     const int q = (int)solves.size();
     if (q >= MAX_SOLVES) {
         set_error("too many pressure solves enqueued without synchronisation");
@@ -300,6 +351,9 @@ int Solver::project_solve(float dt_) {
         if (int e = sweeps(n, &c->done, b)) return e;
         it += n;
         if (int e = launch_residual_check(*this, dt_, c, done_host_dev, seq, tol, cap, n, 0)) return e;
+        // warm_start = N >= 2: the learned correction again every N sweeps
+        if (ws_net && s.warm_start >= 2 && it % s.warm_start == 0 && it < cap)
+            if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap)) return e;
         BLZ_HIP(hipEventRecord(batch_ev[b % (SOLVE_LOOKAHEAD + 1)], stream));
         if (b >= SOLVE_LOOKAHEAD) BLZ_HIP(hipEventSynchronize(batch_ev[(b - SOLVE_LOOKAHEAD) % (SOLVE_LOOKAHEAD + 1)]));
     }
@@ -629,49 +683,20 @@ int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], i
 
 int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep) {
     CHECK_HANDLE(h);
-    Solver& s = h->s;
-    static const int cand[] = {2, 4, 8, 102, 103, 202, 203, 402, 403};
-    sweeps = std::max(2, sweeps & ~1);
-    const float dt = s.s.dt;
-    BLZ_HIP(hipStreamSynchronize(s.stream));
-    BLZ_HIP(hipMemcpyAsync(s.tmp, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
-    hipEvent_t e0, e1;
-    BLZ_HIP(hipEventCreate(&e0));
-    BLZ_HIP(hipEventCreate(&e1));
-    const int keep = s.rbgs_variant;
-    int best = keep, rc = BLZ_OK, k = 0;
-    float bestms = 1e30f;
-    for (int v : cand) {
-        if (v >= 100 && !blz::fused_available(s)) continue;
-        s.rbgs_variant = v;
-        int launches = 0, kind = 0;
-        rc = blz::launch_rbgs_sweeps(s, dt, 2, s.zero_flag, &launches, &kind);
-        if (!rc) rc = hipEventRecord(e0, s.stream) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
-        if (!rc) rc = blz::launch_rbgs_sweeps(s, dt, sweeps, s.zero_flag, &launches, &kind);
-        if (!rc) rc = hipEventRecord(e1, s.stream) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
-        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? BLZ_OK : BLZ_E_HIP;
-        if (rc) break;
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)sweeps;
-        if (ms_per_sweep) ms_per_sweep[k] = ms;
-        ++k;
-        if (ms < bestms) {
-            bestms = ms;
-            best = v;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipMemcpyAsync(s.prb, s.tmp, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream);
-    BLZ_HIP(hipStreamSynchronize(s.stream));
-    s.rbgs_variant = rc ? keep : best;
-    if (rc) {
-        blz::set_error("autotune failed");
-        return rc;
-    }
+    int best = 0;
+    if (int e = h->s.autotune(sweeps, &best, ms_per_sweep)) return e;
+    h->s.rbgs_variant = best;
     if (variant) *variant = best;
     return BLZ_OK;
+}
+
+int blz_warm_start_apply(blz_solver* h, float dt, const float* pred, double* before, double* after, int* kept) {
+    CHECK_HANDLE(h);
+    if (!pred || !before || !after || !kept || !(dt > 0.0f)) {
+        blz::set_error("blz_warm_start_apply: bad argument");
+        return BLZ_E_INVALID;
+    }
+    return blz::warm_start_apply_given(h->s, dt, pred, before, after, kept);
 }
 
 int blz_set_rbgs_variant(blz_solver* h, int variant) {
@@ -680,6 +705,7 @@ int blz_set_rbgs_variant(blz_solver* h, int variant) {
         blz::set_error("unknown RBGS variant");
         return BLZ_E_INVALID;
     }
+    h->s.rbgs_auto = 0;
     h->s.rbgs_variant = variant;
     return BLZ_OK;
 }

@@ -799,6 +799,8 @@ __global__ void k_ctl_reset(SolveCtl* c) {
     c->res = -1.0;
     c->res0 = -1.0;
     c->res_ws = -1.0;
+    c->ws_kept = 0;
+    c->ws_rejected = 0;
 }
 
 // pressure_gradient_update_kernel :601-669 for the faces along AXIS,
@@ -1001,8 +1003,8 @@ static int fused_dispatch(Solver& s, int v, float scale, int pairs, const int* s
 
 int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* launches, int* kind) {
     const float scale = (s.s.density * s.dx * s.dx) / dt; // Bridson p.75, :530
-    int v = s.rbgs_variant;
-    if (v == 0) v = fused_available(s) ? 103 : 8;
+    int v = s.rbgs_variant ? s.rbgs_variant : s.rbgs_auto;
+    if (v == 0) v = 8;
     if (v >= 100 && !fused_available(s)) v = 8;
     if (v >= 100) {
         const int pairs = sweeps / 2;

@@ -46,7 +46,8 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
 // voxel l%16, channels 4*(l/16)..+3), B = 16 input x 16 output channels
 // (lane l: output channel l%16, input channels 4*(l/16)..+3), D = 16
 // voxels x 16 output channels (lane l: output channel l%16, voxels
-// 4*(l/16)..+3). Optional epilogue: ReLU, or a product with (M > 0).
+// 4*(l/16)..+3). Optional epilogue: leaky ReLU, or the leaky ReLU's
+// derivative at the activation M (1 where M > 0, NET_LEAK elsewhere).
 template <int NT>
 __global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, int Cx, int xoff, int Cin,
                                                    const bf16* __restrict__ Wt, bf16* __restrict__ Y, int Cy, int yoff,
@@ -90,8 +91,8 @@ __global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, i
             if (vo >= nvox) continue;
             float r = acc[t][q];
             const int co = t * 16 + li;
-            if (relu) r = fmaxf(r, 0.0f);
-            if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : 0.0f;
+            if (relu) r = r > 0.0f ? r : NET_LEAK * r;
+            if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : NET_LEAK * r;
             Y[vo * Cy + yoff + co] = f2bf(r);
         }
     }
@@ -199,7 +200,7 @@ __global__ void k_pool_bwd(const bf16* __restrict__ gP, int Cp, int poff, bf16* 
     int b, z, y, x;
     const long long vp = parent_of(v, fine, b, z, y, x);
     const float r = bf2f(gS[v * Cgs + gsoff + c]) + bf2f(gP[vp * Cp + poff + c]) * 0.125f;
-    gS[v * Cgs + gsoff + c] = f2bf(bf2f(S[v * Cs + soff + c]) > 0.0f ? r : 0.0f);
+    gS[v * Cgs + gsoff + c] = f2bf(bf2f(S[v * Cs + soff + c]) > 0.0f ? r : NET_LEAK * r);
 }
 
 // nearest 2x upsampling of a coarse view into a fine view
@@ -231,7 +232,7 @@ __global__ void k_up_bwd(const bf16* __restrict__ gU, int Cu, int uoff, bf16* __
                              2 * x + (q & 1);
         s += bf2f(gU[vf * Cu + uoff + c]);
     }
-    gB[vc * Cb + boff + c] = f2bf(bf2f(Bm[vc * Cbm + bmoff + c]) > 0.0f ? s : 0.0f);
+    gB[vc * Cb + boff + c] = f2bf(bf2f(Bm[vc * Cbm + bmoff + c]) > 0.0f ? s : NET_LEAK * s);
 }
 
 // 1x1 head: out[v] = sum_c X[v][c] w[c] (FP32)
@@ -275,7 +276,8 @@ __global__ void __launch_bounds__(256) k_head_bwd(const float* __restrict__ gout
         for (int c = 0; c < NET_HEAD; ++c) {
             const float xv = bf2f(X[v * NET_HEAD + c]);
             acc[c] = fmaf(go, xv, acc[c]);
-            gX[v * NET_HEAD + c] = f2bf(xv > 0.0f ? go * bf2f(f2bf(w[c])) : 0.0f);
+            const float gx = go * bf2f(f2bf(w[c]));
+            gX[v * NET_HEAD + c] = f2bf(xv > 0.0f ? gx : NET_LEAK * gx);
         }
     }
 #pragma unroll
@@ -286,11 +288,29 @@ __global__ void __launch_bounds__(256) k_head_bwd(const float* __restrict__ gout
     }
 }
 
+// global gradient-norm clipping: *scale = min(1, maxnorm / ||G||) (one block)
+__global__ void __launch_bounds__(1024) k_gclip(const float* __restrict__ G, size_t n, float maxnorm,
+                                                float* __restrict__ scale) {
+    double a = 0.0;
+    for (size_t t = threadIdx.x; t < n; t += 1024) a += (double)G[t] * (double)G[t];
+    for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int q = 0; q < 16; ++q) tot += sm[q];
+        const double nrm = sqrt(tot);
+        *scale = (nrm > (double)maxnorm && nrm == nrm) ? (float)((double)maxnorm / nrm) : (nrm == nrm ? 1.0f : 0.0f);
+    }
+}
+
 __global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
-                       float* __restrict__ V, size_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+                       float* __restrict__ V, size_t n, float lr, float b1, float b2, float eps, float c1, float c2,
+                       const float* __restrict__ gscale) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const float gg = G[t];
+    const float gg = G[t] * *gscale;
     const float m = b1 * M[t] + (1.0f - b1) * gg;
     const float v = b2 * V[t] + (1.0f - b2) * gg * gg;
     M[t] = m;
@@ -472,7 +492,7 @@ int Net::ensure_dims(const NetDims& d) {
                           {&g_a3, n3 * 64}, {&g_p2, n3 * 32},   {&g_a2, n2 * 32},  {&g_p1, n2 * 16},  {&g_a1, n1 * 16}};
     size_t bytes = 0;
     for (const Item& it : items) bytes += ((size_t)it.n * 2 + 255) / 256 * 256;
-    const size_t fbytes = ((size_t)n1 * 4 + 255) / 256 * 256;
+    const size_t fbytes = ((size_t)(n1 + 1) * 4 + 255) / 256 * 256; // + gradient clip scale after out
     bytes += 3 * fbytes;
     if (bytes > act_bytes) {
         if (act) NET_HIP(hipFree(act));
@@ -595,8 +615,10 @@ int Net::backward() {
 int Net::adam(float lr, float b1, float b2, float eps) {
     ++adam_t;
     const float c1 = 1.0f - std::pow(b1, (float)adam_t), c2 = 1.0f - std::pow(b2, (float)adam_t);
+    float* gscale = out + dims.n(); // one float after the prediction (see ensure_dims)
+    hipLaunchKernelGGL(k_gclip, dim3(1), dim3(1024), 0, stream, G, nparam, 1.0f, gscale);
     hipLaunchKernelGGL(k_adam, dim3(nblk((long long)nparam, 256)), dim3(256), 0, stream, P, G, M, V, nparam, lr, b1,
-                       b2, eps, c1, c2);
+                       b2, eps, c1, c2, gscale);
     NET_HIP(hipGetLastError());
     return refresh_weights();
 }

@@ -8,9 +8,12 @@
 //     to a 4x coarser grid (sum over each 4x4x4 block / 4, which is the
 //     right-hand side of the coarse Poisson problem for a smooth error), with
 //     a solid mask (1 where a block touches a domain wall or lies outside);
-//  2. normalise by s = max |coarse r| and run the U-Net (bias-free, so its
-//     output scales with s);
-//  3. save p, add s * trilinear(prediction) to p;
+//  2. normalise by s = RMS of the coarse r and run the U-Net (bias-free with
+//     leaky ReLUs, so its output scales with s);
+//  3. save p, prolongate the prediction trilinearly to e and add alpha e,
+//     alpha = <r, A e> / <A e, A e> (A x = n x - sum_nbr x, FP64 sums): the
+//     scale minimising the residual's 2-norm, so the network only has to
+//     get the shape of the correction right;
 //  4. evaluate the FP64 residual: the prediction is kept only if it is lower
 //     than before, otherwise p is restored (RBGS stays the judge and the
 //     fallback). The sweeps then run to the same tolerance as without it.
@@ -66,11 +69,28 @@ __global__ void __launch_bounds__(BS) k_ws_restrict(RB p, RB rhs, RBL L, int W, 
     const float rc = 0.25f * sum;
     res[t] = rc;
     mask[t] = wall ? 1.0f : 0.0f;
-    // |x| bit patterns order like the values (NaN above +inf)
-    atomicMax(amax, __float_as_uint(rc) & 0x7fffffffu);
 }
 
-// network input: (r / s, mask) in BF16, 16 channels
+// input scale s = RMS of the coarse residual over the real coarse cells (one
+// block, fixed order: deterministic); stored as float bits in *amax
+__global__ void __launch_bounds__(1024) k_ws_rms(const float* __restrict__ res, int n, int nreal,
+                                                 unsigned* __restrict__ amax, const int* __restrict__ stop) {
+    if (*stop) return;
+    double a = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) a += (double)res[t] * (double)res[t];
+    for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int q = 0; q < 16; ++q) tot += sm[q];
+        const float r = (float)sqrt(tot / (double)max(nreal, 1));
+        *amax = __float_as_uint(r);
+    }
+}
+
+// network input: (r / s, mask) in BF16, 16 channels (s = input scale)
 __global__ void k_ws_pack(const float* __restrict__ res, const float* __restrict__ mask,
                           const unsigned* __restrict__ amax, bf16* __restrict__ x0, int n) {
     const int t = blockIdx.x * BS + threadIdx.x;
@@ -93,14 +113,12 @@ __global__ void k_ws_export(const float* __restrict__ res, const float* __restri
     out2[n + t] = mask[t];
 }
 
-// p += s * trilinear(prediction) at every fine cell (natural order thread)
-__global__ void __launch_bounds__(BS) k_ws_prolong(RB p, RBL L, int W, int H, int D, const float* __restrict__ pred,
-                                                   int Wc, int Hc, int Dc, int Wn, int Hn,
-                                                   const unsigned* __restrict__ amax, const SolveCtl* __restrict__ c) {
+// e = trilinear(prediction) at every fine cell, natural order, into tmp
+__global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W, int H, int D,
+                                                   const float* __restrict__ pred, int Wc, int Hc, int Dc, int Wn,
+                                                   int Hn, const SolveCtl* __restrict__ c) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= W * H * D || c->done) return;
-    const float s = __uint_as_float(*amax);
-    if (!(s > 0.0f) || !(s <= 3.4e38f)) return;
     const int i = t % W, j = (t / W) % H, k = t / (W * H);
     auto axis = [](int q, int n, int& a0, int& a1, float& w) {
         float f = ((float)q + 0.5f) * 0.25f - 0.5f;
@@ -120,9 +138,79 @@ __global__ void __launch_bounds__(BS) k_ws_prolong(RB p, RBL L, int W, int H, in
     const float c01 = P(x0, y0, z1) * (1.0f - wx) + P(x1, y0, z1) * wx;
     const float c11 = P(x0, y1, z1) * (1.0f - wx) + P(x1, y1, z1) * wx;
     const float c0 = c00 * (1.0f - wy) + c10 * wy, c1 = c01 * (1.0f - wy) + c11 * wy;
-    const float corr = s * (c0 * (1.0f - wz) + c1 * wz);
-    float& dst = p.c[(i + j) & 1][L.at(i, j, k)];
-    dst = dst + corr;
+    const float v = c0 * (1.0f - wz) + c1 * wz;
+    e[t] = (v == v && fabsf(v) <= 3.4e38f) ? v : 0.0f;
+}
+
+// scale of the correction minimising the residual's 2-norm: with A x =
+// n x - sum_nbr x and r = scale*rhs - A p, alpha = <r, A e> / <A e, A e>;
+// per-block partial sums (FP64) of both products
+__global__ void __launch_bounds__(BS) k_ws_dots(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
+                                                const float* __restrict__ e, double* __restrict__ partial,
+                                                const SolveCtl* __restrict__ c) {
+    if (c->done) return;
+    double num = 0.0, den = 0.0;
+    const int n = W * H * D;
+    for (int t = blockIdx.x * BS + threadIdx.x; t < n; t += gridDim.x * BS) {
+        const int i = t % W, j = (t / W) % H, k = t / (W * H);
+        const double pc = pget(p, L, i, j, k);
+        double r = (double)scale * pget(rhs, L, i, j, k);
+        const double ec = e[t];
+        double ae = 0.0, cnt = 0.0;
+        if (i > 0) { r += pget(p, L, i - 1, j, k); ae -= e[t - 1]; cnt += 1.0; }
+        if (i < W - 1) { r += pget(p, L, i + 1, j, k); ae -= e[t + 1]; cnt += 1.0; }
+        if (j > 0) { r += pget(p, L, i, j - 1, k); ae -= e[t - W]; cnt += 1.0; }
+        if (j < H - 1) { r += pget(p, L, i, j + 1, k); ae -= e[t + W]; cnt += 1.0; }
+        if (k > 0) { r += pget(p, L, i, j, k - 1); ae -= e[t - W * H]; cnt += 1.0; }
+        if (k < D - 1) { r += pget(p, L, i, j, k + 1); ae -= e[t + W * H]; cnt += 1.0; }
+        r -= cnt * pc;
+        ae += cnt * ec;
+        num += r * ae;
+        den += ae * ae;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        num += __shfl_xor(num, o, 64);
+        den += __shfl_xor(den, o, 64);
+    }
+    __shared__ double sn[BS / 64], sd[BS / 64];
+    if ((threadIdx.x & 63) == 0) {
+        sn[threadIdx.x >> 6] = num;
+        sd[threadIdx.x >> 6] = den;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < (int)(BS / 64); ++q) {
+            a += sn[q];
+            b += sd[q];
+        }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// p += alpha e (alpha from the partial sums, 0 when undefined)
+__global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int D, const float* __restrict__ e,
+                                                 const double* __restrict__ partial, int nb,
+                                                 const SolveCtl* __restrict__ c) {
+    if (c->done) return;
+    __shared__ float alpha;
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < nb; ++q) {
+            a += partial[2 * q];
+            b += partial[2 * q + 1];
+        }
+        const double al = b > 0.0 ? a / b : 0.0;
+        alpha = (al == al && fabs(al) < 1e30) ? (float)al : 0.0f;
+    }
+    __syncthreads();
+    const int n = W * H * D;
+    for (int t = blockIdx.x * BS + threadIdx.x; t < n; t += gridDim.x * BS) {
+        const int i = t % W, j = (t / W) % H, k = t / (W * H);
+        float& dst = p.c[(i + j) & 1][L.at(i, j, k)];
+        dst = dst + alpha * e[t];
+    }
 }
 
 // residual after the correction: keep it if lower, else mark for restore
@@ -143,8 +231,10 @@ __global__ void __launch_bounds__(BS) k_res_final_ws(const double* __restrict__ 
         if (r < c->res) {
             c->res = r;
             c->ws = 1;
+            c->ws_kept += 1;
         } else {
             c->ws = 2;
+            c->ws_rejected += 1;
         }
         if (!(c->res > tol) || c->iters >= cap) {
             c->done = 1;
@@ -212,6 +302,7 @@ static int ws_restrict(Solver& s, float dt, const int* stop) {
     hipLaunchKernelGGL(k_ws_zero, dim3(1), dim3(1), 0, s.stream, amax);
     hipLaunchKernelGGL(k_ws_restrict, dim3(nb(nn)), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, s.Wc,
                        s.Hc, s.Dc, s.Wn, s.Hn, s.Dn, res, mask, amax, stop);
+    hipLaunchKernelGGL(k_ws_rms, dim3(1), dim3(1024), 0, s.stream, res, (int)nn, s.Wc * s.Hc * s.Dc, amax, stop);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
@@ -227,16 +318,61 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
                        (int)nn);
     BLZ_HIP(hipGetLastError());
     if (int e = s.ws_net->forward()) return e;
-    // save the pressure planes, then add the prediction
+    // save the pressure planes, then add the prediction with its fitted scale
     float* backup = s.ws_backup ? s.ws_backup : s.prb + 4 * s.Np;
     BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
-    hipLaunchKernelGGL(k_ws_prolong, dim3(nb((size_t)s.W * s.H * s.D)), dim3(BS), 0, s.stream, s.p, s.L, s.W, s.H,
-                       s.D, s.ws_net->out, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, amax, c);
+    const size_t N = (size_t)s.W * s.H * s.D;
+    float* e = s.tmp;
+    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out, s.Wc,
+                       s.Hc, s.Dc, s.Wn, s.Hn, c);
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
+    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 2);
+    hipLaunchKernelGGL(k_ws_dots, dim3(nbd), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, e,
+                       s.red + 8, c);
+    hipLaunchKernelGGL(k_ws_apply, dim3((unsigned)std::min<size_t>(nb(N), 4096)), dim3(BS), 0, s.stream, s.p, s.L,
+                       s.W, s.H, s.D, e, s.red + 8, nbd, c);
     BLZ_HIP(hipGetLastError());
     if (int e = launch_residual_ws(s, dt, c, host_done, seq, tol, cap)) return e;
     hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
                        backup, s.prb, 2 * s.Np);
     BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+// the correction step with a given coarse prediction (network grid, x
+// fastest) instead of the network's: restrict, then steps 3-4 of the
+// warm start, synchronously; for measuring what an exact coarse solve gains
+int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* before, double* after, int* kept) {
+    if (!s.ws_net) {
+        set_error("no warm-start network attached");
+        return BLZ_E_INVALID;
+    }
+    const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
+    SolveCtl* c = s.ctl;
+    if (int e = launch_ctl_reset(s, c)) return e;
+    if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, -1.0, 1 << 30, 0, 1)) return e;
+    BLZ_HIP(hipMemcpyAsync(s.ws_net->out, host_pred, nn * sizeof(float), hipMemcpyHostToDevice, s.stream));
+    float* backup = s.ws_backup ? s.ws_backup : s.prb + 4 * s.Np;
+    BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
+    const size_t N = (size_t)s.W * s.H * s.D;
+    float* e = s.tmp;
+    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out, s.Wc,
+                       s.Hc, s.Dc, s.Wn, s.Hn, c);
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
+    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 2);
+    hipLaunchKernelGGL(k_ws_dots, dim3(nbd), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, e,
+                       s.red + 8, c);
+    hipLaunchKernelGGL(k_ws_apply, dim3((unsigned)std::min<size_t>(nb(N), 4096)), dim3(BS), 0, s.stream, s.p, s.L,
+                       s.W, s.H, s.D, e, s.red + 8, nbd, c);
+    BLZ_HIP(hipGetLastError());
+    if (int e2 = launch_residual_ws(s, dt, c, s.done_host_dev, 0, -1.0, 1 << 30)) return e2;
+    hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
+                       backup, s.prb, 2 * s.Np);
+    BLZ_HIP(hipMemcpyAsync(s.ctl_host, c, sizeof(SolveCtl), hipMemcpyDeviceToHost, s.stream));
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    *before = s.ctl_host[0].res0;
+    *after = s.ctl_host[0].res_ws;
+    *kept = s.ctl_host[0].ws == 1;
     return BLZ_OK;
 }
 

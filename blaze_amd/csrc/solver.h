@@ -68,10 +68,10 @@ struct SolveCtl {
     int done;   // 1 once the residual <= tol or the sweep cap is reached
     int iters;  // sweeps counted so far
     int checks; // residual evaluations so far
-    int ws;     // learned warm start: 0 not applied, 1 kept, 2 rejected (pressure restored)
+    int ws;     // last learned correction: 0 none, 1 kept, 2 rejected (pressure restored)
     double res, res0; // last and first residual
-    double res_ws;    // residual right after the learned correction
-    double pad;
+    double res_ws;    // residual right after the last learned correction
+    int ws_kept, ws_rejected; // learned corrections kept / rejected in this solve
 };
 constexpr int MAX_SOLVES = 64;     // pressure solves per step (bounds max_substeps)
 constexpr int SOLVE_LOOKAHEAD = 2; // sweep batches the host may enqueue past the last one it has seen finish
@@ -145,7 +145,8 @@ struct Solver {
     std::vector<hipEvent_t> evpool;
     size_t ev_used = 0;
     std::vector<Span> spans;
-    int rbgs_variant = 0;
+    int rbgs_variant = 0; // requested variant (0 = autotuned)
+    int rbgs_auto = 0;    // autotuned variant when rbgs_variant == 0
     // learned warm start (attached network; activations of its own)
     const Net* ws_src = nullptr;
     Net* ws_net = nullptr;     // view: shares ws_src's weights, owns activations
@@ -169,6 +170,7 @@ struct Solver {
     int substep(float time, float dt);
     int run_stage(int stage, float time, float dt);
     int project_solve(float dt);
+    int autotune(int sweeps, int* best, float* ms_per_sweep);
     int enqueue_solve_stats();
     int max_abs_velocity(float* out);
 };
@@ -196,6 +198,7 @@ int launch_residual_ws_final(Solver& s, SolveCtl* c, int* host_done, int seq, do
 int warm_start_attach(Solver& s, const Net* net);
 int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
 int warm_start_input(Solver& s, float dt, float* host_in2);
+int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* before, double* after, int* kept);
 int launch_pressure_update(Solver& s, float dt);
 int launch_advect(Solver& s, float dt);
 int launch_p2g(Solver& s);

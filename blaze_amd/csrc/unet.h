@@ -14,8 +14,8 @@
 //   s2 -pool-> p2(32) -e3a-> a3(64) -e3b-> b3(64)
 //   [up(b3), s2](96) -d2a-> a4(32) -d2b-> b4(32)
 //   [up(b4), s1](48) -d1a-> a5(16) -d1b-> b5(16) -1x1-> out(1, FP32)
-// with ReLU after every 3x3x3 convolution, 2x2x2 average pooling and
-// nearest-neighbour 2x upsampling.
+// with a leaky ReLU (slope NET_LEAK below 0) after every 3x3x3
+// convolution, 2x2x2 average pooling and nearest-neighbour 2x upsampling.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,6 +49,7 @@ constexpr ConvSpec kConv[L_NCONV] = {{16, 16}, {16, 16}, {16, 32}, {32, 32}, {32
                                      {64, 64}, {96, 32}, {32, 32}, {48, 16}, {16, 16}};
 constexpr int NET_IN_CH = 16;  // x0 channels (2 used)
 constexpr int NET_HEAD = 16;   // 1x1 head inputs
+constexpr float NET_LEAK = 0.1f; // leaky ReLU slope (keeps f(a x) = a f(x), a > 0)
 
 struct Net {
     hipStream_t stream = nullptr;

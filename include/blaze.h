@@ -104,7 +104,8 @@ typedef struct blz_settings {
     float cfl_number;         /* >0: substeps n = ceil(max|u| dt / (cfl dx)) */
     int max_substeps;         /* cap on n (default 8) */
     float smoke_density;      /* >0: enable smoke density grid, splat value */
-    int warm_start;           /* 0 plain RBGS, 1 learned warm start (see blz_set_warm_start_model) */
+    int warm_start;           /* 0 plain RBGS; 1 learned correction before the sweeps; N >= 2 also
+                                 after every N sweeps (blz_set_warm_start attaches the network) */
 } blz_settings;
 
 /* per-step statistics of the last blz_step / blz_run_stage call */
@@ -114,8 +115,8 @@ typedef struct blz_stats {
     int rbgs_iters_last;      /* RBGS sweeps in the last substep's projection */
     double residual;          /* ||div u||_inf proxy after the last projection (FP64), -1 if not computed */
     double residual_initial;  /* same measure before the first sweep of the last projection */
-    int warm_start_used;      /* 1 when the learned prediction was kept */
-    int warm_start_fallbacks; /* number of rejected predictions so far */
+    int warm_start_used;      /* learned corrections kept in the last solve */
+    int warm_start_fallbacks; /* learned corrections rejected so far (pressure restored) */
     double stage_ms[BLZ_NUM_STAGES]; /* device time per stage from HIP events (0 if timing off) */
     double source_ms;         /* device time of the last blz_add_source */
     double step_ms;           /* device time of the last blz_step */
@@ -175,8 +176,9 @@ int blz_set_timing(blz_solver* h, int on);
 /* RBGS kernel variant (no reference counterpart): 2, 4, 8 = two in-place
    colour passes per sweep with that many four-slab groups in flight per
    thread; 102, 103 = fused whole-sweep kernel (2 or 3 groups in flight),
-   sweeps in pairs through a spare plane set; 0 = default (103 where the
-   spare planes exist, else 8). */
+   sweeps in pairs through a spare plane set, with 1/2/4 waves per block for
+   x = 1/2/4 in 10x; 0 = default: the autotuner's choice for this grid, made
+   at the first solve (every variant gives the same bits). */
 int blz_set_rbgs_variant(blz_solver* h, int variant);
 /* Autotuner (no reference counterpart): time `sweeps` sweeps of every
    variant above on this grid (pressure restored afterwards) and keep the
@@ -212,6 +214,11 @@ int blz_set_warm_start(blz_solver* h, blz_net* net);
 /* The network input for the current pressure and rhs (for generating
    training data): in2 = 2 x net_dims values; dims reported in (W, H, D). */
 int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]);
+/* The correction step with a caller-given coarse prediction (network grid,
+   x fastest) in place of the network's: fitted scale, FP64 residual before
+   and after, kept only if lower (else the pressure is restored). For
+   measuring what an exact coarse solve would gain (ablations). */
+int blz_warm_start_apply(blz_solver* h, float dt, const float* pred, double* before, double* after, int* kept);
 
 /* ---- batched scenes (no reference counterpart; the reference steps one
    scene per process, src/main.cu:72-117) ----

@@ -14,13 +14,15 @@ Network (channels-last, params vector layout identical to the library):
   x0(16: res, mask, 0..) -e1a-> a1(16) -e1b-> s1(16) -pool-> p1 -e2a-> a2(32)
   -e2b-> s2(32) -pool-> p2 -e3a-> a3(64) -e3b-> b3(64) -up-> [u2, s2](96)
   -d2a-> a4(32) -d2b-> b4(32) -up-> [u1, s1](48) -d1a-> a5(16) -d1b-> b5(16)
-  -1x1-> out; ReLU after every 3x3x3 convolution; no biases.
+  -1x1-> out; leaky ReLU (slope LEAK below 0) after every 3x3x3
+  convolution; no biases.
 """
 import numpy as np
 
 CONV = [(16, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (32, 32), (48, 16), (16, 16)]
 NAMES = ["e1a", "e1b", "e2a", "e2b", "e3a", "e3b", "d2a", "d2b", "d1a", "d1b"]
 HEAD = 16
+LEAK = 0.1
 
 
 def offsets():
@@ -123,7 +125,7 @@ def forward(params, x_in, quant=True):
     x0[..., 0] = x_in[:, 0]
     x0[..., 1] = x_in[:, 1]
     x0 = q(x0)
-    relu = lambda a: np.maximum(a, 0.0)
+    relu = lambda a: np.where(a > 0, a, LEAK * a)
     c = {"x0": x0, "w": wq}
     c["a1"] = q(relu(conv3(x0, wq[0])))
     c["s1"] = q(relu(conv3(c["a1"], wq[1])))
@@ -160,7 +162,7 @@ def backward(params, x_in, target, quant=True):
         gout = gout.astype(np.float32).astype(np.float64)
     wq = c["w"]
     grads = [None] * len(CONV)
-    mask = lambda g, a: g * (a > 0)
+    mask = lambda g, a: g * np.where(a > 0, 1.0, LEAK)
     # head
     g_head = np.einsum("bzyx,bzyxc->c", gout, c["b5"])
     g_b5 = q(mask(gout[..., None] * c["head"], c["b5"]))
