@@ -392,7 +392,7 @@ def batched(a):
         s.max_iter_solve = 100
         s.residual_check_every = 10
         s.cfl_number = 0.0
-        s.warm_start = 1
+        s.warm_start = 0 if a.no_warm_start else 1
         g = B.ParticleGen()
         g.count = int(20000 * (n / 64) ** 3)
         g.seed = 1 + q
@@ -403,8 +403,10 @@ def batched(a):
         sets.append(s)
         parts.append(B.generate_particles(g))
     b = B.Batch(sets)
-    net = WS.load_net()
-    b.set_warm_start(net)
+    net = None
+    if not a.no_warm_start:
+        net = WS.load_net()
+        b.set_warm_start(net)
     for q in range(a.scenes):
         b.scenes[q].add_source(parts[q])
     v = b.autotune(10)
@@ -421,7 +423,8 @@ def batched(a):
     vox = st.voxels
     res = [b.scenes[q].stats().residual for q in range(a.scenes)]
     out = {
-        "metric": "voxel-steps/s, 64 batched scenes of 32^3..128^3 (one captured-graph step, warm start on)",
+        "metric": f"voxel-steps/s, {a.scenes} batched scenes of 32^3..128^3 (one captured-graph step, "
+                  f"warm start {'off' if a.no_warm_start else 'on'})",
         "value": vox * a.steps / elapsed,
         "unit": "voxel-steps/s",
         "n_gpus": 1,

@@ -206,8 +206,8 @@ int Batch::enqueue_step(hipStream_t st) {
         const int cap = s.s.max_iter_solve;
         if (int e = launch_ctl_reset(s, c)) return e;
         if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, tol, cap, 0, 1)) return e;
-        if (s.ws_net && s.s.warm_start)
-            if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap)) return e;
+        if (s.ws_net && s.s.warm_start && cap > 0)
+            if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap, every)) return e;
         BLZ_HIP(hipEventRecord(join[q], s.stream));
         BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
     }
@@ -232,7 +232,7 @@ int Batch::enqueue_step(hipStream_t st) {
             BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
             if (s.ws_net && s.s.warm_start >= 2 && it % s.s.warm_start == 0 && it < s.s.max_iter_solve)
                 if (int e = warm_start_apply(s, s.s.dt, s.ctl, s.done_host_dev, 0, (double)s.s.residual_tol,
-                                             s.s.max_iter_solve))
+                                             s.s.max_iter_solve, every))
                     return e;
             BLZ_HIP(hipEventRecord(join[q], s.stream));
             BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));

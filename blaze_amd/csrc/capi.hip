@@ -34,7 +34,6 @@ Solver::~Solver() {
     if (ctl_host) (void)hipHostFree(ctl_host);
     if (ws_net) delete ws_net;
     if (ws_buf) (void)hipFree(ws_buf);
-    if (ws_backup) (void)hipFree(ws_backup);
     if (zero_flag) (void)hipFree(zero_flag);
     if (done_host) (void)hipHostFree((void*)done_host);
     for (hipEvent_t e : batch_ev)
@@ -127,7 +126,7 @@ int Solver::init(const blz_settings& st_) {
     const size_t tail = (size_t)2 * L.Wp * L.Hp * 4;
     prb_floats = (6 * Np + tail) * 4 < ((size_t)1 << 32) && W <= 4096 ? 6 * Np + tail : 4 * Np;
     if (hipMalloc(&prb, prb_floats * sizeof(float)) != hipSuccess || hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess ||
-        hipMalloc(&red, (8 + RESIDUAL_BLOCKS) * sizeof(double)) != hipSuccess) {
+        hipMalloc(&red, (8 + 2 * RESIDUAL_BLOCKS) * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc failed for solver scratch");
         return BLZ_E_NOMEM;
     }
@@ -137,7 +136,7 @@ int Solver::init(const blz_settings& st_) {
     rhs.c[1] = prb + 3 * Np;
     // pressure zeroed once (:715-716): +0 inside, -0 in the ghost shell
     if (int e = launch_init_pressure(*this)) return e;
-    BLZ_HIP(hipMemset(red, 0, (8 + RESIDUAL_BLOCKS) * sizeof(double)));
+    BLZ_HIP(hipMemset(red, 0, (8 + 2 * RESIDUAL_BLOCKS) * sizeof(double)));
     BLZ_HIP(hipStreamSynchronize(stream));
     BLZ_HIP(hipHostMalloc((void**)&red_host, 8 * sizeof(double), hipHostMallocDefault));
     if (hipMalloc(&ctl, MAX_SOLVES * sizeof(SolveCtl)) != hipSuccess ||
@@ -342,18 +341,24 @@ int Solver::project_solve(float dt_) {
     SolveCtl* c = ctl + q;
     if (int e = launch_ctl_reset(*this, c)) return e;
     if (int e = launch_residual_check(*this, dt_, c, done_host_dev, seq, tol, cap, 0, 1)) return e;
-    if (ws_net && s.warm_start)
-        if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap)) return e;
+    int it0 = 0;
+    if (ws_net && s.warm_start && cap > 0) {
+        it0 = std::min(every, cap);
+        if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap, it0)) return e;
+    }
     int b = 0;
-    for (int it = 0; it < cap; ++b) {
+    for (int it = it0; it < cap; ++b) {
         if (*done_host == seq) break;
         const int n = std::min(every, cap - it);
         if (int e = sweeps(n, &c->done, b)) return e;
         it += n;
         if (int e = launch_residual_check(*this, dt_, c, done_host_dev, seq, tol, cap, n, 0)) return e;
         // warm_start = N >= 2: the learned correction again every N sweeps
-        if (ws_net && s.warm_start >= 2 && it % s.warm_start == 0 && it < cap)
-            if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap)) return e;
+        if (ws_net && s.warm_start >= 2 && it % s.warm_start == 0 && it < cap) {
+            const int k = std::min(every, cap - it);
+            if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap, k)) return e;
+            it += k;
+        }
         BLZ_HIP(hipEventRecord(batch_ev[b % (SOLVE_LOOKAHEAD + 1)], stream));
         if (b >= SOLVE_LOOKAHEAD) BLZ_HIP(hipEventSynchronize(batch_ev[(b - SOLVE_LOOKAHEAD) % (SOLVE_LOOKAHEAD + 1)]));
     }

@@ -940,19 +940,44 @@ int launch_divergence(Solver& s) {
 }
 
 template <int PG>
-static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop) {
+static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop, int set = 0) {
     const size_t cols = (size_t)(s.W / 2) * s.H;
     const dim3 grid(blocks_for(cols, 64)), block(64);
     const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
     const unsigned plane_bytes = (unsigned)(s.Np * 4);
+    const unsigned p0 = set ? 4u * plane_bytes : 0u; // pressure planes: home (0, 1) or spare (4, 5)
+    const unsigned nbytes = (unsigned)(set ? s.prb_floats * 4 : 4 * s.Np * 4);
     for (int l = 0; l < sweeps; ++l) {
         for (int c = 0; c < 2; ++c) {
-            hipLaunchKernelGGL((k_rbgs_pass<PG>), grid, block, 0, s.stream, s.prb, 4 * plane_bytes, c * plane_bytes,
-                               (c ^ 1) * plane_bytes, (2 + c) * plane_bytes, s.W, s.H, s.D, s.L.Wp, s.L.Hp, c, scale,
-                               tab, stop);
+            hipLaunchKernelGGL((k_rbgs_pass<PG>), grid, block, 0, s.stream, s.prb, nbytes, p0 + c * plane_bytes,
+                               p0 + (c ^ 1) * plane_bytes, (2 + c) * plane_bytes, s.W, s.H, s.D, s.L.Wp, s.L.Hp, c,
+                               scale, tab, stop);
             BLZ_HIP(hipGetLastError());
         }
     }
+    return BLZ_OK;
+}
+
+// `sweeps` two-pass sweeps on the spare pressure planes (the warm start's
+// uncorrected path; same operations as on the home planes)
+int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop) {
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
+    return rbgs_passes<8>(s, scale, sweeps, stop, 1);
+}
+
+// residual of the spare planes into partial, for the warm start's judge
+int launch_residual_partial(Solver& s, float dt, bool spare, double* partial, const int* stop, int* nb_out) {
+    const double cc = (double)dt / ((double)s.s.density * (double)s.dx * (double)s.dx);
+    const int nb = residual_blocks(s);
+    RB pp = s.p;
+    if (spare) {
+        pp.c[0] = s.prb + 4 * s.Np;
+        pp.c[1] = s.prb + 5 * s.Np;
+    }
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, pp, s.rhs, s.L, s.W, s.H, s.D, cc, cb_a(s),
+                       cb_b(s), partial, stop);
+    BLZ_HIP(hipGetLastError());
+    *nb_out = nb;
     return BLZ_OK;
 }
 

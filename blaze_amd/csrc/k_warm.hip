@@ -14,9 +14,12 @@
 //     alpha = <r, A e> / <A e, A e> (A x = n x - sum_nbr x, FP64 sums): the
 //     scale minimising the residual's 2-norm, so the network only has to
 //     get the shape of the correction right;
-//  4. evaluate the FP64 residual: the prediction is kept only if it is lower
-//     than before, otherwise p is restored (RBGS stays the judge and the
-//     fallback). The sweeps then run to the same tolerance as without it.
+//  4. the judge: the corrected pressure (home planes) and the uncorrected one
+//     (saved to the spare planes) both get the next batch of Gauss-Seidel
+//     sweeps, and the one with the lower FP64 residual continues -- on a
+//     rejection the solve continues exactly (bitwise) as without the
+//     prediction, having spent one batch of sweeps twice. The sweeps then
+//     run to the same tolerance as without it.
 // All decisions are taken on the device (no host round trip).
 #include "solver.h"
 
@@ -213,26 +216,37 @@ __global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int 
     }
 }
 
-// residual after the correction: keep it if lower, else mark for restore
-__global__ void __launch_bounds__(BS) k_res_final_ws(const double* __restrict__ partial, int n, SolveCtl* c,
-                                                     int* host_done, int seq, double tol, int cap) {
-    if (c->done) return;
+__device__ __forceinline__ double block_max(const double* __restrict__ partial, int n) {
     double mx = 0.0;
     for (int q = threadIdx.x; q < n; q += BS) mx = fmax(mx, partial[q]);
     for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
     __shared__ double sm[BS / 64];
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
     __syncthreads();
+    double r = sm[0];
+    for (int q = 1; q < (int)(BS / 64); ++q) r = fmax(r, sm[q]);
+    __syncthreads();
+    return r;
+}
+
+// the judge: residuals of the corrected (A) and uncorrected (B) pressure
+// after the same sweeps; the lower one continues (B wins ties)
+__global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, int na, const double* __restrict__ pb,
+                                                 int nb_, SolveCtl* c, int* host_done, int seq, double tol, int cap,
+                                                 int add) {
+    if (c->done) return;
+    const double ra = block_max(pa, na);
+    const double rb = block_max(pb, nb_);
     if (threadIdx.x == 0) {
-        double r = sm[0];
-        for (int q = 1; q < (int)(BS / 64); ++q) r = fmax(r, sm[q]);
-        c->checks += 1;
-        c->res_ws = r;
-        if (r < c->res) {
-            c->res = r;
+        c->iters += add;
+        c->checks += 2;
+        c->res_ws = ra;
+        if (ra < rb) {
+            c->res = ra;
             c->ws = 1;
             c->ws_kept += 1;
         } else {
+            c->res = rb;
             c->ws = 2;
             c->ws_rejected += 1;
         }
@@ -244,7 +258,26 @@ __global__ void __launch_bounds__(BS) k_res_final_ws(const double* __restrict__ 
     }
 }
 
-// restore the saved pressure when the prediction was rejected
+// the "given prediction" path keeps the immediate comparison (no sweeps)
+__global__ void __launch_bounds__(BS) k_res_final_ws(const double* __restrict__ partial, int n, SolveCtl* c,
+                                                     int* host_done, int seq, double tol, int cap) {
+    if (c->done) return;
+    const double r = block_max(partial, n);
+    if (threadIdx.x == 0) {
+        c->checks += 1;
+        c->res_ws = r;
+        if (r < c->res) {
+            c->res = r;
+            c->ws = 1;
+            c->ws_kept += 1;
+        } else {
+            c->ws = 2;
+            c->ws_rejected += 1;
+        }
+    }
+}
+
+// continue from the uncorrected pressure when the prediction was rejected
 __global__ void __launch_bounds__(BS) k_ws_restore(const SolveCtl* __restrict__ c, const float* __restrict__ backup,
                                                    float* __restrict__ p, size_t n) {
     if (c->ws != 2) return;
@@ -252,6 +285,9 @@ __global__ void __launch_bounds__(BS) k_ws_restore(const SolveCtl* __restrict__ 
 }
 
 __global__ void k_ws_zero(unsigned* a) { *a = 0u; }
+
+// a new application: no decision yet (the restore only follows this one's)
+__global__ void k_ws_begin(SolveCtl* c) { c->ws = 0; }
 
 inline unsigned nb(size_t n) { return (unsigned)((n + BS - 1) / BS); }
 
@@ -275,9 +311,9 @@ int warm_start_attach(Solver& s, const Net* net) {
         set_error("hipMalloc failed for warm-start buffers");
         return BLZ_E_NOMEM;
     }
-    if (!fused_available(s) && !s.ws_backup && hipMalloc(&s.ws_backup, 2 * s.Np * sizeof(float)) != hipSuccess) {
-        set_error("hipMalloc failed for the warm-start pressure backup");
-        return BLZ_E_NOMEM;
+    if (!fused_available(s)) {
+        set_error("grid too large for the warm start's second pressure plane set");
+        return BLZ_E_INVALID;
     }
     Net* v = new (std::nothrow) Net();
     if (!v) {
@@ -307,20 +343,22 @@ static int ws_restrict(Solver& s, float dt, const int* stop) {
     return BLZ_OK;
 }
 
-int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap) {
+int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge) {
     if (!s.ws_net) return BLZ_OK;
     const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
     float* res = s.ws_buf;
     float* mask = s.ws_buf + nn;
     unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
+    hipLaunchKernelGGL(k_ws_begin, dim3(1), dim3(1), 0, s.stream, c);
     if (int e = ws_restrict(s, dt, &c->done)) return e;
     hipLaunchKernelGGL(k_ws_pack, dim3(nb(nn * NET_IN_CH)), dim3(BS), 0, s.stream, res, mask, amax, s.ws_net->x0,
                        (int)nn);
     BLZ_HIP(hipGetLastError());
     if (int e = s.ws_net->forward()) return e;
-    // save the pressure planes, then add the prediction with its fitted scale
-    float* backup = s.ws_backup ? s.ws_backup : s.prb + 4 * s.Np;
-    BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
+    // the uncorrected pressure goes to the spare planes, the corrected one
+    // (prediction with its fitted scale) stays in the home planes
+    float* spare = s.prb + 4 * s.Np;
+    BLZ_HIP(hipMemcpyAsync(spare, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
     const size_t N = (size_t)s.W * s.H * s.D;
     float* e = s.tmp;
     hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out, s.Wc,
@@ -332,9 +370,21 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
     hipLaunchKernelGGL(k_ws_apply, dim3((unsigned)std::min<size_t>(nb(N), 4096)), dim3(BS), 0, s.stream, s.p, s.L,
                        s.W, s.H, s.D, e, s.red + 8, nbd, c);
     BLZ_HIP(hipGetLastError());
-    if (int e = launch_residual_ws(s, dt, c, host_done, seq, tol, cap)) return e;
+    // the judge: the same sweeps on both, the lower residual continues
+    int launches = 0, kind = 0;
+    const int keep = s.rbgs_variant;
+    s.rbgs_variant = 8; // in place on the home planes (the spare ones hold the other path)
+    int rc = launch_rbgs_sweeps(s, dt, judge, &c->done, &launches, &kind);
+    s.rbgs_variant = keep;
+    if (rc) return rc;
+    if (int e2 = launch_rbgs_spare(s, dt, judge, &c->done)) return e2;
+    int na = 0, nb2 = 0;
+    if (int e2 = launch_residual_partial(s, dt, false, s.red + 8, &c->done, &na)) return e2;
+    if (int e2 = launch_residual_partial(s, dt, true, s.red + 8 + RESIDUAL_BLOCKS, &c->done, &nb2)) return e2;
+    hipLaunchKernelGGL(k_ws_judge, dim3(1), dim3(BS), 0, s.stream, s.red + 8, na, s.red + 8 + RESIDUAL_BLOCKS, nb2, c,
+                       host_done, seq, tol, cap, judge);
     hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
-                       backup, s.prb, 2 * s.Np);
+                       spare, s.prb, 2 * s.Np);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
@@ -352,7 +402,7 @@ int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* 
     if (int e = launch_ctl_reset(s, c)) return e;
     if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, -1.0, 1 << 30, 0, 1)) return e;
     BLZ_HIP(hipMemcpyAsync(s.ws_net->out, host_pred, nn * sizeof(float), hipMemcpyHostToDevice, s.stream));
-    float* backup = s.ws_backup ? s.ws_backup : s.prb + 4 * s.Np;
+    float* backup = s.prb + 4 * s.Np;
     BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
     const size_t N = (size_t)s.W * s.H * s.D;
     float* e = s.tmp;

@@ -153,7 +153,6 @@ struct Solver {
     int Wc = 0, Hc = 0, Dc = 0; // coarse grid (4x), real cells
     int Wn = 0, Hn = 0, Dn = 0; // network grid (coarse rounded up to multiples of 4)
     float* ws_buf = nullptr;   // coarse residual, mask (Wn*Hn*Dn each), scale bits
-    float* ws_backup = nullptr; // pressure backup when there are no spare planes
     // captured (batched) steps: noise offsets of wind and turbulence read
     // from device memory instead of kernel arguments
     const float* dev_off = nullptr;
@@ -195,8 +194,12 @@ int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int 
 int launch_ctl_reset(Solver& s, SolveCtl* c);
 int launch_residual_ws(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
 int launch_residual_ws_final(Solver& s, SolveCtl* c, int* host_done, int seq, double tol, int cap, int nb_partial);
+int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop);
+int launch_residual_partial(Solver& s, float dt, bool spare, double* partial, const int* stop, int* nb_out);
+int residual_blocks(const Solver& s);
 int warm_start_attach(Solver& s, const Net* net);
-int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
+// learned correction + `judge` sweeps on both paths (counted once)
+int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge);
 int warm_start_input(Solver& s, float dt, float* host_in2);
 int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* before, double* after, int* kept);
 int launch_pressure_update(Solver& s, float dt);

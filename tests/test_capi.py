@@ -40,7 +40,8 @@ def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "layout.c"
     fields = {"blz_settings": [f[0] for f in B.Settings._fields_],
               "blz_stats": [f[0] for f in B.Stats._fields_],
-              "blz_particle_gen": [f[0] for f in B.ParticleGen._fields_]}
+              "blz_particle_gen": [f[0] for f in B.ParticleGen._fields_],
+              "blz_batch_stats": [f[0] for f in B.BatchStats._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
@@ -52,7 +53,8 @@ def test_struct_layouts_match_c(tmp_path):
     subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
     out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True)
                .stdout.splitlines())
-    for st, cls in (("blz_settings", B.Settings), ("blz_stats", B.Stats), ("blz_particle_gen", B.ParticleGen)):
+    for st, cls in (("blz_settings", B.Settings), ("blz_stats", B.Stats), ("blz_particle_gen", B.ParticleGen),
+                    ("blz_batch_stats", B.BatchStats)):
         assert int(out[st]) == C.sizeof(cls), st
         for f in fields[st]:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, f"{st}.{f}"
