@@ -38,7 +38,7 @@ struct Batch {
     std::vector<FusedScene> fs_h;
     std::vector<ResScene> rs_h;
     int fs_blocks = 0, rs_blocks = 0;
-    int every = 10, nbatch = 0;
+    int every = 10;
     int variant = 103;
     int nodes = 0;
     double capture_ms = 0.0, step_ms = 0.0;
@@ -108,11 +108,9 @@ int Batch::init(const blz_settings* st, int n) {
             set_error("batched scene too large for the fused sweep planes");
             return BLZ_E_INVALID;
         }
-        h->s.dev_off = off_dev + 2 * q;
         hipEvent_t e;
         BLZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         join.push_back(e);
-        nbatch = std::max(nbatch, h->s.s.max_iter_solve / every);
     }
     return build_tables(variant);
 }
@@ -211,29 +209,47 @@ int Batch::enqueue_step(hipStream_t st) {
         BLZ_HIP(hipEventRecord(join[q], s.stream));
         BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
     }
-    // every scene's sweeps and checks; a converged scene's blocks return at entry
-    for (int b = 0; b < nbatch; ++b) {
+    // every scene's sweeps and checks; a converged scene's blocks return at
+    // entry. it[q] follows scene q's sweep count as Solver::project_solve
+    // counts it (the warm start's judge batch included), so re-applications
+    // of the learned correction happen after the same sweeps as alone.
+    std::vector<int> it(n);
+    for (int q = 0; q < n; ++q) {
+        const Solver& s = sc[q]->s;
+        it[q] = (s.ws_net && s.s.warm_start && s.s.max_iter_solve > 0) ? every : 0;
+    }
+    auto pending = [&]() {
+        for (int q = 0; q < n; ++q)
+            if (it[q] < sc[q]->s.s.max_iter_solve) return true;
+        return false;
+    };
+    while (pending()) {
         for (int l = 0; l < every / 2; ++l) {
             if (int e = launch_fused_multi(st, variant, fs, n, fs_blocks)) return e;
             if (int e = launch_fused_multi(st, variant, fs + n, n, fs_blocks)) return e;
         }
         if (int e = launch_residual_multi(st, rs, n, rs_blocks, every, 0)) return e;
         // warm_start = N >= 2: each scene's learned correction every N sweeps
-        const int it = (b + 1) * every;
+        std::vector<char> again(n, 0);
         bool any = false;
         for (int q = 0; q < n; ++q) {
             const Solver& s = sc[q]->s;
-            any |= s.ws_net && s.s.warm_start >= 2 && it % s.s.warm_start == 0 && it < s.s.max_iter_solve;
+            if (it[q] >= s.s.max_iter_solve) continue;
+            it[q] += every;
+            again[q] = s.ws_net && s.s.warm_start >= 2 && it[q] % s.s.warm_start == 0 && it[q] < s.s.max_iter_solve;
+            any |= again[q] != 0;
         }
         if (!any) continue;
         BLZ_HIP(hipEventRecord(fork, st));
         for (int q = 0; q < n; ++q) {
             Solver& s = sc[q]->s;
             BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
-            if (s.ws_net && s.s.warm_start >= 2 && it % s.s.warm_start == 0 && it < s.s.max_iter_solve)
+            if (again[q]) {
                 if (int e = warm_start_apply(s, s.s.dt, s.ctl, s.done_host_dev, 0, (double)s.s.residual_tol,
                                              s.s.max_iter_solve, every))
                     return e;
+                it[q] += every;
+            }
             BLZ_HIP(hipEventRecord(join[q], s.stream));
             BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
         }
@@ -265,9 +281,13 @@ int Batch::capture(int which) {
     }
     auto t0 = std::chrono::steady_clock::now();
     hipGraph_t graph = nullptr;
+    // captured noise kernels read their offsets from device memory; outside
+    // the graph (blz_step on a scene handle) they take the time argument
+    for (int q = 0; q < n; ++q) sc[q]->s.dev_off = off_dev + 2 * q;
     BLZ_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
     int e = enqueue_step(root);
     hipError_t ce = hipStreamEndCapture(root, &graph);
+    for (int q = 0; q < n; ++q) sc[q]->s.dev_off = nullptr;
     if (e) {
         if (graph) (void)hipGraphDestroy(graph);
         return e;

@@ -96,3 +96,39 @@ def test_single_grid_autotune_keeps_results():
     g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
     o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
     compare(f"autotuned v{v}", g.field(B.FIELD_P), o.field(B.FIELD_P))
+
+
+def test_batch_with_warm_start_and_handle_steps():
+    """With the learned warm start inside the captured graph, each scene still
+    equals the same scene stepped alone with the same network; afterwards a
+    scene handle stepped on its own (outside the graph, wind noise from its
+    time argument) stays equal to the single solver."""
+    print("[batch] warm start", flush=True)
+    net = B.Net(seed=4)
+    cases = [scene(r, 10 + i) for i, r in enumerate(SHAPES[:3])]
+    for s, _ in cases:
+        s.warm_start = 20
+    b = B.Batch([c[0] for c in cases])
+    b.set_warm_start(net)
+    singles = [B.Solver(c[0]) for c in cases]
+    for i, (s, part) in enumerate(cases):
+        singles[i].set_warm_start(net)
+        b.scenes[i].add_source(part)
+        singles[i].add_source(part)
+    t = np.float32(0.0)
+    for step in range(3):
+        b.step(float(t))
+        for g in singles:
+            g.step(float(t))
+        t = np.float32(t + np.float32(cases[0][0].dt))
+    kept = sum(b.scenes[i].stats().warm_start_used for i in range(len(cases)))
+    rej = sum(b.scenes[i].stats().warm_start_fallbacks for i in range(len(cases)))
+    print(f"[batch] warm start kept {kept} rejected {rej}", flush=True)
+    for i in range(len(cases)):
+        assert b.scenes[i].stats().warm_start_fallbacks == singles[i].stats().warm_start_fallbacks
+        for w in (B.FIELD_T, B.FIELD_U, B.FIELD_V, B.FIELD_W, B.FIELD_P):
+            compare(f"batch+ws scene {i} field {w}", b.scenes[i].field(w), singles[i].field(w))
+    for i in range(len(cases)):
+        b.scenes[i].step(float(t))
+        singles[i].step(float(t))
+        compare(f"handle step scene {i}", b.scenes[i].field(B.FIELD_U), singles[i].field(B.FIELD_U))
