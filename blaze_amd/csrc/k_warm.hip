@@ -5,15 +5,18 @@
 // Before the sweeps of a residual-driven solve:
 //  1. restrict the residual the previous pressure leaves,
 //       r = scale*rhs + sum_nbr p - n p        (scale = rho dx^2 / dt)
-//     to a 4x coarser grid (sum over each 4x4x4 block / 4, which is the
-//     right-hand side of the coarse Poisson problem for a smooth error), with
-//     a solid mask (1 where a block touches a domain wall or lies outside);
+//     to a coarser grid, F = 4 or more (doubled until at most 32 coarse cells
+//     per axis, the scale the network was trained at): sum over each F^3
+//     block / F, which is the right-hand side of the coarse Poisson problem
+//     for a smooth error; with a solid mask (1 where a block touches a domain
+//     wall or lies outside);
 //  2. normalise by s = RMS of the coarse r and run the U-Net (bias-free with
 //     leaky ReLUs, so its output scales with s);
 //  3. save p, prolongate the prediction trilinearly to e and add alpha e,
-//     alpha = <r, A e> / <A e, A e> (A x = n x - sum_nbr x, FP64 sums): the
-//     scale minimising the residual's 2-norm, so the network only has to
-//     get the shape of the correction right;
+//     alpha = <r, e - mean e> / <e, A e> (A x = n x - sum_nbr x, FP64
+//     sums): the scale minimising the error's energy norm, so the network
+//     only has to get the shape of the correction right and no step can
+//     raise the A-norm of the error;
 //  4. the judge: the corrected pressure (home planes) and the uncorrected one
 //     (saved to the spare planes) both get the next batch of Gauss-Seidel
 //     sweeps, and the one with the lower FP64 residual continues -- on a
@@ -40,9 +43,9 @@ __device__ __forceinline__ float pget(const RB& p, const RBL& L, int i, int j, i
 }
 
 // thread per network cell (I, J, K)
-__global__ void __launch_bounds__(BS) k_ws_restrict(RB p, RB rhs, RBL L, int W, int H, int D, float scale, int Wc,
-                                                    int Hc, int Dc, int Wn, int Hn, int Dn, float* __restrict__ res,
-                                                    float* __restrict__ mask, unsigned* __restrict__ amax,
+__global__ void __launch_bounds__(BS) k_ws_restrict(RB p, RB rhs, RBL L, int W, int H, int D, float scale, int F,
+                                                    int Wc, int Hc, int Dc, int Wn, int Hn, int Dn,
+                                                    float* __restrict__ res, float* __restrict__ mask,
                                                     const int* __restrict__ stop) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= Wn * Hn * Dn || *stop) return;
@@ -54,9 +57,9 @@ __global__ void __launch_bounds__(BS) k_ws_restrict(RB p, RB rhs, RBL L, int W, 
     }
     float sum = 0.0f;
     bool wall = false;
-    for (int k = 4 * K; k < min(4 * K + 4, D); ++k)
-        for (int j = 4 * J; j < min(4 * J + 4, H); ++j)
-            for (int i = 4 * I; i < min(4 * I + 4, W); ++i) {
+    for (int k = F * K; k < min(F * K + F, D); ++k)
+        for (int j = F * J; j < min(F * J + F, H); ++j)
+            for (int i = F * I; i < min(F * I + F, W); ++i) {
                 const float pc = pget(p, L, i, j, k);
                 float r = scale * pget(rhs, L, i, j, k);
                 float n = 0.0f;
@@ -69,7 +72,7 @@ __global__ void __launch_bounds__(BS) k_ws_restrict(RB p, RB rhs, RBL L, int W, 
                 sum += r - n * pc;
                 wall |= n < 6.0f;
             }
-    const float rc = 0.25f * sum;
+    const float rc = sum / (float)F;
     res[t] = rc;
     mask[t] = wall ? 1.0f : 0.0f;
 }
@@ -118,13 +121,13 @@ __global__ void k_ws_export(const float* __restrict__ res, const float* __restri
 
 // e = trilinear(prediction) at every fine cell, natural order, into tmp
 __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W, int H, int D,
-                                                   const float* __restrict__ pred, int Wc, int Hc, int Dc, int Wn,
-                                                   int Hn, const SolveCtl* __restrict__ c) {
+                                                   const float* __restrict__ pred, float invF, int Wc, int Hc, int Dc,
+                                                   int Wn, int Hn, const SolveCtl* __restrict__ c) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= W * H * D || c->done) return;
     const int i = t % W, j = (t / W) % H, k = t / (W * H);
-    auto axis = [](int q, int n, int& a0, int& a1, float& w) {
-        float f = ((float)q + 0.5f) * 0.25f - 0.5f;
+    auto axis = [invF](int q, int n, int& a0, int& a1, float& w) {
+        float f = ((float)q + 0.5f) * invF - 0.5f;
         f = fminf(fmaxf(f, 0.0f), (float)(n - 1));
         a0 = (int)f;
         a1 = min(a0 + 1, n - 1);
@@ -145,14 +148,19 @@ __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W,
     e[t] = (v == v && fabsf(v) <= 3.4e38f) ? v : 0.0f;
 }
 
-// scale of the correction minimising the residual's 2-norm: with A x =
-// n x - sum_nbr x and r = scale*rhs - A p, alpha = <r, A e> / <A e, A e>;
-// per-block partial sums (FP64) of both products
+// scale of the correction minimising the error's energy norm: with A x =
+// n x - sum_nbr x (symmetric, positive semi-definite, constants in its null
+// space) and r = scale*rhs - A p, the error p* - p - alpha e has the least
+// A-norm at alpha = <r, e'> / <e, A e>, e' = e minus its mean (a constant
+// changes no residual). Unlike the residual-norm fit, this step can never
+// raise the A-norm of the error, i.e. it cannot add smooth error that the
+// sweeps remove slowly. Per-block partial sums (FP64) of <r,e>, <e,Ae>,
+// sum r and sum e.
 __global__ void __launch_bounds__(BS) k_ws_dots(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
                                                 const float* __restrict__ e, double* __restrict__ partial,
                                                 const SolveCtl* __restrict__ c) {
     if (c->done) return;
-    double num = 0.0, den = 0.0;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int n = W * H * D;
     for (int t = blockIdx.x * BS + threadIdx.x; t < n; t += gridDim.x * BS) {
         const int i = t % W, j = (t / W) % H, k = t / (W * H);
@@ -168,27 +176,22 @@ __global__ void __launch_bounds__(BS) k_ws_dots(RB p, RB rhs, RBL L, int W, int 
         if (k < D - 1) { r += pget(p, L, i, j, k + 1); ae -= e[t + W * H]; cnt += 1.0; }
         r -= cnt * pc;
         ae += cnt * ec;
-        num += r * ae;
-        den += ae * ae;
+        v[0] += r * ec;
+        v[1] += ec * ae;
+        v[2] += r;
+        v[3] += ec;
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        num += __shfl_xor(num, o, 64);
-        den += __shfl_xor(den, o, 64);
-    }
-    __shared__ double sn[BS / 64], sd[BS / 64];
-    if ((threadIdx.x & 63) == 0) {
-        sn[threadIdx.x >> 6] = num;
-        sd[threadIdx.x >> 6] = den;
-    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+    __shared__ double sv[4][BS / 64];
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 4; ++q) sv[q][threadIdx.x >> 6] = v[q];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int q = 0; q < (int)(BS / 64); ++q) {
-            a += sn[q];
-            b += sd[q];
-        }
-        partial[2 * blockIdx.x] = a;
-        partial[2 * blockIdx.x + 1] = b;
+    if (threadIdx.x < 4) {
+        double a = 0.0;
+        for (int w = 0; w < (int)(BS / 64); ++w) a += sv[threadIdx.x][w];
+        partial[4 * blockIdx.x + threadIdx.x] = a;
     }
 }
 
@@ -199,12 +202,14 @@ __global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int 
     if (c->done) return;
     __shared__ float alpha;
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
+        double re = 0.0, eae = 0.0, sr = 0.0, se = 0.0;
         for (int q = 0; q < nb; ++q) {
-            a += partial[2 * q];
-            b += partial[2 * q + 1];
+            re += partial[4 * q];
+            eae += partial[4 * q + 1];
+            sr += partial[4 * q + 2];
+            se += partial[4 * q + 3];
         }
-        const double al = b > 0.0 ? a / b : 0.0;
+        const double al = eae > 0.0 ? (re - sr * se / ((double)W * H * D)) / eae : 0.0;
         alpha = (al == al && fabs(al) < 1e30) ? (float)al : 0.0f;
     }
     __syncthreads();
@@ -300,9 +305,13 @@ int warm_start_attach(Solver& s, const Net* net) {
     }
     s.ws_src = net;
     if (!net) return BLZ_OK;
-    s.Wc = (s.W + 3) / 4;
-    s.Hc = (s.H + 3) / 4;
-    s.Dc = (s.D + 3) / 4;
+    // coarsening factor: 4, doubled until the coarse grid has at most 32
+    // cells per axis (the scale of the training grids)
+    s.ws_f = 4;
+    while (std::max(s.W, std::max(s.H, s.D)) > 32 * s.ws_f) s.ws_f *= 2;
+    s.Wc = (s.W + s.ws_f - 1) / s.ws_f;
+    s.Hc = (s.H + s.ws_f - 1) / s.ws_f;
+    s.Dc = (s.D + s.ws_f - 1) / s.ws_f;
     s.Wn = std::max(4, (s.Wc + 3) / 4 * 4);
     s.Hn = std::max(4, (s.Hc + 3) / 4 * 4);
     s.Dn = std::max(4, (s.Dc + 3) / 4 * 4);
@@ -336,8 +345,8 @@ static int ws_restrict(Solver& s, float dt, const int* stop) {
     unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
     const float scale = (s.s.density * s.dx * s.dx) / dt;
     hipLaunchKernelGGL(k_ws_zero, dim3(1), dim3(1), 0, s.stream, amax);
-    hipLaunchKernelGGL(k_ws_restrict, dim3(nb(nn)), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, s.Wc,
-                       s.Hc, s.Dc, s.Wn, s.Hn, s.Dn, res, mask, amax, stop);
+    hipLaunchKernelGGL(k_ws_restrict, dim3(nb(nn)), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale,
+                       s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, s.Dn, res, mask, stop);
     hipLaunchKernelGGL(k_ws_rms, dim3(1), dim3(1024), 0, s.stream, res, (int)nn, s.Wc * s.Hc * s.Dc, amax, stop);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
@@ -361,10 +370,10 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
     BLZ_HIP(hipMemcpyAsync(spare, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
     const size_t N = (size_t)s.W * s.H * s.D;
     float* e = s.tmp;
-    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out, s.Wc,
-                       s.Hc, s.Dc, s.Wn, s.Hn, c);
+    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out,
+                       1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
     const float scale = (s.s.density * s.dx * s.dx) / dt;
-    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 2);
+    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 4);
     hipLaunchKernelGGL(k_ws_dots, dim3(nbd), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, e,
                        s.red + 8, c);
     hipLaunchKernelGGL(k_ws_apply, dim3((unsigned)std::min<size_t>(nb(N), 4096)), dim3(BS), 0, s.stream, s.p, s.L,
@@ -406,10 +415,10 @@ int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* 
     BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
     const size_t N = (size_t)s.W * s.H * s.D;
     float* e = s.tmp;
-    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out, s.Wc,
-                       s.Hc, s.Dc, s.Wn, s.Hn, c);
+    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out,
+                       1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
     const float scale = (s.s.density * s.dx * s.dx) / dt;
-    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 2);
+    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 4);
     hipLaunchKernelGGL(k_ws_dots, dim3(nbd), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, e,
                        s.red + 8, c);
     hipLaunchKernelGGL(k_ws_apply, dim3((unsigned)std::min<size_t>(nb(N), 4096)), dim3(BS), 0, s.stream, s.p, s.L,

@@ -150,7 +150,8 @@ struct Solver {
     // learned warm start (attached network; activations of its own)
     const Net* ws_src = nullptr;
     Net* ws_net = nullptr;     // view: shares ws_src's weights, owns activations
-    int Wc = 0, Hc = 0, Dc = 0; // coarse grid (4x), real cells
+    int ws_f = 4;               // coarsening factor of the warm start
+    int Wc = 0, Hc = 0, Dc = 0; // coarse grid, real cells
     int Wn = 0, Hn = 0, Dn = 0; // network grid (coarse rounded up to multiples of 4)
     float* ws_buf = nullptr;   // coarse residual, mask (Wn*Hn*Dn each), scale bits
     // captured (batched) steps: noise offsets of wind and turbulence read

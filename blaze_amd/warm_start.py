@@ -67,10 +67,12 @@ def coarse_target(x, coarse_dims, sweeps):
     return out
 
 
-def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, log=None):
+def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, log=None, points=1, between=50):
     """Run `scene` (a .blz path) for `frames` frames stage by stage; at each
-    `every`-th frame record (input, target). `res` overrides the grid
-    (downsampled runs keep the domain and scale dx)."""
+    `every`-th frame record (input, target) at `points` moments of the
+    pressure solve: before its sweeps and after each further `between`
+    sweeps (the residuals a repeated correction meets). `res` overrides the
+    grid (downsampled runs keep the domain and scale dx)."""
     s, _, _, gen = parse_scene(scene)
     if res is not None:
         f = s.grid_res[0] / res[0]
@@ -81,6 +83,9 @@ def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, lo
             gen.pscale = gen.pscale * f
     s.cfl_number = 0.0
     s.warm_start = 0
+    if points > 1:
+        s.residual_tol = 0.0
+        s.max_iter_solve = between
     parts = generate_particles(gen)
     g = Solver(s)
     net = seed_net or Net(seed=0)
@@ -96,10 +101,13 @@ def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, lo
         for st in _force_stages(s):
             g.run_stage(st, float(t), dt)
         g.run_stage(STAGE_DIVERGENCE, float(t), dt)
-        if fr % every == 0:
-            x = g.warm_start_input(dt)
-            if np.isfinite(x).all() and np.abs(x[0]).max() > 0:
-                pairs.append((x, coarse_target(x, cdims, sweeps)))
+        for c in range(points):
+            if fr % every == 0:
+                x = g.warm_start_input(dt)
+                if np.isfinite(x).all() and np.abs(x[0]).max() > 0:
+                    pairs.append((x, coarse_target(x, cdims, sweeps)))
+            if c < points - 1:
+                g.run_stage(STAGE_PRESSURE, float(t), dt)
         g.run_stage(STAGE_PRESSURE, float(t), dt)
         g.run_stage(STAGE_PROJECT_UPDATE, float(t), dt)
         g.run_stage(STAGE_ADVECT, float(t), dt)
@@ -173,13 +181,15 @@ def main(argv=None):
 
     root = os.path.dirname(PKG)
     ap = argparse.ArgumentParser(description="train the warm-start U-Net on solver-generated pairs")
-    ap.add_argument("--frames", type=int, default=120)
+    ap.add_argument("--frames", type=int, default=100)
     ap.add_argument("--every", type=int, default=2)
     ap.add_argument("--sweeps", type=int, default=2000, help="coarse Gauss-Seidel sweeps per target")
-    ap.add_argument("--steps", type=int, default=1500)
-    ap.add_argument("--batch", type=int, default=4)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--points", type=int, default=3, help="pairs per solve: before the sweeps and after each --between sweeps")
+    ap.add_argument("--between", type=int, default=50)
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -190,7 +200,8 @@ def main(argv=None):
                                                                       (64, 64, 64))]
     pairs = []
     for path, res in scenes:
-        pairs += scene_pairs(path, a.frames, res=res, sweeps=a.sweeps, every=a.every, log=log)
+        pairs += scene_pairs(path, a.frames, res=res, sweeps=a.sweeps, every=a.every, log=log, points=a.points,
+                             between=a.between)
     log(f"[data] {len(pairs)} pairs in {time.time() - t0:.1f}s")
     rng = np.random.default_rng(a.seed)
     perm = rng.permutation(len(pairs))
