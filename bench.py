@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--warm-every", type=int, default=1,
                     help="warm_start setting of that leg (1: before the sweeps, N >= 2: also every N sweeps)")
+    ap.add_argument("--warm-parts", type=int, default=3,
+                    help="warm-start corrections: 1 network, 2 global, 3 both")
     ap.add_argument("--batched", action="store_true",
                     help="instead: 64 scenes of 32^3..128^3 in one captured-graph step (warm start on)")
     ap.add_argument("--scenes", type=int, default=64)
@@ -192,13 +194,21 @@ def main():
         import blaze_amd.warm_start as WS
 
         if os.path.exists(WS.WEIGHTS):
-            warm = run_leg(a, s, parts, WS.load_net())
+            net = WS.load_net()
+            warm = {"scene_tol": run_leg(a, s, parts, net)}
+            # equal residual: both solvers run to the tolerance the plain
+            # solve reached (median over the timed frames; same cap)
+            teq = float(np.median(plain["residuals"]))
+            warm["tol_eq"] = teq
+            warm["plain_eq"] = run_leg(a, s, parts, None, tol=teq)
+            warm["warm_eq"] = run_leg(a, s, parts, net, tol=teq)
     report(a, s, gen, parts, plain, warm)
 
 
-def run_leg(a, s, parts, net):
+def run_leg(a, s, parts, net, tol=None):
     """Warmup frames, then a.steps timed frames of the scene; net != None:
-    with the learned warm start (same tolerance and cap)."""
+    with the learned warm start (same tolerance and cap); tol: another
+    residual tolerance for the leg."""
     import numpy as np
 
     import blaze_amd as B
@@ -206,17 +216,19 @@ def run_leg(a, s, parts, net):
     W, H, D = s.grid_res
     N3 = W * H * D
     s = s.copy()
+    if tol is not None:
+        s.residual_tol = tol
     if net is not None:
         s.warm_start = a.warm_every
     g = B.Solver(s)
     g.set_timing(True)
     g.set_rbgs_variant(a.rbgs_variant)
     if net is not None:
-        g.set_warm_start(net)
+        g.set_warm_start(net, network=bool(a.warm_parts & 1), global_correction=bool(a.warm_parts & 2))
     t = np.float32(0.0)
     frame = s.source_range[0]
     sr_end = s.source_range[1]
-    tag = "warm" if net is not None else "plain"
+    tag = ("warm" if net is not None else "plain") + ("" if tol is None else "@eq")
 
     def one_frame():
         nonlocal t, frame
@@ -344,20 +356,26 @@ def report(a, s, gen, parts, L, warm):
         "cpu_baseline": cpu,
     }
     if warm is not None:
-        wv = N3 * warm["tot_sub"] / warm["elapsed"]
+        def leg(L):
+            return {"value": N3 * L["tot_sub"] / L["elapsed"], "ms_per_step": L["elapsed"] * 1e3 / a.steps,
+                    "rbgs_sweeps": L["tot_iters"], "substeps": L["tot_sub"],
+                    "mean_final_residual": sum(L["residuals"]) / max(1, len(L["residuals"])),
+                    "corrections_kept": L["kept"], "corrections_rejected": L["rejected"]}
+        ws, pe, we = leg(warm["scene_tol"]), leg(warm["plain_eq"]), leg(warm["warm_eq"])
         out["warm_start"] = {
-            "value": wv,
-            "unit": "voxel-steps/s",
-            "ratio_vs_plain": wv / value,
-            "ms_per_step": warm["elapsed"] * 1e3 / a.steps,
-            "rbgs_sweeps": warm["tot_iters"],
-            "final_residual": warm["residuals"][-1] if warm["residuals"] else None,
-            "plain_final_residual": residuals[-1] if residuals else None,
-            "mean_final_residual_ratio": (sum(warm["residuals"]) / max(1e-300, sum(residuals))
-                                          if residuals and warm["residuals"] else None),
-            "corrections_kept": warm["kept"],
-            "corrections_rejected": warm["rejected"],
-            "setting": f"warm_start={a.warm_every}, weights blaze_amd/warm_start_weights.npy",
+            "setting": f"warm_start={a.warm_every}, corrections: "
+                       + {1: "network", 2: "global", 3: "network + global"}[a.warm_parts]
+                       + ", weights blaze_amd/warm_start_weights.npy",
+            "at_scene_tolerance": dict(ws, ratio_vs_plain=ws["value"] / value,
+                                       plain_mean_final_residual=sum(residuals) / max(1, len(residuals))),
+            "at_equal_residual": {
+                "tolerance": warm["tol_eq"],
+                "note": "both solvers to the median final residual of the plain run (same cap)",
+                "plain": pe,
+                "warm": we,
+                "speedup": we["value"] / pe["value"],
+                "sweeps_ratio": pe["rbgs_sweeps"] / max(1, we["rbgs_sweeps"]),
+            },
         }
     print(json.dumps(out), flush=True)
 

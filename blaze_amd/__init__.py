@@ -131,6 +131,7 @@ def _load():
         "blz_net_train_step": ([vp, P(C.c_float), P(C.c_float), P(C.c_int), C.c_float, C.c_int,
                                 P(C.c_double)], C.c_int),
         "blz_set_warm_start": ([vp, vp], C.c_int),
+        "blz_set_warm_start_parts": ([vp, C.c_int], C.c_int),
         "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
         "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
         "blz_warm_start_apply": ([vp, C.c_float, P(C.c_float), P(C.c_double), P(C.c_double), P(C.c_int)], C.c_int),
@@ -289,11 +290,15 @@ class Solver:
         cand = [2, 4, 8, 102, 103, 202, 203, 402, 403]
         return v.value, {c: ms[i] for i, c in enumerate(cand) if ms[i] > 0}
 
-    def set_warm_start(self, net):
+    def set_warm_start(self, net, network: bool = True, global_correction: bool = True):
         """Attach a Net (None detaches); used when settings.warm_start != 0
-        and residual_tol > 0."""
+        and residual_tol > 0. network / global_correction select the parts
+        of the correction (at least one)."""
         self._net = net
         _check(_lib.blz_set_warm_start(self._h, net._h if net is not None else None), "blz_set_warm_start")
+        if net is not None:
+            _check(_lib.blz_set_warm_start_parts(self._h, int(network) | (2 * int(global_correction))),
+                   "blz_set_warm_start_parts")
 
     def warm_start_dims(self):
         nd, cd = (C.c_int * 3)(), (C.c_int * 3)()

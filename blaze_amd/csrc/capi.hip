@@ -661,6 +661,11 @@ int blz_set_warm_start(blz_solver* h, blz_net* net) {
     return blz::warm_start_attach(h->s, net ? &net->n : nullptr);
 }
 
+int blz_set_warm_start_parts(blz_solver* h, int parts) {
+    CHECK_HANDLE(h);
+    return blz::warm_start_set_parts(h->s, parts);
+}
+
 int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]) {
     CHECK_HANDLE(h);
     Solver& s = h->s;

@@ -6,6 +6,7 @@
 // -ffp-contract=off and IEEE division/sqrt, results are bit-identical to
 // a serial CPU evaluation of the same formulas.
 #pragma once
+#include <cmath>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -40,11 +41,32 @@ __device__ __forceinline__ f3 g2o(const Gv& g, float dx, int i, int j, int k) {
     return mk3(((float)i + g.ox) * dx, ((float)j + g.oy) * dx, ((float)k + g.oz) * dx);
 }
 
+// object -> grid units: p / dx (IEEE division), or p * 2^k when dx = 2^-k,
+// which gives the same bits (scaling by a power of two is exact, and
+// overflow, zeros, infinities and NaNs come out alike)
+struct DivDx {
+    float dx;
+    __device__ __forceinline__ float operator()(float v) const { return v / dx; }
+};
+struct MulDx {
+    float inv;
+    __device__ __forceinline__ float operator()(float v) const { return v * inv; }
+};
+// true (and inv = 1/dx, exact) when dx is a power of two
+inline bool dx_pow2(float dx, float* inv) {
+    int e = 0;
+    const float m = std::frexp(dx, &e);
+    if (!(dx > 0.0f) || m != 0.5f) return false;
+    *inv = std::ldexp(1.0f, 1 - e);
+    return std::isfinite(*inv) && *inv * dx == 1.0f;
+}
+
 // trilinear_interpolate, src/Grid.h:94-108
-__device__ __forceinline__ float trilinear(const Gv& g, float dx, f3 p) {
-    const float gX = fmaxf(fminf(p.x / dx - g.ox, (float)g.nx - 1.001f), 0.0f);
-    const float gY = fmaxf(fminf(p.y / dx - g.oy, (float)g.ny - 1.001f), 0.0f);
-    const float gZ = fmaxf(fminf(p.z / dx - g.oz, (float)g.nz - 1.001f), 0.0f);
+template <class S>
+__device__ __forceinline__ float trilinear_s(const Gv& g, const S& sc, f3 p) {
+    const float gX = fmaxf(fminf(sc(p.x) - g.ox, (float)g.nx - 1.001f), 0.0f);
+    const float gY = fmaxf(fminf(sc(p.y) - g.oy, (float)g.ny - 1.001f), 0.0f);
+    const float gZ = fmaxf(fminf(sc(p.z) - g.oz, (float)g.nz - 1.001f), 0.0f);
     const int i = (int)gX, j = (int)gY, k = (int)gZ;
     const float tx = gX - (float)i, ty = gY - (float)j, tz = gZ - (float)k;
     const int sy = g.nx, sz = g.nx * g.ny;
@@ -55,6 +77,7 @@ __device__ __forceinline__ float trilinear(const Gv& g, float dx, f3 p) {
     const float c11 = b[sy + sz] * (1.0f - tx) + b[sy + sz + 1] * tx;
     return (1.0f - tz) * ((1.0f - ty) * c00 + ty * c10) + tz * ((1.0f - ty) * c01 + ty * c11);
 }
+__device__ __forceinline__ float trilinear(const Gv& g, float dx, f3 p) { return trilinear_s(g, DivDx{dx}, p); }
 
 // cerp / cerpc, src/Grid.h:148-163 (cubic Lagrange on nodes -1,0,1,2)
 __device__ __forceinline__ float cerp(float q0, float q1, float q2, float q3, float s) {
@@ -83,11 +106,11 @@ __device__ __forceinline__ float cw_apply(const CW& c, float q0, float q1, float
 
 // tricubic_interpolate, src/Grid.h:165-222: x cerps, then y, then z; with
 // clamp, every 1-D stage is clamped at 0 (cerpc).
-template <bool CLAMP>
-__device__ __forceinline__ float tricubic(const Gv& g, float dx, f3 p) {
-    const float gX = fmaxf(fminf(p.x / dx - g.ox, (float)g.nx - 1.001f), 0.0f);
-    const float gY = fmaxf(fminf(p.y / dx - g.oy, (float)g.ny - 1.001f), 0.0f);
-    const float gZ = fmaxf(fminf(p.z / dx - g.oz, (float)g.nz - 1.001f), 0.0f);
+template <bool CLAMP, class S>
+__device__ __forceinline__ float tricubic_s(const Gv& g, const S& sc, f3 p) {
+    const float gX = fmaxf(fminf(sc(p.x) - g.ox, (float)g.nx - 1.001f), 0.0f);
+    const float gY = fmaxf(fminf(sc(p.y) - g.oy, (float)g.ny - 1.001f), 0.0f);
+    const float gZ = fmaxf(fminf(sc(p.z) - g.oz, (float)g.nz - 1.001f), 0.0f);
     const int i1 = (int)gX, j1 = (int)gY, k1 = (int)gZ;
     const int sy = g.nx, sz = g.nx * g.ny;
     const int ix[4] = {max(i1 - 1, 0), i1, i1 + 1, min(i1 + 2, g.nx - 1)};
@@ -109,6 +132,10 @@ __device__ __forceinline__ float tricubic(const Gv& g, float dx, f3 p) {
     }
     float v = cw_apply(cz, rz[0], rz[1], rz[2], rz[3]);
     return CLAMP ? fmaxf(v, 0.0f) : v;
+}
+template <bool CLAMP>
+__device__ __forceinline__ float tricubic(const Gv& g, float dx, f3 p) {
+    return tricubic_s<CLAMP>(g, DivDx{dx}, p);
 }
 
 // fit, src/FluidSolver.cu:250-252

@@ -5,6 +5,8 @@
 // launched for T (clamped), U, V, W (:942-945), then swap_grids_kernel
 // (:458-465). Every grid reads the same front velocities and writes its back
 // buffer; the host swaps front/back pointers afterwards (no device kernel).
+// Positions are scaled to grid units by p / dx, or by p * 2^k when dx =
+// 2^-k (the same bits, without 30 IEEE divisions per cell).
 #include "solver.h"
 
 namespace blz {
@@ -13,44 +15,86 @@ namespace {
 
 constexpr unsigned BS = 256;
 
-template <bool CLAMP>
-__global__ void __launch_bounds__(BS) k_advect(Gv U, Gv V, Gv W, Gv G, float* __restrict__ out, float dx, float a,
-                                               float b, float c1, float c2, float c3, float dt) {
+// Ralston RK3 back-trace of cell (i, j, k) of grid G through the front
+// velocities (trilinear), in object space
+template <class S>
+__device__ __forceinline__ f3 backtrace(const Gv& U, const Gv& V, const Gv& W, const Gv& G, const S& sc, float dx,
+                                        int i, int j, int k, float a, float b, float c1, float c2, float c3,
+                                        float dt) {
+    f3 pos = g2o(G, dx, i, j, k);
+    const f3 k1 = mk3(trilinear_s(U, sc, pos), trilinear_s(V, sc, pos), trilinear_s(W, sc, pos));
+    const f3 p1 = mk3(pos.x - a * k1.x, pos.y - a * k1.y, pos.z - a * k1.z);
+    const f3 k2 = mk3(trilinear_s(U, sc, p1), trilinear_s(V, sc, p1), trilinear_s(W, sc, p1));
+    const f3 p2 = mk3(pos.x - b * k2.x, pos.y - b * k2.y, pos.z - b * k2.z);
+    const f3 k3 = mk3(trilinear_s(U, sc, p2), trilinear_s(V, sc, p2), trilinear_s(W, sc, p2));
+    pos.x -= dt * (c1 * k1.x + c2 * k2.x + c3 * k3.x);
+    pos.y -= dt * (c1 * k1.y + c2 * k2.y + c3 * k3.y);
+    pos.z -= dt * (c1 * k1.z + c2 * k2.z + c3 * k3.z);
+    return pos;
+}
+
+template <bool CLAMP, class S>
+__global__ void __launch_bounds__(BS) k_advect(Gv U, Gv V, Gv W, Gv G, float* __restrict__ out, S sc, float dx,
+                                               float a, float b, float c1, float c2, float c3, float dt) {
     const int n = G.nx * G.ny * G.nz;
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int i = t % G.nx, j = (t / G.nx) % G.ny, k = t / (G.nx * G.ny);
-    f3 pos = g2o(G, dx, i, j, k);
-    const f3 k1 = mk3(trilinear(U, dx, pos), trilinear(V, dx, pos), trilinear(W, dx, pos));
-    const f3 p1 = mk3(pos.x - a * k1.x, pos.y - a * k1.y, pos.z - a * k1.z);
-    const f3 k2 = mk3(trilinear(U, dx, p1), trilinear(V, dx, p1), trilinear(W, dx, p1));
-    const f3 p2 = mk3(pos.x - b * k2.x, pos.y - b * k2.y, pos.z - b * k2.z);
-    const f3 k3 = mk3(trilinear(U, dx, p2), trilinear(V, dx, p2), trilinear(W, dx, p2));
-    pos.x -= dt * (c1 * k1.x + c2 * k2.x + c3 * k3.x);
-    pos.y -= dt * (c1 * k1.y + c2 * k2.y + c3 * k3.y);
-    pos.z -= dt * (c1 * k1.z + c2 * k2.z + c3 * k3.z);
-    out[t] = tricubic<CLAMP>(G, dx, pos);
+    const f3 pos = backtrace(U, V, W, G, sc, dx, i, j, k, a, b, c1, c2, c3, dt);
+    out[t] = tricubic_s<CLAMP>(G, sc, pos);
+}
+
+// temperature and smoke density share the cell-centred grid, hence the
+// back-trace: one trace, two clamped tricubic samples (the same values as
+// two separate launches)
+template <class S>
+__global__ void __launch_bounds__(BS) k_advect_pair(Gv U, Gv V, Gv W, Gv G1, Gv G2, float* __restrict__ out1,
+                                                    float* __restrict__ out2, S sc, float dx, float a, float b,
+                                                    float c1, float c2, float c3, float dt) {
+    const int n = G1.nx * G1.ny * G1.nz;
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % G1.nx, j = (t / G1.nx) % G1.ny, k = t / (G1.nx * G1.ny);
+    const f3 pos = backtrace(U, V, W, G1, sc, dx, i, j, k, a, b, c1, c2, c3, dt);
+    out1[t] = tricubic_s<true>(G1, sc, pos);
+    out2[t] = tricubic_s<true>(G2, sc, pos);
+}
+
+template <class S>
+int advect_all(Solver& s, const S& sc, float dt) {
+    const Gv U = s.f[F_U].view(), V = s.f[F_V].view(), W = s.f[F_W].view();
+    const float a = 0.5f * dt, b = 0.75f * dt;
+    const float c1 = 2.0f / 9.0f, c2 = 3.0f / 9.0f, c3 = 4.0f / 9.0f;
+    for (int g = F_T; g < F_N; ++g) {
+        Field& G = s.f[g];
+        const size_t n = G.count();
+        if (g == F_S && !s.smoke) continue;
+        const Gv tv = s.f[F_T].view(), dv = s.f[F_S].view();
+        const bool pair = s.smoke && tv.nx == dv.nx && tv.ny == dv.ny && tv.nz == dv.nz && tv.ox == dv.ox &&
+                          tv.oy == dv.oy && tv.oz == dv.oz;
+        if (g == F_S && pair) continue; // with T
+        if (g == F_T && pair) {
+            Field& D = s.f[F_S];
+            hipLaunchKernelGGL(k_advect_pair<S>, dim3(blocks_for(n, BS)), dim3(BS), 0, s.stream, U, V, W, G.view(),
+                               D.view(), G.back(), D.back(), sc, s.dx, a, b, c1, c2, c3, dt);
+        } else if (g == F_T || g == F_S) {
+            hipLaunchKernelGGL((k_advect<true, S>), dim3(blocks_for(n, BS)), dim3(BS), 0, s.stream, U, V, W, G.view(),
+                               G.back(), sc, s.dx, a, b, c1, c2, c3, dt);
+        } else {
+            hipLaunchKernelGGL((k_advect<false, S>), dim3(blocks_for(n, BS)), dim3(BS), 0, s.stream, U, V, W,
+                               G.view(), G.back(), sc, s.dx, a, b, c1, c2, c3, dt);
+        }
+        BLZ_HIP(hipGetLastError());
+    }
+    return BLZ_OK;
 }
 
 } // namespace
 
 int launch_advect(Solver& s, float dt) {
-    const Gv U = s.f[F_U].view(), V = s.f[F_V].view(), W = s.f[F_W].view();
-    const float a = 0.5f * dt, b = 0.75f * dt;
-    const float c1 = 2.0f / 9.0f, c2 = 3.0f / 9.0f, c3 = 4.0f / 9.0f;
-    for (int g = F_T; g < F_N; ++g) {
-        if (g == F_S && !s.smoke) continue;
-        Field& G = s.f[g];
-        const size_t n = G.count();
-        const bool clamp = (g == F_T || g == F_S);
-        if (clamp)
-            hipLaunchKernelGGL(k_advect<true>, dim3(blocks_for(n, BS)), dim3(BS), 0, s.stream, U, V, W, G.view(),
-                               G.back(), s.dx, a, b, c1, c2, c3, dt);
-        else
-            hipLaunchKernelGGL(k_advect<false>, dim3(blocks_for(n, BS)), dim3(BS), 0, s.stream, U, V, W, G.view(),
-                               G.back(), s.dx, a, b, c1, c2, c3, dt);
-        BLZ_HIP(hipGetLastError());
-    }
+    float inv = 0.0f;
+    const int rc = dx_pow2(s.dx, &inv) ? advect_all(s, MulDx{inv}, dt) : advect_all(s, DivDx{s.dx}, dt);
+    if (rc) return rc;
     for (int g = F_T; g < F_N; ++g)
         if (g != F_S || s.smoke) s.f[g].swap();
     return BLZ_OK;

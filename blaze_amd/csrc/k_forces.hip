@@ -25,30 +25,33 @@ __global__ void __launch_bounds__(BS) k_scale(float* __restrict__ a, size_t n, f
 }
 
 // V faces: V += dt * ((beta * T~) * g), T~ trilinear at the face position
+template <class S>
 __global__ void __launch_bounds__(BS) k_buoyancy(Gv T, float* __restrict__ V, int vnx, int vny, int vnz, float vox,
-                                                 float voy, float voz, float dx, float beta, float g, float dt) {
+                                                 float voy, float voz, S sc, float dx, float beta, float g,
+                                                 float dt) {
     const int n = vnx * vny * vnz;
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int i = t % vnx, j = (t / vnx) % vny, k = t / (vnx * vny);
     const f3 pos = mk3(((float)i + vox) * dx, ((float)j + voy) * dx, ((float)k + voz) * dx);
-    const float temp = trilinear(T, dx, pos);
+    const float temp = trilinear_s(T, sc, pos);
     const float force = beta * temp * g;
     V[t] += dt * force;
 }
 
 // computeVorticity :174-201 at cell (i,j,k) of the T grid
-__device__ __forceinline__ f3 vort_at(const Gv& T, const Gv& U, const Gv& V, const Gv& W, float dx, int i, int j,
-                                      int k) {
+template <class S>
+__device__ __forceinline__ f3 vort_at(const Gv& T, const Gv& U, const Gv& V, const Gv& W, const S& sc, float dx,
+                                      int i, int j, int k) {
     const float x0 = ((float)(i - 1) + T.ox) * dx, x1 = ((float)i + T.ox) * dx, x2 = ((float)(i + 1) + T.ox) * dx;
     const float y0 = ((float)(j - 1) + T.oy) * dx, y1 = ((float)j + T.oy) * dx, y2 = ((float)(j + 1) + T.oy) * dx;
     const float z0 = ((float)(k - 1) + T.oz) * dx, z1 = ((float)k + T.oz) * dx, z2 = ((float)(k + 1) + T.oz) * dx;
-    const float w121 = trilinear(W, dx, mk3(x1, y2, z1)), w101 = trilinear(W, dx, mk3(x1, y0, z1));
-    const float v112 = trilinear(V, dx, mk3(x1, y1, z2)), v110 = trilinear(V, dx, mk3(x1, y1, z0));
-    const float u112 = trilinear(U, dx, mk3(x1, y1, z2)), u110 = trilinear(U, dx, mk3(x1, y1, z0));
-    const float w211 = trilinear(W, dx, mk3(x2, y1, z1)), w011 = trilinear(W, dx, mk3(x0, y1, z1));
-    const float v211 = trilinear(V, dx, mk3(x2, y1, z1)), v011 = trilinear(V, dx, mk3(x0, y1, z1));
-    const float u121 = trilinear(U, dx, mk3(x1, y2, z1)), u101 = trilinear(U, dx, mk3(x1, y0, z1));
+    const float w121 = trilinear_s(W, sc, mk3(x1, y2, z1)), w101 = trilinear_s(W, sc, mk3(x1, y0, z1));
+    const float v112 = trilinear_s(V, sc, mk3(x1, y1, z2)), v110 = trilinear_s(V, sc, mk3(x1, y1, z0));
+    const float u112 = trilinear_s(U, sc, mk3(x1, y1, z2)), u110 = trilinear_s(U, sc, mk3(x1, y1, z0));
+    const float w211 = trilinear_s(W, sc, mk3(x2, y1, z1)), w011 = trilinear_s(W, sc, mk3(x0, y1, z1));
+    const float v211 = trilinear_s(V, sc, mk3(x2, y1, z1)), v011 = trilinear_s(V, sc, mk3(x0, y1, z1));
+    const float u121 = trilinear_s(U, sc, mk3(x1, y2, z1)), u101 = trilinear_s(U, sc, mk3(x1, y0, z1));
     const float t = 2.0f * dx;
     return mk3((w121 - w101 - (v112 - v110)) / t, (u112 - u110 - (w211 - w011)) / t,
                (v211 - v011 - (u121 - u101)) / t);
@@ -57,13 +60,14 @@ __device__ __forceinline__ f3 vort_at(const Gv& T, const Gv& U, const Gv& V, con
 // omega and |omega| for cells 1..N-2 on each axis (tmp planes 0..3). The
 // reference evaluates omega 7 times per cell; it is a pure function of the
 // cell, so evaluating it once and reading it back gives the same bits.
-__global__ void __launch_bounds__(BS) k_vort_omega(Gv T, Gv U, Gv V, Gv W, float dx, float* __restrict__ tmp,
+template <class S>
+__global__ void __launch_bounds__(BS) k_vort_omega(Gv T, Gv U, Gv V, Gv W, S sc, float dx, float* __restrict__ tmp,
                                                    int nW, int nH, int nD) {
     const int iw = nW - 2, ih = nH - 2, id = nD - 2;
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= iw * ih * id) return;
     const int i = 1 + t % iw, j = 1 + (t / iw) % ih, k = 1 + t / (iw * ih);
-    const f3 om = vort_at(T, U, V, W, dx, i, j, k);
+    const f3 om = vort_at(T, U, V, W, sc, dx, i, j, k);
     const size_t N = (size_t)nW * nH * nD;
     const size_t c = (size_t)i + (size_t)j * nW + (size_t)k * nW * nH;
     tmp[c] = om.x;
@@ -73,7 +77,8 @@ __global__ void __launch_bounds__(BS) k_vort_omega(Gv T, Gv U, Gv V, Gv W, float
 }
 
 // confinement increments 0.5*dt*f for active cells 2..N-3 (tmp planes 4..6)
-__global__ void __launch_bounds__(BS) k_vort_force(Gv T, Gv U, Gv V, Gv W, float dx, float* __restrict__ tmp,
+template <class S>
+__global__ void __launch_bounds__(BS) k_vort_force(Gv T, Gv U, Gv V, Gv W, S sc, float dx, float* __restrict__ tmp,
                                                    int nW, int nH, int nD, float conf, float dt, float eps) {
     const int iw = nW - 4, ih = nH - 4, id = nD - 4;
     const int t = blockIdx.x * BS + threadIdx.x;
@@ -88,7 +93,7 @@ __global__ void __launch_bounds__(BS) k_vort_force(Gv T, Gv U, Gv V, Gv W, float
     const f3 gn = mk3((nrm[c + 1] - nrm[c - 1]) / t2, (nrm[c + sy] - nrm[c - sy]) / t2,
                       (nrm[c + sz] - nrm[c - sz]) / t2);
     const f3 pos = g2o(T, dx, i, j, k);
-    const f3 vel = mk3(trilinear(U, dx, pos), trilinear(V, dx, pos), trilinear(W, dx, pos));
+    const f3 vel = mk3(trilinear_s(U, sc, pos), trilinear_s(V, sc, pos), trilinear_s(W, sc, pos));
     const float speed = len3(vel);
     const float onorm = nrm[c] + eps;
     if (onorm > speed) om = mk3(speed * om.x / onorm, speed * om.y / onorm, speed * om.z / onorm);
@@ -237,11 +242,17 @@ int launch_drag(Solver& s, float dt) {
     return BLZ_OK;
 }
 
-int launch_buoyancy(Solver& s, float dt) {
+template <class S>
+static int buoyancy(Solver& s, const S& sc, float dt) {
     const Field& V = s.f[F_V];
-    LAUNCH(k_buoyancy, V.count(), s.f[F_T].view(), V.front(), V.nx, V.ny, V.nz, V.ox, V.oy, V.oz, s.dx,
+    LAUNCH(k_buoyancy<S>, V.count(), s.f[F_T].view(), V.front(), V.nx, V.ny, V.nz, V.ox, V.oy, V.oz, sc, s.dx,
            s.s.buoyancy, s.s.gravity, dt);
     return BLZ_OK;
+}
+
+int launch_buoyancy(Solver& s, float dt) {
+    float inv = 0.0f;
+    return dx_pow2(s.dx, &inv) ? buoyancy(s, MulDx{inv}, dt) : buoyancy(s, DivDx{s.dx}, dt);
 }
 
 static int gather3(Solver& s, bool range2) {
@@ -263,9 +274,20 @@ int launch_vorticity(Solver& s, float dt) {
     if (s.W < 5 || s.H < 5 || s.D < 5) return BLZ_OK; // no active cell
     const float eps = 1e-20f * (1.0f / (s.dx * dt));
     const Gv T = s.f[F_T].view(), U = s.f[F_U].view(), V = s.f[F_V].view(), W = s.f[F_W].view();
-    LAUNCH(k_vort_omega, (size_t)(s.W - 2) * (s.H - 2) * (s.D - 2), T, U, V, W, s.dx, s.tmp, s.W, s.H, s.D);
-    LAUNCH(k_vort_force, (size_t)(s.W - 4) * (s.H - 4) * (s.D - 4), T, U, V, W, s.dx, s.tmp, s.W, s.H, s.D,
-           s.s.vorticity_conf, dt, eps);
+    float inv = 0.0f;
+    if (dx_pow2(s.dx, &inv)) {
+        const MulDx sc{inv};
+        LAUNCH(k_vort_omega<MulDx>, (size_t)(s.W - 2) * (s.H - 2) * (s.D - 2), T, U, V, W, sc, s.dx, s.tmp, s.W,
+               s.H, s.D);
+        LAUNCH(k_vort_force<MulDx>, (size_t)(s.W - 4) * (s.H - 4) * (s.D - 4), T, U, V, W, sc, s.dx, s.tmp, s.W,
+               s.H, s.D, s.s.vorticity_conf, dt, eps);
+    } else {
+        const DivDx sc{s.dx};
+        LAUNCH(k_vort_omega<DivDx>, (size_t)(s.W - 2) * (s.H - 2) * (s.D - 2), T, U, V, W, sc, s.dx, s.tmp, s.W,
+               s.H, s.D);
+        LAUNCH(k_vort_force<DivDx>, (size_t)(s.W - 4) * (s.H - 4) * (s.D - 4), T, U, V, W, sc, s.dx, s.tmp, s.W,
+               s.H, s.D, s.s.vorticity_conf, dt, eps);
+    }
     return gather3(s, true);
 }
 

@@ -17,6 +17,16 @@
 //     sums): the scale minimising the error's energy norm, so the network
 //     only has to get the shape of the correction right and no step can
 //     raise the A-norm of the error;
+//  3b. the global correction: block sums of the residual left after step 3
+//     over a grid of at most 16^3 blocks, the Galerkin coarse problem for
+//     piecewise-constant blocks solved by conjugate gradients in one
+//     workgroup, prolongated and added with its energy-optimal scale the
+//     same way. It sets the smoothest error modes, which Gauss-Seidel
+//     barely reduces on large grids (at 256^3 a few hundred sweeps leave
+//     them almost untouched) and on which a local network is least
+//     reliable: without it, a prediction whose smooth part is slightly off
+//     can win the short judge below and still leave the solve worse after
+//     hundreds of sweeps;
 //  4. the judge: the corrected pressure (home planes) and the uncorrected one
 //     (saved to the spare planes) both get the next batch of Gauss-Seidel
 //     sweeps, and the one with the lower FP64 residual continues -- on a
@@ -38,43 +48,96 @@ __device__ __forceinline__ bf16 f2bf_w(float f) {
     return (bf16)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-__device__ __forceinline__ float pget(const RB& p, const RBL& L, int i, int j, int k) {
-    return p.c[(i + j) & 1][L.at(i, j, k)];
+
+// r = scale*rhs + sum_nbr p - n p for every cell, FP32, written in natural
+// order (x fastest). Thread per (x pair m, row j, k group g), reading the
+// float4 groups of the split layout (cells 2m and 2m+1 of 4 k); a missing
+// neighbour is a -0 ghost, the identity of the sum, so all six slots are
+// summed in the order i-1, i+1, j-1, j+1, k-1, k+1.
+__global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
+                                                 float* __restrict__ r, const int* __restrict__ stop) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    if (*stop) return;
+    const int Wh = W >> 1;
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= Wh * H * L.Dg) return;
+    const int m = t % Wh, j = (t / Wh) % H, g = t / (Wh * H);
+    const int c0 = j & 1, c1 = c0 ^ 1;
+    const f4* P0 = (const f4*)p.c[c0];
+    const f4* P1 = (const f4*)p.c[c1];
+    const f4* R0 = (const f4*)rhs.c[c0];
+    const f4* R1 = (const f4*)rhs.c[c1];
+    const int gs = L.Wp * L.Hp;
+    const int e = (j + 1) * L.Wp + (m + 1) + (g + 1) * gs;
+    const f4 cur0 = P0[e], cur1 = P1[e], prv0 = P0[e - gs], prv1 = P1[e - gs], nxt0 = P0[e + gs], nxt1 = P1[e + gs];
+    const f4 l1 = P1[e - 1], r0 = P0[e + 1], n1 = P1[e - L.Wp], s1 = P1[e + L.Wp], n0 = P0[e - L.Wp],
+             s0 = P0[e + L.Wp], q0 = R0[e], q1 = R1[e];
+    const int i0 = 2 * m, i1 = i0 + 1;
+    const float nxy0 = (float)((int)(i0 > 0) + 1 + (int)(j > 0) + (int)(j < H - 1));
+    const float nxy1 = (float)(1 + (int)(i1 < W - 1) + (int)(j > 0) + (int)(j < H - 1));
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl) {
+        const int k = 4 * g + sl;
+        if (k >= D) break;
+        const float dn0 = sl > 0 ? cur0[sl - 1] : prv0[3], up0 = sl < 3 ? cur0[sl + 1] : nxt0[0];
+        const float dn1 = sl > 0 ? cur1[sl - 1] : prv1[3], up1 = sl < 3 ? cur1[sl + 1] : nxt1[0];
+        const float nz = (float)((int)(k > 0) + (int)(k < D - 1));
+        float a = scale * q0[sl];
+        a += l1[sl];
+        a += cur1[sl];
+        a += n1[sl];
+        a += s1[sl];
+        a += dn0;
+        a += up0;
+        float b = scale * q1[sl];
+        b += cur0[sl];
+        b += r0[sl];
+        b += n0[sl];
+        b += s0[sl];
+        b += dn1;
+        b += up1;
+        const size_t o = ((size_t)k * H + j) * W + i0;
+        r[o] = a - (nxy0 + nz) * cur0[sl];
+        r[o + 1] = b - (nxy1 + nz) * cur1[sl];
+    }
 }
 
-// thread per network cell (I, J, K)
-__global__ void __launch_bounds__(BS) k_ws_restrict(RB p, RB rhs, RBL L, int W, int H, int D, float scale, int F,
-                                                    int Wc, int Hc, int Dc, int Wn, int Hn, int Dn,
-                                                    float* __restrict__ res, float* __restrict__ mask,
-                                                    const int* __restrict__ stop) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= Wn * Hn * Dn || *stop) return;
+// one workgroup per network cell (I, J, K): r summed over its F^3 block
+// (per-thread FP32 partial sums over a fixed subset, then a fixed FP64
+// tree: deterministic) / F, and the mask (1 where the block touches a wall);
+// padding cells get 0 and mask 1
+__global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r, int W, int H, int D, int F, int Wc,
+                                                    int Hc, int Dc, int Wn, int Hn, float* __restrict__ res,
+                                                    float* __restrict__ mask, const int* __restrict__ stop) {
+    if (*stop) return;
+    const int t = blockIdx.x;
     const int I = t % Wn, J = (t / Wn) % Hn, K = t / (Wn * Hn);
     if (I >= Wc || J >= Hc || K >= Dc) {
-        res[t] = 0.0f;
-        mask[t] = 1.0f;
+        if (threadIdx.x == 0) {
+            res[t] = 0.0f;
+            mask[t] = 1.0f;
+        }
         return;
     }
+    const int ex = min(F, W - F * I), ey = min(F, H - F * J), ez = min(F, D - F * K);
+    const int m = ex * ey * ez;
     float sum = 0.0f;
-    bool wall = false;
-    for (int k = F * K; k < min(F * K + F, D); ++k)
-        for (int j = F * J; j < min(F * J + F, H); ++j)
-            for (int i = F * I; i < min(F * I + F, W); ++i) {
-                const float pc = pget(p, L, i, j, k);
-                float r = scale * pget(rhs, L, i, j, k);
-                float n = 0.0f;
-                if (i > 0) { r += pget(p, L, i - 1, j, k); n += 1.0f; }
-                if (i < W - 1) { r += pget(p, L, i + 1, j, k); n += 1.0f; }
-                if (j > 0) { r += pget(p, L, i, j - 1, k); n += 1.0f; }
-                if (j < H - 1) { r += pget(p, L, i, j + 1, k); n += 1.0f; }
-                if (k > 0) { r += pget(p, L, i, j, k - 1); n += 1.0f; }
-                if (k < D - 1) { r += pget(p, L, i, j, k + 1); n += 1.0f; }
-                sum += r - n * pc;
-                wall |= n < 6.0f;
-            }
-    const float rc = sum / (float)F;
-    res[t] = rc;
-    mask[t] = wall ? 1.0f : 0.0f;
+    for (int q = threadIdx.x; q < m; q += BS) {
+        const int i = F * I + q % ex, j = F * J + (q / ex) % ey, k = F * K + q / (ex * ey);
+        sum += r[((size_t)k * H + j) * W + i];
+    }
+    double acc = (double)sum;
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double sm[BS / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int q = 0; q < (int)(BS / 64); ++q) tot += sm[q];
+        res[t] = (float)(tot / (double)F);
+        const bool wall = I == 0 || J == 0 || K == 0 || F * (I + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
+        mask[t] = wall ? 1.0f : 0.0f;
+    }
 }
 
 // input scale s = RMS of the coarse residual over the real coarse cells (one
@@ -148,77 +211,136 @@ __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W,
     e[t] = (v == v && fabsf(v) <= 3.4e38f) ? v : 0.0f;
 }
 
-// scale of the correction minimising the error's energy norm: with A x =
-// n x - sum_nbr x (symmetric, positive semi-definite, constants in its null
-// space) and r = scale*rhs - A p, the error p* - p - alpha e has the least
-// A-norm at alpha = <r, e'> / <e, A e>, e' = e minus its mean (a constant
-// changes no residual). Unlike the residual-norm fit, this step can never
-// raise the A-norm of the error, i.e. it cannot add smooth error that the
-// sweeps remove slowly. Per-block partial sums (FP64) of <r,e>, <e,Ae>,
-// sum r and sum e.
-__global__ void __launch_bounds__(BS) k_ws_dots(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
-                                                const float* __restrict__ e, double* __restrict__ partial,
-                                                const SolveCtl* __restrict__ c) {
+// A x at natural cell t = (i, j, k): n x - sum of the in-domain neighbours
+__device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int i, int j, int k, int W, int H,
+                                     int D) {
+    const float xc = x[t];
+    float s = 0.0f, n = 0.0f;
+    const size_t WH = (size_t)W * H;
+    if (i > 0) { s += x[t - 1]; n += 1.0f; }
+    if (i < W - 1) { s += x[t + 1]; n += 1.0f; }
+    if (j > 0) { s += x[t - W]; n += 1.0f; }
+    if (j < H - 1) { s += x[t + W]; n += 1.0f; }
+    if (k > 0) { s += x[t - WH]; n += 1.0f; }
+    if (k < D - 1) { s += x[t + WH]; n += 1.0f; }
+    return n * xc - s;
+}
+
+// One workgroup per block of the global grid (F^3 cells, clipped), all on
+// natural-order arrays: with the residual r' = r - alpha1 A e1 (e1 the
+// correction already chosen, or none) and the direction e2 (or none):
+//   partial[4b..4b+3] = <r', e2>, <e2, A e2>, sum r', sum e2   (FP64)
+//   sg[2b], sg[2b+1]  = sum of r' and of A e2 over the block   (when sg)
+// The scale alpha = <r', e2 - mean e2> / <e2, A e2> minimises the error's
+// energy norm along e2 (A symmetric positive semi-definite with constants
+// in its null space; a constant changes no residual): unlike a fit of the
+// residual's norm, such a step cannot raise the A-norm of the error, i.e.
+// cannot add smooth error that the sweeps remove slowly. The block sums of
+// r' are the global correction's right-hand side. Fixed reduction order.
+__global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r, const float* __restrict__ e1,
+                                                     const double* __restrict__ alpha1,
+                                                     const float* __restrict__ e2, int W, int H, int D, int F,
+                                                     int Gw, int Gh, double* __restrict__ partial,
+                                                     double* __restrict__ sg, const SolveCtl* __restrict__ c) {
     if (c->done) return;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    const int n = W * H * D;
-    for (int t = blockIdx.x * BS + threadIdx.x; t < n; t += gridDim.x * BS) {
-        const int i = t % W, j = (t / W) % H, k = t / (W * H);
-        const double pc = pget(p, L, i, j, k);
-        double r = (double)scale * pget(rhs, L, i, j, k);
-        const double ec = e[t];
-        double ae = 0.0, cnt = 0.0;
-        if (i > 0) { r += pget(p, L, i - 1, j, k); ae -= e[t - 1]; cnt += 1.0; }
-        if (i < W - 1) { r += pget(p, L, i + 1, j, k); ae -= e[t + 1]; cnt += 1.0; }
-        if (j > 0) { r += pget(p, L, i, j - 1, k); ae -= e[t - W]; cnt += 1.0; }
-        if (j < H - 1) { r += pget(p, L, i, j + 1, k); ae -= e[t + W]; cnt += 1.0; }
-        if (k > 0) { r += pget(p, L, i, j, k - 1); ae -= e[t - W * H]; cnt += 1.0; }
-        if (k < D - 1) { r += pget(p, L, i, j, k + 1); ae -= e[t + W * H]; cnt += 1.0; }
-        r -= cnt * pc;
-        ae += cnt * ec;
-        v[0] += r * ec;
-        v[1] += ec * ae;
-        v[2] += r;
-        v[3] += ec;
+    const int b = blockIdx.x;
+    const int I = b % Gw, J = (b / Gw) % Gh, K = b / (Gw * Gh);
+    const int ex = min(F, W - F * I), ey = min(F, H - F * J), ez = min(F, D - F * K);
+    const int m = ex * ey * ez;
+    const float a1 = e1 ? (float)*alpha1 : 0.0f;
+    float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int q = threadIdx.x; q < m; q += BS) {
+        const int i = F * I + q % ex, j = F * J + (q / ex) % ey, k = F * K + q / (ex * ey);
+        const size_t t = ((size_t)k * H + j) * W + i;
+        float rr = r[t];
+        if (e1) rr -= a1 * lap(e1, t, i, j, k, W, H, D);
+        v[2] += rr;
+        if (e2) {
+            const float ec = e2[t];
+            const float ae = lap(e2, t, i, j, k, W, H, D);
+            v[0] += rr * ec;
+            v[1] += ec * ae;
+            v[3] += ec;
+            v[4] += ae;
+        }
     }
+    double d[5];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
-    __shared__ double sv[4][BS / 64];
+    for (int q = 0; q < 5; ++q) {
+        d[q] = (double)v[q];
+        for (int o = 32; o >= 1; o >>= 1) d[q] += __shfl_xor(d[q], o, 64);
+    }
+    __shared__ double sv[5][BS / 64];
     if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 4; ++q) sv[q][threadIdx.x >> 6] = v[q];
+        for (int q = 0; q < 5; ++q) sv[q][threadIdx.x >> 6] = d[q];
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < 5) {
         double a = 0.0;
         for (int w = 0; w < (int)(BS / 64); ++w) a += sv[threadIdx.x][w];
-        partial[4 * blockIdx.x + threadIdx.x] = a;
+        if (threadIdx.x < 4) partial[4 * b + threadIdx.x] = a;
+        if (sg && threadIdx.x == 2) sg[2 * b] = a;
+        if (sg && threadIdx.x == 4) sg[2 * b + 1] = a;
     }
 }
 
-// p += alpha e (alpha from the partial sums, 0 when undefined)
-__global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int D, const float* __restrict__ e,
-                                                 const double* __restrict__ partial, int nb,
-                                                 const SolveCtl* __restrict__ c) {
+// alpha from the per-block partial sums (one workgroup, fixed order: every
+// run gives the same alpha); 0 when undefined
+__global__ void __launch_bounds__(BS) k_ws_alpha(const double* __restrict__ partial, int nb, double n,
+                                                 double* __restrict__ alpha, const SolveCtl* __restrict__ c) {
     if (c->done) return;
-    __shared__ float alpha;
-    if (threadIdx.x == 0) {
-        double re = 0.0, eae = 0.0, sr = 0.0, se = 0.0;
-        for (int q = 0; q < nb; ++q) {
-            re += partial[4 * q];
-            eae += partial[4 * q + 1];
-            sr += partial[4 * q + 2];
-            se += partial[4 * q + 3];
-        }
-        const double al = eae > 0.0 ? (re - sr * se / ((double)W * H * D)) / eae : 0.0;
-        alpha = (al == al && fabs(al) < 1e30) ? (float)al : 0.0f;
-    }
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nb; q += BS)
+        for (int k = 0; k < 4; ++k) v[k] += partial[4 * q + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    __shared__ double sv[4][BS / 64];
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k) sv[k][threadIdx.x >> 6] = v[k];
     __syncthreads();
-    const int n = W * H * D;
-    for (int t = blockIdx.x * BS + threadIdx.x; t < n; t += gridDim.x * BS) {
-        const int i = t % W, j = (t / W) % H, k = t / (W * H);
-        float& dst = p.c[(i + j) & 1][L.at(i, j, k)];
-        dst = dst + alpha * e[t];
+    if (threadIdx.x == 0) {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k)
+            for (int w = 0; w < (int)(BS / 64); ++w) a[k] += sv[k][w];
+        const double al = a[1] > 0.0 ? (a[0] - a[2] * a[3] / n) / a[1] : 0.0;
+        *alpha = (al == al && fabs(al) < 1e30) ? al : 0.0;
     }
+}
+
+// p += alpha1 e1 + alpha2 e2 (either may be absent); thread per (x pair,
+// row, k group) of the split layout: float4 read-modify-write of both
+// colours, natural-order reads of e (8 contiguous bytes per thread and k)
+__global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int D, const float* __restrict__ e1,
+                                                 const double* __restrict__ alpha1, const float* __restrict__ e2,
+                                                 const double* __restrict__ alpha2, const SolveCtl* __restrict__ c) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    if (c->done) return;
+    const int Wh = W >> 1;
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= Wh * H * L.Dg) return;
+    const int m = t % Wh, j = (t / Wh) % H, g = t / (Wh * H);
+    const float a1 = e1 ? (float)*alpha1 : 0.0f, a2 = e2 ? (float)*alpha2 : 0.0f;
+    const int c0 = j & 1;
+    f4* P0 = (f4*)p.c[c0];
+    f4* P1 = (f4*)p.c[c0 ^ 1];
+    const int e = (j + 1) * L.Wp + (m + 1) + (g + 1) * L.Wp * L.Hp;
+    f4 x0 = P0[e], x1 = P1[e];
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl) {
+        const int k = 4 * g + sl;
+        if (k >= D) break;
+        const size_t o = ((size_t)k * H + j) * W + 2 * m;
+        if (e1) {
+            x0[sl] = x0[sl] + a1 * e1[o];
+            x1[sl] = x1[sl] + a1 * e1[o + 1];
+        }
+        if (e2) {
+            x0[sl] = x0[sl] + a2 * e2[o];
+            x1[sl] = x1[sl] + a2 * e2[o + 1];
+        }
+    }
+    P0[e] = x0;
+    P1[e] = x1;
 }
 
 __device__ __forceinline__ double block_max(const double* __restrict__ partial, int n) {
@@ -289,12 +411,142 @@ __global__ void __launch_bounds__(BS) k_ws_restore(const SolveCtl* __restrict__ 
     for (size_t t = (size_t)blockIdx.x * BS + threadIdx.x; t < n; t += (size_t)gridDim.x * BS) p[t] = backup[t];
 }
 
+// Global correction: solve the Galerkin coarse problem
+//     A_G x = b - (sum b / (W H D)) * |block|,
+//     b = block sums of r' = r - alpha1 A e1 (sum r - alpha1 sum A e1)
+// with A_G = P^T A P for the piecewise-constant prolongation P: the graph
+// Laplacian of the G grid whose edge weights are the numbers of fine faces
+// two blocks share (A symmetric positive semi-definite with constants in
+// its null space; subtracting the constant part of the fine residual makes
+// the right-hand side consistent). Conjugate gradients in one workgroup,
+// x0 = 0, at most `iters` iterations or until |r| <= 1e-4 |r0|; search
+// direction in LDS for the neighbour reads, FP64 dot products. Every thread
+// computes the same scalars, so the loop bounds are uniform.
+constexpr int GT = 1024;
+constexpr int GPT = WS_GMAX / GT;
+__global__ void __launch_bounds__(GT) k_ws_gsolve(const double* __restrict__ sg, const double* __restrict__ alpha1,
+                                                  float* __restrict__ x_out, int Gw, int Gh, int Gd, int F, int W,
+                                                  int H, int D, int iters, const SolveCtl* __restrict__ c) {
+    if (c->done) return;
+    const double a1 = alpha1 ? *alpha1 : 0.0;
+    __shared__ float sp[WS_GMAX];
+    __shared__ double red[GT / 64];
+    const int n = Gw * Gh * Gd;
+    auto bsum = [&](double v) -> double {
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        __syncthreads(); // every thread has read the previous sum
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        double s = 0.0;
+        for (int q = 0; q < GT / 64; ++q) s += red[q];
+        return s;
+    };
+    float x[GPT], r[GPT], d[GPT], wxy[GPT][3];
+    int nb[GPT][6];
+    double sb = 0.0;
+#pragma unroll
+    for (int q = 0; q < GPT; ++q) {
+        const int g = threadIdx.x + q * GT;
+        x[q] = r[q] = d[q] = 0.0f;
+        wxy[q][0] = wxy[q][1] = wxy[q][2] = 0.0f;
+        for (int f = 0; f < 6; ++f) nb[q][f] = -1;
+        if (g < n) {
+            const int I = g % Gw, J = (g / Gw) % Gh, K = g / (Gw * Gh);
+            const int ex = min(F, W - F * I), ey = min(F, H - F * J), ez = min(F, D - F * K);
+            wxy[q][0] = (float)(ey * ez); // faces shared with the x neighbours
+            wxy[q][1] = (float)(ex * ez);
+            wxy[q][2] = (float)(ex * ey);
+            if (I > 0) nb[q][0] = g - 1;
+            if (I < Gw - 1) nb[q][1] = g + 1;
+            if (J > 0) nb[q][2] = g - Gw;
+            if (J < Gh - 1) nb[q][3] = g + Gw;
+            if (K > 0) nb[q][4] = g - Gw * Gh;
+            if (K < Gd - 1) nb[q][5] = g + Gw * Gh;
+            const double bg = sg[2 * g] - a1 * sg[2 * g + 1];
+            r[q] = (float)bg;
+            sb += bg;
+        }
+    }
+    const double mean = bsum(sb) / ((double)W * (double)H * (double)D);
+    double rr = 0.0;
+#pragma unroll
+    for (int q = 0; q < GPT; ++q) {
+        const int g = threadIdx.x + q * GT;
+        if (g < n) {
+            const int I = g % Gw, J = (g / Gw) % Gh, K = g / (Gw * Gh);
+            const double vol = (double)min(F, W - F * I) * min(F, H - F * J) * min(F, D - F * K);
+            r[q] = (float)((double)r[q] - mean * vol);
+            d[q] = r[q];
+            rr += (double)r[q] * r[q];
+        }
+    }
+    rr = bsum(rr);
+    const double stop = rr * 1e-8;
+    for (int it = 0; it < iters && rr > stop && rr > 0.0; ++it) {
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+            const int g = threadIdx.x + q * GT;
+            if (g < n) sp[g] = d[q];
+        }
+        __syncthreads();
+        float ad[GPT];
+        double pap = 0.0;
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+            float a = 0.0f;
+            for (int f = 0; f < 6; ++f)
+                if (nb[q][f] >= 0) a += wxy[q][f >> 1] * (d[q] - sp[nb[q][f]]);
+            ad[q] = a;
+            pap += (double)d[q] * a;
+        }
+        pap = bsum(pap); // its first barrier also orders the LDS reads before the next writes
+        if (!(pap > 0.0)) break;
+        const double al = rr / pap;
+        double rn = 0.0;
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+            x[q] += (float)al * d[q];
+            r[q] -= (float)al * ad[q];
+            rn += (double)r[q] * r[q];
+        }
+        rn = bsum(rn);
+        const float be = (float)(rn / rr);
+        rr = rn;
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) d[q] = r[q] + be * d[q];
+    }
+#pragma unroll
+    for (int q = 0; q < GPT; ++q) {
+        const int g = threadIdx.x + q * GT;
+        if (g < n) x_out[g] = x[q];
+    }
+}
+
 __global__ void k_ws_zero(unsigned* a) { *a = 0u; }
 
 // a new application: no decision yet (the restore only follows this one's)
 __global__ void k_ws_begin(SolveCtl* c) { c->ws = 0; }
 
 inline unsigned nb(size_t n) { return (unsigned)((n + BS - 1) / BS); }
+
+// ws_buf layout (floats unless noted): coarse residual and mask (nn each),
+// 64 words of scalars, the global grid's solution (WS_GMAX), then FP64: 4
+// partial sums and 2 block sums per global-grid block, and the two scales.
+// s.tmp holds natural-order fields: r, then the network's and the global
+// correction's prolongations (3 of its 7 cell planes).
+inline size_t ws_nn(const Solver& s) { return (size_t)s.Wn * s.Hn * s.Dn; }
+inline float* ws_xg(const Solver& s) { return s.ws_buf + 2 * ws_nn(s) + 64; }
+inline double* ws_part(const Solver& s) { return (double*)(ws_xg(s) + WS_GMAX); }
+inline double* ws_sg(const Solver& s) { return ws_part(s) + 4 * WS_GMAX; }
+inline double* ws_alpha(const Solver& s) { return ws_sg(s) + 2 * WS_GMAX; }
+inline size_t ws_buf_bytes(const Solver& s) {
+    return (2 * ws_nn(s) + 64 + WS_GMAX) * sizeof(float) + (6 * WS_GMAX + 8) * sizeof(double);
+}
+inline size_t ncell(const Solver& s) { return (size_t)s.W * s.H * s.D; }
+inline float* ws_r(const Solver& s) { return s.tmp; }
+inline float* ws_enet(const Solver& s) { return s.tmp + ncell(s); }
+inline float* ws_eg(const Solver& s) { return s.tmp + 2 * ncell(s); }
+inline unsigned split_threads(const Solver& s) { return (unsigned)((s.W >> 1) * s.H * s.L.Dg); }
 
 } // namespace
 
@@ -312,11 +564,16 @@ int warm_start_attach(Solver& s, const Net* net) {
     s.Wc = (s.W + s.ws_f - 1) / s.ws_f;
     s.Hc = (s.H + s.ws_f - 1) / s.ws_f;
     s.Dc = (s.D + s.ws_f - 1) / s.ws_f;
+    // global correction grid: blocks of ws_g^3 cells, at most 16 per axis
+    s.ws_g = 1;
+    while (std::max(s.W, std::max(s.H, s.D)) > 16 * s.ws_g) s.ws_g *= 2;
+    s.Gw = (s.W + s.ws_g - 1) / s.ws_g;
+    s.Gh = (s.H + s.ws_g - 1) / s.ws_g;
+    s.Gd = (s.D + s.ws_g - 1) / s.ws_g;
     s.Wn = std::max(4, (s.Wc + 3) / 4 * 4);
     s.Hn = std::max(4, (s.Hc + 3) / 4 * 4);
     s.Dn = std::max(4, (s.Dc + 3) / 4 * 4);
-    const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
-    if (!s.ws_buf && hipMalloc(&s.ws_buf, (2 * nn + 64) * sizeof(float)) != hipSuccess) {
+    if (!s.ws_buf && hipMalloc(&s.ws_buf, ws_buf_bytes(s)) != hipSuccess) {
         set_error("hipMalloc failed for warm-start buffers");
         return BLZ_E_NOMEM;
     }
@@ -338,16 +595,71 @@ int warm_start_attach(Solver& s, const Net* net) {
     return BLZ_OK;
 }
 
+// r (natural order, into s.tmp) and the network input: r restricted to the
+// network grid, its RMS as the input scale
 static int ws_restrict(Solver& s, float dt, const int* stop) {
-    const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
+    const size_t nn = ws_nn(s);
     float* res = s.ws_buf;
     float* mask = s.ws_buf + nn;
     unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
     const float scale = (s.s.density * s.dx * s.dx) / dt;
+    hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
+                       scale, ws_r(s), stop);
     hipLaunchKernelGGL(k_ws_zero, dim3(1), dim3(1), 0, s.stream, amax);
-    hipLaunchKernelGGL(k_ws_restrict, dim3(nb(nn)), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale,
-                       s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, s.Dn, res, mask, stop);
+    hipLaunchKernelGGL(k_ws_restrict, dim3((unsigned)nn), dim3(BS), 0, s.stream, ws_r(s), s.W, s.H, s.D, s.ws_f, s.Wc,
+                       s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
     hipLaunchKernelGGL(k_ws_rms, dim3(1), dim3(1024), 0, s.stream, res, (int)nn, s.Wc * s.Hc * s.Dc, amax, stop);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int warm_start_set_parts(Solver& s, int parts) {
+    if (parts < 1 || parts > 3) {
+        set_error("warm-start parts must be 1 (network), 2 (global) or 3 (both)");
+        return BLZ_E_INVALID;
+    }
+    s.ws_parts = parts;
+    return BLZ_OK;
+}
+
+// The corrections on the home planes (the spare ones already hold the
+// uncorrected pressure), from r in s.tmp (computed from the home planes):
+// first the network's coarse prediction (when pred != nullptr), prolongated
+// trilinearly, with its energy-optimal scale; then the global one -- the
+// Galerkin solve on the G grid of the residual the first leaves, which sets
+// the smoothest error modes (the ones Gauss-Seidel reduces slowest, and the
+// ones a network's prediction is least reliable on), prolongated and scaled
+// the same way against that residual; both added in one pass.
+static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
+    const size_t N = ncell(s);
+    const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
+    double* a_net = ws_alpha(s);
+    double* a_g = ws_alpha(s) + 1;
+    const float* enet = pred ? ws_enet(s) : nullptr;
+    const bool glob = s.ws_parts & 2;
+    if (pred) {
+        hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, ws_enet(s), s.W, s.H, s.D, pred,
+                           1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
+        hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, ws_enet(s),
+                           s.W, s.H, s.D, s.ws_g, s.Gw, s.Gh, ws_part(s), glob ? ws_sg(s) : nullptr, c);
+        hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
+    } else if (glob) {
+        hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, nullptr, s.W,
+                           s.H, s.D, s.ws_g, s.Gw, s.Gh, ws_part(s), ws_sg(s), c);
+    }
+    if (glob) {
+        // right-hand side: block sums of r - alpha_net A e_net (sum r' and
+        // alpha_net * sum A e_net per block)
+        hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), pred ? a_net : nullptr, ws_xg(s),
+                           s.Gw, s.Gh, s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
+        hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, ws_eg(s), s.W, s.H, s.D, ws_xg(s),
+                           1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
+        hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), enet, a_net, ws_eg(s), s.W, s.H,
+                           s.D, s.ws_g, s.Gw, s.Gh, ws_part(s), nullptr, c);
+        hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
+    }
+    hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.L, s.W, s.H, s.D, enet,
+                       a_net, glob ? ws_eg(s) : nullptr, a_g, c);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
@@ -359,30 +671,30 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
     float* mask = s.ws_buf + nn;
     unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
     hipLaunchKernelGGL(k_ws_begin, dim3(1), dim3(1), 0, s.stream, c);
-    if (int e = ws_restrict(s, dt, &c->done)) return e;
-    hipLaunchKernelGGL(k_ws_pack, dim3(nb(nn * NET_IN_CH)), dim3(BS), 0, s.stream, res, mask, amax, s.ws_net->x0,
-                       (int)nn);
-    BLZ_HIP(hipGetLastError());
-    if (int e = s.ws_net->forward()) return e;
+    const bool use_net = s.ws_parts & 1;
+    if (!use_net) {
+        const float scale = (s.s.density * s.dx * s.dx) / dt;
+        hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
+                           s.D, scale, ws_r(s), &c->done);
+    } else {
+        if (int e = ws_restrict(s, dt, &c->done)) return e;
+        hipLaunchKernelGGL(k_ws_pack, dim3(nb(nn * NET_IN_CH)), dim3(BS), 0, s.stream, res, mask, amax,
+                           s.ws_net->x0, (int)nn);
+        BLZ_HIP(hipGetLastError());
+        if (int e = s.ws_net->forward()) return e;
+    }
     // the uncorrected pressure goes to the spare planes, the corrected one
-    // (prediction with its fitted scale) stays in the home planes
+    // stays in the home planes
     float* spare = s.prb + 4 * s.Np;
     BLZ_HIP(hipMemcpyAsync(spare, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
-    const size_t N = (size_t)s.W * s.H * s.D;
-    float* e = s.tmp;
-    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out,
-                       1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
-    const float scale = (s.s.density * s.dx * s.dx) / dt;
-    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 4);
-    hipLaunchKernelGGL(k_ws_dots, dim3(nbd), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, e,
-                       s.red + 8, c);
-    hipLaunchKernelGGL(k_ws_apply, dim3((unsigned)std::min<size_t>(nb(N), 4096)), dim3(BS), 0, s.stream, s.p, s.L,
-                       s.W, s.H, s.D, e, s.red + 8, nbd, c);
-    BLZ_HIP(hipGetLastError());
+    if (int e = ws_correct(s, c, use_net ? s.ws_net->out : nullptr)) return e;
     // the judge: the same sweeps on both, the lower residual continues
     int launches = 0, kind = 0;
     const int keep = s.rbgs_variant;
-    s.rbgs_variant = 8; // in place on the home planes (the spare ones hold the other path)
+    // in place on the home planes (the spare ones hold the other path): a
+    // two-pass variant, the chosen one when it is one
+    const int v = keep ? keep : s.rbgs_auto;
+    s.rbgs_variant = (v == 2 || v == 4) ? v : 8;
     int rc = launch_rbgs_sweeps(s, dt, judge, &c->done, &launches, &kind);
     s.rbgs_variant = keep;
     if (rc) return rc;
@@ -399,8 +711,9 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
 }
 
 // the correction step with a given coarse prediction (network grid, x
-// fastest) instead of the network's: restrict, then steps 3-4 of the
-// warm start, synchronously; for measuring what an exact coarse solve gains
+// fastest) instead of the network's, followed by the global correction when
+// enabled, judged by the immediate FP64 residual, synchronously; for
+// measuring what an exact coarse solve gains
 int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* before, double* after, int* kept) {
     if (!s.ws_net) {
         set_error("no warm-start network attached");
@@ -413,17 +726,10 @@ int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* 
     BLZ_HIP(hipMemcpyAsync(s.ws_net->out, host_pred, nn * sizeof(float), hipMemcpyHostToDevice, s.stream));
     float* backup = s.prb + 4 * s.Np;
     BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
-    const size_t N = (size_t)s.W * s.H * s.D;
-    float* e = s.tmp;
-    hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, s.ws_net->out,
-                       1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
     const float scale = (s.s.density * s.dx * s.dx) / dt;
-    const int nbd = (int)std::min<size_t>(nb(N), RESIDUAL_BLOCKS / 4);
-    hipLaunchKernelGGL(k_ws_dots, dim3(nbd), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, scale, e,
-                       s.red + 8, c);
-    hipLaunchKernelGGL(k_ws_apply, dim3((unsigned)std::min<size_t>(nb(N), 4096)), dim3(BS), 0, s.stream, s.p, s.L,
-                       s.W, s.H, s.D, e, s.red + 8, nbd, c);
-    BLZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
+                       scale, ws_r(s), &c->done);
+    if (int e = ws_correct(s, c, (s.ws_parts & 1) ? s.ws_net->out : nullptr)) return e;
     if (int e2 = launch_residual_ws(s, dt, c, s.done_host_dev, 0, -1.0, 1 << 30)) return e2;
     hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
                        backup, s.prb, 2 * s.Np);

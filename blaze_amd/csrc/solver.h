@@ -73,6 +73,7 @@ struct SolveCtl {
     double res_ws;    // residual right after the last learned correction
     int ws_kept, ws_rejected; // learned corrections kept / rejected in this solve
 };
+constexpr int WS_GMAX = 4096;      // cells of the warm start's global-correction grid (16^3)
 constexpr int MAX_SOLVES = 64;     // pressure solves per step (bounds max_substeps)
 constexpr int SOLVE_LOOKAHEAD = 2; // sweep batches the host may enqueue past the last one it has seen finish
 
@@ -151,6 +152,9 @@ struct Solver {
     const Net* ws_src = nullptr;
     Net* ws_net = nullptr;     // view: shares ws_src's weights, owns activations
     int ws_f = 4;               // coarsening factor of the warm start
+    int ws_parts = 3;           // bit 0: network correction, bit 1: global correction
+    int ws_g = 1;               // block size of the global correction's grid
+    int Gw = 1, Gh = 1, Gd = 1; // that grid (at most WS_GMAX cells)
     int Wc = 0, Hc = 0, Dc = 0; // coarse grid, real cells
     int Wn = 0, Hn = 0, Dn = 0; // network grid (coarse rounded up to multiples of 4)
     float* ws_buf = nullptr;   // coarse residual, mask (Wn*Hn*Dn each), scale bits
@@ -194,6 +198,7 @@ int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int 
                           int first);
 int launch_ctl_reset(Solver& s, SolveCtl* c);
 int launch_residual_ws(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
+int warm_start_set_parts(Solver& s, int parts);
 int launch_residual_ws_final(Solver& s, SolveCtl* c, int* host_done, int seq, double tol, int cap, int nb_partial);
 int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop);
 int launch_residual_partial(Solver& s, float dt, bool spare, double* partial, const int* stop, int* nb_out);

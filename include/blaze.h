@@ -211,6 +211,11 @@ int blz_net_train_step(blz_net* net, const float* in2, const float* target, cons
    if the FP64 residual drops (else the pressure is restored). The network
    must outlive the attachment. */
 int blz_set_warm_start(blz_solver* h, blz_net* net);
+/* Which corrections a warm start applies (no reference counterpart):
+   bit 0 the network's, bit 1 the global one (an exact Galerkin solve on a
+   grid of at most 16^3 blocks of the residual the first leaves, which sets
+   the smoothest error modes); default 3. Both are judged together. */
+int blz_set_warm_start_parts(blz_solver* h, int parts);
 /* The network input for the current pressure and rhs (for generating
    training data): in2 = 2 x net_dims values; dims reported in (W, H, D). */
 int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]);

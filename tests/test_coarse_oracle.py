@@ -1,0 +1,105 @@
+"""CPU pins of the global-correction oracle (oracle/coarse_oracle.py): each
+check is fixed by mathematics independent of the oracle's own formulas."""
+import itertools
+
+import numpy as np
+import pytest
+
+C = pytest.importorskip("oracle.coarse_oracle")
+
+
+def inject(x, shape, F):
+    """Piecewise-constant prolongation P (each block's value on its cells)."""
+    D, H, W = shape
+    out = np.repeat(np.repeat(np.repeat(x, F, 0), F, 1), F, 2)
+    return out[:D, :H, :W]
+
+
+@pytest.mark.parametrize("shape,F", [((10, 7, 5), 4), ((9, 6, 5), 2), ((8, 8, 8), 1)])
+def test_galerkin_operator_is_PtAP(shape, F):
+    """A_G x equals P^T A P x with P built by repetition and P^T by block
+    sums, on grids whose far blocks are clipped."""
+    rng = np.random.default_rng(1)
+    G = C.block_grid(shape, F)
+    x = rng.standard_normal(G)
+    ref = C.block_sums(C.laplacian(inject(x, shape, F)), F)
+    np.testing.assert_allclose(C.galerkin_apply(x, shape, F), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_coarse_solve_matches_dense_pseudo_inverse():
+    shape, F = (9, 6, 5), 2
+    G = C.block_grid(shape, F)
+    n = int(np.prod(G))
+    A = np.zeros((n, n))
+    for q in range(n):
+        u = np.zeros(n)
+        u[q] = 1.0
+        A[:, q] = C.galerkin_apply(u.reshape(G), shape, F).ravel()
+    assert np.allclose(A, A.T)
+    rng = np.random.default_rng(2)
+    r = rng.standard_normal(shape)
+    b = C.coarse_rhs(r, F)
+    assert abs(b.sum()) < 1e-10 * np.abs(b).sum()   # consistent: orthogonal to the null space
+    x = C.coarse_solve(b, shape, F, iters=5000, rtol2=1e-30)
+    xd = np.linalg.pinv(A) @ b.ravel()
+    np.testing.assert_allclose(x.ravel() - x.mean(), xd - xd.mean(), rtol=1e-6, atol=1e-9 * np.abs(xd).max())
+
+
+def test_unit_blocks_give_the_exact_solution():
+    """F = 1: the block grid is the fine grid, so the correction solves the
+    (consistent) problem exactly and its scale is 1."""
+    rng = np.random.default_rng(3)
+    shape = (12, 10, 9)
+    rhs = rng.standard_normal(shape)
+    rhs -= rhs.mean()
+    p = rng.standard_normal(shape)
+    scale = 0.37
+    r0 = np.abs(C.residual(p, rhs, scale)).max()
+    pc, al = C.global_correction(p, rhs, scale, 1, iters=5000, rtol2=1e-30)
+    assert abs(al - 1.0) < 1e-8
+    assert np.abs(C.residual(pc, rhs, scale)).max() < 1e-8 * r0
+
+
+def test_energy_scale_minimises_error_energy():
+    """With a known solution p*, the scale minimises ||p* - p - a e||_A over a
+    brute-force grid of a."""
+    rng = np.random.default_rng(4)
+    shape = (10, 8, 6)
+    ps = rng.standard_normal(shape)
+    scale = 1.0
+    rhs = C.laplacian(ps) / scale
+    p = ps + rng.standard_normal(shape)
+    e = rng.standard_normal(shape)
+    r = C.residual(p, rhs, scale)
+    al = C.energy_alpha(r, e)
+
+    def energy(a):
+        d = ps - p - a * e
+        return float((d * C.laplacian(d)).sum())
+
+    grid = np.linspace(al - 1.0, al + 1.0, 2001)
+    best = grid[np.argmin([energy(a) for a in grid])]
+    assert abs(best - al) <= 1e-3
+    assert energy(al) <= energy(0.0)
+
+
+def test_prolongation_reproduces_linear_functions():
+    """Block values of a linear function (at block centres) prolongate to the
+    same linear function on every fine cell between the first and last
+    block centres."""
+    shape, F = (16, 12, 8), 4
+    G = C.block_grid(shape, F)
+    cz, cy, cx = [(np.arange(g) + 0.5) * F - 0.5 for g in G]
+    lin = lambda z, y, x: 0.3 * z - 1.2 * y + 2.0 * x + 0.7
+    x = lin(cz[:, None, None], cy[None, :, None], cx[None, None, :])
+    out = C.prolong(x, shape, F)
+    for k, j, i in itertools.product(range(shape[0]), range(shape[1]), range(shape[2])):
+        if cz[0] <= k <= cz[-1] and cy[0] <= j <= cy[-1] and cx[0] <= i <= cx[-1]:
+            assert abs(out[k, j, i] - lin(k, j, i)) < 1e-9
+
+
+def test_block_size_rule():
+    assert C.block_size((256, 256, 256)) == 16
+    assert C.block_size((128, 64, 32)) == 8
+    assert C.block_size((16, 9, 3)) == 1
+    assert C.block_size((17, 4, 4)) == 2

@@ -70,8 +70,10 @@ STAGE_CASES = [
 
 
 @pytest.mark.parametrize("name,stage,kw", STAGE_CASES, ids=[c[0] for c in STAGE_CASES])
-@pytest.mark.parametrize("res", [(32, 24, 20), (16, 16, 16)], ids=["32x24x20", "16c"])
+@pytest.mark.parametrize("res", [(32, 24, 20), (16, 16, 16), (24, 20, 12)], ids=["32x24x20", "16c", "24x20x12"])
 def test_stage_bitwise(name, stage, kw, res):
+    """dx = 1/W: a power of two for the first two shapes (grid scaling by
+    multiplication), not for the third (IEEE division)."""
     print(f"[stage] {name} res={res}", flush=True)
     s = settings(B, res, **kw)
     st = random_state(s, seed=sum(name.encode()), vel_scale=0.7, smoke=s.smoke_density > 0)

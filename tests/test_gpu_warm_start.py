@@ -58,10 +58,60 @@ def test_detached_or_disabled_warm_start_is_the_plain_path():
     assert g.stats().warm_start_used == 0 and g.stats().warm_start_fallbacks == 0
 
 
-def run_frames(scene, res, frames, tol, cap, net, start=0, every_ws=1):
+@pytest.mark.parametrize("res", [(16, 12, 10), (64, 48, 40), (66, 20, 36)])
+def test_global_correction_matches_oracle(res):
+    """The global correction alone (no network), applied to a smooth
+    problem, against oracle/coarse_oracle.py: the FP64 residual after it and
+    the corrected pressure (FP32 kernels and CG to 1e-6 against float64).
+    (16, 12, 10) has unit blocks, where the correction is the exact solve."""
+    C = pytest.importorskip("oracle.coarse_oracle")
+    print(f"[ws] global correction {res}", flush=True)
+    W, H, D = res
+    s = settings(B, res, max_iter_solve=10, residual_tol=1e-7, residual_check_every=10, warm_start=1,
+                 closed_bounds=(1, 1, 1, 1, 1, 1))
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    # a smooth bump whose largest residual lies inside (the prolongation is
+    # flat within half a block of the walls, where it changes no residual)
+    sig2 = 2.0 * (max(res) / 6.0) ** 2
+    rhs = (np.exp(-((x - 0.45 * W) ** 2 + (y - 0.55 * H) ** 2 + (z - 0.5 * D) ** 2) / sig2)
+           + 0.002 * np.random.default_rng(5).standard_normal((D, H, W)))
+    rhs = (rhs - rhs.mean()).astype(np.float32)   # consistent: no residual part outside A's range
+    amp = 0.01 if max(res) <= 16 else 0.0
+    p0 = (amp * np.random.default_rng(6).standard_normal((D, H, W))).astype(np.float32)
+    g = B.Solver(s)
+    g.set_field(B.FIELD_RHS, rhs)
+    g.set_field(B.FIELD_P, p0)
+    g.set_warm_start(B.Net(seed=0), network=False, global_correction=True)
+    (wn, hn, dn), _ = g.warm_start_dims()
+    dt = 1.0 / 24.0
+    before, after, kept = g.warm_start_apply(dt, np.zeros((dn, hn, wn), np.float32))
+    scale = float(np.float32(np.float32(s.density * s.dx) * np.float32(s.dx)) / np.float32(dt))
+    c = dt / (float(s.density) * float(s.dx) * float(s.dx))
+    F = C.block_size((D, H, W))
+    pc, al = C.global_correction(p0, rhs, scale, F)
+    want_before = np.abs(C.residual(p0, rhs, scale)).max() * c
+    want_after = np.abs(C.residual(pc, rhs, scale)).max() * c
+    print(f"[ws] F={F} alpha {al:.4f} residual {before:.4e} -> {after:.4e} (oracle {want_before:.4e} -> "
+          f"{want_after:.4e}) kept {kept}", flush=True)
+    assert before == pytest.approx(want_before, rel=1e-5)
+    assert abs(after - want_after) <= 2e-3 * want_before
+    if F == 1:
+        assert after <= 1e-3 * before and kept
+    # with larger blocks the trilinear correction's Laplacian concentrates on
+    # the block-centre planes, so the immediate max-norm residual need not
+    # drop (the judge's sweeps spread it); the residual after the correction
+    # pins restriction, coarse solve, prolongation and scale all the same
+    if kept:
+        dg = g.field(B.FIELD_P).astype(np.float64) - p0
+        dw = pc - p0
+        assert np.linalg.norm(dg - dw) <= 2e-3 * np.linalg.norm(dw)
+
+
+def run_frames(scene, res, frames, tol, cap, net, start=0, every_ws=1, network=True, global_correction=True):
     """Frames `start`..`start+frames-1` of `scene` at grid `res`, stage by
     stage; returns the sweeps of each frame's solve, the kept/rejected
-    counts of learned corrections and each frame's final FP64 residual."""
+    counts of corrections and each frame's final FP64 residual above the
+    floor its right-hand side sets."""
     s, _, _, gen = B.parse_scene(os.path.join(SCENES, scene))
     f = s.grid_res[0] / res[0]
     for a in range(3):
@@ -76,7 +126,7 @@ def run_frames(scene, res, frames, tol, cap, net, start=0, every_ws=1):
     parts = B.generate_particles(gen)
     g = B.Solver(s)
     if net is not None:
-        g.set_warm_start(net)
+        g.set_warm_start(net, network=network, global_correction=global_correction)
     dt = float(s.dt)
     t = np.float32(0.0)
     sweeps, kept, res = [], 0, []
@@ -85,12 +135,15 @@ def run_frames(scene, res, frames, tol, cap, net, start=0, every_ws=1):
         for stg in W._force_stages(s):
             g.run_stage(stg, float(t), dt)
         g.run_stage(B.STAGE_DIVERGENCE, float(t), dt)
+        # with open faces the divergence does not sum to zero: the residual's
+        # constant part |mean(rhs)| stays whatever the pressure (the floor)
+        floor = abs(float(np.mean(g.field(B.FIELD_RHS), dtype=np.float64)))
         g.run_stage(B.STAGE_PRESSURE, float(t), dt)
         st = g.stats()
         if fr >= start:
             sweeps.append(st.rbgs_iters_last)
             kept += st.warm_start_used
-            res.append(st.residual)
+            res.append(st.residual - floor)
         g.run_stage(B.STAGE_PROJECT_UPDATE, float(t), dt)
         g.run_stage(B.STAGE_ADVECT, float(t), dt)
         t = np.float32(t + np.float32(dt))
@@ -98,13 +151,14 @@ def run_frames(scene, res, frames, tol, cap, net, start=0, every_ws=1):
 
 
 def test_trained_network_ablation():
-    """Ablation of the learned part on the solver's own data (plume, 32^3):
-    the network trained on pairs from the first frames predicts held-out
-    corrections far better than the same network untrained, and a solve
-    that applies it (every 50 sweeps, judged by the FP64 residual) ends no
-    worse than the plain solve of the same frames at equal sweeps. The
-    measured gains are printed: the final-residual ratio warm/plain and the
-    kept/rejected counts."""
+    """Ablation of the warm start's parts on the solver's own data (plume):
+    the network trained on pairs from the first frames at 32^3 predicts
+    held-out corrections far better than the same network untrained; in
+    64^3 solves that apply the corrections every 50 sweeps (judged by the
+    FP64 residual) the global correction lowers the final residual (above
+    the floor the open faces set) well below the plain solve's, and the
+    network, alone or with it, ends no worse than the solve without it. The
+    measured ratios are printed."""
     print("[ws] ablation: data", flush=True)
     pairs = W.scene_pairs(os.path.join(SCENES, "plume64.blz"), 48, res=(32, 32, 32), sweeps=1500, every=1)
     train_pairs, held = pairs[:40], pairs[40:]
@@ -114,10 +168,17 @@ def test_trained_network_ablation():
     e1 = W.relative_error(net, held)
     print(f"[ws] held-out shape error: untrained {e0:.3f} trained {e1:.3f}", flush=True)
     assert e1 < 0.85 * e0
-    plain = run_frames("plume64.blz", (32, 32, 32), 12, 1e-12, 300, None, start=40)
-    warm = run_frames("plume64.blz", (32, 32, 32), 12, 1e-12, 300, net, start=40, every_ws=50)
-    ratio = float(np.mean(warm[3]) / np.mean(plain[3]))
-    print(f"[ws] final residual plain {np.mean(plain[3]):.3e} warm {np.mean(warm[3]):.3e} ratio {ratio:.3f} "
-          f"kept {warm[1]} rejected {warm[2]}", flush=True)
-    assert warm[1] + warm[2] > 0
-    assert ratio <= 1.25
+    args = ("plume64.blz", (64, 64, 64), 8, 1e-12, 300)
+    plain = run_frames(*args, None, start=24)
+    runs = {name: run_frames(*args, net, start=24, every_ws=50, network=nw, global_correction=gl)
+            for name, nw, gl in (("net", True, False), ("global", False, True), ("net+global", True, True))}
+    rp = float(np.mean(plain[3]))
+    print(f"[ws] plain: mean final residual above the floor {rp:.3e}", flush=True)
+    ratio = {k: float(np.mean(v[3])) / rp for k, v in runs.items()}
+    for k, v in runs.items():
+        print(f"[ws] {k}: final residual ratio to plain {ratio[k]:.3e} kept {v[1]} rejected {v[2]}", flush=True)
+    for v in runs.values():
+        assert v[1] + v[2] > 0
+    assert ratio["net"] <= 1.25
+    assert ratio["global"] <= 0.5
+    assert ratio["net+global"] <= 1.25 * ratio["global"]
