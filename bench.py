@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--probe-sweeps", type=int, default=20)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start leg")
-    ap.add_argument("--warm-every", type=int, default=1,
+    ap.add_argument("--warm-every", type=int, default=50,
                     help="warm_start setting of that leg (1: before the sweeps, N >= 2: also every N sweeps)")
     ap.add_argument("--warm-parts", type=int, default=3,
                     help="warm-start corrections: 1 network, 2 global, 3 both")

@@ -246,7 +246,7 @@ int Batch::enqueue_step(hipStream_t st) {
             BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
             if (again[q]) {
                 if (int e = warm_start_apply(s, s.s.dt, s.ctl, s.done_host_dev, 0, (double)s.s.residual_tol,
-                                             s.s.max_iter_solve, every))
+                                             s.s.max_iter_solve, every, true))
                     return e;
                 it[q] += every;
             }

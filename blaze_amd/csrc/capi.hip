@@ -220,6 +220,7 @@ int Solver::enqueue_solve_stats() {
 }
 
 int Solver::finalize() {
+    if (!solves.empty()) st.warm_start_used = 0;
     for (size_t q = 0; q < solves.size(); ++q) {
         int it = solves[q];
         double r = -1.0, r0 = -1.0;
@@ -236,7 +237,7 @@ int Solver::finalize() {
         st.residual = r;
         st.residual_initial = r0;
         if (solves[q] >= 0) {
-            st.warm_start_used = ctl_host[q].ws_kept;
+            st.warm_start_used += ctl_host[q].ws_kept;
             st.warm_start_fallbacks += ctl_host[q].ws_rejected;
         }
     }
@@ -356,7 +357,7 @@ int Solver::project_solve(float dt_) {
         // warm_start = N >= 2: the learned correction again every N sweeps
         if (ws_net && s.warm_start >= 2 && it % s.warm_start == 0 && it < cap) {
             const int k = std::min(every, cap - it);
-            if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap, k)) return e;
+            if (int e = warm_start_apply(*this, dt_, c, done_host_dev, seq, tol, cap, k, true)) return e;
             it += k;
         }
         BLZ_HIP(hipEventRecord(batch_ev[b % (SOLVE_LOOKAHEAD + 1)], stream));

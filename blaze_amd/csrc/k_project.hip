@@ -770,6 +770,7 @@ __device__ __forceinline__ void res_final_block(const double* __restrict__ parti
         } else {
             c->iters += add;
         }
+        c->res_prev = first ? -1.0 : c->res;
         c->res = r;
         c->checks += 1;
         if (!(r > tol) || c->iters >= cap) {
@@ -801,6 +802,8 @@ __global__ void k_ctl_reset(SolveCtl* c) {
     c->res_ws = -1.0;
     c->ws_kept = 0;
     c->ws_rejected = 0;
+    c->res_prev = -1.0;
+    c->skip_spare = 1;
 }
 
 // pressure_gradient_update_kernel :601-669 for the faces along AXIS,

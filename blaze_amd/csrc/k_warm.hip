@@ -106,7 +106,8 @@ __global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int
 // (per-thread FP32 partial sums over a fixed subset, then a fixed FP64
 // tree: deterministic) / F, and the mask (1 where the block touches a wall);
 // padding cells get 0 and mask 1
-__global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r, int W, int H, int D, int F, int Wc,
+__global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r, int W, int H, int D, int F, int lf,
+                                                    int Wc,
                                                     int Hc, int Dc, int Wn, int Hn, float* __restrict__ res,
                                                     float* __restrict__ mask, const int* __restrict__ stop) {
     if (*stop) return;
@@ -119,12 +120,10 @@ __global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r,
         }
         return;
     }
-    const int ex = min(F, W - F * I), ey = min(F, H - F * J), ez = min(F, D - F * K);
-    const int m = ex * ey * ez;
     float sum = 0.0f;
-    for (int q = threadIdx.x; q < m; q += BS) {
-        const int i = F * I + q % ex, j = F * J + (q / ex) % ey, k = F * K + q / (ex * ey);
-        sum += r[((size_t)k * H + j) * W + i];
+    for (int q = threadIdx.x; q < (1 << (3 * lf)); q += BS) {
+        const int i = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
+        if (i < W && j < H && k < D) sum += r[((size_t)k * H + j) * W + i];
     }
     double acc = (double)sum;
     for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
@@ -182,13 +181,13 @@ __global__ void k_ws_export(const float* __restrict__ res, const float* __restri
     out2[n + t] = mask[t];
 }
 
-// e = trilinear(prediction) at every fine cell, natural order, into tmp
+// e = trilinear(prediction) at every fine cell, natural order; grid (x
+// blocks, H, D): the y and z weights are uniform over a workgroup
 __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W, int H, int D,
                                                    const float* __restrict__ pred, float invF, int Wc, int Hc, int Dc,
                                                    int Wn, int Hn, const SolveCtl* __restrict__ c) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= W * H * D || c->done) return;
-    const int i = t % W, j = (t / W) % H, k = t / (W * H);
+    const int i = blockIdx.x * BS + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= W || c->done) return;
     auto axis = [invF](int q, int n, int& a0, int& a1, float& w) {
         float f = ((float)q + 0.5f) * invF - 0.5f;
         f = fminf(fmaxf(f, 0.0f), (float)(n - 1));
@@ -208,7 +207,7 @@ __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W,
     const float c11 = P(x0, y1, z1) * (1.0f - wx) + P(x1, y1, z1) * wx;
     const float c0 = c00 * (1.0f - wy) + c10 * wy, c1 = c01 * (1.0f - wy) + c11 * wy;
     const float v = c0 * (1.0f - wz) + c1 * wz;
-    e[t] = (v == v && fabsf(v) <= 3.4e38f) ? v : 0.0f;
+    e[((size_t)k * H + j) * W + i] = (v == v && fabsf(v) <= 3.4e38f) ? v : 0.0f;
 }
 
 // A x at natural cell t = (i, j, k): n x - sum of the in-domain neighbours
@@ -240,17 +239,17 @@ __device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int 
 __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r, const float* __restrict__ e1,
                                                      const double* __restrict__ alpha1,
                                                      const float* __restrict__ e2, int W, int H, int D, int F,
-                                                     int Gw, int Gh, double* __restrict__ partial,
+                                                     int lf, int Gw, int Gh, double* __restrict__ partial,
                                                      double* __restrict__ sg, const SolveCtl* __restrict__ c) {
     if (c->done) return;
     const int b = blockIdx.x;
     const int I = b % Gw, J = (b / Gw) % Gh, K = b / (Gw * Gh);
-    const int ex = min(F, W - F * I), ey = min(F, H - F * J), ez = min(F, D - F * K);
-    const int m = ex * ey * ez;
     const float a1 = e1 ? (float)*alpha1 : 0.0f;
     float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (int q = threadIdx.x; q < m; q += BS) {
-        const int i = F * I + q % ex, j = F * J + (q / ex) % ey, k = F * K + q / (ex * ey);
+    // F = 2^lf: the block's F^3 cells by shifts, cells past the far faces skipped
+    for (int q = threadIdx.x; q < (1 << (3 * lf)); q += BS) {
+        const int i = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
+        if (i >= W || j >= H || k >= D) continue;
         const size_t t = ((size_t)k * H + j) * W + i;
         float rr = r[t];
         if (e1) rr -= a1 * lap(e1, t, i, j, k, W, H, D);
@@ -368,6 +367,7 @@ __global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, 
         c->iters += add;
         c->checks += 2;
         c->res_ws = ra;
+        c->res_prev = c->res;
         if (ra < rb) {
             c->res = ra;
             c->ws = 1;
@@ -377,6 +377,61 @@ __global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, 
             c->ws = 2;
             c->ws_rejected += 1;
         }
+        if (!(c->res > tol) || c->iters >= cap) {
+            c->done = 1;
+            __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __threadfence_system();
+        }
+    }
+}
+
+// The predicted judge (re-applications within a solve): the corrected
+// pressure after the batch of sweeps (A) against the plain path's residual
+// extrapolated from the last check, res * min(1, res / res_prev). Gauss-Seidel
+// converges ever more slowly, so the extrapolation is optimistic for the
+// plain path and the judge conservative. Kept: the solve continues from A,
+// the uncorrected sweeps are skipped. Rejected: the uncorrected pressure
+// gets the same batch of sweeps (k_ws_judge_fin takes its residual), so the
+// solve continues exactly as without the correction.
+__global__ void __launch_bounds__(BS) k_ws_judge_pred(const double* __restrict__ pa, int na, SolveCtl* c,
+                                                      int* host_done, int seq, double tol, int cap, int add) {
+    if (c->done) return;
+    const double ra = block_max(pa, na);
+    if (threadIdx.x == 0) {
+        const double rho = (c->res_prev > 0.0 && c->res >= 0.0) ? fmin(1.0, c->res / c->res_prev) : 1.0;
+        const double pred = c->res * rho;
+        c->checks += 1;
+        c->res_ws = ra;
+        if (ra < pred) {
+            c->res_prev = c->res;
+            c->res = ra;
+            c->iters += add;
+            c->ws = 1;
+            c->ws_kept += 1;
+            c->skip_spare = 1;
+            if (!(c->res > tol) || c->iters >= cap) {
+                c->done = 1;
+                __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __threadfence_system();
+            }
+        } else {
+            c->ws = 2;
+            c->ws_rejected += 1;
+            c->skip_spare = 0;
+        }
+    }
+}
+
+// after a rejection by the predicted judge: the uncorrected path's residual
+__global__ void __launch_bounds__(BS) k_ws_judge_fin(const double* __restrict__ pb, int nb_, SolveCtl* c,
+                                                     int* host_done, int seq, double tol, int cap, int add) {
+    if (c->done || c->ws != 2) return;
+    const double rb = block_max(pb, nb_);
+    if (threadIdx.x == 0) {
+        c->res_prev = c->res;
+        c->res = rb;
+        c->iters += add;
+        c->checks += 1;
         if (!(c->res > tol) || c->iters >= cap) {
             c->done = 1;
             __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -522,10 +577,85 @@ __global__ void __launch_bounds__(GT) k_ws_gsolve(const double* __restrict__ sg,
     }
 }
 
+// The same Galerkin problem when every block is whole (each axis a multiple
+// of F): A_G = F^2 (L_x + L_y + L_z), L the 1-D Neumann graph Laplacian,
+// diagonalised by the orthonormal DCT-II, c_a(i) = s_a cos(pi a (i + 1/2) /
+// n), eigenvalue 2 - 2 cos(pi a / n). Solved exactly in one workgroup:
+// transform b along x, y, z, divide by F^2 (lx + ly + lz) (the constant
+// mode, absent from a consistent b, is set to 0), transform back. FP32 in
+// LDS with the coefficient tables computed in FP64.
+__global__ void __launch_bounds__(GT) k_ws_gdct(const double* __restrict__ sg, const double* __restrict__ alpha1,
+                                                float* __restrict__ x_out, int Gw, int Gh, int Gd, int F, int W,
+                                                int H, int D, const SolveCtl* __restrict__ c) {
+    if (c->done) return;
+    __shared__ float A[WS_GMAX], Bf[WS_GMAX];
+    __shared__ float cs[3][16][16]; // cs[axis][a][i]
+    __shared__ float lam[3][16];
+    __shared__ double red[GT / 64];
+    const int n = Gw * Gh * Gd;
+    const int dims[3] = {Gw, Gh, Gd};
+    for (int q = threadIdx.x; q < 3 * 256; q += GT) {
+        const int ax = q >> 8, a = (q >> 4) & 15, i = q & 15;
+        const int m = dims[ax];
+        if (a < m && i < m) {
+            const double sa = a == 0 ? sqrt(1.0 / m) : sqrt(2.0 / m);
+            cs[ax][a][i] = (float)(sa * cos(M_PI * a * (i + 0.5) / m));
+            if (i == 0) lam[ax][a] = (float)(2.0 - 2.0 * cos(M_PI * a / m));
+        }
+    }
+    // b = sum r' per block, minus its constant part (uniform blocks: the mean)
+    const double a1 = alpha1 ? *alpha1 : 0.0;
+    double sb = 0.0;
+    for (int g = threadIdx.x; g < n; g += GT) {
+        const double bg = sg[2 * g] - a1 * sg[2 * g + 1];
+        A[g] = (float)bg;
+        sb += bg;
+    }
+    for (int o = 32; o >= 1; o >>= 1) sb += __shfl_xor(sb, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sb;
+    __syncthreads();
+    double tot = 0.0;
+    for (int q = 0; q < GT / 64; ++q) tot += red[q];
+    const float mean = (float)(tot / n);
+    for (int g = threadIdx.x; g < n; g += GT) A[g] -= mean;
+    __syncthreads();
+    // one 1-D transform along axis ax (forward: sum_i c[a][i] v[i]; inverse:
+    // sum_a c[a][i] v[a]) from src into dst
+    auto pass = [&](const float* src, float* dst, int ax, bool inv) {
+        const int m = dims[ax];
+        const int st = ax == 0 ? 1 : (ax == 1 ? Gw : Gw * Gh);
+        for (int g = threadIdx.x; g < n; g += GT) {
+            const int co = ax == 0 ? g % Gw : (ax == 1 ? (g / Gw) % Gh : g / (Gw * Gh));
+            const int base = g - co * st;
+            float acc = 0.0f;
+            for (int q = 0; q < m; ++q) acc += (inv ? cs[ax][q][co] : cs[ax][co][q]) * src[base + q * st];
+            dst[g] = acc;
+        }
+        __syncthreads();
+    };
+    pass(A, Bf, 0, false);
+    pass(Bf, A, 1, false);
+    pass(A, Bf, 2, false);
+    const float w = (float)F * (float)F;
+    for (int g = threadIdx.x; g < n; g += GT) {
+        const int a = g % Gw, b = (g / Gw) % Gh, cc = g / (Gw * Gh);
+        const float l = lam[0][a] + lam[1][b] + lam[2][cc];
+        Bf[g] = l > 0.0f ? Bf[g] / (w * l) : 0.0f;
+    }
+    __syncthreads();
+    pass(Bf, A, 2, true);
+    pass(A, Bf, 1, true);
+    pass(Bf, A, 0, true);
+    for (int g = threadIdx.x; g < n; g += GT) x_out[g] = A[g];
+}
+
 __global__ void k_ws_zero(unsigned* a) { *a = 0u; }
 
 // a new application: no decision yet (the restore only follows this one's)
-__global__ void k_ws_begin(SolveCtl* c) { c->ws = 0; }
+__global__ void k_ws_begin(SolveCtl* c) {
+    c->ws = 0;
+    c->skip_spare = 1;
+}
 
 inline unsigned nb(size_t n) { return (unsigned)((n + BS - 1) / BS); }
 
@@ -547,6 +677,12 @@ inline float* ws_r(const Solver& s) { return s.tmp; }
 inline float* ws_enet(const Solver& s) { return s.tmp + ncell(s); }
 inline float* ws_eg(const Solver& s) { return s.tmp + 2 * ncell(s); }
 inline unsigned split_threads(const Solver& s) { return (unsigned)((s.W >> 1) * s.H * s.L.Dg); }
+inline dim3 prolong_grid(const Solver& s) { return dim3((unsigned)((s.W + BS - 1) / BS), (unsigned)s.H, (unsigned)s.D); }
+inline int ilog2(int f) {
+    int l = 0;
+    while ((1 << l) < f) ++l;
+    return l;
+}
 
 } // namespace
 
@@ -606,7 +742,8 @@ static int ws_restrict(Solver& s, float dt, const int* stop) {
     hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
                        scale, ws_r(s), stop);
     hipLaunchKernelGGL(k_ws_zero, dim3(1), dim3(1), 0, s.stream, amax);
-    hipLaunchKernelGGL(k_ws_restrict, dim3((unsigned)nn), dim3(BS), 0, s.stream, ws_r(s), s.W, s.H, s.D, s.ws_f, s.Wc,
+    hipLaunchKernelGGL(k_ws_restrict, dim3((unsigned)nn), dim3(BS), 0, s.stream, ws_r(s), s.W, s.H, s.D, s.ws_f,
+                       ilog2(s.ws_f), s.Wc,
                        s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
     hipLaunchKernelGGL(k_ws_rms, dim3(1), dim3(1024), 0, s.stream, res, (int)nn, s.Wc * s.Hc * s.Dc, amax, stop);
     BLZ_HIP(hipGetLastError());
@@ -638,24 +775,28 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
     const float* enet = pred ? ws_enet(s) : nullptr;
     const bool glob = s.ws_parts & 2;
     if (pred) {
-        hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, ws_enet(s), s.W, s.H, s.D, pred,
+        hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, ws_enet(s), s.W, s.H, s.D, pred,
                            1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
         hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, ws_enet(s),
-                           s.W, s.H, s.D, s.ws_g, s.Gw, s.Gh, ws_part(s), glob ? ws_sg(s) : nullptr, c);
+                           s.W, s.H, s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), glob ? ws_sg(s) : nullptr, c);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
     } else if (glob) {
         hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, nullptr, s.W,
-                           s.H, s.D, s.ws_g, s.Gw, s.Gh, ws_part(s), ws_sg(s), c);
+                           s.H, s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), ws_sg(s), c);
     }
     if (glob) {
         // right-hand side: block sums of r - alpha_net A e_net (sum r' and
         // alpha_net * sum A e_net per block)
-        hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), pred ? a_net : nullptr, ws_xg(s),
-                           s.Gw, s.Gh, s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
-        hipLaunchKernelGGL(k_ws_prolong, dim3(nb(N)), dim3(BS), 0, s.stream, ws_eg(s), s.W, s.H, s.D, ws_xg(s),
+        if (s.W % s.ws_g == 0 && s.H % s.ws_g == 0 && s.D % s.ws_g == 0)
+            hipLaunchKernelGGL(k_ws_gdct, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), pred ? a_net : nullptr,
+                               ws_xg(s), s.Gw, s.Gh, s.Gd, s.ws_g, s.W, s.H, s.D, c);
+        else
+            hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), pred ? a_net : nullptr,
+                               ws_xg(s), s.Gw, s.Gh, s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
+        hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, ws_eg(s), s.W, s.H, s.D, ws_xg(s),
                            1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
         hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), enet, a_net, ws_eg(s), s.W, s.H,
-                           s.D, s.ws_g, s.Gw, s.Gh, ws_part(s), nullptr, c);
+                           s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), nullptr, c);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
     }
     hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.L, s.W, s.H, s.D, enet,
@@ -664,7 +805,8 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
     return BLZ_OK;
 }
 
-int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge) {
+int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
+                     bool predicted) {
     if (!s.ws_net) return BLZ_OK;
     const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
     float* res = s.ws_buf;
@@ -698,12 +840,23 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
     int rc = launch_rbgs_sweeps(s, dt, judge, &c->done, &launches, &kind);
     s.rbgs_variant = keep;
     if (rc) return rc;
-    if (int e2 = launch_rbgs_spare(s, dt, judge, &c->done)) return e2;
     int na = 0, nb2 = 0;
-    if (int e2 = launch_residual_partial(s, dt, false, s.red + 8, &c->done, &na)) return e2;
-    if (int e2 = launch_residual_partial(s, dt, true, s.red + 8 + RESIDUAL_BLOCKS, &c->done, &nb2)) return e2;
-    hipLaunchKernelGGL(k_ws_judge, dim3(1), dim3(BS), 0, s.stream, s.red + 8, na, s.red + 8 + RESIDUAL_BLOCKS, nb2, c,
-                       host_done, seq, tol, cap, judge);
+    if (predicted) {
+        if (int e2 = launch_residual_partial(s, dt, false, s.red + 8, &c->done, &na)) return e2;
+        hipLaunchKernelGGL(k_ws_judge_pred, dim3(1), dim3(BS), 0, s.stream, s.red + 8, na, c, host_done, seq, tol,
+                           cap, judge);
+        if (int e2 = launch_rbgs_spare(s, dt, judge, &c->skip_spare)) return e2;
+        if (int e2 = launch_residual_partial(s, dt, true, s.red + 8 + RESIDUAL_BLOCKS, &c->skip_spare, &nb2))
+            return e2;
+        hipLaunchKernelGGL(k_ws_judge_fin, dim3(1), dim3(BS), 0, s.stream, s.red + 8 + RESIDUAL_BLOCKS, nb2, c,
+                           host_done, seq, tol, cap, judge);
+    } else {
+        if (int e2 = launch_rbgs_spare(s, dt, judge, &c->done)) return e2;
+        if (int e2 = launch_residual_partial(s, dt, false, s.red + 8, &c->done, &na)) return e2;
+        if (int e2 = launch_residual_partial(s, dt, true, s.red + 8 + RESIDUAL_BLOCKS, &c->done, &nb2)) return e2;
+        hipLaunchKernelGGL(k_ws_judge, dim3(1), dim3(BS), 0, s.stream, s.red + 8, na, s.red + 8 + RESIDUAL_BLOCKS,
+                           nb2, c, host_done, seq, tol, cap, judge);
+    }
     hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
                        spare, s.prb, 2 * s.Np);
     BLZ_HIP(hipGetLastError());

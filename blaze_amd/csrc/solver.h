@@ -72,6 +72,9 @@ struct SolveCtl {
     double res, res0; // last and first residual
     double res_ws;    // residual right after the last learned correction
     int ws_kept, ws_rejected; // learned corrections kept / rejected in this solve
+    double res_prev;          // residual of the check before the last one (-1: none)
+    int skip_spare;           // 1: the predicted judge kept the correction (no uncorrected sweeps)
+    int pad_;
 };
 constexpr int WS_GMAX = 4096;      // cells of the warm start's global-correction grid (16^3)
 constexpr int MAX_SOLVES = 64;     // pressure solves per step (bounds max_substeps)
@@ -205,7 +208,8 @@ int launch_residual_partial(Solver& s, float dt, bool spare, double* partial, co
 int residual_blocks(const Solver& s);
 int warm_start_attach(Solver& s, const Net* net);
 // learned correction + `judge` sweeps on both paths (counted once)
-int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge);
+int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
+                     bool predicted = false);
 int warm_start_input(Solver& s, float dt, float* host_in2);
 int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* before, double* after, int* kept);
 int launch_pressure_update(Solver& s, float dt);

@@ -115,7 +115,7 @@ typedef struct blz_stats {
     int rbgs_iters_last;      /* RBGS sweeps in the last substep's projection */
     double residual;          /* ||div u||_inf proxy after the last projection (FP64), -1 if not computed */
     double residual_initial;  /* same measure before the first sweep of the last projection */
-    int warm_start_used;      /* learned corrections kept in the last solve */
+    int warm_start_used;      /* warm-start corrections kept in the solves of the last step (or stage) */
     int warm_start_fallbacks; /* learned corrections rejected so far (pressure restored) */
     double stage_ms[BLZ_NUM_STAGES]; /* device time per stage from HIP events (0 if timing off) */
     double source_ms;         /* device time of the last blz_add_source */

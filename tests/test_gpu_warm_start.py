@@ -32,7 +32,7 @@ def test_rejected_prediction_gives_the_plain_result_bitwise():
         o.set_field(w, a)
     net = B.Net(seed=1)
     net.set_params(net.params() * np.float32(1e30))
-    g.set_warm_start(net)
+    g.set_warm_start(net, network=True, global_correction=False)
     g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
     o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
     compare("ws-rejected P", g.field(B.FIELD_P), o.field(B.FIELD_P))
@@ -40,6 +40,31 @@ def test_rejected_prediction_gives_the_plain_result_bitwise():
     assert gs.warm_start_used == 0 and gs.warm_start_fallbacks == 1
     assert gs.rbgs_iters_last == os_["rbgs_iters_last"]
     assert gs.residual == os_["residual"] and gs.residual_initial == os_["residual_initial"]
+
+
+def test_repeated_empty_corrections_keep_the_plain_result_bitwise():
+    """Re-applications every 20 sweeps go through the predicted judge
+    (plain residual extrapolated from the last check); whether it keeps an
+    empty correction or rejects it (then the uncorrected pressure gets the
+    batch of sweeps), the solve stays the oracle's plain solve bit for bit."""
+    print("[ws] repeated empty corrections", flush=True)
+    s = settings(B, (32, 24, 20), max_iter_solve=80, residual_tol=1e-9, residual_check_every=10, warm_start=20)
+    st = random_state(s, seed=7)
+    g, o = B.Solver(s), O.Oracle(s)
+    for w, a in st.items():
+        g.set_field(w, a)
+        o.set_field(w, a)
+    net = B.Net(seed=1)
+    net.set_params(net.params() * np.float32(1e30))
+    g.set_warm_start(net, network=True, global_correction=False)
+    g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    compare("ws-repeated P", g.field(B.FIELD_P), o.field(B.FIELD_P))
+    gs, os_ = g.stats(), o.stats()
+    print(f"[ws] kept {gs.warm_start_used} rejected {gs.warm_start_fallbacks}", flush=True)
+    assert gs.warm_start_used + gs.warm_start_fallbacks >= 3
+    assert gs.rbgs_iters_last == os_["rbgs_iters_last"]
+    assert gs.residual == os_["residual"]
 
 
 def test_detached_or_disabled_warm_start_is_the_plain_path():
