@@ -458,7 +458,7 @@ def batched(a):
                    "voxels": vox, "parallelism": "single GPU"},
         "batch": {"autotuned_variant": v, "graph_nodes": st.graph_nodes, "capture_ms": st.capture_ms,
                   "tune_ms_per_sweep": {st.tune_variant[k]: st.tune_ms[k] for k in range(6)},
-                  "max_final_residual": max(res)},
+                  "max_final_residual": max(res), "mean_final_residual": sum(res) / len(res)},
     }
     print(json.dumps(out), flush=True)
 
