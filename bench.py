@@ -54,6 +54,10 @@ def parse():
                     help="warm_start setting of that leg (1: before the sweeps, N >= 2: also every N sweeps)")
     ap.add_argument("--warm-parts", type=int, default=3,
                     help="warm-start corrections: 1 network, 2 global, 3 both")
+    ap.add_argument("--no-render", action="store_true", help="skip the render of the plain leg's final state")
+    ap.add_argument("--render-reps", type=int, default=3)
+    ap.add_argument("--render-cpu-rows", type=int, default=8,
+                    help="rows of the CPU oracle's render sample (spread evenly over the image)")
     ap.add_argument("--batched", action="store_true",
                     help="instead: 64 scenes of 32^3..128^3 in one captured-graph step (warm start on)")
     ap.add_argument("--scenes", type=int, default=64)
@@ -274,10 +278,63 @@ def run_leg(a, s, parts, net, tol=None):
             f"{st.residual_checks} residual {st.residual:.3e} step {st.step_ms:.1f} ms", a.quiet)
     g.sync()
     elapsed = time.perf_counter() - t0
+    rend = None
+    if net is None and tol is None and not a.no_render:
+        rend = render_leg(a, g, s)
     return dict(elapsed=elapsed, tot_sub=tot_sub, tot_iters=tot_iters, tot_checks=tot_checks, tot_launch=tot_launch,
                 tot_timed_sweeps=tot_timed_sweeps, tot_rbgs_ms=tot_rbgs_ms, kernel_kind=kernel_kind,
                 stage_ms=stage_ms, residuals=residuals, kept=kept,
-                rejected=g.stats().warm_start_fallbacks - rejected0, variant=g.stats().rbgs_kernel)
+                rejected=g.stats().warm_start_fallbacks - rejected0, variant=g.stats().rbgs_kernel, render=rend)
+
+
+def render_leg(a, g, s):
+    """Render the final state with the scene's render settings (the frame
+    loop's consumer): wall time per frame (scatter build, blur passes, ray
+    march, copy of the image to the host), and the image for the CPU
+    comparison."""
+    import blaze_amd as B
+
+    rs = B.parse_render(a.scene)
+    img = g.render(rs)  # first call allocates the scatter volumes and the frame
+    t0 = time.perf_counter()
+    for _ in range(a.render_reps):
+        img = g.render(rs)
+    ms = (time.perf_counter() - t0) * 1e3 / a.render_reps
+    log(f"[render] {rs.width}x{rs.height}: {ms:.2f} ms per frame, lit {(img.sum(axis=2) > 0).mean():.2f}, "
+        f"max {img.max():.3f}", a.quiet)
+    return dict(ms=ms, settings=rs, image=img, T=g.field(B.FIELD_T),
+                dom=(list(s.domain_min), list(s.domain_max)), dx=float(s.dx))
+
+
+def render_report(a, R):
+    """GPU render time, the CPU oracle on an evenly spread sample of rows
+    (scaled to the frame) and the sampled rows compared bit for bit."""
+    import numpy as np
+
+    rs, img = R["settings"], R["image"]
+    out = {"workload": f"render of the final 256^3 state, {rs.width}x{rs.height}, {rs.light_count} lights, "
+                       f"multi-scatter {rs.ms_blur_iter} blur passes (scene render keys)",
+           "ms_per_frame": R["ms"], "pixels_per_s": rs.width * rs.height / (R["ms"] / 1e3),
+           "lit_fraction": float((img.sum(axis=2) > 0).mean())}
+    if not a.no_cpu_baseline:
+        import oracle as O
+
+        n = max(1, a.render_cpu_rows)
+        stride = max(1, rs.height // n)
+        row0 = stride // 2
+        t0 = time.perf_counter()
+        ref = O.render(R["T"], R["dx"], R["dom"][0], R["dom"][1], rs, rows=(row0, rs.height, stride))
+        t = time.perf_counter() - t0
+        rows = list(range(row0, rs.height, stride))
+        same = bool(np.array_equal(img[rows].view(np.uint32), ref[rows].view(np.uint32)))
+        cpu_ms = t * 1e3 * rs.height / len(rows)
+        out["cpu_baseline"] = {"ms_per_frame": cpu_ms, "cores": O.max_threads(), "kind": "port",
+                               "sample": f"oracle render of {len(rows)} rows (every {stride}th from row {row0}, "
+                                         f"scatter volume included) scaled to {rs.height} rows"}
+        out["sampled_rows_bitwise_equal_to_oracle"] = same
+        log(f"[render] cpu oracle {cpu_ms:.0f} ms per frame (sample of {len(rows)} rows), rows equal {same}",
+            a.quiet)
+    return out
 
 
 def report(a, s, gen, parts, L, warm):
@@ -355,6 +412,8 @@ def report(a, s, gen, parts, L, warm):
         },
         "cpu_baseline": cpu,
     }
+    if L.get("render") is not None:
+        out["render"] = render_report(a, L["render"])
     if warm is not None:
         def leg(L):
             return {"value": N3 * L["tot_sub"] / L["elapsed"], "ms_per_step": L["elapsed"] * 1e3 / a.steps,

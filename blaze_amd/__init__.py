@@ -78,6 +78,24 @@ class BatchStats(C.Structure):
     ]
 
 
+class RenderSettings(C.Structure):
+    """blz_render_settings: camera, lights, sampling, shader, colour ramp and
+    multi-scatter terms of the renderer (the .blz render keys)."""
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("cam_focal", C.c_float), ("cam_aperture", C.c_float),
+        ("cam_u", C.c_float * 3), ("cam_v", C.c_float * 3), ("cam_w", C.c_float * 3), ("cam_trans", C.c_float * 3),
+        ("p_step", C.c_float), ("s_step", C.c_float), ("cutoff", C.c_float), ("light_count", C.c_int),
+        ("light_dir", (C.c_float * 3) * 10), ("light_color", (C.c_float * 3) * 10),
+        ("light_intensity", C.c_float * 10), ("light_exposure", C.c_float * 10), ("light_samples", C.c_int * 10),
+        ("density_scale", C.c_float), ("shadow_density_scale", C.c_float),
+        ("volume_color", C.c_float * 3), ("scattering", C.c_float * 3),
+        ("absorption", C.c_float), ("gain", C.c_float), ("emission_scale", C.c_float),
+        ("ramp_size", C.c_int), ("ramp_keys", C.c_float * 10), ("ramp_colors", (C.c_float * 3) * 10),
+        ("temp_range", C.c_float * 2), ("ms_scale", C.c_float), ("ms_mask", C.c_float * 2),
+        ("ms_color", C.c_float * 3), ("ms_blur_iter", C.c_int),
+    ]
+
+
 class ParticleGen(C.Structure):
     _fields_ = [
         ("count", C.c_int), ("seed", C.c_ulonglong), ("center", C.c_float * 3),
@@ -132,6 +150,9 @@ def _load():
                                 P(C.c_double)], C.c_int),
         "blz_set_warm_start": ([vp, vp], C.c_int),
         "blz_set_warm_start_parts": ([vp, C.c_int], C.c_int),
+        "blz_render_settings_default": ([P(RenderSettings)], C.c_int),
+        "blz_parse_render": ([C.c_char_p, P(RenderSettings)], C.c_int),
+        "blz_render": ([vp, P(RenderSettings), P(C.c_float)], C.c_int),
         "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
         "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
         "blz_warm_start_apply": ([vp, C.c_float, P(C.c_float), P(C.c_double), P(C.c_double), P(C.c_int)], C.c_int),
@@ -175,6 +196,19 @@ def parse_scene(path: str):
     _check(_lib.blz_parse_scene(path.encode(), C.byref(s), d, 4096, pf, 4096, C.byref(g), C.byref(has)),
            f"parse_scene({path})")
     return s, d.value.decode(), pf.value.decode(), (g if has.value else None)
+
+
+def default_render_settings() -> RenderSettings:
+    r = RenderSettings()
+    _check(_lib.blz_render_settings_default(C.byref(r)), "blz_render_settings_default")
+    return r
+
+
+def parse_render(path: str) -> RenderSettings:
+    """The render keys of a .blz (camera, lights, shader, ramp, multi-scatter)."""
+    r = RenderSettings()
+    _check(_lib.blz_parse_render(path.encode(), C.byref(r)), f"parse_render({path})")
+    return r
 
 
 def generate_particles(gen: ParticleGen) -> np.ndarray:
@@ -289,6 +323,12 @@ class Solver:
         _check(_lib.blz_autotune_rbgs(self._h, sweeps, C.byref(v), ms), "blz_autotune_rbgs")
         cand = [2, 4, 8, 102, 103, 202, 203, 402, 403]
         return v.value, {c: ms[i] for i, c in enumerate(cand) if ms[i] > 0}
+
+    def render(self, settings: RenderSettings) -> np.ndarray:
+        """(height, width, 3) float32 image of the current temperature."""
+        out = np.empty((settings.height, settings.width, 3), np.float32)
+        _check(_lib.blz_render(self._h, C.byref(settings), out.ctypes.data_as(C.POINTER(C.c_float))), "blz_render")
+        return out
 
     def set_warm_start(self, net, network: bool = True, global_correction: bool = True):
         """Attach a Net (None detaches); used when settings.warm_start != 0

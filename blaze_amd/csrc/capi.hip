@@ -34,6 +34,7 @@ Solver::~Solver() {
     if (ctl_host) (void)hipHostFree(ctl_host);
     if (ws_net) delete ws_net;
     if (ws_buf) (void)hipFree(ws_buf);
+    if (rnd_buf) (void)hipFree(rnd_buf);
     if (zero_flag) (void)hipFree(zero_flag);
     if (done_host) (void)hipHostFree((void*)done_host);
     for (hipEvent_t e : batch_ev)
@@ -660,6 +661,15 @@ int blz_set_warm_start(blz_solver* h, blz_net* net) {
     CHECK_HANDLE(h);
     BLZ_HIP(hipStreamSynchronize(h->s.stream));
     return blz::warm_start_attach(h->s, net ? &net->n : nullptr);
+}
+
+int blz_render(blz_solver* h, const blz_render_settings* r, float* rgb) {
+    CHECK_HANDLE(h);
+    if (!r || !rgb) {
+        blz::set_error("blz_render: null argument");
+        return BLZ_E_INVALID;
+    }
+    return blz::render_frame(h->s, *r, rgb);
 }
 
 int blz_set_warm_start_parts(blz_solver* h, int parts) {

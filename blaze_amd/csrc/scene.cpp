@@ -292,3 +292,165 @@ extern "C" int blz_generate_particles(const blz_particle_gen* g, float* part8) {
     }
     return BLZ_OK;
 }
+
+// ---- render settings (src/Parser.h:109-170: lights, cam, sampling, shader,
+// colour ramp, multi-scatter; all required there and here) ----
+
+extern "C" int blz_render_settings_default(blz_render_settings* r) {
+    if (!r) {
+        blz::set_error("blz_render_settings_default: null settings");
+        return BLZ_E_INVALID;
+    }
+    std::memset(r, 0, sizeof(*r));
+    r->width = 64;
+    r->height = 64;
+    r->cam_focal = 50.0f;
+    r->cam_aperture = 41.4214f;
+    r->cam_u[0] = 1.0f;
+    r->cam_v[1] = 1.0f;
+    r->cam_w[2] = 1.0f;
+    r->cam_trans[0] = 0.5f;
+    r->cam_trans[1] = 0.5f;
+    r->cam_trans[2] = 3.0f;
+    r->p_step = 1.0f / 64.0f;
+    r->s_step = 2.0f / 64.0f;
+    r->cutoff = 0.01f;
+    r->density_scale = 1.0f;
+    r->shadow_density_scale = 1.0f;
+    for (int a = 0; a < 3; ++a) {
+        r->volume_color[a] = 1.0f;
+        r->scattering[a] = 1.0f;
+    }
+    r->ramp_size = 1;
+    r->ramp_colors[0][0] = r->ramp_colors[0][1] = r->ramp_colors[0][2] = 1.0f;
+    r->temp_range[1] = 1.0f;
+    r->ms_mask[1] = 1.0f;
+    return BLZ_OK;
+}
+
+extern "C" int blz_parse_render(const char* path, blz_render_settings* out) {
+    if (!path || !out) {
+        blz::set_error("blz_parse_render: null argument");
+        return BLZ_E_INVALID;
+    }
+    std::ifstream f(path, std::ios::binary);
+    if (!f.good()) {
+        blz::set_error(std::string("scene file does not exist: ") + path);
+        return BLZ_E_IO;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    Value root;
+    std::string err;
+    if (!blz::json::parse(ss.str(), root, err) || !root.is_object()) {
+        blz::set_error(std::string("malformed .blz ") + path + ": " + (err.empty() ? "not an object" : err));
+        return BLZ_E_PARSE;
+    }
+    blz_render_settings r;
+    blz_render_settings_default(&r);
+    Reader rd(root);
+    rd.arr("renderRes", 2, nullptr, &r.width);
+    rd.integer("lightCount", &r.light_count);
+    rd.num("primarySampleRate", &r.p_step);
+    rd.num("shadowSamplingRate", &r.s_step);
+    rd.num("cutoff", &r.cutoff);
+    rd.num("densityScale", &r.density_scale);
+    rd.num("shadowDensityScale", &r.shadow_density_scale);
+    rd.arr("volumeColor", 3, r.volume_color, nullptr);
+    rd.arr("scatteringColor", 3, r.scattering, nullptr);
+    rd.num("absorption", &r.absorption);
+    rd.num("gain", &r.gain);
+    rd.num("emissionScale", &r.emission_scale);
+    rd.arr("shaderTempRange", 2, r.temp_range, nullptr);
+    rd.num("multiScatterScale", &r.ms_scale);
+    rd.arr("multiScatterDensityMask", 2, r.ms_mask, nullptr);
+    rd.arr("multiScatterColor", 3, r.ms_color, nullptr);
+    rd.integer("multiScatterBlurIter", &r.ms_blur_iter);
+    std::string bad;
+    if (rd.ok && (r.light_count < 0 || r.light_count > BLZ_MAX_LIGHTS)) bad = "lightCount (0..10)";
+    // lights[i]: light_exposure, light_intensity, light_samples, light_dir, light_color
+    const Value* lights = rd.ok ? rd.get("lights", true) : nullptr;
+    if (rd.ok && bad.empty()) {
+        if (!lights || !lights->is_array() || (int)lights->arr.size() < r.light_count) {
+            bad = "lights (fewer than lightCount)";
+        } else {
+            for (int i = 0; i < r.light_count && bad.empty(); ++i) {
+                Reader l(lights->arr[i]);
+                l.num("light_exposure", &r.light_exposure[i]);
+                l.num("light_intensity", &r.light_intensity[i]);
+                l.integer("light_samples", &r.light_samples[i]);
+                l.arr("light_dir", 3, r.light_dir[i], nullptr);
+                l.arr("light_color", 3, r.light_color[i], nullptr);
+                if (!l.ok) bad = "lights[" + std::to_string(i) + "]." + l.missing;
+            }
+        }
+    }
+    // cam: focal, aperture, rotMat rows -> u, v, w, t
+    if (rd.ok && bad.empty()) {
+        const Value* cam = rd.get("cam", true);
+        if (cam && cam->is_object()) {
+            Reader c(*cam);
+            c.num("focal", &r.cam_focal);
+            c.num("aperture", &r.cam_aperture);
+            c.arr("t", 3, r.cam_trans, nullptr);
+            const Value* rm = c.get("rotMat", true);
+            if (c.ok && (!rm || !rm->is_array() || rm->arr.size() < 3)) bad = "cam.rotMat (3 rows)";
+            float* rows[3] = {r.cam_u, r.cam_v, r.cam_w};
+            for (int q = 0; q < 3 && c.ok && bad.empty(); ++q) {
+                const Value& row = rm->arr[q];
+                if (!row.is_array() || row.arr.size() < 3) {
+                    bad = "cam.rotMat row";
+                    break;
+                }
+                for (int a = 0; a < 3; ++a) {
+                    if (!row.arr[a].is_number()) {
+                        bad = "cam.rotMat element";
+                        break;
+                    }
+                    rows[q][a] = (float)row.arr[a].num;
+                }
+            }
+            if (!c.ok && bad.empty()) bad = "cam." + c.missing;
+        } else if (rd.ok) {
+            bad = "cam (object)";
+        }
+    }
+    // shaderColorRamp: count, keys, colors
+    if (rd.ok && bad.empty()) {
+        const Value* ramp = rd.get("shaderColorRamp", true);
+        if (ramp && ramp->is_object()) {
+            Reader c(*ramp);
+            c.integer("count", &r.ramp_size);
+            if (c.ok && (r.ramp_size < 1 || r.ramp_size > BLZ_MAX_RAMP)) bad = "shaderColorRamp.count (1..10)";
+            if (c.ok && bad.empty()) {
+                c.arr("keys", r.ramp_size, r.ramp_keys, nullptr);
+                const Value* cols = c.get("colors", true);
+                if (c.ok && (!cols || !cols->is_array() || (int)cols->arr.size() < r.ramp_size)) {
+                    bad = "shaderColorRamp.colors";
+                } else {
+                    for (int q = 0; q < r.ramp_size && c.ok && bad.empty(); ++q) {
+                        const Value& col = cols->arr[q];
+                        if (!col.is_array() || col.arr.size() < 3) {
+                            bad = "shaderColorRamp.colors row";
+                            break;
+                        }
+                        for (int a = 0; a < 3; ++a) r.ramp_colors[q][a] = (float)col.arr[a].num;
+                    }
+                }
+            }
+            if (!c.ok && bad.empty()) bad = "shaderColorRamp." + c.missing;
+        } else if (rd.ok) {
+            bad = "shaderColorRamp (object)";
+        }
+    }
+    if (!rd.ok) {
+        blz::set_error(std::string("missing or invalid .blz render key: ") + rd.missing);
+        return BLZ_E_PARSE;
+    }
+    if (!bad.empty()) {
+        blz::set_error(std::string("missing or invalid .blz render key: ") + bad);
+        return BLZ_E_PARSE;
+    }
+    *out = r;
+    return BLZ_OK;
+}

@@ -161,6 +161,8 @@ struct Solver {
     int Wc = 0, Hc = 0, Dc = 0; // coarse grid, real cells
     int Wn = 0, Hn = 0, Dn = 0; // network grid (coarse rounded up to multiples of 4)
     float* ws_buf = nullptr;   // coarse residual, mask (Wn*Hn*Dn each), scale bits
+    float* rnd_buf = nullptr;  // renderer: two scatter volumes and the frame buffer
+    size_t rnd_bytes = 0;
     // captured (batched) steps: noise offsets of wind and turbulence read
     // from device memory instead of kernel arguments
     const float* dev_off = nullptr;
@@ -201,6 +203,7 @@ int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int 
                           int first);
 int launch_ctl_reset(Solver& s, SolveCtl* c);
 int launch_residual_ws(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
+int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb);
 int warm_start_set_parts(Solver& s, int parts);
 int launch_residual_ws_final(Solver& s, SolveCtl* c, int* host_done, int seq, double tol, int cap, int nb_partial);
 int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop);

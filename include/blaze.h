@@ -14,7 +14,9 @@
  *     step()                                src/FluidSolver.cu:858-955  -> blz_step
  *     getTempGrid()                         src/FluidSolver.h:108       -> blz_get_field(BLZ_FIELD_T)
  *   struct SceneSettings (solver subset)    src/SceneSettings.h:32-71   -> blz_settings
- *   Parser::sceneParse                      src/Parser.h:41-172         -> blz_parse_scene
+ *   Parser::sceneParse                      src/Parser.h:41-172         -> blz_parse_scene,
+ *                                                                          blz_parse_render
+ *   RenderEngine::render                    src/RenderEngine.cu:312-365 -> blz_render
  *   Parser::sourceParticleParse             src/Parser.h:174-211        -> blz_read_particles
  *
  * Threading: one handle per host thread. A handle owns all of its device
@@ -186,6 +188,48 @@ int blz_set_rbgs_variant(blz_solver* h, int variant);
    order 2, 4, 8, 102, 103, 202, 203, 402, 403 (fused ones skipped when the
    spare planes do not exist). */
 int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep);
+
+/* ---- renderer: the solver's consumer (RenderEngine::render,
+   src/RenderEngine.cu:312-365, render_kernel :149-225; Camera, Light,
+   Shader, ColorRamp in src/Camera.h, src/Light.h, src/Shader.h,
+   src/ColorRamp.h). Ray-marches the temperature grid as emission +
+   single-scattered light from directional lights with shadow rays, plus a
+   blurred 4x-downsampled multi-scatter term. Texture filtering is explicit
+   FP32 trilinear with clamped edges (the reference's linear-filtered,
+   clamp-addressed textures, whose hardware weights are 8-bit fixed point). */
+#define BLZ_MAX_LIGHTS 10
+#define BLZ_MAX_RAMP 10
+typedef struct blz_render_settings {
+    int width, height;                              /* renderRes */
+    float cam_focal, cam_aperture;                  /* cam.focal, cam.aperture */
+    float cam_u[3], cam_v[3], cam_w[3];             /* rows of cam.rotMat */
+    float cam_trans[3];                             /* cam.t */
+    float p_step, s_step, cutoff;                   /* primarySampleRate, shadowSamplingRate, cutoff */
+    int light_count;                                /* lightCount (<= BLZ_MAX_LIGHTS) */
+    float light_dir[BLZ_MAX_LIGHTS][3];
+    float light_color[BLZ_MAX_LIGHTS][3];
+    float light_intensity[BLZ_MAX_LIGHTS];
+    float light_exposure[BLZ_MAX_LIGHTS];
+    int light_samples[BLZ_MAX_LIGHTS];              /* read, unused (as the reference) */
+    float density_scale, shadow_density_scale;      /* densityScale, shadowDensityScale */
+    float volume_color[3], scattering[3];           /* volumeColor, scatteringColor */
+    float absorption, gain, emission_scale;         /* absorption, gain, emissionScale */
+    int ramp_size;                                  /* shaderColorRamp.count (<= BLZ_MAX_RAMP) */
+    float ramp_keys[BLZ_MAX_RAMP];
+    float ramp_colors[BLZ_MAX_RAMP][3];
+    float temp_range[2];                            /* shaderTempRange */
+    float ms_scale;                                 /* multiScatterScale */
+    float ms_mask[2];                               /* multiScatterDensityMask */
+    float ms_color[3];                              /* multiScatterColor */
+    int ms_blur_iter;                               /* multiScatterBlurIter */
+} blz_render_settings;
+int blz_render_settings_default(blz_render_settings* r);
+/* The render keys of a .blz (src/Parser.h:109-170); all required. */
+int blz_parse_render(const char* path, blz_render_settings* out);
+/* Render the solver's current temperature (RenderEngine::render): rgb
+   receives width*height*3 floats, pixel (i, j) at 3*(i + width*j). Width
+   and height must be multiples of 8 (as the reference's 8x8 launch). */
+int blz_render(blz_solver* h, const blz_render_settings* r, float* rgb);
 
 /* ---- learned warm start (no reference counterpart; the reference solve
    always starts from the previous pressure, src/FluidSolver.cu:517-599) ----

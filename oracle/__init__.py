@@ -39,6 +39,23 @@ class _Settings(C.Structure):
     ]
 
 
+class _RenderSettings(C.Structure):
+    """blz_render_settings (include/blaze.h), declared again here."""
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("cam_focal", C.c_float), ("cam_aperture", C.c_float),
+        ("cam_u", C.c_float * 3), ("cam_v", C.c_float * 3), ("cam_w", C.c_float * 3), ("cam_trans", C.c_float * 3),
+        ("p_step", C.c_float), ("s_step", C.c_float), ("cutoff", C.c_float), ("light_count", C.c_int),
+        ("light_dir", (C.c_float * 3) * 10), ("light_color", (C.c_float * 3) * 10),
+        ("light_intensity", C.c_float * 10), ("light_exposure", C.c_float * 10), ("light_samples", C.c_int * 10),
+        ("density_scale", C.c_float), ("shadow_density_scale", C.c_float),
+        ("volume_color", C.c_float * 3), ("scattering", C.c_float * 3),
+        ("absorption", C.c_float), ("gain", C.c_float), ("emission_scale", C.c_float),
+        ("ramp_size", C.c_int), ("ramp_keys", C.c_float * 10), ("ramp_colors", (C.c_float * 3) * 10),
+        ("temp_range", C.c_float * 2), ("ms_scale", C.c_float), ("ms_mask", C.c_float * 2),
+        ("ms_color", C.c_float * 3), ("ms_blur_iter", C.c_int),
+    ]
+
+
 def _lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"oracle: {LIB_PATH} missing; run make -C oracle")
@@ -66,6 +83,8 @@ def _lib():
         "orc_noise_hash": ([C.c_uint], C.c_uint),
         "orc_lattice_hash": ([f, f, f, f], C.c_uint),
         "orc_sample": ([vp, i, i, i, d, d, d], d),
+        "orc_render": ([P(f), i, i, i, f, P(f), P(f), P(_RenderSettings), P(f), i, i, i], i),
+        "orc_bexp": ([f], f),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -157,3 +176,28 @@ class Oracle:
 
     def sample(self, which: int, pos, cubic=False, clamp=False) -> float:
         return lib.orc_sample(self._h, which, int(cubic), int(clamp), *[float(v) for v in pos])
+
+
+def render(T, dx, dom_min, dom_max, settings, rows=None):
+    """Image (height, width, 3) float32 of the temperature grid T (D, H, W)
+    with the render settings (any ctypes struct of the blz_render_settings
+    layout); rows = (row0, row1[, stride]) renders only rows row0, row0 +
+    stride, ... below row1 (the others stay 0)."""
+    rs = _RenderSettings()
+    C.memmove(C.byref(rs), C.byref(settings), C.sizeof(_RenderSettings))
+    T = np.ascontiguousarray(T, np.float32)
+    D, H, W = T.shape
+    out = np.zeros((rs.height, rs.width, 3), np.float32)
+    lo = (C.c_float * 3)(*dom_min)
+    hi = (C.c_float * 3)(*dom_max)
+    r0, r1, st = (tuple(rows) + (1,))[:3] if rows is not None else (0, rs.height, 1)
+    fp = C.POINTER(C.c_float)
+    rc = lib.orc_render(T.ctypes.data_as(fp), W, H, D, C.c_float(dx), lo, hi, C.byref(rs),
+                        out.ctypes.data_as(fp), int(r0), int(r1), int(st))
+    if rc != 0:
+        raise ValueError("orc_render: invalid arguments")
+    return out
+
+
+def bexp(x: float) -> float:
+    return lib.orc_bexp(C.c_float(x))

@@ -8,6 +8,7 @@
 #include <new>
 
 #include "blaze_oracle.hpp"
+#include "render_oracle.hpp"
 
 namespace {
 
@@ -177,5 +178,19 @@ double orc_sample(void* p, int which, int cubic, int clamp, double x, double y, 
     auto pos = orc::mk<float>((float)x, (float)y, (float)z);
     return cubic ? g->tricubic(pos, clamp != 0) : g->trilinear(pos);
 }
+
+// the renderer over a caller-given temperature grid (rows row0, row0 +
+// stride, ... below row1)
+int orc_render(const float* T, int W, int H, int D, float dx, const float* dmin, const float* dmax,
+               const blz_render_settings* r, float* rgb, int row0, int row1, int stride) {
+    if (!T || !dmin || !dmax || !r || !rgb || W <= 0 || H <= 0 || D <= 0 || r->width <= 0 || r->height <= 0 ||
+        row0 < 0 || row1 > r->height || row0 > row1 || stride < 1 || !(r->p_step > 0.0f) || !(r->s_step > 0.0f) ||
+        r->light_count < 0 || r->light_count > BLZ_MAX_LIGHTS || r->ramp_size < 1 || r->ramp_size > BLZ_MAX_RAMP)
+        return -1;
+    rnd::render(T, W, H, D, dx, dmin, dmax, *r, rgb, row0, row1, stride);
+    return 0;
+}
+
+float orc_bexp(float x) { return rnd::bexp(x); }
 
 } // extern "C"
