@@ -153,6 +153,7 @@ def _load():
         "blz_render_settings_default": ([P(RenderSettings)], C.c_int),
         "blz_parse_render": ([C.c_char_p, P(RenderSettings)], C.c_int),
         "blz_render": ([vp, P(RenderSettings), P(C.c_float)], C.c_int),
+        "blz_write_image": ([C.c_char_p, P(C.c_float), C.c_int, C.c_int], C.c_int),
         "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
         "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
         "blz_warm_start_apply": ([vp, C.c_float, P(C.c_float), P(C.c_double), P(C.c_double), P(C.c_int)], C.c_int),
@@ -209,6 +210,13 @@ def parse_render(path: str) -> RenderSettings:
     r = RenderSettings()
     _check(_lib.blz_parse_render(path.encode(), C.byref(r)), f"parse_render({path})")
     return r
+
+
+def write_image(path: str, img: np.ndarray):
+    """(height, width, 3) float image -> .ppm / .png / .exr by extension."""
+    a = np.ascontiguousarray(img, np.float32)
+    h, w, _ = a.shape
+    _check(_lib.blz_write_image(path.encode(), a.ctypes.data_as(C.POINTER(C.c_float)), w, h), f"write_image({path})")
 
 
 def generate_particles(gen: ParticleGen) -> np.ndarray:

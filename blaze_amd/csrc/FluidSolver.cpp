@@ -61,6 +61,8 @@ void Parser::sceneParse(SceneSettings* scene, const char* sceneFile) {
             scene->renderRes.y = (int)v->arr[1].num;
         }
     }
+    // renderer keys: all present, or the scene is simulated without frames
+    scene->hasRender = blz_parse_render(sceneFile, &scene->render) == BLZ_OK;
     scene->sourceAnimated = s.source_animated != 0;
     scene->sourceRange = {s.source_range[0], s.source_range[1]};
     scene->sourceMaxParticleCount = s.source_max_particle_count;

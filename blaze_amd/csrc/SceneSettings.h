@@ -2,7 +2,8 @@
 //
 // Mirrors the member names of the reference struct (asiroisvigneux/Blaze
 // src/SceneSettings.h:32-71) for the solver and source fields, with plain
-// host vector types in place of CUDA's. Renderer fields are out of scope.
+// host vector types in place of CUDA's; the renderer's keys are kept as the
+// C-ABI blz_render_settings.
 // toC() yields the C-ABI blz_settings consumed by the HIP library.
 #pragma once
 
@@ -66,6 +67,9 @@ struct SceneSettings {
     int warmStart;
     bool hasParticleGen;
     blz_particle_gen particleGen;
+    // renderer keys (src/SceneSettings.h:73-107), parsed when present
+    bool hasRender;
+    blz_render_settings render;
 
     blz_settings toC() const {
         blz_settings s;

@@ -16,6 +16,8 @@ class Timer {
     int iter = 0;
     float time = 0.0f;
     blz_stats last{};
+    double render_ms = 0.0; // last RenderEngine::render (wall, includes the image copy)
+    double write_ms = 0.0;  // last RenderEngine::writeToDisk
 
     std::string getStats() const {
         static const char* names[BLZ_NUM_STAGES] = {"cooldown", "drag", "vorticity", "buoyancy", "wind",
@@ -34,6 +36,10 @@ class Timer {
         }
         std::snprintf(buf, sizeof(buf), "  step     %8.3f ms\n", last.step_ms);
         msg += buf;
+        if (render_ms > 0.0) {
+            std::snprintf(buf, sizeof(buf), "  render   %8.3f ms\n  write    %8.3f ms\n", render_ms, write_ms);
+            msg += buf;
+        }
         return msg;
     }
 };

@@ -1,8 +1,9 @@
 // main.cpp — `blaze [-l] scene.blz`: runs the solver frame loop of a scene.
 //
-// Reference: src/main.cu:32-117 (argParse, frame loop). The renderer is out
-// of scope; each frame prints the solver statistics, and -l appends them to
-// <sceneDir>/../logs/log_<scene>.txt like the reference Logger.
+// Reference: src/main.cu:32-117 (argParse, frame loop). Frames inside the
+// render range are rendered and written (RenderEngine) when the scene has
+// the render keys; each frame prints the solver statistics, and -l appends
+// them to <sceneDir>/../logs/log_<scene>.txt like the reference Logger.
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -10,6 +11,7 @@
 
 #include "FluidSolver.h"
 #include "Parser.h"
+#include "RenderEngine.h"
 
 static void printHelp() {
     std::printf("blaze (MI355X) solver\n"
@@ -59,18 +61,25 @@ int main(int argc, char* argv[]) {
     }
     Timer tmr;
     FluidSolver solver(&tmr, &scn);
+    RenderEngine* renderer = scn.hasRender ? new RenderEngine(&tmr, scn) : nullptr;
+    if (!renderer) std::printf("scene has no render keys: frames are simulated, not rendered\n");
     tmr.time = 0.0f;
     tmr.iter = scn.sourceRange.x;
     const int last = scn.renderRange.y > 0 ? scn.renderRange.y : scn.sourceRange.y;
     while (tmr.iter <= last) {
         if (tmr.iter <= scn.sourceRange.y) solver.addSource();
         solver.step();
+        if (renderer && tmr.iter >= scn.renderRange.x && tmr.iter <= scn.renderRange.y) {
+            renderer->render(&solver);
+            renderer->writeToDisk();
+        }
         const std::string msg = tmr.getStats();
         std::fputs(msg.c_str(), stdout);
         if (log) std::fputs(msg.c_str(), log);
         tmr.iter++;
         tmr.time += scn.dt;
     }
+    delete renderer;
     if (log) std::fclose(log);
     return 0;
 }

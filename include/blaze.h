@@ -17,6 +17,7 @@
  *   Parser::sceneParse                      src/Parser.h:41-172         -> blz_parse_scene,
  *                                                                          blz_parse_render
  *   RenderEngine::render                    src/RenderEngine.cu:312-365 -> blz_render
+ *   RenderEngine::writeToDisk               src/RenderEngine.cu:367-476 -> blz_write_image
  *   Parser::sourceParticleParse             src/Parser.h:174-211        -> blz_read_particles
  *
  * Threading: one handle per host thread. A handle owns all of its device
@@ -230,6 +231,11 @@ int blz_parse_render(const char* path, blz_render_settings* out);
    receives width*height*3 floats, pixel (i, j) at 3*(i + width*j). Width
    and height must be multiples of 8 (as the reference's 8x8 launch). */
 int blz_render(blz_solver* h, const blz_render_settings* r, float* rgb);
+
+/* Frame writer (RenderEngine::writeToDisk, src/RenderEngine.cu:367-476): by
+   extension .ppm (ASCII P3, int(255.99 v)), .png (8-bit RGBA, (int)(255 v)
+   clamped) or .exr (FLOAT B, G, R, uncompressed); rgb as blz_render. */
+int blz_write_image(const char* path, const float* rgb, int width, int height);
 
 /* ---- learned warm start (no reference counterpart; the reference solve
    always starts from the previous pressure, src/FluidSolver.cu:517-599) ----
