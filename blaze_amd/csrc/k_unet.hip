@@ -50,9 +50,11 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
 // derivative at the activation M (1 where M > 0, NET_LEAK elsewhere).
 template <int NT>
 __global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, int Cx, int xoff, int Cin,
-                                                   const bf16* __restrict__ Wt, bf16* __restrict__ Y, int Cy, int yoff,
-                                                   const bf16* __restrict__ Mk, int Cm, int moff, int relu, Geo g,
-                                                   long long nvox) {
+                                                   const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y, int Cy,
+                                                   int yoff, const bf16* __restrict__ Mk, int Cm, int moff, int relu,
+                                                   Geo g, long long nvox) {
+    // this workgroup's output channels: tiles blockIdx.y*NT .. +NT-1 of 16
+    const int c0 = blockIdx.y * NT * 16;
     const int lane = threadIdx.x & 63;
     const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
     if (v0 >= nvox) return;
@@ -72,7 +74,7 @@ __global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, i
         const bool inb = vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
         const long long vv = va + dz * HW + dy * g.W + dx;
         const bf16* xp = X + (inb ? vv : 0) * Cx + xoff + kb * 4;
-        const bf16* wp = Wt + ((size_t)tap * (NT * 16) + li) * Cin + kb * 4;
+        const bf16* wp = Wt + ((size_t)tap * Cout + c0 + li) * Cin + kb * 4;
         for (int c = 0; c < nchunk; ++c) {
             s4v a = {0, 0, 0, 0};
             if (inb) a = *(const s4v*)(xp + c * 16);
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, i
             const long long vo = v0 + kb * 4 + q;
             if (vo >= nvox) continue;
             float r = acc[t][q];
-            const int co = t * 16 + li;
+            const int co = c0 + t * 16 + li;
             if (relu) r = r > 0.0f ? r : NET_LEAK * r;
             if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : NET_LEAK * r;
             Y[vo * Cy + yoff + co] = f2bf(r);
@@ -366,12 +368,17 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     const Geo g{d.B, d.D, d.H, d.W};
     const long long nv = d.n();
     const unsigned grid = nblk((nv + 15) / 16, 4);
+    // small grids: one 16-channel tile per wave (grid.y tiles) for more
+    // waves in flight; every output keeps its accumulation order
+    const bool split = (nv + 15) / 16 < 2048 && cout > 16;
+    const int nt = split ? 1 : cout / 16;
+    const dim3 gr(grid, split ? (unsigned)(cout / 16) : 1u);
 #define CONV_CASE(NT)                                                                                              \
     case NT:                                                                                                       \
-        hipLaunchKernelGGL(k_conv3_fwd<NT>, dim3(grid), dim3(256), 0, n.stream, x.p, x.C, x.off, cin, w, y.p, y.C, \
+        hipLaunchKernelGGL(k_conv3_fwd<NT>, gr, dim3(256), 0, n.stream, x.p, x.C, x.off, cin, w, cout, y.p, y.C,   \
                            y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv);                                  \
         break;
-    switch (cout / 16) {
+    switch (nt) {
         CONV_CASE(1)
         CONV_CASE(2)
         CONV_CASE(3)
