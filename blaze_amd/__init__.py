@@ -153,6 +153,8 @@ def _load():
         "blz_render_settings_default": ([P(RenderSettings)], C.c_int),
         "blz_parse_render": ([C.c_char_p, P(RenderSettings)], C.c_int),
         "blz_render": ([vp, P(RenderSettings), P(C.c_float)], C.c_int),
+        "blz_stage_source": ([vp, P(C.c_float), C.c_int], C.c_int),
+        "blz_add_source_staged": ([vp], C.c_int),
         "blz_write_image": ([C.c_char_p, P(C.c_float), C.c_int, C.c_int], C.c_int),
         "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
         "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
@@ -331,6 +333,18 @@ class Solver:
         _check(_lib.blz_autotune_rbgs(self._h, sweeps, C.byref(v), ms), "blz_autotune_rbgs")
         cand = [2, 4, 8, 102, 103, 202, 203, 402, 403]
         return v.value, {c: ms[i] for i, c in enumerate(cand) if ms[i] > 0}
+
+    def stage_source(self, part8: np.ndarray):
+        """Start the upload of the next frame's particles (N, 8) on the copy
+        stream; safe to call from another thread while step() runs."""
+        a = np.ascontiguousarray(part8, np.float32)
+        self._staged = a
+        _check(_lib.blz_stage_source(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), int(a.shape[0])),
+               "blz_stage_source")
+
+    def add_source_staged(self):
+        """addSource with the frame staged last."""
+        _check(_lib.blz_add_source_staged(self._h), "blz_add_source_staged")
 
     def render(self, settings: RenderSettings) -> np.ndarray:
         """(height, width, 3) float32 image of the current temperature."""

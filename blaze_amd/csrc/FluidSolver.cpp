@@ -127,12 +127,39 @@ FluidSolver::FluidSolver(Timer* tmr, SceneSettings* scn) : mScn(scn), mTmr(tmr) 
     check(blz_set_timing(mH, 1), "FluidSolver");
 }
 
-FluidSolver::~FluidSolver() { blz_destroy(mH); }
+FluidSolver::~FluidSolver() {
+    if (mPrefetch.valid()) mPrefetch.wait();
+    blz_destroy(mH);
+}
 
-// addSource, src/FluidSolver.cu:770-821
+// addSource, src/FluidSolver.cu:770-821. Animated .part sources: the frame
+// was read and its upload started by the previous call's worker (or is read
+// now); the worker then reads frame + 1 and stages it while this frame's
+// step runs (the reference parses and copies synchronously).
 void FluidSolver::addSource() {
-    if (!mScn->sourceAnimated ||
-        (mScn->sourceAnimated && mTmr->iter >= mScn->sourceRange.x && mTmr->iter <= mScn->sourceRange.y)) {
+    const bool inRange = mTmr->iter >= mScn->sourceRange.x && mTmr->iter <= mScn->sourceRange.y;
+    if (mScn->sourceAnimated && !mScn->hasParticleGen) {
+        if (!inRange) return;
+        if (mPrefetch.valid()) check(mPrefetch.get(), "addSource (staged upload)");
+        if (mPrefetchFrame != mTmr->iter) {
+            Parser::sourceParticleParse(mScn, mPart, mParticleCount, mTmr->iter);
+            check(blz_stage_source(mH, mPart.data(), mParticleCount), "addSource");
+        }
+        check(blz_add_source_staged(mH), "addSource");
+        const int next = mTmr->iter + 1;
+        mPrefetchFrame = -1;
+        if (next <= mScn->sourceRange.y) {
+            mPrefetchFrame = next;
+            mPrefetch = std::async(std::launch::async, [this, next]() -> int {
+                std::vector<float> buf;
+                int n = 0;
+                Parser::sourceParticleParse(mScn, buf, n, next);
+                return blz_stage_source(mH, buf.data(), n);
+            });
+        }
+        return;
+    }
+    if (!mScn->sourceAnimated || inRange) {
         bool fresh = false;
         if (!mSingleFrameSourceInit || mScn->sourceAnimated) {
             Parser::sourceParticleParse(mScn, mPart, mParticleCount, mTmr->iter);

@@ -7,6 +7,7 @@
 // front buffer instead of a device-heap Grid object.
 #pragma once
 
+#include <future>
 #include <vector>
 
 #include "../../include/blaze.h"
@@ -36,4 +37,8 @@ class FluidSolver {
     bool mSingleFrameSourceInit = false;
     int mParticleCount = 0;
     std::vector<float> mPart;
+    // animated .part sources: the next frame is read and its upload started
+    // on a worker thread while the current frame steps
+    std::future<int> mPrefetch;
+    int mPrefetchFrame = -1;
 };

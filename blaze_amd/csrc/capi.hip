@@ -35,6 +35,13 @@ Solver::~Solver() {
     if (ws_net) delete ws_net;
     if (ws_buf) (void)hipFree(ws_buf);
     if (rnd_buf) (void)hipFree(rnd_buf);
+    for (int q = 0; q < 2; ++q) {
+        if (stage_ev[q]) (void)hipEventSynchronize(stage_ev[q]);
+        if (stage_host[q]) (void)hipHostFree(stage_host[q]);
+        if (stage_dev[q]) (void)hipFree(stage_dev[q]);
+        if (stage_ev[q]) (void)hipEventDestroy(stage_ev[q]);
+    }
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (zero_flag) (void)hipFree(zero_flag);
     if (done_host) (void)hipHostFree((void*)done_host);
     for (hipEvent_t e : batch_ev)
@@ -476,6 +483,70 @@ int Solver::add_source(const float* part8, int n) {
     return BLZ_OK;
 }
 
+// Stage the next source frame: copy into the pinned slot and start its
+// upload on the copy stream (returns before the upload ends). The slot's
+// previous upload is waited for first; the solver stream is not touched,
+// so this may run on another host thread while a step is in flight.
+int Solver::stage_source(const float* part8, int n) {
+    if (n < 0 || (n > 0 && !part8)) {
+        set_error("stage_source: bad particle array");
+        return BLZ_E_INVALID;
+    }
+    if (!copy_stream) BLZ_HIP(hipStreamCreate(&copy_stream));
+    const int q = stage_next;
+    if (!stage_ev[q]) BLZ_HIP(hipEventCreateWithFlags(&stage_ev[q], hipEventDisableTiming));
+    BLZ_HIP(hipEventSynchronize(stage_ev[q]));
+    if (n > stage_cap[q]) {
+        if (stage_host[q]) BLZ_HIP(hipHostFree(stage_host[q]));
+        if (stage_dev[q]) BLZ_HIP(hipFree(stage_dev[q]));
+        stage_host[q] = nullptr;
+        stage_dev[q] = nullptr;
+        stage_cap[q] = 0;
+        if (hipHostMalloc(&stage_host[q], (size_t)n * 8 * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&stage_dev[q], (size_t)n * 8 * sizeof(float)) != hipSuccess) {
+            set_error("allocation failed for staged particles");
+            return BLZ_E_NOMEM;
+        }
+        stage_cap[q] = n;
+    }
+    if (n > 0) {
+        std::memcpy(stage_host[q], part8, (size_t)n * 8 * sizeof(float));
+        BLZ_HIP(hipMemcpyAsync(stage_dev[q], stage_host[q], (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice,
+                               copy_stream));
+    }
+    BLZ_HIP(hipEventRecord(stage_ev[q], copy_stream));
+    stage_count[q] = n;
+    stage_ready = q;
+    stage_next = q ^ 1;
+    return BLZ_OK;
+}
+
+// addSource with the frame staged last: the solver stream waits for its
+// upload (device-side), takes it over, and splats as add_source does
+int Solver::add_source_staged() {
+    if (stage_ready < 0) {
+        set_error("add_source_staged: nothing staged");
+        return BLZ_E_INVALID;
+    }
+    const int q = stage_ready, n = stage_count[q];
+    if (n > part_cap) {
+        if (part) BLZ_HIP(hipFree(part));
+        part = nullptr;
+        part_cap = 0;
+        if (hipMalloc(&part, (size_t)n * 8 * sizeof(float)) != hipSuccess) {
+            set_error("hipMalloc failed for particles");
+            return BLZ_E_NOMEM;
+        }
+        part_cap = n;
+    }
+    BLZ_HIP(hipStreamWaitEvent(stream, stage_ev[q], 0));
+    if (n > 0)
+        BLZ_HIP(hipMemcpyAsync(part, stage_dev[q], (size_t)n * 8 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    npart = n;
+    stage_ready = -1;
+    return add_source(nullptr, 0);
+}
+
 } // namespace blz
 
 using blz::Solver;
@@ -661,6 +732,16 @@ int blz_set_warm_start(blz_solver* h, blz_net* net) {
     CHECK_HANDLE(h);
     BLZ_HIP(hipStreamSynchronize(h->s.stream));
     return blz::warm_start_attach(h->s, net ? &net->n : nullptr);
+}
+
+int blz_stage_source(blz_solver* h, const float* part8, int count) {
+    CHECK_HANDLE(h);
+    return h->s.stage_source(part8, count);
+}
+
+int blz_add_source_staged(blz_solver* h) {
+    CHECK_HANDLE(h);
+    return h->s.add_source_staged();
 }
 
 int blz_render(blz_solver* h, const blz_render_settings* r, float* rgb) {

@@ -134,6 +134,16 @@ struct Solver {
     // particles
     float* part = nullptr;
     int npart = 0, part_cap = 0;
+    // staged particle uploads (double-buffered pinned host memory, a copy
+    // stream at default priority, one event per slot)
+    float* stage_host[2] = {nullptr, nullptr};
+    float* stage_dev[2] = {nullptr, nullptr};
+    int stage_cap[2] = {0, 0};
+    int stage_count[2] = {0, 0};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    int stage_next = 0;   // slot the next stage_source fills
+    int stage_ready = -1; // slot staged last (-1: none)
     // particle-to-grid scratch (grow-only)
     void* p2g_buf = nullptr;
     size_t p2g_bytes = 0;
@@ -175,6 +185,8 @@ struct Solver {
     ~Solver();
     int init(const blz_settings& st);
     int add_source(const float* part8, int n);
+    int stage_source(const float* part8, int n);
+    int add_source_staged();
     int step_enqueue(float time, bool sync);
     int substep(float time, float dt);
     int run_stage(int stage, float time, float dt);

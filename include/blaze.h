@@ -162,6 +162,16 @@ int blz_destroy(blz_solver* h);
 /* Particle-to-grid source (addSource). part8 != NULL uploads count particles
    first; part8 == NULL re-splats the previously uploaded set. */
 int blz_add_source(blz_solver* h, const float* part8, int count);
+/* Animated sources without a synchronous upload (no reference counterpart;
+   the reference parses and copies each frame inside addSource,
+   src/FluidSolver.cu:781-791): blz_stage_source copies the next frame's
+   particles into one of two pinned buffers and starts the upload on a copy
+   stream, returning at once; it touches no state a step uses, so it may run
+   on another host thread while blz_step is in flight. blz_add_source_staged
+   then splats the frame staged last (the solver stream waits for its
+   upload on the device). Same result as blz_add_source with that array. */
+int blz_stage_source(blz_solver* h, const float* part8, int count);
+int blz_add_source_staged(blz_solver* h);
 /* One frame (step): forces, projection, advection, with CFL substeps when
    enabled. time is the frame's simulation time (Timer::time). */
 int blz_step(blz_solver* h, float time);
