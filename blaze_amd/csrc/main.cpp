@@ -2,14 +2,16 @@
 //
 // Reference: src/main.cu:32-117 (argParse, frame loop). Frames inside the
 // render range are rendered and written (RenderEngine) when the scene has
-// the render keys; each frame prints the solver statistics, and -l appends
-// them to <sceneDir>/../logs/log_<scene>.txt like the reference Logger.
+// the render keys; each frame prints the stats table of the reference Timer,
+// and -l appends the scene summary and the tables to the timestamped log of
+// the reference Logger.
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <string>
 
 #include "FluidSolver.h"
+#include "Logger.h"
 #include "Parser.h"
 #include "RenderEngine.h"
 
@@ -54,10 +56,10 @@ int main(int argc, char* argv[]) {
     if (!argParse(argc, argv, scenePath, writeLog)) return 1;
     SceneSettings scn;
     Parser::sceneParse(&scn, scenePath.c_str());
-    std::FILE* log = nullptr;
+    Logger* log = nullptr;
     if (writeLog) {
-        std::string path = std::string(scn.sceneDir) + "/../logs/log_" + scn.sceneName + ".txt";
-        log = std::fopen(path.c_str(), "a");
+        log = new Logger(scn.sceneDir, scn.sceneName);
+        log->append(Logger::sceneInfo(scn));
     }
     Timer tmr;
     FluidSolver solver(&tmr, &scn);
@@ -75,11 +77,11 @@ int main(int argc, char* argv[]) {
         }
         const std::string msg = tmr.getStats();
         std::fputs(msg.c_str(), stdout);
-        if (log) std::fputs(msg.c_str(), log);
+        if (log) log->append(msg);
         tmr.iter++;
         tmr.time += scn.dt;
     }
     delete renderer;
-    if (log) std::fclose(log);
+    delete log;
     return 0;
 }

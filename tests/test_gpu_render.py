@@ -127,9 +127,11 @@ def test_render_rejects_bad_sizes():
 
 
 def test_cli_renders_frames(tmp_path):
-    """`blaze scene.blz` renders and writes each frame of the render range
+    """`blaze -l scene.blz` renders and writes each frame of the render range
     (src/main.cu:89-106, RenderEngine::writeToDisk naming): the plume at
-    32^3, frames 1-3 as PNG under <sceneDir>/../render/."""
+    32^3, frames 1-3 as PNG under <sceneDir>/../render/, the per-frame stats
+    table on stdout and, with the scene summary, in the timestamped log under
+    <sceneDir>/../logs/ (src/Timer.h:80-132, src/Logger.h:52-63)."""
     import json
     import subprocess
     import zlib
@@ -148,10 +150,14 @@ def test_cli_renders_frames(tmp_path):
     scene = tmp_path / "scenes" / "t.blz"
     scene.write_text(json.dumps(d))
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "blaze_amd", "bin", "blaze")
-    r = subprocess.run([exe, str(scene)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "-l", str(scene)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-400:], flush=True)
     assert r.returncode == 0, r.stderr
-    assert "render" in r.stdout
+    assert "Frame: 003" in r.stdout and "Gauss-Seidel Solve:" in r.stdout and "Render:" in r.stdout
+    logs = list((tmp_path / "logs").glob("log_*.txt"))
+    assert len(logs) == 1
+    text = logs[0].read_text()
+    assert text.startswith("sceneName:") and text.count("Frame: ") == 3 and "Total Time:" in text
     for f in (1, 2, 3):
         data = (tmp_path / "render" / f"t_{f:04d}.png").read_bytes()
         assert data[:8] == b"\x89PNG\r\n\x1a\n"
