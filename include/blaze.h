@@ -250,8 +250,10 @@ int blz_write_image(const char* path, const float* rgb, int width, int height);
 /* ---- learned warm start (no reference counterpart; the reference solve
    always starts from the previous pressure, src/FluidSolver.cu:517-599) ----
    A 3-level 3D U-Net (16/32/64 channels, BF16 weights, FP32 accumulation)
-   on a 4x coarser grid maps (residual of the previous pressure, solid mask)
-   to a pressure correction. Network grids are (batch, D, H, W) with D, H,
+   on a coarser grid (F = 4, doubled until at most 32 cells per axis) maps
+   (residual of the current pressure, solid mask) to a pressure correction,
+   followed by an exact Galerkin solve on at most 16^3 blocks (the global
+   correction). Network grids are (batch, D, H, W) with D, H,
    W multiples of 4; inputs are two FP32 channels (residual, mask), each
    batch*D*H*W values, x fastest. */
 typedef struct blz_net blz_net;
@@ -267,9 +269,12 @@ int blz_net_forward(blz_net* net, const float* in2, const int dims[4], float* ou
 int blz_net_train_step(blz_net* net, const float* in2, const float* target, const int dims[4], float lr,
                        int update, double* loss);
 /* Attach a network (NULL detaches). With warm_start != 0 and residual_tol
-   > 0, each solve applies the correction before its sweeps and keeps it only
-   if the FP64 residual drops (else the pressure is restored). The network
-   must outlive the attachment. */
+   > 0, each solve applies the corrections before its sweeps (and, for
+   warm_start = N >= 2, after every N sweeps) and keeps them only if the
+   FP64 residual after the next batch of sweeps beats the uncorrected path
+   (measured on the first application, extrapolated from the last check on
+   re-applications); a rejection continues exactly as without them. The
+   network must outlive the attachment. */
 int blz_set_warm_start(blz_solver* h, blz_net* net);
 /* Which corrections a warm start applies (no reference counterpart):
    bit 0 the network's, bit 1 the global one (an exact Galerkin solve on a
@@ -280,9 +285,10 @@ int blz_set_warm_start_parts(blz_solver* h, int parts);
    training data): in2 = 2 x net_dims values; dims reported in (W, H, D). */
 int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]);
 /* The correction step with a caller-given coarse prediction (network grid,
-   x fastest) in place of the network's: fitted scale, FP64 residual before
-   and after, kept only if lower (else the pressure is restored). For
-   measuring what an exact coarse solve would gain (ablations). */
+   x fastest) in place of the network's, scaled to the least error energy and
+   followed by the global correction when enabled: FP64 residual before and
+   right after, kept only if lower (else the pressure is restored). For
+   ablations (an exact coarse solve in place of the network). */
 int blz_warm_start_apply(blz_solver* h, float dt, const float* pred, double* before, double* after, int* kept);
 
 /* ---- batched scenes (no reference counterpart; the reference steps one
