@@ -236,6 +236,35 @@ __device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int 
 // residual's norm, such a step cannot raise the A-norm of the error, i.e.
 // cannot add smooth error that the sweeps remove slowly. The block sums of
 // r' are the global correction's right-hand side. Fixed reduction order.
+typedef float f4w __attribute__((ext_vector_type(4)));
+
+// A x for the four cells (i0..i0+3, j, k), i0 a multiple of 4 and W a
+// multiple of 4: the same per-cell formula as lap(), from float4 rows
+__device__ __forceinline__ void lap4(const float* __restrict__ x, size_t t, int i0, int j, int k, int W, int H,
+                                     int D, const f4w& cx, float out[4]) {
+    const size_t WH = (size_t)W * H;
+    const f4w zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const f4w yn = j > 0 ? *(const f4w*)(x + t - W) : zero;
+    const f4w ys = j < H - 1 ? *(const f4w*)(x + t + W) : zero;
+    const f4w zd = k > 0 ? *(const f4w*)(x + t - WH) : zero;
+    const f4w zu = k < D - 1 ? *(const f4w*)(x + t + WH) : zero;
+    const float xl = i0 > 0 ? x[t - 1] : 0.0f;
+    const float xr = i0 + 4 < W ? x[t + 4] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = i0 + q;
+        float s = 0.0f, n = 0.0f;
+        if (i > 0) { s += q > 0 ? cx[q - 1] : xl; n += 1.0f; }
+        if (i < W - 1) { s += q < 3 ? cx[q + 1] : xr; n += 1.0f; }
+        if (j > 0) { s += yn[q]; n += 1.0f; }
+        if (j < H - 1) { s += ys[q]; n += 1.0f; }
+        if (k > 0) { s += zd[q]; n += 1.0f; }
+        if (k < D - 1) { s += zu[q]; n += 1.0f; }
+        out[q] = n * cx[q] - s;
+    }
+}
+
+template <bool V4>
 __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r, const float* __restrict__ e1,
                                                      const double* __restrict__ alpha1,
                                                      const float* __restrict__ e2, int W, int H, int D, int F,
@@ -246,21 +275,53 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
     const int I = b % Gw, J = (b / Gw) % Gh, K = b / (Gw * Gh);
     const float a1 = e1 ? (float)*alpha1 : 0.0f;
     float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    // F = 2^lf: the block's F^3 cells by shifts, cells past the far faces skipped
-    for (int q = threadIdx.x; q < (1 << (3 * lf)); q += BS) {
-        const int i = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
-        if (i >= W || j >= H || k >= D) continue;
-        const size_t t = ((size_t)k * H + j) * W + i;
-        float rr = r[t];
-        if (e1) rr -= a1 * lap(e1, t, i, j, k, W, H, D);
-        v[2] += rr;
-        if (e2) {
-            const float ec = e2[t];
-            const float ae = lap(e2, t, i, j, k, W, H, D);
-            v[0] += rr * ec;
-            v[1] += ec * ae;
-            v[3] += ec;
-            v[4] += ae;
+    if (V4) {
+        // four x-consecutive cells per step (F >= 4, W a multiple of 4)
+        for (int q = 4 * threadIdx.x; q < (1 << (3 * lf)); q += 4 * BS) {
+            const int i0 = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
+            if (i0 >= W || j >= H || k >= D) continue;
+            const size_t t = ((size_t)k * H + j) * W + i0;
+            const f4w r4 = *(const f4w*)(r + t);
+            float rr[4] = {r4[0], r4[1], r4[2], r4[3]};
+            if (e1) {
+                const f4w c1 = *(const f4w*)(e1 + t);
+                float l1[4];
+                lap4(e1, t, i0, j, k, W, H, D, c1, l1);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rr[u] -= a1 * l1[u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[2] += rr[u];
+            if (e2) {
+                const f4w c2 = *(const f4w*)(e2 + t);
+                float l2[4];
+                lap4(e2, t, i0, j, k, W, H, D, c2, l2);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    v[0] += rr[u] * c2[u];
+                    v[1] += c2[u] * l2[u];
+                    v[3] += c2[u];
+                    v[4] += l2[u];
+                }
+            }
+        }
+    } else {
+        // F = 2^lf: the block's F^3 cells by shifts, cells past the far faces skipped
+        for (int q = threadIdx.x; q < (1 << (3 * lf)); q += BS) {
+            const int i = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
+            if (i >= W || j >= H || k >= D) continue;
+            const size_t t = ((size_t)k * H + j) * W + i;
+            float rr = r[t];
+            if (e1) rr -= a1 * lap(e1, t, i, j, k, W, H, D);
+            v[2] += rr;
+            if (e2) {
+                const float ec = e2[t];
+                const float ae = lap(e2, t, i, j, k, W, H, D);
+                v[0] += rr * ec;
+                v[1] += ec * ae;
+                v[3] += ec;
+                v[4] += ae;
+            }
         }
     }
     double d[5];
@@ -683,6 +744,12 @@ inline int ilog2(int f) {
     while ((1 << l) < f) ++l;
     return l;
 }
+// the float4 block pass where rows split into whole float4s
+using BlockDotsFn = void (*)(const float*, const float*, const double*, const float*, int, int, int, int, int, int,
+                             int, double*, double*, const SolveCtl*);
+inline BlockDotsFn blockdots_kernel(const Solver& s) {
+    return (s.W % 4 == 0 && s.ws_g >= 4) ? k_ws_blockdots<true> : k_ws_blockdots<false>;
+}
 
 } // namespace
 
@@ -777,11 +844,11 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
     if (pred) {
         hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, ws_enet(s), s.W, s.H, s.D, pred,
                            1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
-        hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, ws_enet(s),
+        hipLaunchKernelGGL(blockdots_kernel(s), dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, ws_enet(s),
                            s.W, s.H, s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), glob ? ws_sg(s) : nullptr, c);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
     } else if (glob) {
-        hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, nullptr, s.W,
+        hipLaunchKernelGGL(blockdots_kernel(s), dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, nullptr, s.W,
                            s.H, s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), ws_sg(s), c);
     }
     if (glob) {
@@ -795,7 +862,7 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
                                ws_xg(s), s.Gw, s.Gh, s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
         hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, ws_eg(s), s.W, s.H, s.D, ws_xg(s),
                            1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
-        hipLaunchKernelGGL(k_ws_blockdots, dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), enet, a_net, ws_eg(s), s.W, s.H,
+        hipLaunchKernelGGL(blockdots_kernel(s), dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), enet, a_net, ws_eg(s), s.W, s.H,
                            s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), nullptr, c);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
     }
