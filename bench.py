@@ -26,7 +26,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 RBGS_BYTES_PER_VOXEL_SWEEP = 12.0  # read p, read rhs, write p
 RBGS_KERNELS = {1: ("k_rbgs_pass", "RBGS colour pass, two per sweep", "rbgs_pass_traffic.json"),
-                2: ("k_rbgs_fused", "RBGS fused sweep, one per sweep", "rbgs_fused_traffic.json")}
+                2: ("k_rbgs_fused", "RBGS fused sweep, one per sweep", "rbgs_fused_traffic.json"),
+                3: ("k_rbgs_pair", "RBGS row-pair fused sweep, one per sweep", "rbgs_pair_traffic.json")}
 # compulsory bytes per voxel of the non-solve part of a substep of this
 # workload: cooling 8, vorticity 24, buoyancy 12, divergence 16, update 28,
 # advection 40 (T, U, V, W, density)

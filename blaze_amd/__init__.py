@@ -329,9 +329,9 @@ class Solver:
         """Time every RBGS variant on this grid and keep the fastest; returns
         (variant, {variant: ms per sweep})."""
         v = C.c_int()
-        ms = (C.c_float * 9)()
+        cand = [2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1203, 1204, 1403, 1404]
+        ms = (C.c_float * len(cand))()
         _check(_lib.blz_autotune_rbgs(self._h, sweeps, C.byref(v), ms), "blz_autotune_rbgs")
-        cand = [2, 4, 8, 102, 103, 202, 203, 402, 403]
         return v.value, {c: ms[i] for i, c in enumerate(cand) if ms[i] > 0}
 
     def stage_source(self, part8: np.ndarray):

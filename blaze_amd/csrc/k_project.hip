@@ -622,6 +622,294 @@ __global__ void __launch_bounds__(64 * WPB) k_rbgs_fused_multi(const FusedScene*
                    e.scale, tab, wv);
 }
 
+// ---- row-pair fused sweep ------------------------------------------------
+// One whole sweep per launch (ping-pong as above), with the rows of a pair
+// (j, j+1), j even, in one wave: lanes 0-30 hold cells 30s .. 30s+30 of
+// row j, lanes 32-62 cells 30s-1 .. 30s+29 of row j+1 (lanes 31 and 63
+// carry no cell). In both halves the end lanes are red, so every red
+// neighbour of a black cell in x is a lane of the same half, and its red
+// neighbour in the other row of the pair is a lane of the other half (lane
+// +-33). Segments overlap by one red cell, computed by both waves from the
+// same inputs in the same order and stored by both with the same bits.
+// Per slab group every lane runs two update chains:
+//   chain A (red type): a red lane's own cell; a black lane's red neighbour
+//           outside the pair (row j-1 for row j, row j+2 for row j+1);
+//   chain C (black type): a black lane's own cell, from the new reds of
+//           lanes -+1 (x), the other half and its own chain A (y), all by
+//           ds_bpermute, and its own new k-1 and old k+1 values.
+// Each chain follows its cell's reference operation order, so every value
+// equals the two-pass sweep's bits. A chain A whose cell lies outside the
+// grid publishes -0, the ghost value, so no neighbour needs a mask. 8 loads
+// per group (chain A: own old value for k+1, rhs, four black neighbours;
+// chain C: own old value, rhs), one store; lanes without a cell write -0
+// into a ghost slot of their row. Groups away from slab 0 and slab D-1 skip
+// the k-edge selects. Division and the exact fallback as in the fused
+// sweep above.
+constexpr int PNL = 8; // loads per group
+constexpr int PNU = 2; // of which the k+1 sources, issued first
+enum { P_PA = 0, P_PC, P_RA, P_XL, P_XR, P_YN, P_YS, P_RC };
+
+template <int PG, int U, bool FIRST>
+struct PairWait {
+    static constexpr int N = FIRST && U < PG - 1 ? (PNL - PNU) + (PG - 2 - U) * PNL + U * (1 + PNL)
+                                                 : (PNL - PNU) + (PG - 2) * (PNL + 1);
+};
+
+struct PairArgs {
+    i4v rs;
+    int vo[PNL];      // per-lane byte offsets of each load in group 0 (plane excluded)
+    unsigned pl[PNL]; // plane byte offset of each load
+    int vst;          // per-lane byte offset of the store (plane included)
+    int D, Dg, gstride;
+    float scale;
+    float dmA, ymA, deA, yeA, dmC, ymC, deC, yeC; // denominators/reciprocals, mid and k-edge
+    int srcL, srcR, srcN, srcS; // bpermute addresses of the lanes holding the x-1, x+1, y-1, y+1 reds
+    int pubA;                   // chain A's cell is in the grid (else it publishes -0)
+    int ownRed;                 // the lane stores chain A's published value (red cells, lanes without a cell)
+    int needA, needC;           // chain results that are used
+};
+
+template <int PG>
+__device__ __forceinline__ void pair_load(const PairArgs& a, f4v (&sl)[PNL], int g) {
+    const int so = min(g, a.Dg) * a.gstride;
+#pragma unroll
+    for (int q = 0; q < PNL; ++q) sl[q] = ld4_async(a.vo[q], a.rs, (int)(a.pl[q] + (unsigned)so));
+}
+
+template <int N>
+__device__ __forceinline__ void pair_wait(f4v (&cur)[PNL], f4v (&nxt)[PNL]) {
+    asm volatile("s_waitcnt vmcnt(%10)"
+                 : "+v"(cur[0]), "+v"(cur[1]), "+v"(cur[2]), "+v"(cur[3]), "+v"(cur[4]), "+v"(cur[5]), "+v"(cur[6]),
+                   "+v"(cur[7]), "+v"(nxt[0]), "+v"(nxt[1])
+                 : "n"(N));
+}
+
+__device__ __forceinline__ float lane_get(float v, int src_addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_addr, __float_as_int(v)));
+}
+
+// division of the fast run is exact for this numerator (see fdiv_unsafe)
+// (bitwise, not short-circuit, operators: lane masks without branches)
+__device__ __forceinline__ bool fdiv_ok(float num) {
+    const float mag = fabsf(num);
+    const bool lo = mag >= 0x1p-125f, hi = mag <= 3.40282347e38f, pz = __float_as_uint(num) == 0u;
+    return (lo & hi) | pz;
+}
+
+// The arithmetic of one group. BOUND: the group holds slab 0, slab D-1 or
+// slabs k >= D (k-edge denominators; -0 stored for k >= D); other groups
+// skip those selects. Returns false (fast run) when some lane needs the
+// exact division.
+template <bool EXACT, bool BOUND>
+__device__ __forceinline__ bool pair_math(const PairArgs& a, const f4v* c, const f4v* n, int k0, float& prevA,
+                                          float& prevC, f4v& out) {
+    const int D = a.D;
+    float pA = prevA, pC = prevC;
+    float rA[4], nA[4], pub[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = k0 + s;
+        const bool edge = BOUND && ((k == 0) | (k == D - 1));
+        const float up = s < 3 ? c[P_PA][s + 1] : n[P_PA][0];
+        float x = a.scale * c[P_RA][s];
+        x = x + c[P_XL][s];
+        x = x + c[P_XR][s];
+        x = x + c[P_YN][s];
+        x = x + c[P_YS][s];
+        nA[s] = x + pA;
+        nA[s] = nA[s] + up;
+        if (EXACT) rA[s] = div_small(nA[s], edge ? a.deA : a.dmA, edge ? a.yeA : a.ymA);
+        else rA[s] = fdiv_fast(nA[s], edge ? a.deA : a.dmA, edge ? a.yeA : a.ymA);
+        pA = rA[s];
+        pub[s] = a.pubA != 0 ? rA[s] : -0.0f; // -0 (the ghost value) for cells outside the grid
+    }
+    float Lv[4], Rv[4], Nv[4], Sv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        Lv[s] = lane_get(pub[s], a.srcL);
+        Rv[s] = lane_get(pub[s], a.srcR);
+        Nv[s] = lane_get(pub[s], a.srcN);
+        Sv[s] = lane_get(pub[s], a.srcS);
+    }
+    asm volatile("" : "+v"(Lv[0]), "+v"(Lv[1]), "+v"(Lv[2]), "+v"(Lv[3]), "+v"(Rv[0]), "+v"(Rv[1]), "+v"(Rv[2]),
+                 "+v"(Rv[3]), "+v"(Nv[0]), "+v"(Nv[1]), "+v"(Nv[2]), "+v"(Nv[3]), "+v"(Sv[0]), "+v"(Sv[1]),
+                 "+v"(Sv[2]), "+v"(Sv[3]));
+    const bool mA = a.needA != 0, mC = a.needC != 0; // lane masks, loop-invariant
+    bool bad = false;                                 // the lane needs the exact division
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = k0 + s;
+        const bool edge = BOUND && ((k == 0) | (k == D - 1));
+        const float up = s < 3 ? c[P_PC][s + 1] : n[P_PC][0];
+        float x = a.scale * c[P_RC][s];
+        x = x + Lv[s];
+        x = x + Rv[s];
+        x = x + Nv[s];
+        x = x + Sv[s];
+        float nC = x + pC;
+        nC = nC + up;
+        float rC;
+        if (EXACT) {
+            rC = div_small(nC, edge ? a.deC : a.dmC, edge ? a.yeC : a.ymC);
+        } else {
+            rC = fdiv_fast(nC, edge ? a.deC : a.dmC, edge ? a.yeC : a.ymC);
+            const bool okA = fdiv_ok(nA[s]), okC = fdiv_ok(nC);
+            if (!BOUND || k < D) bad = bad | (mA & !okA) | (mC & !okC);
+        }
+        const float v = a.ownRed != 0 ? pub[s] : rC;
+        out[s] = (BOUND && k >= D) ? -0.0f : v;
+        pC = rC;
+    }
+    if (!EXACT && __builtin_amdgcn_ballot_w64(bad)) return false;
+    prevA = pA;
+    prevC = pC;
+    return true;
+}
+
+template <int PG, int U, bool EXACT, bool FIRST>
+__device__ __forceinline__ bool pair_group(const PairArgs& a, f4v (&sl)[PG][PNL], int g, float& prevA, float& prevC) {
+    constexpr int UN = (U + 1) % PG;
+    pair_wait<PairWait<PG, U, FIRST>::N>(sl[U], sl[UN]);
+    f4v out;
+    const bool bound = g == 0 || 4 * g + 4 >= a.D; // wave-uniform
+    const bool done = bound ? pair_math<EXACT, true>(a, sl[U], sl[UN], 4 * g, prevA, prevC, out)
+                            : pair_math<EXACT, false>(a, sl[U], sl[UN], 4 * g, prevA, prevC, out);
+    if (!done) return false;
+    st4_async(out, a.vst, a.rs, min(g, a.Dg) * a.gstride);
+    pair_load<PG>(a, sl[U], g + PG);
+    return true;
+}
+
+template <int PG, bool EXACT, bool FIRST, int U>
+__device__ __forceinline__ bool pair_ring(const PairArgs& a, f4v (&sl)[PG][PNL], int gb, float& pA, float& pC,
+                                          int* failed) {
+    if constexpr (U == PG) {
+        return true;
+    } else {
+        if (!pair_group<PG, U, EXACT, FIRST>(a, sl, gb + U, pA, pC)) {
+            *failed = gb + U;
+            return false;
+        }
+        return pair_ring<PG, EXACT, FIRST, U + 1>(a, sl, gb, pA, pC, failed);
+    }
+}
+
+__device__ __forceinline__ void hold_pair_slot(f4v (&r)[PNL]) {
+    asm volatile("s_waitcnt vmcnt(0)"
+                 : "+v"(r[0]), "+v"(r[1]), "+v"(r[2]), "+v"(r[3]), "+v"(r[4]), "+v"(r[5]), "+v"(r[6]), "+v"(r[7]));
+}
+
+// Walk groups g_start.. of the wave. Returns Dg, or for a fast run the first
+// group where an exact division is needed (that group was not stored).
+template <int PG, bool EXACT>
+__device__ __forceinline__ int pair_run(const PairArgs& a, int g_start, float& pA, float& pC) {
+    f4v sl[PG][PNL];
+#pragma unroll
+    for (int u = 0; u < PG; ++u) pair_load<PG>(a, sl[u], g_start + u);
+    int result = a.Dg;
+    int failed = a.Dg;
+    bool ok = pair_ring<PG, EXACT, true, 0>(a, sl, g_start, pA, pC, &failed);
+    for (int gb = g_start + PG; ok && gb < a.Dg; gb += PG) ok = pair_ring<PG, EXACT, false, 0>(a, sl, gb, pA, pC, &failed);
+    if (!ok) result = failed;
+#pragma unroll
+    for (int u = 0; u < PG; ++u) hold_pair_slot(sl[u]);
+    return result;
+}
+
+// waves of a row-pair sweep: 30-cell segments per row pair
+__host__ __device__ __forceinline__ int pair_segments(int W) { return (W + 29) / 30; }
+
+// one wave wv of a row-pair fused sweep
+template <int PG>
+__device__ __forceinline__ void pair_wave(float* __restrict__ base, unsigned nbytes, unsigned srcR, unsigned srcB,
+                                          unsigned rhsR, unsigned rhsB, unsigned dstR, unsigned dstB, int W, int H,
+                                          int D, int Wp, int Hp, float scale, const DivTab& tab, int wv) {
+    const int nseg = pair_segments(W);
+    const int jp = wv / nseg, sg = wv - jp * nseg;
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5, l = lane & 31;
+    const int jr = 2 * jp + half;     // the lane's row
+    const int i = 30 * sg - half + l; // the lane's cell column
+    const bool active = l < 31 && i >= 0 && i < W && jr < H;
+    const bool red = ((i + jr) & 1) == 0;
+    auto off = [&](int ii, int jj) -> int { return 16 * (((Hp + (jj + 1)) * Wp) + ((ii >> 1) + 1)); };
+    PairArgs a;
+    const unsigned long long b64 = (unsigned long long)base;
+    a.rs[0] = (int)(unsigned)(b64 & 0xffffffffull);
+    a.rs[1] = (int)(unsigned)(b64 >> 32);
+    a.rs[2] = (int)nbytes;
+    a.rs[3] = 0x00020000;
+    a.D = D;
+    a.Dg = (D + 3) >> 2;
+    a.gstride = Hp * Wp * 16;
+    a.scale = scale;
+    // addressed cells, clamped into the grid (a clamped chain's result is
+    // never used)
+    const int ic = min(max(i, 0), W - 1);
+    const int jc = min(jr, H - 1);
+    const int ja = red ? jc : min(max(half == 0 ? jr - 1 : jr + 1, 0), H - 1);
+    a.vo[P_PA] = off(ic, ja);
+    a.pl[P_PA] = srcR;
+    a.vo[P_RA] = off(ic, ja);
+    a.pl[P_RA] = rhsR;
+    a.vo[P_XL] = off(ic - 1, ja);
+    a.vo[P_XR] = off(ic + 1, ja);
+    a.vo[P_YN] = off(ic, ja - 1);
+    a.vo[P_YS] = off(ic, ja + 1);
+    a.pl[P_XL] = a.pl[P_XR] = a.pl[P_YN] = a.pl[P_YS] = srcB;
+    a.vo[P_PC] = off(ic, jc);
+    a.pl[P_PC] = srcB;
+    a.vo[P_RC] = off(ic, jc);
+    a.pl[P_RC] = rhsB;
+    {
+        const int cnt = cell_cnt(ic, ja, W, H);
+        a.dmA = (float)(cnt + 2);
+        a.deA = (float)(cnt + 1);
+        a.ymA = rcp_of(cnt + 2, tab);
+        a.yeA = rcp_of(cnt + 1, tab);
+    }
+    {
+        const int cnt = cell_cnt(ic, jc, W, H);
+        a.dmC = (float)(cnt + 2);
+        a.deC = (float)(cnt + 1);
+        a.ymC = rcp_of(cnt + 2, tab);
+        a.yeC = rcp_of(cnt + 1, tab);
+    }
+    // neighbour reds: x from lanes -+1 of the half; in y the lane's own
+    // chain A on the side outside the pair, the other half's lane across
+    const int partner = half == 0 ? min(lane + 33, 63) : max(lane - 33, 0);
+    a.srcL = 4 * max(lane - 1, 0);
+    a.srcR = 4 * min(lane + 1, 63);
+    a.srcN = 4 * (half == 0 ? lane : partner);
+    a.srcS = 4 * (half == 0 ? partner : lane);
+    // a neighbour outside the grid is a lane without a cell (x: cell -1 or
+    // W, or lanes 31/63; y: the missing row of the pair) or the lane's own
+    // chain A on a row outside the grid: all publish -0
+    a.needA = active && (red || (half == 0 ? jr > 0 : jr < H - 1));
+    a.pubA = a.needA;
+    a.needC = active && !red;
+    a.ownRed = red || !active;
+    // store: the lane's cell in its colour's dst plane (the red cell of a
+    // segment overlap is stored by both waves with the same bits); lanes
+    // without a cell write -0 into the ghost slot past the end of their row
+    a.vst = (int)(active ? (red ? dstR : dstB) + (unsigned)off(i, jr) : dstR + (unsigned)off(W, jc));
+    float pA = -0.0f, pC = -0.0f;
+    const int g = pair_run<PG, false>(a, 0, pA, pC);
+    if (g < a.Dg) pair_run<PG, true>(a, g, pA, pC);
+}
+
+template <int PG, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_rbgs_pair(float* __restrict__ base, unsigned nbytes, unsigned srcR,
+                                                        unsigned srcB, unsigned rhsR, unsigned rhsB, unsigned dstR,
+                                                        unsigned dstB, int W, int H, int D, int Wp, int Hp,
+                                                        float scale, DivTab tab, const int* __restrict__ stop,
+                                                        int nwaves) {
+    if (*stop) return; // the solve has converged
+    const int wv = xcd_block(blockIdx.x, gridDim.x) * WPB + (int)(threadIdx.x >> 6);
+    if (wv >= nwaves) return;
+    pair_wave<PG>(base, nbytes, srcR, srcB, rhsR, rhsB, dstR, dstB, W, H, D, Wp, Hp, scale, tab, wv);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -1006,20 +1294,47 @@ static int fused_pairs(Solver& s, float scale, int pairs, const int* stop) {
     return BLZ_OK;
 }
 
+// the same pairs with the row-pair fused sweep
+template <int PG, int WPB>
+static int pair_pairs(Solver& s, float scale, int pairs, const int* stop) {
+    const int nwaves = pair_segments(s.W) * ((s.H + 1) / 2);
+    const dim3 grid((unsigned)((nwaves + WPB - 1) / WPB)), block(64 * WPB);
+    const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
+    const unsigned pb = (unsigned)(s.Np * 4);
+    const unsigned nbytes = (unsigned)(s.prb_floats * 4);
+    for (int l = 0; l < pairs; ++l) {
+        for (int h = 0; h < 2; ++h) {
+            const unsigned src = h == 0 ? 0u : 4u * pb, dst = h == 0 ? 4u * pb : 0u;
+            hipLaunchKernelGGL((k_rbgs_pair<PG, WPB>), grid, block, 0, s.stream, s.prb, nbytes, src, src + pb,
+                               2u * pb, 3u * pb, dst, dst + pb, s.W, s.H, s.D, s.L.Wp, s.L.Hp, scale, tab, stop,
+                               nwaves);
+            BLZ_HIP(hipGetLastError());
+        }
+    }
+    return BLZ_OK;
+}
+
 bool fused_available(const Solver& s) { return s.prb_floats >= 6 * s.Np; }
 
 // Gauss-Seidel sweeps. variant: 2/4/8 = two in-place colour passes per
 // sweep with that many slab groups in flight; 100*w + g (w = 1, 2, 4 waves
 // per block, g = 2, 3 groups in flight) = fused sweeps in pairs, an odd
-// last sweep as two colour passes; 0 = the default (103 where the spare
-// planes exist, else 8).
+// last sweep as two colour passes; 1000 + 100*w + g (g = 3, 4) = the same
+// with the row-pair fused sweep; 0 = the autotuner's choice (8 where it
+// has not run).
 bool rbgs_variant_valid(int v) {
     return v == 0 || v == 2 || v == 4 || v == 8 || v == 102 || v == 103 || v == 202 || v == 203 || v == 402 ||
-           v == 403;
+           v == 403 || v == 1103 || v == 1104 || v == 1203 || v == 1204 || v == 1403 || v == 1404;
 }
 
 static int fused_dispatch(Solver& s, int v, float scale, int pairs, const int* stop) {
     switch (v) {
+    case 1103: return pair_pairs<3, 1>(s, scale, pairs, stop);
+    case 1104: return pair_pairs<4, 1>(s, scale, pairs, stop);
+    case 1203: return pair_pairs<3, 2>(s, scale, pairs, stop);
+    case 1204: return pair_pairs<4, 2>(s, scale, pairs, stop);
+    case 1403: return pair_pairs<3, 4>(s, scale, pairs, stop);
+    case 1404: return pair_pairs<4, 4>(s, scale, pairs, stop);
     case 102: return fused_pairs<2, 1>(s, scale, pairs, stop);
     case 202: return fused_pairs<2, 2>(s, scale, pairs, stop);
     case 203: return fused_pairs<3, 2>(s, scale, pairs, stop);
@@ -1037,7 +1352,7 @@ int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* la
     if (v >= 100) {
         const int pairs = sweeps / 2;
         *launches = 2 * pairs + 2 * (sweeps & 1);
-        *kind = 2;
+        *kind = v >= 1000 ? 3 : 2;
         if (int e = fused_dispatch(s, v, scale, pairs, stop)) return e;
         return (sweeps & 1) ? rbgs_passes<8>(s, scale, 1, stop) : BLZ_OK;
     }

@@ -92,18 +92,21 @@ def test_stage_bitwise(name, stage, kw, res):
             assert gs.residual_initial == os_["residual_initial"]
 
 
-RBGS_SHAPES = [(32, 24, 20), (64, 8, 40), (16, 10, 21), (32, 16, 64), (130, 6, 9), (128, 5, 8), (66, 4, 5)]
+RBGS_SHAPES = [(32, 24, 20), (64, 8, 40), (16, 10, 21), (32, 16, 64), (130, 6, 9), (128, 5, 8), (66, 4, 5),
+               (60, 7, 6)]
 
 
-@pytest.mark.parametrize("variant", [2, 4, 8, 102, 103])
+@pytest.mark.parametrize("variant", [2, 4, 8, 102, 103, 1103, 1204])
 @pytest.mark.parametrize("res", RBGS_SHAPES, ids=["x".join(map(str, r)) for r in RBGS_SHAPES])
 @pytest.mark.parametrize("iters", [9, 8])
 def test_rbgs_variants_bitwise(variant, res, iters):
     """Every RBGS kernel variant gives the oracle's sweeps bit for bit: the
-    two-pass in-place kernels (2/4/8 slab groups in flight) and the fused
-    ping-pong sweeps (102/103; an odd count ends with a two-pass sweep).
-    Widths 130, 128 and 66 put the fused kernel's row over several waves,
-    with a partly filled last wave."""
+    two-pass in-place kernels (2/4/8 slab groups in flight), the fused
+    ping-pong sweeps (102/103; an odd count ends with a two-pass sweep) and
+    the row-pair fused sweeps (1103/1204). Widths 130, 128 and 66 put the
+    fused kernel's row over several waves, with a partly filled last wave;
+    odd heights (5, 7 rows) leave the row-pair kernel a pair with one row,
+    and width 60 fills its 30-cell segments exactly."""
     print(f"[rbgs] v{variant} res={res} iters={iters}", flush=True)
     s = settings(B, res, max_iter_solve=iters)
     st = random_state(s, seed=variant + iters)
@@ -114,7 +117,7 @@ def test_rbgs_variants_bitwise(variant, res, iters):
     compare(f"rbgs v{variant}", g.field(B.FIELD_P), o.field(B.FIELD_P))
 
 
-@pytest.mark.parametrize("variant", [8, 103])
+@pytest.mark.parametrize("variant", [8, 103, 1104])
 @pytest.mark.parametrize("kind", ["tiny", "negzero"])
 def test_rbgs_exact_division_fallback(variant, kind):
     """Numerators outside the fast division's exact range (|num| < 2^-125,
