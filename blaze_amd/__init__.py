@@ -329,7 +329,7 @@ class Solver:
         """Time every RBGS variant on this grid and keep the fastest; returns
         (variant, {variant: ms per sweep})."""
         v = C.c_int()
-        cand = [2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1203, 1204, 1403, 1404]
+        cand = [2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1105, 1106, 1203, 1204, 1403, 1404]
         ms = (C.c_float * len(cand))()
         _check(_lib.blz_autotune_rbgs(self._h, sweeps, C.byref(v), ms), "blz_autotune_rbgs")
         return v.value, {c: ms[i] for i, c in enumerate(cand) if ms[i] > 0}

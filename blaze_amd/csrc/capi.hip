@@ -268,7 +268,7 @@ void clear_step_stats(blz_stats& st) {
 // rhs (the pressure is restored afterwards; all variants give the same
 // bits); *best = the fastest. Synchronises the stream.
 int Solver::autotune(int sweeps, int* best, float* ms_per_sweep) {
-    static const int cand[] = {2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1203, 1204, 1403, 1404};
+    static const int cand[] = {2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1105, 1106, 1203, 1204, 1403, 1404};
     sweeps = std::max(2, sweeps & ~1);
     const float dt = s.dt;
     BLZ_HIP(hipStreamSynchronize(stream));

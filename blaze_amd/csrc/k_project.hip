@@ -1319,18 +1319,21 @@ bool fused_available(const Solver& s) { return s.prb_floats >= 6 * s.Np; }
 // Gauss-Seidel sweeps. variant: 2/4/8 = two in-place colour passes per
 // sweep with that many slab groups in flight; 100*w + g (w = 1, 2, 4 waves
 // per block, g = 2, 3 groups in flight) = fused sweeps in pairs, an odd
-// last sweep as two colour passes; 1000 + 100*w + g (g = 3, 4) = the same
+// last sweep as two colour passes; 1000 + 100*w + g (g = 3..6) = the same
 // with the row-pair fused sweep; 0 = the autotuner's choice (8 where it
 // has not run).
 bool rbgs_variant_valid(int v) {
     return v == 0 || v == 2 || v == 4 || v == 8 || v == 102 || v == 103 || v == 202 || v == 203 || v == 402 ||
-           v == 403 || v == 1103 || v == 1104 || v == 1203 || v == 1204 || v == 1403 || v == 1404;
+           v == 403 || v == 1103 || v == 1104 || v == 1105 || v == 1106 || v == 1203 || v == 1204 || v == 1403 ||
+           v == 1404;
 }
 
 static int fused_dispatch(Solver& s, int v, float scale, int pairs, const int* stop) {
     switch (v) {
     case 1103: return pair_pairs<3, 1>(s, scale, pairs, stop);
     case 1104: return pair_pairs<4, 1>(s, scale, pairs, stop);
+    case 1105: return pair_pairs<5, 1>(s, scale, pairs, stop);
+    case 1106: return pair_pairs<6, 1>(s, scale, pairs, stop);
     case 1203: return pair_pairs<3, 2>(s, scale, pairs, stop);
     case 1204: return pair_pairs<4, 2>(s, scale, pairs, stop);
     case 1403: return pair_pairs<3, 4>(s, scale, pairs, stop);

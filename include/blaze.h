@@ -190,16 +190,17 @@ int blz_set_timing(blz_solver* h, int on);
    colour passes per sweep with that many four-slab groups in flight per
    thread; 102, 103 = fused whole-sweep kernel (2 or 3 groups in flight),
    sweeps in pairs through a spare plane set, with 1/2/4 waves per block for
-   x = 1/2/4 in 10x; 1x03, 1x04 = row-pair fused whole-sweep kernel (two
-   rows per wave, 3 or 4 groups in flight, x = 1/2/4 waves per block);
+   x = 1/2/4 in 10x; 1x03 .. 1x06 = row-pair fused whole-sweep kernel (two
+   rows per wave, 3 to 6 groups in flight, x = 1/2/4 waves per block);
    0 = default: the autotuner's choice for this grid, made at the first
    solve (every variant gives the same bits). */
 int blz_set_rbgs_variant(blz_solver* h, int variant);
 /* Autotuner (no reference counterpart): time `sweeps` sweeps of every
    variant above on this grid (pressure restored afterwards) and keep the
-   fastest; ms_per_sweep (optional, 15 entries) receives each time in the
-   order 2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1203, 1204,
-   1403, 1404 (fused ones skipped when the spare planes do not exist). */
+   fastest; ms_per_sweep (optional, 17 entries) receives each time in the
+   order 2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1105, 1106,
+   1203, 1204, 1403, 1404 (fused ones skipped when the spare planes do not
+   exist). */
 int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep);
 
 /* ---- renderer: the solver's consumer (RenderEngine::render,
