@@ -688,8 +688,11 @@ __device__ __forceinline__ float lane_get(float v, int src_addr) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src_addr, __float_as_int(v)));
 }
 
-// division of the fast run is exact for this numerator (see fdiv_unsafe)
-// (bitwise, not short-circuit, operators: lane masks without branches)
+// the fast run's division is exact for this numerator (+0 or 2^-125 <=
+// |num| <= FLT_MAX; see fdiv_unsafe). A -0 numerator needs every term to be
+// -0, which a solver state meets only when degenerate, so it takes the
+// exact run rather than a sign copy on every division. Bitwise, not
+// short-circuit, operators: lane masks without branches.
 __device__ __forceinline__ bool fdiv_ok(float num) {
     const float mag = fabsf(num);
     const bool lo = mag >= 0x1p-125f, hi = mag <= 3.40282347e38f, pz = __float_as_uint(num) == 0u;
@@ -1178,6 +1181,10 @@ __global__ void __launch_bounds__(BS) k_maxabs(const float* __restrict__ a, size
     }
 }
 
+__global__ void k_fill_floats(float* __restrict__ d, size_t n, float v) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) d[t] = v;
+}
+
 __global__ void k_fill_doubles(double* d, int n, double v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) d[t] = v;
@@ -1214,6 +1221,16 @@ static inline int4 cb_a(const Solver& s) {
 static inline int2 cb_b(const Solver& s) {
     const int* c = s.s.closed_bounds;
     return make_int2(c[4], c[5]);
+}
+
+// the home pressure planes (ghosts included) set to v: the autotuner's
+// timing state
+int launch_fill_pressure(Solver& s, float v) {
+    const size_t n = 2 * s.Np;
+    const unsigned nb = (unsigned)std::min<size_t>(blocks_for(n, BS), 8192);
+    hipLaunchKernelGGL(k_fill_floats, dim3(nb), dim3(BS), 0, s.stream, s.prb, n, v);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
 }
 
 int launch_init_pressure(Solver& s) {
