@@ -181,6 +181,8 @@ __global__ void k_ws_export(const float* __restrict__ res, const float* __restri
     out2[n + t] = mask[t];
 }
 
+typedef float f4w __attribute__((ext_vector_type(4)));
+
 // e = trilinear(prediction) at every fine cell, natural order; grid (x
 // blocks, H, D): the y and z weights are uniform over a workgroup
 __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W, int H, int D,
@@ -210,6 +212,50 @@ __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W,
     e[((size_t)k * H + j) * W + i] = (v == v && fabsf(v) <= 3.4e38f) ? v : 0.0f;
 }
 
+// the same at four x-consecutive cells per thread (W a multiple of 4), one
+// float4 store, threads over (x/4, y, z)
+__global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W, int H, int D,
+                                                    const float* __restrict__ pred, float invF, int Wc, int Hc,
+                                                    int Dc, int Wn, int Hn, const SolveCtl* __restrict__ c) {
+    if (c->done) return;
+    const int W4 = W >> 2;
+    const size_t t = (size_t)blockIdx.x * BS + threadIdx.x;
+    if (t >= (size_t)W4 * H * D) return;
+    const int q = (int)(t % (size_t)W4);
+    const size_t row = t / (size_t)W4;
+    const int j = (int)(row % (size_t)H), k = (int)(row / (size_t)H);
+    auto axis = [invF](int x, int n, int& a0, int& a1, float& wt) {
+        float f = ((float)x + 0.5f) * invF - 0.5f;
+        f = fminf(fmaxf(f, 0.0f), (float)(n - 1));
+        a0 = (int)f;
+        a1 = min(a0 + 1, n - 1);
+        wt = f - (float)a0;
+    };
+    int y0, y1, z0, z1;
+    float wy, wz;
+    axis(j, Hc, y0, y1, wy);
+    axis(k, Dc, z0, z1, wz);
+    const float* p00 = pred + ((size_t)z0 * Hn + y0) * Wn;
+    const float* p10 = pred + ((size_t)z0 * Hn + y1) * Wn;
+    const float* p01 = pred + ((size_t)z1 * Hn + y0) * Wn;
+    const float* p11 = pred + ((size_t)z1 * Hn + y1) * Wn;
+    f4w out;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        int x0, x1;
+        float wx;
+        axis(4 * q + u, Wc, x0, x1, wx);
+        const float c00 = p00[x0] * (1.0f - wx) + p00[x1] * wx;
+        const float c10 = p10[x0] * (1.0f - wx) + p10[x1] * wx;
+        const float c01 = p01[x0] * (1.0f - wx) + p01[x1] * wx;
+        const float c11 = p11[x0] * (1.0f - wx) + p11[x1] * wx;
+        const float c0 = c00 * (1.0f - wy) + c10 * wy, c1 = c01 * (1.0f - wy) + c11 * wy;
+        const float val = c0 * (1.0f - wz) + c1 * wz;
+        out[u] = (val == val && fabsf(val) <= 3.4e38f) ? val : 0.0f;
+    }
+    *(f4w*)(e + ((size_t)k * H + j) * W + 4 * q) = out;
+}
+
 // A x at natural cell t = (i, j, k): n x - sum of the in-domain neighbours
 __device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int i, int j, int k, int W, int H,
                                      int D) {
@@ -236,7 +282,6 @@ __device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int 
 // residual's norm, such a step cannot raise the A-norm of the error, i.e.
 // cannot add smooth error that the sweeps remove slowly. The block sums of
 // r' are the global correction's right-hand side. Fixed reduction order.
-typedef float f4w __attribute__((ext_vector_type(4)));
 
 // A x for the four cells (i0..i0+3, j, k), i0 a multiple of 4 and W a
 // multiple of 4: the same per-cell formula as lap(), from float4 rows
@@ -264,6 +309,35 @@ __device__ __forceinline__ void lap4(const float* __restrict__ x, size_t t, int 
     }
 }
 
+// the block-pass sums of the four cells (i0..i0+3, j, k) into v
+__device__ __forceinline__ void blockdots_cell4(const float* __restrict__ r, const float* __restrict__ e1, float a1,
+                                                const float* __restrict__ e2, size_t t, int i0, int j, int k, int W,
+                                                int H, int D, float (&v)[6]) {
+    const f4w r4 = *(const f4w*)(r + t);
+    float rr[4] = {r4[0], r4[1], r4[2], r4[3]};
+    if (e1) {
+        const f4w c1 = *(const f4w*)(e1 + t);
+        float l1[4];
+        lap4(e1, t, i0, j, k, W, H, D, c1, l1);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) rr[u] -= a1 * l1[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[2] += rr[u];
+    if (e2) {
+        const f4w c2 = *(const f4w*)(e2 + t);
+        float l2[4];
+        lap4(e2, t, i0, j, k, W, H, D, c2, l2);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[0] += rr[u] * c2[u];
+            v[1] += c2[u] * l2[u];
+            v[3] += c2[u];
+            v[4] += l2[u];
+        }
+    }
+}
+
 template <bool V4>
 __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r, const float* __restrict__ e1,
                                                      const double* __restrict__ alpha1,
@@ -280,30 +354,7 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
         for (int q = 4 * threadIdx.x; q < (1 << (3 * lf)); q += 4 * BS) {
             const int i0 = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
             if (i0 >= W || j >= H || k >= D) continue;
-            const size_t t = ((size_t)k * H + j) * W + i0;
-            const f4w r4 = *(const f4w*)(r + t);
-            float rr[4] = {r4[0], r4[1], r4[2], r4[3]};
-            if (e1) {
-                const f4w c1 = *(const f4w*)(e1 + t);
-                float l1[4];
-                lap4(e1, t, i0, j, k, W, H, D, c1, l1);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) rr[u] -= a1 * l1[u];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[2] += rr[u];
-            if (e2) {
-                const f4w c2 = *(const f4w*)(e2 + t);
-                float l2[4];
-                lap4(e2, t, i0, j, k, W, H, D, c2, l2);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    v[0] += rr[u] * c2[u];
-                    v[1] += c2[u] * l2[u];
-                    v[3] += c2[u];
-                    v[4] += l2[u];
-                }
-            }
+            blockdots_cell4(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
         }
     } else {
         // F = 2^lf: the block's F^3 cells by shifts, cells past the far faces skipped
@@ -340,6 +391,56 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
         if (threadIdx.x < 4) partial[4 * b + threadIdx.x] = a;
         if (sg && threadIdx.x == 2) sg[2 * b] = a;
         if (sg && threadIdx.x == 4) sg[2 * b + 1] = a;
+    }
+}
+
+// The same sums with one workgroup of RW waves per row of global blocks
+// (J, K): cells x in [0, W), y in [F*J, F*J+F), z in [F*K, F*K+F), clipped;
+// lane = float4 column (W <= 256, W a multiple of 4, 4 <= F), so each wave
+// reads whole 1 KB rows instead of one block's 4F-byte pieces; waves take
+// the F x F rows in turn. Block I = lane / (F/4): its lanes are reduced by
+// shuffles, the waves through LDS in a fixed order.
+constexpr int RW = 16;
+__global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __restrict__ r,
+                                                               const float* __restrict__ e1,
+                                                               const double* __restrict__ alpha1,
+                                                               const float* __restrict__ e2, int W, int H, int D,
+                                                               int F, int Gw, int Gh, double* __restrict__ partial,
+                                                               double* __restrict__ sg,
+                                                               const SolveCtl* __restrict__ c) {
+    if (c->done) return;
+    const int J = blockIdx.x % Gh, K = blockIdx.x / Gh;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 4 * lane;
+    const float a1 = e1 ? (float)*alpha1 : 0.0f;
+    float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const int y0 = F * J, z0 = F * K;
+    const int ny = min(F, H - y0), nz = min(F, D - z0);
+    if (i0 < W)
+        for (int q = w; q < ny * nz; q += RW) {
+            const int j = y0 + q % ny, k = z0 + q / ny;
+            blockdots_cell4(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
+        }
+    const int gl = F >> 2; // lanes per block
+    double d[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        d[q] = (double)v[q];
+        for (int o = 1; o < gl; o <<= 1) d[q] += __shfl_xor(d[q], o, 64);
+    }
+    __shared__ double sv[RW][16][5];
+    const int I = lane / gl;
+    if ((lane & (gl - 1)) == 0 && I < Gw)
+        for (int q = 0; q < 5; ++q) sv[w][I][q] = d[q];
+    __syncthreads();
+    if (threadIdx.x < Gw * 5) {
+        const int bi = threadIdx.x / 5, q = threadIdx.x % 5;
+        double a = 0.0;
+        for (int u = 0; u < RW; ++u) a += sv[u][bi][q];
+        const int b = (K * Gh + J) * Gw + bi;
+        if (q < 4) partial[4 * b + q] = a;
+        if (sg && q == 2) sg[2 * b] = a;
+        if (sg && q == 4) sg[2 * b + 1] = a;
     }
 }
 
@@ -744,11 +845,35 @@ inline int ilog2(int f) {
     while ((1 << l) < f) ++l;
     return l;
 }
-// the float4 block pass where rows split into whole float4s
-using BlockDotsFn = void (*)(const float*, const float*, const double*, const float*, int, int, int, int, int, int,
-                             int, double*, double*, const SolveCtl*);
-inline BlockDotsFn blockdots_kernel(const Solver& s) {
-    return (s.W % 4 == 0 && s.ws_g >= 4) ? k_ws_blockdots<true> : k_ws_blockdots<false>;
+// the block pass: a workgroup per row of blocks where a row is at most 64
+// float4s, else a workgroup per block (float4 steps where rows split into
+// whole float4s)
+void launch_blockdots(const Solver& s, const float* r, const float* e1, const double* a1, const float* e2,
+                      double* partial, double* sg, const SolveCtl* c) {
+    if (s.W % 4 == 0 && s.W <= 256 && s.ws_g >= 4 && s.Gw <= 16) {
+        hipLaunchKernelGGL(k_ws_blockdots_rows, dim3((unsigned)(s.Gh * s.Gd)), dim3(64 * RW), 0, s.stream, r, e1, a1,
+                           e2, s.W, s.H, s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c);
+        return;
+    }
+    const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
+    if (s.W % 4 == 0 && s.ws_g >= 4)
+        hipLaunchKernelGGL(k_ws_blockdots<true>, dim3(nbg), dim3(BS), 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
+                           s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, partial, sg, c);
+    else
+        hipLaunchKernelGGL(k_ws_blockdots<false>, dim3(nbg), dim3(BS), 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
+                           s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, partial, sg, c);
+}
+
+// e = trilinear prolongation of a coarse grid (float4 rows where W is a
+// multiple of 4)
+void launch_prolong(const Solver& s, float* e, const float* pred, float invF, int Wc, int Hc, int Dc, int Wn, int Hn,
+                    const SolveCtl* c) {
+    if (s.W % 4 == 0)
+        hipLaunchKernelGGL(k_ws_prolong4, dim3(nb((size_t)(s.W / 4) * s.H * s.D)), dim3(BS), 0, s.stream, e, s.W,
+                           s.H, s.D, pred, invF, Wc, Hc, Dc, Wn, Hn, c);
+    else
+        hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, pred, invF, Wc,
+                           Hc, Dc, Wn, Hn, c);
 }
 
 } // namespace
@@ -842,14 +967,11 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
     const float* enet = pred ? ws_enet(s) : nullptr;
     const bool glob = s.ws_parts & 2;
     if (pred) {
-        hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, ws_enet(s), s.W, s.H, s.D, pred,
-                           1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
-        hipLaunchKernelGGL(blockdots_kernel(s), dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, ws_enet(s),
-                           s.W, s.H, s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), glob ? ws_sg(s) : nullptr, c);
+        launch_prolong(s, ws_enet(s), pred, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
+        launch_blockdots(s, ws_r(s), nullptr, nullptr, ws_enet(s), ws_part(s), glob ? ws_sg(s) : nullptr, c);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
     } else if (glob) {
-        hipLaunchKernelGGL(blockdots_kernel(s), dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), nullptr, nullptr, nullptr, s.W,
-                           s.H, s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), ws_sg(s), c);
+        launch_blockdots(s, ws_r(s), nullptr, nullptr, nullptr, ws_part(s), ws_sg(s), c);
     }
     if (glob) {
         // right-hand side: block sums of r - alpha_net A e_net (sum r' and
@@ -860,10 +982,8 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
         else
             hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), pred ? a_net : nullptr,
                                ws_xg(s), s.Gw, s.Gh, s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
-        hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, ws_eg(s), s.W, s.H, s.D, ws_xg(s),
-                           1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
-        hipLaunchKernelGGL(blockdots_kernel(s), dim3(nbg), dim3(BS), 0, s.stream, ws_r(s), enet, a_net, ws_eg(s), s.W, s.H,
-                           s.D, s.ws_g, ilog2(s.ws_g), s.Gw, s.Gh, ws_part(s), nullptr, c);
+        launch_prolong(s, ws_eg(s), ws_xg(s), 1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
+        launch_blockdots(s, ws_r(s), enet, a_net, ws_eg(s), ws_part(s), nullptr, c);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
     }
     hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.L, s.W, s.H, s.D, enet,
