@@ -102,6 +102,9 @@ __global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int
     }
 }
 
+typedef float f4w __attribute__((ext_vector_type(4)));
+constexpr int RW = 16; // waves per workgroup of the row-layout passes
+
 // one workgroup per network cell (I, J, K): r summed over its F^3 block
 // (per-thread FP32 partial sums over a fixed subset, then a fixed FP64
 // tree: deterministic) / F, and the mask (1 where the block touches a wall);
@@ -136,6 +139,60 @@ __global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r,
         res[t] = (float)(tot / (double)F);
         const bool wall = I == 0 || J == 0 || K == 0 || F * (I + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
         mask[t] = wall ? 1.0f : 0.0f;
+    }
+}
+
+// The same restriction with one workgroup of RW waves per row (J, K) of the
+// network grid: lane = float4 column (W <= 256, W a multiple of 4, F >= 4),
+// waves over the F x F fine rows, so each wave reads whole rows; cell I's
+// F/4 lanes are reduced by shuffles, the waves through LDS in a fixed order.
+__global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __restrict__ r, int W, int H, int D, int F,
+                                                              int Wc, int Hc, int Dc, int Wn, int Hn,
+                                                              float* __restrict__ res, float* __restrict__ mask,
+                                                              const int* __restrict__ stop) {
+    if (*stop) return;
+    const int J = blockIdx.x % Hn, K = blockIdx.x / Hn;
+    const size_t row0 = ((size_t)K * Hn + J) * Wn;
+    if (J >= Hc || K >= Dc) {
+        for (int I = threadIdx.x; I < Wn; I += 64 * RW) {
+            res[row0 + I] = 0.0f;
+            mask[row0 + I] = 1.0f;
+        }
+        return;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 4 * lane;
+    const int y0 = F * J, z0 = F * K;
+    const int ny = min(F, H - y0), nz = min(F, D - z0);
+    float sum = 0.0f;
+    if (i0 < W)
+        for (int q = w; q < ny * nz; q += RW) {
+            const int j = y0 + q % ny, k = z0 + q / ny;
+            const f4w v = *(const f4w*)(r + ((size_t)k * H + j) * W + i0);
+            sum += v[0];
+            sum += v[1];
+            sum += v[2];
+            sum += v[3];
+        }
+    const int gl = F >> 2; // lanes per coarse cell
+    double acc = (double)sum;
+    for (int o = 1; o < gl; o <<= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double sv[RW][64];
+    const int I = lane / gl;
+    if ((lane & (gl - 1)) == 0) sv[w][I] = acc;
+    __syncthreads();
+    if (threadIdx.x < Wn) {
+        const int c = threadIdx.x;
+        if (c >= Wc) {
+            res[row0 + c] = 0.0f;
+            mask[row0 + c] = 1.0f;
+        } else {
+            double tot = 0.0;
+            for (int u = 0; u < RW; ++u) tot += sv[u][c];
+            res[row0 + c] = (float)(tot / (double)F);
+            const bool wall = c == 0 || J == 0 || K == 0 || F * (c + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
+            mask[row0 + c] = wall ? 1.0f : 0.0f;
+        }
     }
 }
 
@@ -181,7 +238,6 @@ __global__ void k_ws_export(const float* __restrict__ res, const float* __restri
     out2[n + t] = mask[t];
 }
 
-typedef float f4w __attribute__((ext_vector_type(4)));
 
 // e = trilinear(prediction) at every fine cell, natural order; grid (x
 // blocks, H, D): the y and z weights are uniform over a workgroup
@@ -400,7 +456,6 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
 // reads whole 1 KB rows instead of one block's 4F-byte pieces; waves take
 // the F x F rows in turn. Block I = lane / (F/4): its lanes are reduced by
 // shuffles, the waves through LDS in a fixed order.
-constexpr int RW = 16;
 __global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __restrict__ r,
                                                                const float* __restrict__ e1,
                                                                const double* __restrict__ alpha1,
@@ -934,9 +989,12 @@ static int ws_restrict(Solver& s, float dt, const int* stop) {
     hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
                        scale, ws_r(s), stop);
     hipLaunchKernelGGL(k_ws_zero, dim3(1), dim3(1), 0, s.stream, amax);
-    hipLaunchKernelGGL(k_ws_restrict, dim3((unsigned)nn), dim3(BS), 0, s.stream, ws_r(s), s.W, s.H, s.D, s.ws_f,
-                       ilog2(s.ws_f), s.Wc,
-                       s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
+    if (s.W % 4 == 0 && s.W <= 256 && s.ws_f >= 4 && s.Wn <= 64)
+        hipLaunchKernelGGL(k_ws_restrict_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream, ws_r(s), s.W,
+                           s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
+    else
+        hipLaunchKernelGGL(k_ws_restrict, dim3((unsigned)nn), dim3(BS), 0, s.stream, ws_r(s), s.W, s.H, s.D, s.ws_f,
+                           ilog2(s.ws_f), s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
     hipLaunchKernelGGL(k_ws_rms, dim3(1), dim3(1024), 0, s.stream, res, (int)nn, s.Wc * s.Hc * s.Dc, amax, stop);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;

@@ -132,6 +132,42 @@ def test_global_correction_matches_oracle(res):
         assert np.linalg.norm(dg - dw) <= 2e-3 * np.linalg.norm(dw)
 
 
+@pytest.mark.parametrize("res", [(64, 48, 40), (256, 32, 24), (66, 20, 36)])
+def test_network_input_is_the_restricted_residual(res):
+    """The network's input (blz_warm_start_input): the FP64 residual of the
+    oracle (oracle/coarse_oracle.py) summed over each network cell's F^3
+    block, divided by F and normalised by its RMS over the real cells; the
+    solid mask on the cells touching a wall and on the padding. Widths 64
+    and 256 take the row-layout restriction, 66 the per-cell one."""
+    C = pytest.importorskip("oracle.coarse_oracle")
+    print(f"[ws] network input {res}", flush=True)
+    W, H, D = res
+    s = settings(B, res, max_iter_solve=10, residual_tol=1e-7, residual_check_every=10, warm_start=1)
+    rng = np.random.default_rng(17)
+    rhs = rng.standard_normal((D, H, W)).astype(np.float32)
+    p0 = (0.01 * rng.standard_normal((D, H, W))).astype(np.float32)
+    g = B.Solver(s)
+    g.set_field(B.FIELD_RHS, rhs)
+    g.set_field(B.FIELD_P, p0)
+    g.set_warm_start(B.Net(seed=0))
+    (wn, hn, dn), (wc, hc, dc) = g.warm_start_dims()
+    dt = 1.0 / 24.0
+    got = g.warm_start_input(dt)
+    F = next(f for f in (1, 2, 4, 8, 16, 32, 64) if (-(-W // f), -(-H // f), -(-D // f)) == (wc, hc, dc))
+    scale = float(np.float32(np.float32(s.density * s.dx) * np.float32(s.dx)) / np.float32(dt))
+    res_c = C.block_sums(C.residual(p0, rhs, scale), F) / F
+    want = res_c / np.sqrt(np.mean(res_c ** 2))
+    K, J, I = np.meshgrid(np.arange(dc), np.arange(hc), np.arange(wc), indexing="ij")
+    wall = (I == 0) | (J == 0) | (K == 0) | (F * (I + 1) >= W) | (F * (J + 1) >= H) | (F * (K + 1) >= D)
+    print(f"[ws] F={F} net {wn}x{hn}x{dn} coarse {wc}x{hc}x{dc} max err "
+          f"{np.abs(got[0, :dc, :hc, :wc] - want).max():.2e}", flush=True)
+    np.testing.assert_allclose(got[0, :dc, :hc, :wc], want, atol=2e-4)
+    assert np.array_equal(got[1, :dc, :hc, :wc], wall.astype(np.float32))
+    pad = np.ones((dn, hn, wn), bool)
+    pad[:dc, :hc, :wc] = False
+    assert not got[0][pad].any() and (got[1][pad] == 1.0).all()
+
+
 def run_frames(scene, res, frames, tol, cap, net, start=0, every_ws=1, network=True, global_correction=True):
     """Frames `start`..`start+frames-1` of `scene` at grid `res`, stage by
     stage; returns the sweeps of each frame's solve, the kept/rejected
