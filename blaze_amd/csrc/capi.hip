@@ -128,11 +128,15 @@ int Solver::init(const blz_settings& st_) {
     L.Hp = H + 2;
     L.Dg = (D + 3) / 4;
     Np = (size_t)L.Wp * L.Hp * (L.Dg + 2) * 4;
-    // six planes when their byte offsets (plus a tail that absorbs the fused
-    // sweep's reads two rows past the last group) fit 32 bits, else four
-    // (fused sweeps unavailable)
+    // eight planes when their byte offsets (plus a tail that absorbs the
+    // fused sweep's reads two rows past the last group) fit 32 bits, else
+    // six (the warm start's judge then sweeps in place), else four (fused
+    // sweeps unavailable)
     const size_t tail = (size_t)2 * L.Wp * L.Hp * 4;
-    prb_floats = (6 * Np + tail) * 4 < ((size_t)1 << 32) && W <= 4096 ? 6 * Np + tail : 4 * Np;
+    const bool fit8 = (8 * Np + tail) * 4 < ((size_t)1 << 32) && W <= 4096;
+    const bool fit6 = (6 * Np + tail) * 4 < ((size_t)1 << 32) && W <= 4096;
+    prb_floats = fit8 ? 8 * Np + tail : (fit6 ? 6 * Np + tail : 4 * Np);
+    judge_set = fit8 ? 6 : 4;
     if (hipMalloc(&prb, prb_floats * sizeof(float)) != hipSuccess || hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess ||
         hipMalloc(&red, (8 + 2 * RESIDUAL_BLOCKS) * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc failed for solver scratch");

@@ -1156,7 +1156,7 @@ __global__ void __launch_bounds__(BS) k_init_pressure(float* __restrict__ prb, s
         const int m = (int)(v % L.Wp), j = (int)((v / L.Wp) % L.Hp), g = (int)(v / ((size_t)L.Wp * L.Hp));
         const int k = (g - 1) * 4 + comp;
         const bool inside = m >= 1 && m <= L.Wh && j >= 1 && j <= L.H && g >= 1 && g <= L.Dg && k < L.D;
-        const bool pset = plane == 0 || plane == 1 || plane == 4 || plane == 5;
+        const bool pset = plane != 2 && plane != 3; // pressure planes: home, spare, judge
         prb[t] = (pset && !inside) ? -0.0f : 0.0f;
     }
 }
@@ -1247,13 +1247,14 @@ int launch_divergence(Solver& s) {
     return BLZ_OK;
 }
 
+// set: first plane of the pressure pair swept (0 home, 4 spare, 6 judge)
 template <int PG>
 static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop, int set = 0) {
     const size_t cols = (size_t)(s.W / 2) * s.H;
     const dim3 grid(blocks_for(cols, 64)), block(64);
     const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
     const unsigned plane_bytes = (unsigned)(s.Np * 4);
-    const unsigned p0 = set ? 4u * plane_bytes : 0u; // pressure planes: home (0, 1) or spare (4, 5)
+    const unsigned p0 = (unsigned)set * plane_bytes;
     const unsigned nbytes = (unsigned)(set ? s.prb_floats * 4 : 4 * s.Np * 4);
     for (int l = 0; l < sweeps; ++l) {
         for (int c = 0; c < 2; ++c) {
@@ -1266,11 +1267,11 @@ static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop, int 
     return BLZ_OK;
 }
 
-// `sweeps` two-pass sweeps on the spare pressure planes (the warm start's
+// `sweeps` two-pass sweeps on the judge's pressure planes (the warm start's
 // uncorrected path; same operations as on the home planes)
 int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop) {
     const float scale = (s.s.density * s.dx * s.dx) / dt;
-    return rbgs_passes<8>(s, scale, sweeps, stop, 1);
+    return rbgs_passes<8>(s, scale, sweeps, stop, s.judge_set);
 }
 
 // residual of the spare planes into partial, for the warm start's judge
@@ -1279,8 +1280,8 @@ int launch_residual_partial(Solver& s, float dt, bool spare, double* partial, co
     const int nb = residual_blocks(s);
     RB pp = s.p;
     if (spare) {
-        pp.c[0] = s.prb + 4 * s.Np;
-        pp.c[1] = s.prb + 5 * s.Np;
+        pp.c[0] = s.prb + (size_t)s.judge_set * s.Np;
+        pp.c[1] = s.prb + (size_t)(s.judge_set + 1) * s.Np;
     }
     hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, pp, s.rhs, s.L, s.W, s.H, s.D, cc, cb_a(s),
                        cb_b(s), partial, stop);

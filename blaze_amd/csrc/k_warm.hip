@@ -1070,18 +1070,19 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
         BLZ_HIP(hipGetLastError());
         if (int e = s.ws_net->forward()) return e;
     }
-    // the uncorrected pressure goes to the spare planes, the corrected one
-    // stays in the home planes
-    float* spare = s.prb + 4 * s.Np;
+    // the uncorrected pressure goes to the judge planes (the spare ones
+    // where no judge planes exist), the corrected one stays in the home planes
+    float* spare = s.prb + (size_t)s.judge_set * s.Np;
     BLZ_HIP(hipMemcpyAsync(spare, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
     if (int e = ws_correct(s, c, use_net ? s.ws_net->out : nullptr)) return e;
     // the judge: the same sweeps on both, the lower residual continues
     int launches = 0, kind = 0;
     const int keep = s.rbgs_variant;
-    // in place on the home planes (the spare ones hold the other path): a
-    // two-pass variant, the chosen one when it is one
+    // on the home planes: the chosen variant when the uncorrected path is in
+    // the judge planes (fused sweeps ping-pong through the spare ones), else
+    // in place with a two-pass variant (the chosen one when it is one)
     const int v = keep ? keep : s.rbgs_auto;
-    s.rbgs_variant = (v == 2 || v == 4) ? v : 8;
+    s.rbgs_variant = s.judge_set == 6 ? (v ? v : 8) : ((v == 2 || v == 4) ? v : 8);
     int rc = launch_rbgs_sweeps(s, dt, judge, &c->done, &launches, &kind);
     s.rbgs_variant = keep;
     if (rc) return rc;

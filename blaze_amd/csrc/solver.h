@@ -115,9 +115,12 @@ struct Solver {
     hipStream_t stream = nullptr;
     Field f[F_N];
     RB p, rhs;              // planes of one allocation: p red, p black, rhs red, rhs black,
-                            // spare p red, spare p black (fused sweeps), then a tail of slack
+                            // spare p red, spare p black (fused sweeps), judge p red, judge p
+                            // black (the warm start's uncorrected copy), then a tail of slack
     float* prb = nullptr;
     size_t prb_floats = 0;  // floats in prb
+    int judge_set = 4;      // first plane of the warm start's uncorrected copy: 6 when the judge
+                            // planes exist (the judge's sweeps may then be fused), else 4
     RBL L{};
     size_t Np = 0;          // floats per padded plane
     float* tmp = nullptr;     // 7 * W*H*D scratch for cell-centred intermediates
