@@ -603,7 +603,7 @@ def batched(a):
         "config": {"workload": f"{a.scenes} scenes, N uniform in [32,128] step 8 (seed 5), RBGS to 1e-4 cap 100",
                    "voxels": vox, "parallelism": "single GPU"},
         "batch": {"autotuned_variant": v, "graph_nodes": st.graph_nodes, "capture_ms": st.capture_ms,
-                  "tune_ms_per_sweep": {st.tune_variant[k]: st.tune_ms[k] for k in range(6)},
+                  "tune_ms_per_sweep": {st.tune_variant[k]: st.tune_ms[k] for k in range(B.BATCH_TUNE)},
                   "max_final_residual": max(res), "mean_final_residual": sum(res) / len(res)},
     }
     print(json.dumps(out), flush=True)

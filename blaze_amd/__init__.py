@@ -29,6 +29,7 @@ FIELDS = {"T": FIELD_T, "U": FIELD_U, "V": FIELD_V, "W": FIELD_W, "density": FIE
 (STAGE_COOLDOWN, STAGE_DRAG, STAGE_VORTICITY, STAGE_BUOYANCY, STAGE_WIND, STAGE_TURBULENCE,
  STAGE_DIVERGENCE, STAGE_PRESSURE, STAGE_PROJECT_UPDATE, STAGE_ADVECT) = range(10)
 NUM_STAGES = 10
+BATCH_TUNE = 10  # BLZ_BATCH_TUNE: batch autotuner candidates
 STAGE_NAMES = ["cooldown", "drag", "vorticity", "buoyancy", "wind", "turbulence", "divergence",
                "pressure", "project_update", "advect"]
 
@@ -74,7 +75,7 @@ class BatchStats(C.Structure):
     _fields_ = [
         ("scenes", C.c_int), ("variant", C.c_int), ("graph_nodes", C.c_int), ("pad_", C.c_int),
         ("voxels", C.c_longlong), ("capture_ms", C.c_double), ("step_ms", C.c_double),
-        ("tune_variant", C.c_int * 6), ("tune_ms", C.c_double * 6),
+        ("tune_variant", C.c_int * BATCH_TUNE), ("tune_ms", C.c_double * BATCH_TUNE),
     ]
 
 
@@ -164,6 +165,7 @@ def _load():
         "blz_batch_solver": ([vp, C.c_int, P(vp)], C.c_int),
         "blz_batch_set_warm_start": ([vp, vp], C.c_int),
         "blz_batch_autotune": ([vp, C.c_int, P(C.c_int)], C.c_int),
+        "blz_batch_set_variant": ([vp, C.c_int], C.c_int),
         "blz_batch_step": ([vp, C.c_float], C.c_int),
         "blz_batch_get_stats": ([vp, P(BatchStats)], C.c_int),
     }
@@ -427,6 +429,10 @@ class Batch:
         v = C.c_int()
         _check(_lib.blz_batch_autotune(self._h, sweeps, C.byref(v)), "blz_batch_autotune")
         return v.value
+
+    def set_variant(self, v: int):
+        """Sweep with candidate v of the autotuner (stats().tune_variant)."""
+        _check(_lib.blz_batch_set_variant(self._h, v), "blz_batch_set_variant")
 
     def step(self, time: float):
         _check(_lib.blz_batch_step(self._h, C.c_float(time)), "blz_batch_step")

@@ -21,6 +21,8 @@ namespace blz {
 int launch_fused_multi(hipStream_t st, int variant, const FusedScene* tab, int n, int blocks);
 int launch_residual_multi(hipStream_t st, const ResScene* tab, int n, int blocks, int add, int first);
 int fused_waves(const Solver& s);
+int pair_waves(const Solver& s);
+constexpr int NTUNE = BLZ_BATCH_TUNE;
 int residual_blocks(const Solver& s);
 
 struct Batch {
@@ -42,7 +44,7 @@ struct Batch {
     int variant = 103;
     int nodes = 0;
     double capture_ms = 0.0, step_ms = 0.0;
-    double tune_ms[6] = {0, 0, 0, 0, 0, 0};
+    double tune_ms[NTUNE] = {};
     const Net* net = nullptr;
 
     ~Batch();
@@ -54,8 +56,10 @@ struct Batch {
     int autotune(int sweeps, int* best);
 };
 
-static const int kVariants[6] = {102, 103, 202, 203, 402, 403};
-static int wpb_of(int v) { return v / 100; }
+// fused (one row per wave) and row-pair candidates; waves per block = the
+// hundreds digit
+static const int kVariants[NTUNE] = {102, 103, 202, 203, 402, 403, 1103, 1104, 1203, 1204};
+static int wpb_of(int v) { return (v / 100) % 10; }
 
 Batch::~Batch() {
     if (root) (void)hipStreamSynchronize(root);
@@ -125,7 +129,7 @@ int Batch::build_tables(int v) {
         Solver& s = sc[q]->s;
         const unsigned pb = (unsigned)(s.Np * 4);
         const float dt = s.s.dt;
-        const int nw = fused_waves(s);
+        const int nw = v >= 1000 ? pair_waves(s) : fused_waves(s);
         for (int h = 0; h < 2; ++h) {
             FusedScene& e = fs_h[h * n + q];
             const unsigned src = h == 0 ? 0u : 4u * pb, dst = h == 0 ? 4u * pb : 0u;
@@ -356,7 +360,7 @@ int Batch::autotune(int sweeps, int* best) {
     int bestv = variant;
     double bestms = 1e30;
     int rc = BLZ_OK;
-    for (int c = 0; c < 6 && !rc; ++c) {
+    for (int c = 0; c < NTUNE && !rc; ++c) {
         const int v = kVariants[c];
         if ((rc = build_tables(v))) break;
         // one untimed pair, then the timed sweeps
@@ -463,6 +467,27 @@ int blz_batch_autotune(blz_batch* b, int sweeps, int* variant) {
     return b->b.autotune(sweeps, variant);
 }
 
+int blz_batch_set_variant(blz_batch* b, int variant) {
+    if (!b) {
+        blz::set_error("null batch");
+        return BLZ_E_INVALID;
+    }
+    const int* end = blz::kVariants + blz::NTUNE;
+    if (std::find(blz::kVariants, end, variant) == end) {
+        blz::set_error("blz_batch_set_variant: not a batch sweep variant (see blz_batch_stats.tune_variant)");
+        return BLZ_E_INVALID;
+    }
+    for (blz_solver* h : b->b.sc) BLZ_HIP(hipStreamSynchronize(h->s.stream));
+    BLZ_HIP(hipStreamSynchronize(b->b.root));
+    if (int e = b->b.build_tables(variant)) return e;
+    for (auto& x : b->b.exec)
+        if (x) {
+            BLZ_HIP(hipGraphExecDestroy(x));
+            x = nullptr;
+        }
+    return BLZ_OK;
+}
+
 int blz_batch_step(blz_batch* b, float time) {
     if (!b) {
         blz::set_error("null batch");
@@ -485,7 +510,7 @@ int blz_batch_get_stats(blz_batch* b, blz_batch_stats* out) {
     long long vox = 0;
     for (blz_solver* h : B.sc) vox += (long long)h->s.W * h->s.H * h->s.D;
     out->voxels = vox;
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < blz::NTUNE; ++c) {
         out->tune_variant[c] = blz::kVariants[c];
         out->tune_ms[c] = B.tune_ms[c];
     }

@@ -913,6 +913,25 @@ __global__ void __launch_bounds__(64 * WPB) k_rbgs_pair(float* __restrict__ base
     pair_wave<PG>(base, nbytes, srcR, srcB, rhsR, rhsB, dstR, dstB, W, H, D, Wp, Hp, scale, tab, wv);
 }
 
+// the row-pair sweep over several grids in one launch (table as for
+// k_rbgs_fused_multi, nwaves counted in row-pair waves)
+template <int PG, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_rbgs_pair_multi(const FusedScene* __restrict__ sc, int n, DivTab tab) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sc[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const FusedScene& e = sc[lo];
+    if (*e.stop) return;
+    const int wv = (b - e.first_block) * WPB + (int)(threadIdx.x >> 6);
+    if (wv >= e.nwaves) return;
+    pair_wave<PG>(e.base, e.nbytes, e.srcR, e.srcB, e.rhsR, e.rhsB, e.dstR, e.dstB, e.W, e.H, e.D, e.Wp, e.Hp,
+                  e.scale, tab, wv);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -1387,6 +1406,7 @@ int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* la
 }
 
 int fused_waves(const Solver& s) { return ((s.W + 63) / 64) * s.H; }
+int pair_waves(const Solver& s) { return pair_segments(s.W) * ((s.H + 1) / 2); }
 
 int residual_blocks(const Solver& s) {
     const size_t items = (size_t)(s.W / 2) * s.H * ((s.L.Dg + RES_SEG - 1) / RES_SEG);
@@ -1401,8 +1421,20 @@ static int fused_multi(hipStream_t st, const FusedScene* tab, int n, int blocks)
     return BLZ_OK;
 }
 
+template <int PG, int WPB>
+static int pair_multi(hipStream_t st, const FusedScene* tab, int n, int blocks) {
+    const DivTab dt{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
+    hipLaunchKernelGGL((k_rbgs_pair_multi<PG, WPB>), dim3((unsigned)blocks), dim3(64 * WPB), 0, st, tab, n, dt);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
 int launch_fused_multi(hipStream_t st, int variant, const FusedScene* tab, int n, int blocks) {
     switch (variant) {
+    case 1103: return pair_multi<3, 1>(st, tab, n, blocks);
+    case 1104: return pair_multi<4, 1>(st, tab, n, blocks);
+    case 1203: return pair_multi<3, 2>(st, tab, n, blocks);
+    case 1204: return pair_multi<4, 2>(st, tab, n, blocks);
     case 102: return fused_multi<2, 1>(st, tab, n, blocks);
     case 202: return fused_multi<2, 2>(st, tab, n, blocks);
     case 203: return fused_multi<3, 2>(st, tab, n, blocks);

@@ -304,6 +304,7 @@ int blz_warm_start_apply(blz_solver* h, float dt, const float* pred, double* bef
    divides their max_iter_solve. Sourcing (blz_add_source on the scene
    handles) happens between steps. */
 typedef struct blz_batch blz_batch;
+#define BLZ_BATCH_TUNE 10 /* batch autotuner candidates */
 typedef struct blz_batch_stats {
     int scenes;
     int variant;              /* fused sweep variant in use (blz_set_rbgs_variant numbering) */
@@ -312,8 +313,8 @@ typedef struct blz_batch_stats {
     long long voxels;         /* cells over all scenes */
     double capture_ms;        /* host time spent capturing and instantiating */
     double step_ms;           /* wall time of the last blz_batch_step replay */
-    int tune_variant[6];      /* autotuner candidates ... */
-    double tune_ms[6];        /* ... and their time per sweep over the batch (0 if not tuned) */
+    int tune_variant[BLZ_BATCH_TUNE]; /* autotuner candidates (fused and row-pair sweeps) ... */
+    double tune_ms[BLZ_BATCH_TUNE];   /* ... and their time per sweep over the batch (0 if not tuned) */
 } blz_batch_stats;
 int blz_batch_create(const blz_settings* settings, int n, blz_batch** out);
 int blz_batch_destroy(blz_batch* b);
@@ -324,6 +325,10 @@ int blz_batch_set_warm_start(blz_batch* b, blz_net* net);
    flight x waves per block) over the whole batch, keep the fastest; the
    pressure fields are restored afterwards. */
 int blz_batch_autotune(blz_batch* b, int sweeps, int* variant);
+/* Use one of the autotuner's candidates (blz_batch_stats.tune_variant) for
+   the batch's sweeps (no reference counterpart; every candidate gives the
+   same bits). */
+int blz_batch_set_variant(blz_batch* b, int variant);
 int blz_batch_step(blz_batch* b, float time);
 int blz_batch_get_stats(blz_batch* b, blz_batch_stats* out);
 /* Self-test of the pressure solve's division by a neighbour count d in

@@ -39,8 +39,10 @@ def scene(res, seed):
     return s, part
 
 
-@pytest.mark.parametrize("tune", [False, True])
+@pytest.mark.parametrize("tune", [False, True, 1104, 1203])
 def test_batch_equals_single_scenes(tune):
+    """tune: False = the default variant, True = the autotuner's choice, a
+    number = that row-pair variant (multi-grid row-pair sweep)."""
     print(f"[batch] tune={tune}", flush=True)
     cases = [scene(r, i) for i, r in enumerate(SHAPES)]
     b = B.Batch([c[0] for c in cases])
@@ -48,11 +50,14 @@ def test_batch_equals_single_scenes(tune):
     for i, (s, part) in enumerate(cases):
         b.scenes[i].add_source(part)
         singles[i].add_source(part)
-    if tune:
+    if tune not in (False, True):
+        b.set_variant(tune)
+        assert b.stats().variant == tune
+    elif tune:
         v = b.autotune(10)
         st = b.stats()
         print(f"[batch] autotuned variant {v}: " + ", ".join(
-            f"{st.tune_variant[k]}={st.tune_ms[k]:.4f}ms" for k in range(6)), flush=True)
+            f"{st.tune_variant[k]}={st.tune_ms[k]:.4f}ms" for k in range(B.BATCH_TUNE)), flush=True)
     t = np.float32(0.0)
     for step in range(3):
         b.step(float(t))
@@ -69,6 +74,13 @@ def test_batch_equals_single_scenes(tune):
     st = b.stats()
     print(f"[batch] nodes {st.graph_nodes} capture {st.capture_ms:.1f} ms step {st.step_ms:.2f} ms", flush=True)
     assert st.graph_nodes > 0
+
+
+def test_batch_rejects_unknown_variant():
+    s, _ = scene((32, 24, 16), 0)
+    b = B.Batch([s])
+    with pytest.raises(B.BlazeError, match="not a batch sweep variant"):
+        b.set_variant(8)
 
 
 def test_batch_rejects_unsupported_settings():
