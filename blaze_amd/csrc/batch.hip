@@ -175,6 +175,9 @@ int Batch::build_tables(int v) {
     rs_blocks = rb;
     BLZ_HIP(hipMemcpy(fs, fs_h.data(), 2 * n * sizeof(FusedScene), hipMemcpyHostToDevice));
     BLZ_HIP(hipMemcpy(rs, rs_h.data(), n * sizeof(ResScene), hipMemcpyHostToDevice));
+    // the scenes' own sweeps (the warm start's judge, on each scene's stream)
+    // use the same variant as a single-grid launch: one launch per sweep
+    for (int q = 0; q < n; ++q) sc[q]->s.rbgs_auto = v;
     variant = v;
     return BLZ_OK;
 }
