@@ -1286,12 +1286,6 @@ static int rbgs_passes(Solver& s, float scale, int sweeps, const int* stop, int 
     return BLZ_OK;
 }
 
-// `sweeps` two-pass sweeps on the judge's pressure planes (the warm start's
-// uncorrected path; same operations as on the home planes)
-int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop) {
-    const float scale = (s.s.density * s.dx * s.dx) / dt;
-    return rbgs_passes<8>(s, scale, sweeps, stop, s.judge_set);
-}
 
 // residual of the spare planes into partial, for the warm start's judge
 int launch_residual_partial(Solver& s, float dt, bool spare, double* partial, const int* stop, int* nb_out) {
@@ -1311,17 +1305,20 @@ int launch_residual_partial(Solver& s, float dt, bool spare, double* partial, co
 
 // Pairs of fused sweeps: home set -> spare set -> home set, so the pressure
 // is in the home planes after every pair (and at every residual check).
+// home: first plane of the pressure pair swept (0, or 6 for the judge's
+// copy); the spare planes (4, 5) are the other half of each pair
 template <int PG, int WPB>
-static int fused_pairs(Solver& s, float scale, int pairs, const int* stop) {
+static int fused_pairs(Solver& s, float scale, int pairs, const int* stop, int home) {
     const int nwr = (s.W + 63) / 64;
     const int nwaves = nwr * s.H;
     const dim3 grid((unsigned)((nwaves + WPB - 1) / WPB)), block(64 * WPB);
     const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
     const unsigned pb = (unsigned)(s.Np * 4);
     const unsigned nbytes = (unsigned)(s.prb_floats * 4);
+    const unsigned hp = (unsigned)home * pb;
     for (int l = 0; l < pairs; ++l) {
         for (int h = 0; h < 2; ++h) {
-            const unsigned src = h == 0 ? 0u : 4u * pb, dst = h == 0 ? 4u * pb : 0u;
+            const unsigned src = h == 0 ? hp : 4u * pb, dst = h == 0 ? 4u * pb : hp;
             hipLaunchKernelGGL((k_rbgs_fused<PG, WPB>), grid, block, 0, s.stream, s.prb, nbytes, src, src + pb,
                                2u * pb, 3u * pb, dst, dst + pb, s.W, s.H, s.D, s.L.Wp, s.L.Hp, scale, tab, stop,
                                nwaves);
@@ -1333,15 +1330,16 @@ static int fused_pairs(Solver& s, float scale, int pairs, const int* stop) {
 
 // the same pairs with the row-pair fused sweep
 template <int PG, int WPB>
-static int pair_pairs(Solver& s, float scale, int pairs, const int* stop) {
+static int pair_pairs(Solver& s, float scale, int pairs, const int* stop, int home) {
     const int nwaves = pair_segments(s.W) * ((s.H + 1) / 2);
     const dim3 grid((unsigned)((nwaves + WPB - 1) / WPB)), block(64 * WPB);
     const DivTab tab{1.0f / 3.0f, 1.0f / 4.0f, 1.0f / 5.0f, 1.0f / 6.0f};
     const unsigned pb = (unsigned)(s.Np * 4);
     const unsigned nbytes = (unsigned)(s.prb_floats * 4);
+    const unsigned hp = (unsigned)home * pb;
     for (int l = 0; l < pairs; ++l) {
         for (int h = 0; h < 2; ++h) {
-            const unsigned src = h == 0 ? 0u : 4u * pb, dst = h == 0 ? 4u * pb : 0u;
+            const unsigned src = h == 0 ? hp : 4u * pb, dst = h == 0 ? 4u * pb : hp;
             hipLaunchKernelGGL((k_rbgs_pair<PG, WPB>), grid, block, 0, s.stream, s.prb, nbytes, src, src + pb,
                                2u * pb, 3u * pb, dst, dst + pb, s.W, s.H, s.D, s.L.Wp, s.L.Hp, scale, tab, stop,
                                nwaves);
@@ -1365,23 +1363,38 @@ bool rbgs_variant_valid(int v) {
            v == 1404;
 }
 
-static int fused_dispatch(Solver& s, int v, float scale, int pairs, const int* stop) {
+static int fused_dispatch(Solver& s, int v, float scale, int pairs, const int* stop, int home = 0) {
     switch (v) {
-    case 1103: return pair_pairs<3, 1>(s, scale, pairs, stop);
-    case 1104: return pair_pairs<4, 1>(s, scale, pairs, stop);
-    case 1105: return pair_pairs<5, 1>(s, scale, pairs, stop);
-    case 1106: return pair_pairs<6, 1>(s, scale, pairs, stop);
-    case 1203: return pair_pairs<3, 2>(s, scale, pairs, stop);
-    case 1204: return pair_pairs<4, 2>(s, scale, pairs, stop);
-    case 1403: return pair_pairs<3, 4>(s, scale, pairs, stop);
-    case 1404: return pair_pairs<4, 4>(s, scale, pairs, stop);
-    case 102: return fused_pairs<2, 1>(s, scale, pairs, stop);
-    case 202: return fused_pairs<2, 2>(s, scale, pairs, stop);
-    case 203: return fused_pairs<3, 2>(s, scale, pairs, stop);
-    case 402: return fused_pairs<2, 4>(s, scale, pairs, stop);
-    case 403: return fused_pairs<3, 4>(s, scale, pairs, stop);
-    default: return fused_pairs<3, 1>(s, scale, pairs, stop);
+    case 1103: return pair_pairs<3, 1>(s, scale, pairs, stop, home);
+    case 1104: return pair_pairs<4, 1>(s, scale, pairs, stop, home);
+    case 1105: return pair_pairs<5, 1>(s, scale, pairs, stop, home);
+    case 1106: return pair_pairs<6, 1>(s, scale, pairs, stop, home);
+    case 1203: return pair_pairs<3, 2>(s, scale, pairs, stop, home);
+    case 1204: return pair_pairs<4, 2>(s, scale, pairs, stop, home);
+    case 1403: return pair_pairs<3, 4>(s, scale, pairs, stop, home);
+    case 1404: return pair_pairs<4, 4>(s, scale, pairs, stop, home);
+    case 102: return fused_pairs<2, 1>(s, scale, pairs, stop, home);
+    case 202: return fused_pairs<2, 2>(s, scale, pairs, stop, home);
+    case 203: return fused_pairs<3, 2>(s, scale, pairs, stop, home);
+    case 402: return fused_pairs<2, 4>(s, scale, pairs, stop, home);
+    case 403: return fused_pairs<3, 4>(s, scale, pairs, stop, home);
+    default: return fused_pairs<3, 1>(s, scale, pairs, stop, home);
     }
+}
+
+// `sweeps` sweeps on the judge's pressure planes (the warm start's
+// uncorrected path; same operations as on the home planes): with judge
+// planes of their own, the chosen fused variant in pairs through the spare
+// planes (free again once the corrected path's sweeps are done), an odd
+// last sweep as two colour passes; else two-pass sweeps in place
+int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop) {
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
+    const int v = s.rbgs_variant ? s.rbgs_variant : s.rbgs_auto;
+    if (s.judge_set == 6 && v >= 100) {
+        if (int e = fused_dispatch(s, v, scale, sweeps / 2, stop, 6)) return e;
+        return (sweeps & 1) ? rbgs_passes<8>(s, scale, 1, stop, 6) : BLZ_OK;
+    }
+    return rbgs_passes<8>(s, scale, sweeps, stop, s.judge_set);
 }
 
 int launch_rbgs_sweeps(Solver& s, float dt, int sweeps, const int* stop, int* launches, int* kind) {
