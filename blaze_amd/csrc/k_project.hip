@@ -953,7 +953,7 @@ __device__ __forceinline__ double cell_res(double pc, double rc, double sum, dou
 // for four slabs); z neighbours come from the previous and next group of the
 // thread's own column. Grid-stride; one partial maximum per block, reduced
 // by k_res_final. `stop` is the solve's done flag: a converged solve skips.
-constexpr int RES_SEG = 4;
+constexpr int RES_SEG = 2;
 
 // block blk of nblk blocks of the residual of one grid
 __device__ __forceinline__ void residual_block(const RB& p, const RB& rhs, const RBL& L, int W, int H, int D,

@@ -100,6 +100,58 @@ __global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, i
     }
 }
 
+// The same convolution for one 16-channel output tile per wave with the
+// input chunk count CH = Cin/16 fixed at compile time: the 27 taps are
+// unrolled and every input load is unconditional (an out-of-grid tap reads
+// voxel 0 and is replaced by zeros), so the whole tap sequence is one block
+// whose loads the scheduler issues ahead of the MFMAs instead of waiting
+// out a load latency per tap. Same MFMA sequence per output as above.
+template <int CH>
+__global__ void __launch_bounds__(256) k_conv3_fwd1(const bf16* __restrict__ X, int Cx, int xoff,
+                                                    const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
+                                                    int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
+                                                    int relu, Geo g, long long nvox) {
+    constexpr int Cin = CH * 16;
+    const int c0 = blockIdx.y * 16;
+    const int lane = threadIdx.x & 63;
+    const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (v0 >= nvox) return;
+    const int li = lane & 15, kb = lane >> 4;
+    const long long va = v0 + li;
+    int b, z, y, x;
+    vox_coords(va < nvox ? va : nvox - 1, g, b, z, y, x);
+    const bool vok = va < nvox;
+    f4m acc = f4m{0.f, 0.f, 0.f, 0.f};
+    const long long HW = (long long)g.H * g.W;
+    const s4v zero = {0, 0, 0, 0};
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap) {
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int zz = z + dz, yy = y + dy, xx = x + dx;
+        const bool inb = vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+        const long long vv = va + dz * HW + dy * g.W + dx;
+        const bf16* xp = X + (inb ? vv : 0) * Cx + xoff + kb * 4;
+        const bf16* wp = Wt + ((size_t)tap * Cout + c0 + li) * Cin + kb * 4;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const s4v ld = *(const s4v*)(xp + c * 16);
+            const s4v a = inb ? ld : zero;
+            const s4v bw = *(const s4v*)(wp + c * 16);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const long long vo = v0 + kb * 4 + q;
+        if (vo >= nvox) continue;
+        float r = acc[q];
+        const int co = c0 + li;
+        if (relu) r = r > 0.0f ? r : NET_LEAK * r;
+        if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : NET_LEAK * r;
+        Y[vo * Cy + yoff + co] = f2bf(r);
+    }
+}
+
 // weight gradient: dW[tap][co][ci] += sum_v G[v][co] X[v+tap][ci]. A wave owns
 // one 16x16 (co, ci) tile and a run of voxels; per 16 voxels the G fragment
 // (A = 16 co x 16 voxels) is loaded once and used for all 27 taps (B = 16
@@ -373,6 +425,23 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     const bool split = (nv + 15) / 16 < 2048 && cout > 16;
     const int nt = split ? 1 : cout / 16;
     const dim3 gr(grid, split ? (unsigned)(cout / 16) : 1u);
+    if (nt == 1 && (cin == 16 || cin == 32 || cin == 48 || cin == 64 || cin == 96)) {
+#define CONV1_CASE(CH)                                                                                             \
+    case CH:                                                                                                       \
+        hipLaunchKernelGGL(k_conv3_fwd1<CH>, gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, y.p, y.C, y.off, \
+                           mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv);                                         \
+        break;
+        switch (cin / 16) {
+            CONV1_CASE(1)
+            CONV1_CASE(2)
+            CONV1_CASE(3)
+            CONV1_CASE(4)
+            CONV1_CASE(6)
+        }
+#undef CONV1_CASE
+        NET_HIP(hipGetLastError());
+        return BLZ_OK;
+    }
 #define CONV_CASE(NT)                                                                                              \
     case NT:                                                                                                       \
         hipLaunchKernelGGL(k_conv3_fwd<NT>, gr, dim3(256), 0, n.stream, x.p, x.C, x.off, cin, w, cout, y.p, y.C,   \
