@@ -39,37 +39,50 @@ __global__ void __launch_bounds__(BS) k_buoyancy(Gv T, float* __restrict__ V, in
     V[t] += dt * force;
 }
 
-// computeVorticity :174-201 at cell (i,j,k) of the T grid
+// computeVorticity :174-201 at cell (i,j,k) of the T grid samples U, V and W
+// (trilinear) at the centres of the six neighbouring cells, positions
+// ((float)(i-+1) + ox) * dx etc. Each such position is, in the same float
+// operations, the centre g2o(T, dx, ...) of that neighbour, so every sample
+// is a pure function of one cell: k_vort_centres samples the three
+// components once per cell (tmp planes 4..6) and k_vort_omega differences
+// them, with the reference's expression order, the same bits as twelve
+// samples per cell at a quarter of the gathers.
 template <class S>
-__device__ __forceinline__ f3 vort_at(const Gv& T, const Gv& U, const Gv& V, const Gv& W, const S& sc, float dx,
-                                      int i, int j, int k) {
-    const float x0 = ((float)(i - 1) + T.ox) * dx, x1 = ((float)i + T.ox) * dx, x2 = ((float)(i + 1) + T.ox) * dx;
-    const float y0 = ((float)(j - 1) + T.oy) * dx, y1 = ((float)j + T.oy) * dx, y2 = ((float)(j + 1) + T.oy) * dx;
-    const float z0 = ((float)(k - 1) + T.oz) * dx, z1 = ((float)k + T.oz) * dx, z2 = ((float)(k + 1) + T.oz) * dx;
-    const float w121 = trilinear_s(W, sc, mk3(x1, y2, z1)), w101 = trilinear_s(W, sc, mk3(x1, y0, z1));
-    const float v112 = trilinear_s(V, sc, mk3(x1, y1, z2)), v110 = trilinear_s(V, sc, mk3(x1, y1, z0));
-    const float u112 = trilinear_s(U, sc, mk3(x1, y1, z2)), u110 = trilinear_s(U, sc, mk3(x1, y1, z0));
-    const float w211 = trilinear_s(W, sc, mk3(x2, y1, z1)), w011 = trilinear_s(W, sc, mk3(x0, y1, z1));
-    const float v211 = trilinear_s(V, sc, mk3(x2, y1, z1)), v011 = trilinear_s(V, sc, mk3(x0, y1, z1));
-    const float u121 = trilinear_s(U, sc, mk3(x1, y2, z1)), u101 = trilinear_s(U, sc, mk3(x1, y0, z1));
-    const float t = 2.0f * dx;
-    return mk3((w121 - w101 - (v112 - v110)) / t, (u112 - u110 - (w211 - w011)) / t,
-               (v211 - v011 - (u121 - u101)) / t);
+__global__ void __launch_bounds__(BS) k_vort_centres(Gv T, Gv U, Gv V, Gv W, S sc, float dx, float* __restrict__ tmp,
+                                                     int nW, int nH, int nD) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= nW * nH * nD) return;
+    const int i = t % nW, j = (t / nW) % nH, k = t / (nW * nH);
+    const size_t N = (size_t)nW * nH * nD;
+    const f3 pos = g2o(T, dx, i, j, k);
+    tmp[4 * N + t] = trilinear_s(U, sc, pos);
+    tmp[5 * N + t] = trilinear_s(V, sc, pos);
+    tmp[6 * N + t] = trilinear_s(W, sc, pos);
 }
 
 // omega and |omega| for cells 1..N-2 on each axis (tmp planes 0..3). The
 // reference evaluates omega 7 times per cell; it is a pure function of the
 // cell, so evaluating it once and reading it back gives the same bits.
-template <class S>
-__global__ void __launch_bounds__(BS) k_vort_omega(Gv T, Gv U, Gv V, Gv W, S sc, float dx, float* __restrict__ tmp,
-                                                   int nW, int nH, int nD) {
+__global__ void __launch_bounds__(BS) k_vort_omega(float dx, float* __restrict__ tmp, int nW, int nH, int nD) {
     const int iw = nW - 2, ih = nH - 2, id = nD - 2;
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= iw * ih * id) return;
     const int i = 1 + t % iw, j = 1 + (t / iw) % ih, k = 1 + t / (iw * ih);
-    const f3 om = vort_at(T, U, V, W, sc, dx, i, j, k);
     const size_t N = (size_t)nW * nH * nD;
-    const size_t c = (size_t)i + (size_t)j * nW + (size_t)k * nW * nH;
+    const size_t sy = nW, sz = (size_t)nW * nH;
+    const size_t c = (size_t)i + (size_t)j * sy + (size_t)k * sz;
+    const float* Uc = tmp + 4 * N;
+    const float* Vc = tmp + 5 * N;
+    const float* Wc = tmp + 6 * N;
+    const float w121 = Wc[c + sy], w101 = Wc[c - sy];
+    const float v112 = Vc[c + sz], v110 = Vc[c - sz];
+    const float u112 = Uc[c + sz], u110 = Uc[c - sz];
+    const float w211 = Wc[c + 1], w011 = Wc[c - 1];
+    const float v211 = Vc[c + 1], v011 = Vc[c - 1];
+    const float u121 = Uc[c + sy], u101 = Uc[c - sy];
+    const float t2 = 2.0f * dx;
+    const f3 om = mk3((w121 - w101 - (v112 - v110)) / t2, (u112 - u110 - (w211 - w011)) / t2,
+                      (v211 - v011 - (u121 - u101)) / t2);
     tmp[c] = om.x;
     tmp[N + c] = om.y;
     tmp[2 * N + c] = om.z;
@@ -77,9 +90,8 @@ __global__ void __launch_bounds__(BS) k_vort_omega(Gv T, Gv U, Gv V, Gv W, S sc,
 }
 
 // confinement increments 0.5*dt*f for active cells 2..N-3 (tmp planes 4..6)
-template <class S>
-__global__ void __launch_bounds__(BS) k_vort_force(Gv T, Gv U, Gv V, Gv W, S sc, float dx, float* __restrict__ tmp,
-                                                   int nW, int nH, int nD, float conf, float dt, float eps) {
+__global__ void __launch_bounds__(BS) k_vort_force(float dx, float* __restrict__ tmp, int nW, int nH, int nD, float conf,
+                                                   float dt, float eps) {
     const int iw = nW - 4, ih = nH - 4, id = nD - 4;
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= iw * ih * id) return;
@@ -92,8 +104,9 @@ __global__ void __launch_bounds__(BS) k_vort_force(Gv T, Gv U, Gv V, Gv W, S sc,
     const float t2 = 2.0f * dx;
     const f3 gn = mk3((nrm[c + 1] - nrm[c - 1]) / t2, (nrm[c + sy] - nrm[c - sy]) / t2,
                       (nrm[c + sz] - nrm[c - sz]) / t2);
-    const f3 pos = g2o(T, dx, i, j, k);
-    const f3 vel = mk3(trilinear_s(U, sc, pos), trilinear_s(V, sc, pos), trilinear_s(W, sc, pos));
+    // the velocity at the cell centre: this cell's k_vort_centres samples
+    // (read before this thread overwrites them with the increments)
+    const f3 vel = mk3(tmp[4 * N + c], tmp[5 * N + c], tmp[6 * N + c]);
     const float speed = len3(vel);
     const float onorm = nrm[c] + eps;
     if (onorm > speed) om = mk3(speed * om.x / onorm, speed * om.y / onorm, speed * om.z / onorm);
@@ -275,19 +288,17 @@ int launch_vorticity(Solver& s, float dt) {
     const float eps = 1e-20f * (1.0f / (s.dx * dt));
     const Gv T = s.f[F_T].view(), U = s.f[F_U].view(), V = s.f[F_V].view(), W = s.f[F_W].view();
     float inv = 0.0f;
+    const size_t n = (size_t)s.W * s.H * s.D;
     if (dx_pow2(s.dx, &inv)) {
         const MulDx sc{inv};
-        LAUNCH(k_vort_omega<MulDx>, (size_t)(s.W - 2) * (s.H - 2) * (s.D - 2), T, U, V, W, sc, s.dx, s.tmp, s.W,
-               s.H, s.D);
-        LAUNCH(k_vort_force<MulDx>, (size_t)(s.W - 4) * (s.H - 4) * (s.D - 4), T, U, V, W, sc, s.dx, s.tmp, s.W,
-               s.H, s.D, s.s.vorticity_conf, dt, eps);
+        LAUNCH(k_vort_centres<MulDx>, n, T, U, V, W, sc, s.dx, s.tmp, s.W, s.H, s.D);
     } else {
         const DivDx sc{s.dx};
-        LAUNCH(k_vort_omega<DivDx>, (size_t)(s.W - 2) * (s.H - 2) * (s.D - 2), T, U, V, W, sc, s.dx, s.tmp, s.W,
-               s.H, s.D);
-        LAUNCH(k_vort_force<DivDx>, (size_t)(s.W - 4) * (s.H - 4) * (s.D - 4), T, U, V, W, sc, s.dx, s.tmp, s.W,
-               s.H, s.D, s.s.vorticity_conf, dt, eps);
+        LAUNCH(k_vort_centres<DivDx>, n, T, U, V, W, sc, s.dx, s.tmp, s.W, s.H, s.D);
     }
+    LAUNCH(k_vort_omega, (size_t)(s.W - 2) * (s.H - 2) * (s.D - 2), s.dx, s.tmp, s.W, s.H, s.D);
+    LAUNCH(k_vort_force, (size_t)(s.W - 4) * (s.H - 4) * (s.D - 4), s.dx, s.tmp, s.W, s.H, s.D,
+           s.s.vorticity_conf, dt, eps);
     return gather3(s, true);
 }
 
