@@ -96,14 +96,15 @@ RBGS_SHAPES = [(32, 24, 20), (64, 8, 40), (16, 10, 21), (32, 16, 64), (130, 6, 9
                (60, 7, 6)]
 
 
-@pytest.mark.parametrize("variant", [2, 4, 8, 102, 103, 1103, 1106, 1204])
+@pytest.mark.parametrize("variant", [2, 4, 8, 102, 103, 1103, 1106, 1204, 1404])
 @pytest.mark.parametrize("res", RBGS_SHAPES, ids=["x".join(map(str, r)) for r in RBGS_SHAPES])
 @pytest.mark.parametrize("iters", [9, 8])
 def test_rbgs_variants_bitwise(variant, res, iters):
     """Every RBGS kernel variant gives the oracle's sweeps bit for bit: the
     two-pass in-place kernels (2/4/8 slab groups in flight), the fused
     ping-pong sweeps (102/103; an odd count ends with a two-pass sweep) and
-    the row-pair fused sweeps (1103/1106/1204). Widths 130, 128 and 66 put the
+    the row-pair fused sweeps (1103/1106/1204/1404: 3, 4 and 6 groups in
+    flight, 1, 2 and 4 waves per block). Widths 130, 128 and 66 put the
     fused kernel's row over several waves, with a partly filled last wave;
     odd heights (5, 7 rows) leave the row-pair kernel a pair with one row,
     and width 60 fills its 30-cell segments exactly."""
