@@ -118,17 +118,27 @@ def test_rbgs_variants_bitwise(variant, res, iters):
     compare(f"rbgs v{variant}", g.field(B.FIELD_P), o.field(B.FIELD_P))
 
 
-@pytest.mark.parametrize("variant", [8, 103, 1104])
-@pytest.mark.parametrize("kind", ["tiny", "negzero"])
+@pytest.mark.parametrize("variant", [8, 103, 1104, 1204])
+@pytest.mark.parametrize("kind", ["tiny", "negzero", "upper"])
 def test_rbgs_exact_division_fallback(variant, kind):
     """Numerators outside the fast division's exact range (|num| < 2^-125,
-    and -0 for the fused kernel) take the exact path: still bit for bit."""
+    and -0 for the fused kernels) take the exact path: still bit for bit.
+    "upper": small normal values below slab 20 and subnormal ones above, so
+    the column's chain decays into the exact range part-way up and the
+    waves switch to the exact run mid-column (loads in flight, carried k-1
+    values and the restart group all exercised)."""
     print(f"[rbgs-exact] v{variant} {kind}", flush=True)
-    s = settings(B, (64, 12, 10), max_iter_solve=6)
+    s = settings(B, (64, 12, 40 if kind == "upper" else 10), max_iter_solve=6)
     st = random_state(s, seed=11)
     if kind == "tiny":
         st[B.FIELD_P] = (st[B.FIELD_P] * np.float32(1e-38)).astype(np.float32)
         st[B.FIELD_RHS] = (st[B.FIELD_RHS] * np.float32(3e-38)).astype(np.float32)
+    elif kind == "upper":
+        for f in (B.FIELD_P, B.FIELD_RHS):
+            a = st[f].astype(np.float64)
+            a[:20] *= 1e-30
+            a[20:] *= 1e-38
+            st[f] = a.astype(np.float32)
     else:
         st[B.FIELD_P] = np.full_like(st[B.FIELD_P], -0.0)
         st[B.FIELD_RHS] = np.full_like(st[B.FIELD_RHS], -0.0)
