@@ -110,17 +110,22 @@ def test_single_grid_autotune_keeps_results():
     compare(f"autotuned v{v}", g.field(B.FIELD_P), o.field(B.FIELD_P))
 
 
-def test_batch_with_warm_start_and_handle_steps():
+@pytest.mark.parametrize("variant", [None, 1204])
+def test_batch_with_warm_start_and_handle_steps(variant):
     """With the learned warm start inside the captured graph, each scene still
     equals the same scene stepped alone with the same network; afterwards a
     scene handle stepped on its own (outside the graph, wind noise from its
-    time argument) stays equal to the single solver."""
-    print("[batch] warm start", flush=True)
+    time argument) stays equal to the single solver. variant: the batch's
+    sweep variant (None: the default one-row fused sweep; 1204: row-pair,
+    also used by each scene's warm-start judge)."""
+    print(f"[batch] warm start variant {variant}", flush=True)
     net = B.Net(seed=4)
     cases = [scene(r, 10 + i) for i, r in enumerate(SHAPES[:3])]
     for s, _ in cases:
         s.warm_start = 20
     b = B.Batch([c[0] for c in cases])
+    if variant is not None:
+        b.set_variant(variant)
     b.set_warm_start(net)
     singles = [B.Solver(c[0]) for c in cases]
     for i, (s, part) in enumerate(cases):
