@@ -346,16 +346,19 @@ int Batch::step(float time) {
     return BLZ_OK;
 }
 
-// time `sweeps` sweeps of every candidate on the batch (pressure restored
-// afterwards), keep the fastest
+// time `sweeps` sweeps of every candidate on the batch (pressure set to 1.0,
+// restored afterwards), keep the fastest
 int Batch::autotune(int sweeps, int* best) {
     const int n = (int)sc.size();
     sweeps = std::max(2, sweeps & ~1);
     for (int q = 0; q < n; ++q) BLZ_HIP(hipStreamSynchronize(sc[q]->s.stream));
+    // timing state as for a single grid (Solver::autotune): pressure 1.0,
+    // so no numerator takes the exact division
     for (int q = 0; q < n; ++q) {
         Solver& s = sc[q]->s;
         BLZ_HIP(hipMemcpyAsync(s.tmp, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, root));
         BLZ_HIP(hipMemsetAsync(s.ctl, 0, sizeof(SolveCtl), root));
+        if (int e = launch_fill_pressure(s, 1.0f, root)) return e;
     }
     hipEvent_t e0, e1;
     BLZ_HIP(hipEventCreate(&e0));

@@ -1244,10 +1244,10 @@ static inline int2 cb_b(const Solver& s) {
 
 // the home pressure planes (ghosts included) set to v: the autotuner's
 // timing state
-int launch_fill_pressure(Solver& s, float v) {
+int launch_fill_pressure(Solver& s, float v, hipStream_t st) {
     const size_t n = 2 * s.Np;
     const unsigned nb = (unsigned)std::min<size_t>(blocks_for(n, BS), 8192);
-    hipLaunchKernelGGL(k_fill_floats, dim3(nb), dim3(BS), 0, s.stream, s.prb, n, v);
+    hipLaunchKernelGGL(k_fill_floats, dim3(nb), dim3(BS), 0, st ? st : s.stream, s.prb, n, v);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }

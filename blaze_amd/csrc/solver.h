@@ -238,7 +238,7 @@ int launch_clear_doubles(Solver& s, double* dev, int n);
 int launch_p_to_natural(Solver& s, float* out, const RB& src);
 int launch_natural_to_rb(Solver& s, RB& dst, const float* in);
 int launch_init_pressure(Solver& s);
-int launch_fill_pressure(Solver& s, float v);
+int launch_fill_pressure(Solver& s, float v, hipStream_t st = nullptr); // on st, else the solver's stream
 int selftest_division(hipStream_t st, int d, unsigned long long* dev_bad);
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
