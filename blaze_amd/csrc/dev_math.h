@@ -61,21 +61,49 @@ inline bool dx_pow2(float dx, float* inv) {
     return std::isfinite(*inv) && *inv * dx == 1.0f;
 }
 
+// Gathers of the samplers below. A field of fewer than 2^29 floats is read
+// with buffer loads: a 32-bit byte offset against a buffer resource built
+// from the field's (uniform) base, so the 64-bit address arithmetic of a
+// pointer gather disappears; the values read are the same. Larger fields
+// take the pointer path (a wave-uniform branch).
+struct GatherB {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ float operator()(int idx) const {
+        // the builtin returns the 32 bits as an integer: reinterpret them
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+    }
+};
+struct GatherP {
+    const float* a;
+    __device__ __forceinline__ float operator()(int idx) const { return a[idx]; }
+};
+__device__ __forceinline__ bool gather_buffered(const Gv& g) {
+    return (long long)g.nx * g.ny * g.nz < (1ll << 29);
+}
+__device__ __forceinline__ GatherB gather_b(const Gv& g) {
+    return GatherB{__builtin_amdgcn_make_buffer_rsrc((void*)g.a, (short)0, g.nx * g.ny * g.nz * 4, 0x00020000)};
+}
+
 // trilinear_interpolate, src/Grid.h:94-108
-template <class S>
-__device__ __forceinline__ float trilinear_s(const Gv& g, const S& sc, f3 p) {
+template <class S, class L>
+__device__ __forceinline__ float trilinear_g(const Gv& g, const L& ldg, const S& sc, f3 p) {
     const float gX = fmaxf(fminf(sc(p.x) - g.ox, (float)g.nx - 1.001f), 0.0f);
     const float gY = fmaxf(fminf(sc(p.y) - g.oy, (float)g.ny - 1.001f), 0.0f);
     const float gZ = fmaxf(fminf(sc(p.z) - g.oz, (float)g.nz - 1.001f), 0.0f);
     const int i = (int)gX, j = (int)gY, k = (int)gZ;
     const float tx = gX - (float)i, ty = gY - (float)j, tz = gZ - (float)k;
     const int sy = g.nx, sz = g.nx * g.ny;
-    const float* b = g.a + i + j * sy + k * sz;
-    const float c00 = b[0] * (1.0f - tx) + b[1] * tx;
-    const float c10 = b[sy] * (1.0f - tx) + b[sy + 1] * tx;
-    const float c01 = b[sz] * (1.0f - tx) + b[sz + 1] * tx;
-    const float c11 = b[sy + sz] * (1.0f - tx) + b[sy + sz + 1] * tx;
+    const int b = i + j * sy + k * sz;
+    const float c00 = ldg(b) * (1.0f - tx) + ldg(b + 1) * tx;
+    const float c10 = ldg(b + sy) * (1.0f - tx) + ldg(b + sy + 1) * tx;
+    const float c01 = ldg(b + sz) * (1.0f - tx) + ldg(b + sz + 1) * tx;
+    const float c11 = ldg(b + sy + sz) * (1.0f - tx) + ldg(b + sy + sz + 1) * tx;
     return (1.0f - tz) * ((1.0f - ty) * c00 + ty * c10) + tz * ((1.0f - ty) * c01 + ty * c11);
+}
+template <class S>
+__device__ __forceinline__ float trilinear_s(const Gv& g, const S& sc, f3 p) {
+    if (gather_buffered(g)) return trilinear_g(g, gather_b(g), sc, p);
+    return trilinear_g(g, GatherP{g.a}, sc, p);
 }
 __device__ __forceinline__ float trilinear(const Gv& g, float dx, f3 p) { return trilinear_s(g, DivDx{dx}, p); }
 
@@ -106,8 +134,8 @@ __device__ __forceinline__ float cw_apply(const CW& c, float q0, float q1, float
 
 // tricubic_interpolate, src/Grid.h:165-222: x cerps, then y, then z; with
 // clamp, every 1-D stage is clamped at 0 (cerpc).
-template <bool CLAMP, class S>
-__device__ __forceinline__ float tricubic_s(const Gv& g, const S& sc, f3 p) {
+template <bool CLAMP, class S, class L>
+__device__ __forceinline__ float tricubic_g(const Gv& g, const L& ldg, const S& sc, f3 p) {
     const float gX = fmaxf(fminf(sc(p.x) - g.ox, (float)g.nx - 1.001f), 0.0f);
     const float gY = fmaxf(fminf(sc(p.y) - g.oy, (float)g.ny - 1.001f), 0.0f);
     const float gZ = fmaxf(fminf(sc(p.z) - g.oz, (float)g.nz - 1.001f), 0.0f);
@@ -123,8 +151,8 @@ __device__ __forceinline__ float tricubic_s(const Gv& g, const S& sc, f3 p) {
         float ry[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float* row = g.a + jy[b] + kz[c];
-            float v = cw_apply(cx, row[ix[0]], row[ix[1]], row[ix[2]], row[ix[3]]);
+            const int row = jy[b] + kz[c];
+            float v = cw_apply(cx, ldg(row + ix[0]), ldg(row + ix[1]), ldg(row + ix[2]), ldg(row + ix[3]));
             ry[b] = CLAMP ? fmaxf(v, 0.0f) : v;
         }
         float v = cw_apply(cy, ry[0], ry[1], ry[2], ry[3]);
@@ -132,6 +160,11 @@ __device__ __forceinline__ float tricubic_s(const Gv& g, const S& sc, f3 p) {
     }
     float v = cw_apply(cz, rz[0], rz[1], rz[2], rz[3]);
     return CLAMP ? fmaxf(v, 0.0f) : v;
+}
+template <bool CLAMP, class S>
+__device__ __forceinline__ float tricubic_s(const Gv& g, const S& sc, f3 p) {
+    if (gather_buffered(g)) return tricubic_g<CLAMP>(g, gather_b(g), sc, p);
+    return tricubic_g<CLAMP>(g, GatherP{g.a}, sc, p);
 }
 template <bool CLAMP>
 __device__ __forceinline__ float tricubic(const Gv& g, float dx, f3 p) {
