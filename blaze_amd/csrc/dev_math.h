@@ -68,14 +68,14 @@ inline bool dx_pow2(float dx, float* inv) {
 // take the pointer path (a wave-uniform branch).
 struct GatherB {
     __amdgpu_buffer_rsrc_t r;
-    __device__ __forceinline__ float operator()(int idx) const {
+    __device__ __forceinline__ float operator()(long long idx) const {
         // the builtin returns the 32 bits as an integer: reinterpret them
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)idx * 4, 0, 0));
     }
 };
 struct GatherP {
     const float* a;
-    __device__ __forceinline__ float operator()(int idx) const { return a[idx]; }
+    __device__ __forceinline__ float operator()(long long idx) const { return a[idx]; }
 };
 __device__ __forceinline__ bool gather_buffered(const Gv& g) {
     return (long long)g.nx * g.ny * g.nz < (1ll << 29);
