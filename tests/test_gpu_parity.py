@@ -118,6 +118,28 @@ def test_rbgs_variants_bitwise(variant, res, iters):
     compare(f"rbgs v{variant}", g.field(B.FIELD_P), o.field(B.FIELD_P))
 
 
+@pytest.mark.parametrize("variant", [0, 8, 103, 1104, 1105, 1204, 1404])
+def test_rbgs_large_bitwise(variant):
+    """128^3 (32 slab groups per column, 64 MB of planes read from HBM
+    rather than cache, so loads are in flight for long): a residual-driven
+    solve of 50 sweeps with five checks equals the oracle bit for bit in
+    pressure, sweep count and residuals. Variant 0: the autotuner's choice."""
+    print(f"[rbgs-large] v{variant} 128^3", flush=True)
+    s = settings(B, (128, 128, 128), closed_bounds=(1, 1, 1, 0, 1, 1), max_iter_solve=50, residual_tol=1e-30,
+                 residual_check_every=10)
+    st = random_state(s, seed=variant + 5)
+    g, o = pair(s, st)
+    if variant:
+        g.set_rbgs_variant(variant)
+    g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    gs, os_ = g.stats(), o.stats()
+    assert gs.rbgs_iters_last == os_["rbgs_iters_last"] == 50
+    assert gs.residual_initial == os_["residual_initial"]
+    assert gs.residual == os_["residual"]
+    compare(f"rbgs-large v{variant}", g.field(B.FIELD_P), o.field(B.FIELD_P))
+
+
 @pytest.mark.parametrize("variant", [8, 103, 1104, 1204])
 @pytest.mark.parametrize("kind", ["tiny", "negzero", "upper"])
 def test_rbgs_exact_division_fallback(variant, kind):
