@@ -118,12 +118,14 @@ def test_rbgs_variants_bitwise(variant, res, iters):
     compare(f"rbgs v{variant}", g.field(B.FIELD_P), o.field(B.FIELD_P))
 
 
-@pytest.mark.parametrize("variant", [0, 8, 103, 1104, 1105, 1204, 1404])
+@pytest.mark.parametrize("variant", [0, 2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1105, 1106, 1203, 1204,
+                                     1403, 1404])
 def test_rbgs_large_bitwise(variant):
     """128^3 (32 slab groups per column, 64 MB of planes read from HBM
     rather than cache, so loads are in flight for long): a residual-driven
     solve of 50 sweeps with five checks equals the oracle bit for bit in
-    pressure, sweep count and residuals. Variant 0: the autotuner's choice."""
+    pressure, sweep count and residuals, for every variant the autotuner can
+    pick; variant 0: its choice for this grid."""
     print(f"[rbgs-large] v{variant} 128^3", flush=True)
     s = settings(B, (128, 128, 128), closed_bounds=(1, 1, 1, 0, 1, 1), max_iter_solve=50, residual_tol=1e-30,
                  residual_check_every=10)
