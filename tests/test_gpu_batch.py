@@ -76,6 +76,36 @@ def test_batch_equals_single_scenes(tune):
     assert st.graph_nodes > 0
 
 
+@pytest.mark.parametrize("variant", [None, 203, 1104, 1204])
+def test_batch_large_scenes_equal_single_scenes(variant):
+    """Grids of 128^3 and 96x80x112 (loads from HBM, 28-32 slab groups per
+    column) through the multi-grid sweep and residual kernels: every scene
+    equals the same scene stepped alone bit for bit. variant None: the
+    default multi-grid sweep."""
+    print(f"[batch-large] variant {variant}", flush=True)
+    cases = [scene(r, 20 + i) for i, r in enumerate([(128, 128, 128), (96, 80, 112)])]
+    b = B.Batch([c[0] for c in cases])
+    singles = [B.Solver(c[0]) for c in cases]
+    for i, (s, part) in enumerate(cases):
+        b.scenes[i].add_source(part)
+        singles[i].add_source(part)
+    if variant is not None:
+        b.set_variant(variant)
+    t = np.float32(0.0)
+    for step in range(3):
+        b.step(float(t))
+        for g in singles:
+            g.step(float(t))
+        t = np.float32(t + np.float32(cases[0][0].dt))
+        for i in range(len(cases)):
+            gs, ss = b.scenes[i].stats(), singles[i].stats()
+            assert gs.rbgs_iters == ss.rbgs_iters, (step, i)
+            assert gs.residual == ss.residual, (step, i)
+    for i in range(len(cases)):
+        for w in (B.FIELD_T, B.FIELD_U, B.FIELD_V, B.FIELD_W, B.FIELD_P):
+            compare(f"batch-large scene {i} field {w}", b.scenes[i].field(w), singles[i].field(w))
+
+
 def test_batch_rejects_unknown_variant():
     s, _ = scene((32, 24, 16), 0)
     b = B.Batch([s])
