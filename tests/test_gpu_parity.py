@@ -306,6 +306,18 @@ def test_frames_explosion_tol_bitwise():
     run_frames(s, gen, 6, 1, check_every=2)
 
 
+def test_frames_explosion256_full_size_bitwise():
+    """The benchmark's scene at its full size: 256^3, 1M particles, cooling,
+    vorticity, buoyancy, the autotuned sweep kernel with residual checks, two
+    frames of one substep each (CFL substepping off and 20 sweeps per solve
+    keep the oracle to seconds; the first solve starts from p = 0, so its
+    early groups take the exact division). Every field bit-exact."""
+    s, gen = scene_settings("explosion256.blz", max_iter_solve=20, residual_tol=1e-4, cfl_number=0.0)
+    assert tuple(s.grid_res) == (256, 256, 256) and gen.count == 1000000
+    print("[frames] explosion256 at full size, 2 frames", flush=True)
+    run_frames(s, gen, 2, 1, check_every=1, fields=(0, 1, 2, 3, 5))
+
+
 def test_determinism_repeat():
     """Two GPU runs of the same frames are bitwise identical."""
     s, gen = scene_settings("plume128.blz", res=(32, 32, 32))
