@@ -105,8 +105,15 @@ class ParticleGen(C.Structure):
     ]
 
 
+E_INVALID, E_HIP, E_NOMEM, E_IO, E_PARSE, E_NUMERIC = -1, -2, -3, -4, -5, -6
+
+
 class BlazeError(RuntimeError):
-    pass
+    """A failed C-ABI call; ``code`` is its BLZ_E_* status."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 def _load():
@@ -182,7 +189,7 @@ _lib, ABI_SYMBOLS = _load()
 def _check(rc: int, what: str):
     if rc != 0:
         msg = _lib.blz_last_error().decode(errors="replace")
-        raise BlazeError(f"{what} failed ({rc}): {msg}")
+        raise BlazeError(f"{what} failed ({rc}): {msg}", rc)
 
 
 def default_settings() -> Settings:

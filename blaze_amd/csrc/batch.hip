@@ -354,6 +354,11 @@ int Batch::autotune(int sweeps, int* best) {
     for (int q = 0; q < n; ++q) BLZ_HIP(hipStreamSynchronize(sc[q]->s.stream));
     // timing state as for a single grid (Solver::autotune): pressure 1.0,
     // so no numerator takes the exact division
+    for (int q = 0; q < n; ++q)
+        if (sc[q]->s.tmp_floats < 2 * sc[q]->s.Np) {
+            set_error("batch autotune: scratch smaller than the pressure planes it saves");
+            return BLZ_E_INVALID;
+        }
     for (int q = 0; q < n; ++q) {
         Solver& s = sc[q]->s;
         BLZ_HIP(hipMemcpyAsync(s.tmp, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, root));

@@ -137,7 +137,10 @@ int Solver::init(const blz_settings& st_) {
     const bool fit6 = (6 * Np + tail) * 4 < ((size_t)1 << 32) && W <= 4096;
     prb_floats = fit8 ? 8 * Np + tail : (fit6 ? 6 * Np + tail : 4 * Np);
     judge_set = fit8 ? 6 : 4;
-    if (hipMalloc(&prb, prb_floats * sizeof(float)) != hipSuccess || hipMalloc(&tmp, 7 * N * sizeof(float)) != hipSuccess ||
+    // tmp also holds the two home pressure planes while an autotuner times
+    // its candidates; on small grids the padded planes outgrow 7 * N
+    tmp_floats = std::max(7 * N, 2 * Np);
+    if (hipMalloc(&prb, prb_floats * sizeof(float)) != hipSuccess || hipMalloc(&tmp, tmp_floats * sizeof(float)) != hipSuccess ||
         hipMalloc(&red, (8 + 2 * RESIDUAL_BLOCKS) * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc failed for solver scratch");
         return BLZ_E_NOMEM;
@@ -253,9 +256,21 @@ int Solver::finalize() {
             st.warm_start_fallbacks += ctl_host[q].ws_rejected;
         }
     }
+    // non-finite state (the residual kernels map NaN to +inf): the
+    // reference prints and exits on device errors (src/utils/helper_cuda.h:
+    // 576-591); here the step reports BLZ_E_NUMERIC and the caller decides
+    int bad = -1;
+    for (size_t q = 0; q < solves.size() && bad < 0; ++q)
+        if (solves[q] >= 0 && !(std::isfinite(ctl_host[q].res0) && std::isfinite(ctl_host[q].res))) bad = (int)q;
     const int e = resolve_spans();
     solves.clear();
-    return e;
+    if (e) return e;
+    if (bad >= 0) {
+        set_error("non-finite state: the pressure residual of solve " + std::to_string(bad) +
+                  " is not finite (NaN or infinity in the velocity, pressure or right-hand side)");
+        return BLZ_E_NUMERIC;
+    }
+    return BLZ_OK;
 }
 
 void clear_step_stats(blz_stats& st) {
@@ -275,6 +290,10 @@ int Solver::autotune(int sweeps, int* best, float* ms_per_sweep) {
     static const int cand[] = {2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1105, 1106, 1203, 1204, 1403, 1404};
     sweeps = std::max(2, sweeps & ~1);
     const float dt = s.dt;
+    if (tmp_floats < 2 * Np) {
+        set_error("autotune: scratch smaller than the pressure planes it saves");
+        return BLZ_E_INVALID;
+    }
     BLZ_HIP(hipStreamSynchronize(stream));
     BLZ_HIP(hipMemcpyAsync(tmp, prb, 2 * Np * sizeof(float), hipMemcpyDeviceToDevice, stream));
     // timing state: pressure 1.0 everywhere (the solver's rhs), so no
@@ -437,6 +456,10 @@ int Solver::step_enqueue(float time, bool sync) {
     if (s.cfl_number > 0.0f) {
         float umax = 0.0f;
         if (int e = max_abs_velocity(&umax)) return e;
+        if (!std::isfinite(umax)) {
+            set_error("non-finite state: max |velocity| is not finite (NaN or infinity in U, V or W)");
+            return BLZ_E_NUMERIC;
+        }
         const float q = std::ceil(umax * dtf / (s.cfl_number * dx));
         n = !(q >= 1.0f) ? 1 : (q > (float)s.max_substeps ? s.max_substeps : (int)q);
     }

@@ -1083,7 +1083,9 @@ __device__ __forceinline__ void res_final_block(const double* __restrict__ parti
         c->res_prev = first ? -1.0 : c->res;
         c->res = r;
         c->checks += 1;
-        if (!(r > tol) || c->iters >= cap) {
+        // a non-finite state (residual +inf) ends the solve at once: the
+        // host reports it after the step instead of sweeping to the cap
+        if (!(r > tol) || c->iters >= cap || r == __longlong_as_double(0x7ff0000000000000LL)) {
             c->done = 1;
             __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __threadfence_system();

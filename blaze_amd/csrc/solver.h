@@ -123,7 +123,9 @@ struct Solver {
                             // planes exist (the judge's sweeps may then be fused), else 4
     RBL L{};
     size_t Np = 0;          // floats per padded plane
-    float* tmp = nullptr;     // 7 * W*H*D scratch for cell-centred intermediates
+    float* tmp = nullptr;     // scratch for cell-centred intermediates (7 * W*H*D floats) and the
+                              // autotuners' saved pressure planes (2 * Np floats): max of the two
+    size_t tmp_floats = 0;
     double* red = nullptr;    // reduction slots (device): 8 results + RESIDUAL_BLOCKS partials
     double* red_host = nullptr; // pinned mirror
     SolveCtl* ctl = nullptr;      // MAX_SOLVES control blocks (device)

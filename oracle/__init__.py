@@ -73,6 +73,8 @@ def _lib():
         "orc_field_dims": ([vp, i, P(i)], i),
         "orc_copy_field": ([vp, i, i, P(d)], i),
         "orc_set_field": ([vp, i, P(d)], i),
+        "orc_set_field_back": ([vp, i, P(d)], i),
+        "orc_curl_noise": ([vp, f], i),
         "orc_get_stats": ([vp, P(i), P(i), P(i), P(d), P(d)], i),
         "orc_residual": ([vp, f], d),
         "orc_cerp_f32": ([f, f, f, f, f], f),
@@ -163,6 +165,17 @@ class Oracle:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(nz, ny, nx)
         if lib.orc_set_field(self._h, which, a.ctypes.data_as(C.POINTER(C.c_double))):
             raise ValueError(f"field {which} unavailable")
+
+    def set_field_back(self, which: int, arr):
+        nx, ny, nz = self.dims(which)
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(nz, ny, nx)
+        if lib.orc_set_field_back(self._h, which, a.ctypes.data_as(C.POINTER(C.c_double))):
+            raise ValueError(f"field {which} unavailable")
+
+    def curl_noise(self, dt: float):
+        """The curl-noise half of the turbulence stage on the current back
+        buffers (pin tests)."""
+        lib.orc_curl_noise(self._h, C.c_float(dt))
 
     def stats(self):
         sub, tot, last = C.c_int(), C.c_int(), C.c_int()

@@ -532,6 +532,12 @@ template <class R> struct Solver {
 
     /* compute_turbulence_kernel + add_curl_noise_kernel :271-335 */
     void turbulence(R time, R dt) {
+        turb_noise(time);
+        curl_noise(dt);
+    }
+
+    /* compute_turbulence_kernel :295-335 for the three face grids */
+    void turb_noise(R time) {
         const R freq = R(1) / (R)s.turbulence_scale;
         const R avgGrid = R(W + H + D) / R(3);
         const R offset = std::fmod(time * avgGrid, ORC_K(R, 25000.0));
@@ -539,6 +545,11 @@ template <class R> struct Solver {
         turb_face(U, 0, offset, freq, amp);
         turb_face(V, 1, offset, freq, amp);
         turb_face(Wg, 2, offset, freq, amp);
+    }
+
+    /* add_curl_noise_kernel :271-293: pseudo-curl of the back buffers (with
+       the reference's V-for-U/W reads kept), added to the front faces */
+    void curl_noise(R dt) {
         const size_t n = (size_t)W * H * D;
         std::vector<R> cx(n), cy(n), cz(n);
         std::vector<unsigned char> act(n, 1);

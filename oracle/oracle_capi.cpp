@@ -130,6 +130,30 @@ int orc_set_field(void* p, int which, const double* in) {
     return 0;
 }
 
+/* set a back buffer (for pinning stages that read them) */
+int orc_set_field_back(void* p, int which, const double* in) {
+    Handle* h = (Handle*)p;
+    if (which == BLZ_FIELD_P || which == BLZ_FIELD_RHS) return BLZ_E_INVALID;
+    if (h->fp64) {
+        auto* v = raw_field(h->d, which, true);
+        if (!v) return BLZ_E_INVALID;
+        std::memcpy(v->data(), in, v->size() * sizeof(double));
+    } else {
+        auto* v = raw_field(h->f, which, true);
+        if (!v) return BLZ_E_INVALID;
+        for (size_t i = 0; i < v->size(); ++i) (*v)[i] = (float)in[i];
+    }
+    return 0;
+}
+
+/* the curl part of the turbulence stage alone (add_curl_noise_kernel,
+   src/FluidSolver.cu:271-293), reading the back buffers as they are */
+int orc_curl_noise(void* p, float dt) {
+    Handle* h = (Handle*)p;
+    if (h->fp64) h->d.curl_noise((double)dt); else h->f.curl_noise(dt);
+    return 0;
+}
+
 int orc_get_stats(void* p, int* substeps, int* iters_total, int* iters_last, double* res, double* res0) {
     Handle* h = (Handle*)p;
     if (h->fp64) {

@@ -76,6 +76,62 @@ def test_parse_committed_scenes(name):
     assert gen is not None and gen.count == raw["particleGen"]["count"]
 
 
+REF_SCENES = "/root/reference/scenes"
+REF_SCENE_NAMES = ["explosion", "fireDebris", "groundFire", "rocketLaunch", "smokePlume", "tornado"]
+
+
+@pytest.mark.parametrize("name", REF_SCENE_NAMES)
+def test_parse_reference_scenes(name):
+    """The reference's own six scene files (its only data) parse through
+    Parser::sceneParse's schema (src/Parser.h:41-172): every solver, source
+    and render key equals an independent JSON reader's value, rounded to
+    float32 as the reference stores it. Skipped where the reference tree is
+    absent."""
+    path = os.path.join(REF_SCENES, name + ".blz")
+    if not os.path.exists(path):
+        pytest.skip("reference scenes not present")
+    raw = json.load(open(path))
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    s, d, pf, gen = B.parse_scene(path)
+    assert gen is None and pf == raw["particleFile"]
+    assert os.path.realpath(d) == os.path.realpath(REF_SCENES)
+    assert list(s.grid_res) == raw["gridRes"]
+    assert list(s.closed_bounds) == raw["closedBounds"]
+    assert s.max_iter_solve == raw["maxIterSolve"]
+    assert s.wind_dir == raw["windDir"]
+    assert s.source_animated == raw["sourceAnimated"]
+    assert s.source_max_particle_count == raw["sourceMaxParticleCount"]
+    # ranges are read as floats and truncated (src/Parser.h:58-64)
+    assert list(s.source_range) == [int(raw["sourceRange"][0]), int(raw["sourceRange"][1])]
+    scalars = {"dx": "dx", "dt": "dt", "density": "density", "gravity": "gravity", "buoyancy": "buoyancy",
+               "cooling_rate": "coolingRate", "vorticity_conf": "vorticityConf", "drag": "drag",
+               "wind_amp": "windAmp", "wind_speed": "windSpeed", "wind_turb_amp": "windTurbAmp",
+               "wind_turb_scale": "windTurbScale", "turbulence_amp": "turbulenceAmp",
+               "turbulence_scale": "turbulenceScale"}
+    for field, key in scalars.items():
+        assert getattr(s, field) == f32(raw[key]), key
+    for field, key in (("domain_min", "domainBboxMin"), ("domain_max", "domainBboxMax"),
+                       ("turb_mask_temp_ramp", "turbMaskTempRamp"), ("turb_mask_vel_ramp", "turbMaskVelRamp")):
+        assert list(getattr(s, field)) == [f32(v) for v in raw[key]], key
+    # extension keys are absent: reference behaviour
+    assert s.noise_seed == 123 and s.residual_tol == 0.0 and s.cfl_number == 0.0 and s.smoke_density == 0.0
+    r = B.parse_render(path)
+    assert [r.width, r.height] == raw["renderRes"]
+    assert r.light_count == raw["lightCount"] == len(raw["lights"])
+    for q, L in enumerate(raw["lights"]):
+        assert list(r.light_dir[q]) == [f32(v) for v in L["light_dir"]]
+        assert list(r.light_color[q]) == [f32(v) for v in L["light_color"]]
+        assert r.light_intensity[q] == f32(L["light_intensity"])
+    ramp = raw["shaderColorRamp"]
+    assert r.ramp_size == ramp["count"]
+    assert list(r.ramp_keys)[: ramp["count"]] == [f32(v) for v in ramp["keys"]]
+    for field, key in (("p_step", "primarySampleRate"), ("s_step", "shadowSamplingRate"), ("cutoff", "cutoff"),
+                       ("density_scale", "densityScale"), ("absorption", "absorption"), ("gain", "gain"),
+                       ("emission_scale", "emissionScale")):
+        assert getattr(r, field) == f32(raw[key]), key
+    assert r.ms_blur_iter == raw["multiScatterBlurIter"]
+
+
 def reference_style_scene(tmp_path, drop=None, extra=None):
     """A .blz with every key of the reference's smokePlume.blz schema
     (solver, source and render keys) but no extension keys."""

@@ -96,15 +96,20 @@ RBGS_SHAPES = [(32, 24, 20), (64, 8, 40), (16, 10, 21), (32, 16, 64), (130, 6, 9
                (60, 7, 6)]
 
 
-@pytest.mark.parametrize("variant", [2, 4, 8, 102, 103, 1103, 1106, 1204, 1404])
+# every variant the autotuner (Solver::autotune) can pick
+ALL_VARIANTS = [2, 4, 8, 102, 103, 202, 203, 402, 403, 1103, 1104, 1105, 1106, 1203, 1204, 1403, 1404]
+
+
+@pytest.mark.parametrize("variant", ALL_VARIANTS)
 @pytest.mark.parametrize("res", RBGS_SHAPES, ids=["x".join(map(str, r)) for r in RBGS_SHAPES])
 @pytest.mark.parametrize("iters", [9, 8])
 def test_rbgs_variants_bitwise(variant, res, iters):
-    """Every RBGS kernel variant gives the oracle's sweeps bit for bit: the
-    two-pass in-place kernels (2/4/8 slab groups in flight), the fused
-    ping-pong sweeps (102/103; an odd count ends with a two-pass sweep) and
-    the row-pair fused sweeps (1103/1106/1204/1404: 3, 4 and 6 groups in
-    flight, 1, 2 and 4 waves per block). Widths 130, 128 and 66 put the
+    """Every RBGS kernel variant the autotuner can pick gives the oracle's
+    sweeps bit for bit on every shape: the two-pass in-place kernels (2/4/8
+    slab groups in flight), the fused ping-pong sweeps (1x2/1x3 with 1, 2 or
+    4 waves per block; an odd count ends with a two-pass sweep) and the
+    row-pair fused sweeps (1x03-1x06: 3 to 6 groups in flight, 1, 2 and 4
+    waves per block). Widths 130, 128 and 66 put the
     fused kernel's row over several waves, with a partly filled last wave;
     odd heights (5, 7 rows) leave the row-pair kernel a pair with one row,
     and width 60 fills its 30-cell segments exactly."""
@@ -316,6 +321,30 @@ def test_frames_explosion256_full_size_bitwise():
     assert tuple(s.grid_res) == (256, 256, 256) and gen.count == 1000000
     print("[frames] explosion256 at full size, 2 frames", flush=True)
     run_frames(s, gen, 2, 1, check_every=1, fields=(0, 1, 2, 3, 5))
+
+
+@pytest.mark.parametrize("res", [(4, 4, 4), (6, 4, 4), (4, 5, 4), (6, 6, 5), (8, 4, 7)],
+                         ids=["4c", "6x4x4", "4x5x4", "6x6x5", "8x4x7"])
+def test_small_grid_autotuned_solve_bitwise(res):
+    """Grids whose padded pressure planes outgrow the cell-centred scratch
+    (2 * Np > 7 * W*H*D): the autotuner, which runs at the first solve and
+    saves the pressure planes while it times its candidates, restores them
+    intact, and the residual-driven solve equals the oracle bit for bit."""
+    print(f"[small-grid autotune] {res}", flush=True)
+    s = settings(B, res, closed_bounds=(1, 1, 1, 0, 1, 1), max_iter_solve=24, residual_tol=1e-30,
+                 residual_check_every=6)
+    st = random_state(s, seed=sum(res))
+    g, o = pair(s, st)
+    g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    assert g.stats().rbgs_iters_last == o.stats()["rbgs_iters_last"] == 24
+    compare(f"small {res}", g.field(B.FIELD_P), o.field(B.FIELD_P))
+    # a whole frame through the drop-in path as well
+    g2, o2 = pair(s, st)
+    g2.step(0.0)
+    o2.step(0.0)
+    for w in (0, 1, 2, 3, 5):
+        compare(f"small {res} frame field {w}", g2.field(w), o2.field(w))
 
 
 def test_determinism_repeat():
