@@ -71,6 +71,14 @@ class Stats(C.Structure):
     ]
 
 
+TRACE_CHECK, TRACE_KEPT, TRACE_REJECTED = 0, 1, 2
+
+
+class TraceEntry(C.Structure):
+    """blz_trace_entry: one FP64 residual evaluation of a solve."""
+    _fields_ = [("iters", C.c_int), ("kind", C.c_int), ("res", C.c_double)]
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("scenes", C.c_int), ("variant", C.c_int), ("graph_nodes", C.c_int), ("pad_", C.c_int),
@@ -145,6 +153,8 @@ def _load():
         "blz_get_stats": ([vp, P(Stats)], C.c_int),
         "blz_set_timing": ([vp, C.c_int], C.c_int),
         "blz_set_rbgs_variant": ([vp, C.c_int], C.c_int),
+        "blz_set_residual_trace": ([vp, C.c_int], C.c_int),
+        "blz_get_residual_trace": ([vp, C.c_int, P(TraceEntry), C.c_int, P(C.c_int)], C.c_int),
         "blz_selftest_division": ([C.c_int, P(C.c_ulonglong)], C.c_int),
         "blz_last_error": ([], C.c_char_p),
         "blz_net_create": ([C.c_ulonglong, P(vp)], C.c_int),
@@ -333,6 +343,19 @@ class Solver:
 
     def set_rbgs_variant(self, v: int):
         _check(_lib.blz_set_rbgs_variant(self._h, v), "blz_set_rbgs_variant")
+
+    def set_residual_trace(self, max_entries: int):
+        """Record up to max_entries residual evaluations per solve (0: off)."""
+        _check(_lib.blz_set_residual_trace(self._h, int(max_entries)), "blz_set_residual_trace")
+
+    def residual_trace(self, solve: int):
+        """History of solve `solve` of the last step: list of (sweeps, kind,
+        residual), kind TRACE_CHECK / TRACE_KEPT / TRACE_REJECTED."""
+        n = C.c_int()
+        _check(_lib.blz_get_residual_trace(self._h, solve, None, 0, C.byref(n)), "blz_get_residual_trace")
+        buf = (TraceEntry * max(1, n.value))()
+        _check(_lib.blz_get_residual_trace(self._h, solve, buf, n.value, C.byref(n)), "blz_get_residual_trace")
+        return [(buf[i].iters, buf[i].kind, buf[i].res) for i in range(n.value)]
 
     def autotune(self, sweeps: int = 20):
         """Time every RBGS variant on this grid and keep the fastest; returns

@@ -32,6 +32,7 @@ Solver::~Solver() {
     if (red_host) (void)hipHostFree(red_host);
     if (ctl) (void)hipFree(ctl);
     if (ctl_host) (void)hipHostFree(ctl_host);
+    if (trace) (void)hipFree(trace);
     if (ws_net) delete ws_net;
     if (ws_buf) (void)hipFree(ws_buf);
     if (rnd_buf) (void)hipFree(rnd_buf);
@@ -263,6 +264,7 @@ int Solver::finalize() {
     for (size_t q = 0; q < solves.size() && bad < 0; ++q)
         if (solves[q] >= 0 && !(std::isfinite(ctl_host[q].res0) && std::isfinite(ctl_host[q].res))) bad = (int)q;
     const int e = resolve_spans();
+    if (!solves.empty()) last_nsolves = (int)solves.size();
     solves.clear();
     if (e) return e;
     if (bad >= 0) {
@@ -737,6 +739,49 @@ int blz_get_stats(blz_solver* h, blz_stats* out) {
 int blz_set_timing(blz_solver* h, int on) {
     CHECK_HANDLE(h);
     h->s.timing = on != 0;
+    return BLZ_OK;
+}
+
+int blz_set_residual_trace(blz_solver* h, int max_entries) {
+    CHECK_HANDLE(h);
+    if (max_entries < 0 || max_entries > (1 << 20)) {
+        blz::set_error("blz_set_residual_trace: max_entries must be in [0, 2^20]");
+        return BLZ_E_INVALID;
+    }
+    Solver& s = h->s;
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    if (s.trace) BLZ_HIP(hipFree(s.trace));
+    s.trace = nullptr;
+    s.trace_cap = 0;
+    if (max_entries > 0) {
+        if (hipMalloc(&s.trace, (size_t)blz::MAX_SOLVES * max_entries * sizeof(blz_trace_entry)) != hipSuccess) {
+            s.trace = nullptr;
+            blz::set_error("hipMalloc failed for the residual history");
+            return BLZ_E_NOMEM;
+        }
+        s.trace_cap = max_entries;
+    }
+    return BLZ_OK;
+}
+
+int blz_get_residual_trace(blz_solver* h, int solve, blz_trace_entry* out, int max, int* n) {
+    CHECK_HANDLE(h);
+    Solver& s = h->s;
+    if (!n || max < 0 || (max > 0 && !out) || solve < 0 || solve >= s.last_nsolves) {
+        blz::set_error("blz_get_residual_trace: bad arguments");
+        return BLZ_E_INVALID;
+    }
+    if (!s.trace) {
+        blz::set_error("blz_get_residual_trace: recording is off (blz_set_residual_trace)");
+        return BLZ_E_INVALID;
+    }
+    // the control blocks of the last step were copied to the host with it
+    const int have = std::min(s.ctl_host[solve].ntrace, s.trace_cap);
+    *n = have;
+    const int m = std::min(have, max);
+    if (m > 0)
+        BLZ_HIP(hipMemcpy(out, s.trace + (size_t)solve * s.trace_cap, (size_t)m * sizeof(blz_trace_entry),
+                          hipMemcpyDeviceToHost));
     return BLZ_OK;
 }
 

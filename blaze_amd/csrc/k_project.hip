@@ -1083,6 +1083,7 @@ __device__ __forceinline__ void res_final_block(const double* __restrict__ parti
         c->res_prev = first ? -1.0 : c->res;
         c->res = r;
         c->checks += 1;
+        trace_push(c, BLZ_TRACE_CHECK, r);
         // a non-finite state (residual +inf) ends the solve at once: the
         // host reports it after the step instead of sweeping to the cap
         if (!(r > tol) || c->iters >= cap || r == __longlong_as_double(0x7ff0000000000000LL)) {
@@ -1104,7 +1105,10 @@ __global__ void __launch_bounds__(BS) k_res_final_multi(const ResScene* __restri
     res_final_block(e.partial, e.nblocks, e.ctl, e.host_done, e.seq, e.tol, e.cap, add, first);
 }
 
-__global__ void k_ctl_reset(SolveCtl* c) {
+__global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap) {
+    c->trace = trace;
+    c->trace_cap = trace_cap;
+    c->ntrace = 0;
     c->done = 0;
     c->iters = 0;
     c->checks = 0;
@@ -1468,7 +1472,10 @@ int launch_residual_multi(hipStream_t st, const ResScene* tab, int n, int blocks
 }
 
 int launch_ctl_reset(Solver& s, SolveCtl* c) {
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s.stream, c);
+    // the residual history slot of this solve (control block q of s.ctl)
+    const ptrdiff_t q = c - s.ctl;
+    blz_trace_entry* tr = (s.trace && q >= 0 && q < MAX_SOLVES) ? s.trace + (size_t)q * s.trace_cap : nullptr;
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s.stream, c, tr, tr ? s.trace_cap : 0);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }

@@ -594,6 +594,7 @@ __global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, 
             c->ws = 2;
             c->ws_rejected += 1;
         }
+        trace_push(c, c->ws == 1 ? BLZ_TRACE_KEPT : BLZ_TRACE_REJECTED, c->res);
         if (!(c->res > tol) || c->iters >= cap) {
             c->done = 1;
             __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -626,6 +627,7 @@ __global__ void __launch_bounds__(BS) k_ws_judge_pred(const double* __restrict__
             c->ws = 1;
             c->ws_kept += 1;
             c->skip_spare = 1;
+            trace_push(c, BLZ_TRACE_KEPT, ra);
             if (!(c->res > tol) || c->iters >= cap) {
                 c->done = 1;
                 __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -649,6 +651,7 @@ __global__ void __launch_bounds__(BS) k_ws_judge_fin(const double* __restrict__ 
         c->res = rb;
         c->iters += add;
         c->checks += 1;
+        trace_push(c, BLZ_TRACE_REJECTED, rb);
         if (!(c->res > tol) || c->iters >= cap) {
             c->done = 1;
             __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -866,6 +869,100 @@ __global__ void __launch_bounds__(GT) k_ws_gdct(const double* __restrict__ sg, c
     for (int g = threadIdx.x; g < n; g += GT) x_out[g] = A[g];
 }
 
+// Exact solve of the network grid's coarse problem, the upper bound of what
+// a perfect network prediction gains (an ablation; parts bit 2):
+// L x = res - mean(res) on the Wc x Hc x Dc coarse cells, L the Neumann
+// graph Laplacian of that grid (the Galerkin operator of piecewise-constant
+// blocks up to the factor F^2, which the energy-optimal scale absorbs),
+// diagonalised by the orthonormal DCT-II as in k_ws_gdct. One workgroup;
+// the grid (at most 32^3) lives in LDS with rows padded to Wc + 1 floats
+// (the x pass, whose lanes take consecutive rows, is then free of bank
+// conflicts); each thread transforms whole lines in registers, zero-padded
+// to 32 taps so the loops unroll.
+constexpr int CD = 32;
+__global__ void __launch_bounds__(GT) k_ws_cdct(const float* __restrict__ res, int Wc, int Hc, int Dc, int Wn, int Hn,
+                                                float* __restrict__ out, const SolveCtl* __restrict__ c) {
+    if (c->done) return;
+    __shared__ float A[(CD + 1) * CD * CD];
+    __shared__ float cs[3][CD][CD]; // cs[axis][a][i], zero outside the axis length
+    __shared__ float lam[3][CD];
+    __shared__ double red[GT / 64];
+    const int dims[3] = {Wc, Hc, Dc};
+    const int Wp = Wc + 1;
+    const int n = Wc * Hc * Dc;
+    for (int q = threadIdx.x; q < 3 * CD * CD; q += GT) {
+        const int ax = q / (CD * CD), a = (q / CD) % CD, i = q % CD;
+        const int m = dims[ax];
+        float v = 0.0f;
+        if (a < m && i < m) {
+            const double sa = a == 0 ? sqrt(1.0 / m) : sqrt(2.0 / m);
+            v = (float)(sa * cos(M_PI * a * (i + 0.5) / m));
+        }
+        cs[ax][a][i] = v;
+        if (i == 0) lam[ax][a] = a < m ? (float)(2.0 - 2.0 * cos(M_PI * a / m)) : 0.0f;
+    }
+    double sb = 0.0;
+    for (int g = threadIdx.x; g < n; g += GT) {
+        const int I = g % Wc, J = (g / Wc) % Hc, K = g / (Wc * Hc);
+        const float v = res[((size_t)K * Hn + J) * Wn + I];
+        A[I + Wp * (J + Hc * K)] = v;
+        sb += (double)v;
+    }
+    for (int o = 32; o >= 1; o >>= 1) sb += __shfl_xor(sb, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sb;
+    __syncthreads();
+    double tot = 0.0;
+    for (int q = 0; q < GT / 64; ++q) tot += red[q];
+    const float mean = (float)(tot / n);
+    for (int g = threadIdx.x; g < n; g += GT) {
+        const int I = g % Wc, J = (g / Wc) % Hc, K = g / (Wc * Hc);
+        A[I + Wp * (J + Hc * K)] -= mean;
+    }
+    __syncthreads();
+    // one 1-D transform of every line along axis ax, in place
+    auto pass = [&](int ax, bool inv) {
+        const int m = dims[ax];
+        const int nl = ax == 0 ? Hc * Dc : (ax == 1 ? Wc * Dc : Wc * Hc);
+        const int st = ax == 0 ? 1 : (ax == 1 ? Wp : Wp * Hc);
+        for (int l = threadIdx.x; l < nl; l += GT) {
+            int base;
+            if (ax == 0) base = Wp * l;                                   // l = J + Hc K
+            else if (ax == 1) base = (l % Wc) + Wp * Hc * (l / Wc);       // l = I + Wc K
+            else base = (l % Wc) + Wp * (l / Wc);                         // l = I + Wc J
+            float v[CD];
+#pragma unroll
+            for (int q = 0; q < CD; ++q) v[q] = q < m ? A[base + q * st] : 0.0f;
+            const float* cax = &cs[ax][0][0];
+            const int so = inv ? 1 : CD, sq = inv ? CD : 1; // cs[ax][o][q] forward, cs[ax][q][o] inverse
+#pragma unroll 1
+            for (int o = 0; o < m; ++o) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int q = 0; q < CD; ++q) acc += cax[o * so + q * sq] * v[q];
+                A[base + o * st] = acc;
+            }
+        }
+        __syncthreads();
+    };
+    pass(0, false);
+    pass(1, false);
+    pass(2, false);
+    for (int g = threadIdx.x; g < n; g += GT) {
+        const int a = g % Wc, b = (g / Wc) % Hc, cc = g / (Wc * Hc);
+        const float l = lam[0][a] + lam[1][b] + lam[2][cc];
+        float& v = A[a + Wp * (b + Hc * cc)];
+        v = l > 0.0f ? v / l : 0.0f;
+    }
+    __syncthreads();
+    pass(2, true);
+    pass(1, true);
+    pass(0, true);
+    for (int g = threadIdx.x; g < n; g += GT) {
+        const int I = g % Wc, J = (g / Wc) % Hc, K = g / (Wc * Hc);
+        out[((size_t)K * Hn + J) * Wn + I] = A[I + Wp * (J + Hc * K)];
+    }
+}
+
 __global__ void k_ws_zero(unsigned* a) { *a = 0u; }
 
 // a new application: no decision yet (the restore only follows this one's)
@@ -1001,8 +1098,14 @@ static int ws_restrict(Solver& s, float dt, const int* stop) {
 }
 
 int warm_start_set_parts(Solver& s, int parts) {
-    if (parts < 1 || parts > 3) {
-        set_error("warm-start parts must be 1 (network), 2 (global) or 3 (both)");
+    // bit 0 network, bit 1 global, bit 2 exact coarse solve in place of the
+    // network, bit 3 the network's correction before the global one
+    if (parts < 1 || parts > 15 || !(parts & 7)) {
+        set_error("warm-start parts: bit 0 network, bit 1 global, bit 2 exact coarse solve, bit 3 network first");
+        return BLZ_E_INVALID;
+    }
+    if ((parts & 4) && (s.W % s.ws_f || s.H % s.ws_f || s.D % s.ws_f || s.Wc > 32 || s.Hc > 32 || s.Dc > 32)) {
+        set_error("warm-start exact coarse solve needs whole blocks and at most 32 coarse cells per axis");
         return BLZ_E_INVALID;
     }
     s.ws_parts = parts;
@@ -1017,13 +1120,12 @@ int warm_start_set_parts(Solver& s, int parts) {
 // the smoothest error modes (the ones Gauss-Seidel reduces slowest, and the
 // ones a network's prediction is least reliable on), prolongated and scaled
 // the same way against that residual; both added in one pass.
-static int ws_correct(Solver& s, SolveCtl* c, const float* pred) {
+static int ws_correct(Solver& s, SolveCtl* c, const float* pred, bool glob) {
     const size_t N = ncell(s);
     const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
     double* a_net = ws_alpha(s);
     double* a_g = ws_alpha(s) + 1;
     const float* enet = pred ? ws_enet(s) : nullptr;
-    const bool glob = s.ws_parts & 2;
     if (pred) {
         launch_prolong(s, ws_enet(s), pred, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
         launch_blockdots(s, ws_r(s), nullptr, nullptr, ws_enet(s), ws_part(s), glob ? ws_sg(s) : nullptr, c);
@@ -1058,23 +1160,48 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
     float* mask = s.ws_buf + nn;
     unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
     hipLaunchKernelGGL(k_ws_begin, dim3(1), dim3(1), 0, s.stream, c);
-    const bool use_net = s.ws_parts & 1;
-    if (!use_net) {
-        const float scale = (s.s.density * s.dx * s.dx) / dt;
-        hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
-                           s.D, scale, ws_r(s), &c->done);
-    } else {
-        if (int e = ws_restrict(s, dt, &c->done)) return e;
+    const bool exact = s.ws_parts & 4;           // exact coarse solve in place of the network
+    const bool coarse = (s.ws_parts & 1) || exact;
+    const bool glob = s.ws_parts & 2;
+    const bool net_first = s.ws_parts & 8;       // the network's correction before the global one
+    // the network's coarse correction of the residual in ws_buf (restricted)
+    auto predict = [&]() -> int {
+        if (exact) {
+            hipLaunchKernelGGL(k_ws_cdct, dim3(1), dim3(GT), 0, s.stream, res, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn,
+                               s.ws_net->out, c);
+            BLZ_HIP(hipGetLastError());
+            return BLZ_OK;
+        }
         hipLaunchKernelGGL(k_ws_pack, dim3(nb(nn * NET_IN_CH)), dim3(BS), 0, s.stream, res, mask, amax,
                            s.ws_net->x0, (int)nn);
         BLZ_HIP(hipGetLastError());
-        if (int e = s.ws_net->forward()) return e;
-    }
+        return s.ws_net->forward();
+    };
     // the uncorrected pressure goes to the judge planes (the spare ones
     // where no judge planes exist), the corrected one stays in the home planes
     float* spare = s.prb + (size_t)s.judge_set * s.Np;
     BLZ_HIP(hipMemcpyAsync(spare, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
-    if (int e = ws_correct(s, c, use_net ? s.ws_net->out : nullptr)) return e;
+    if (coarse && glob && !net_first) {
+        // global first (it sets the smoothest modes), then the network's
+        // correction of the residual the global one leaves (the middle band)
+        const float scale = (s.s.density * s.dx * s.dx) / dt;
+        hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
+                           s.D, scale, ws_r(s), &c->done);
+        if (int e = ws_correct(s, c, nullptr, true)) return e;
+        if (int e = ws_restrict(s, dt, &c->done)) return e;
+        if (int e = predict()) return e;
+        if (int e = ws_correct(s, c, s.ws_net->out, false)) return e;
+    } else {
+        if (!coarse) {
+            const float scale = (s.s.density * s.dx * s.dx) / dt;
+            hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W,
+                               s.H, s.D, scale, ws_r(s), &c->done);
+        } else {
+            if (int e = ws_restrict(s, dt, &c->done)) return e;
+            if (int e = predict()) return e;
+        }
+        if (int e = ws_correct(s, c, coarse ? s.ws_net->out : nullptr, glob)) return e;
+    }
     // the judge: the same sweeps on both, the lower residual continues
     int launches = 0, kind = 0;
     const int keep = s.rbgs_variant;
@@ -1128,7 +1255,7 @@ int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* 
     const float scale = (s.s.density * s.dx * s.dx) / dt;
     hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
                        scale, ws_r(s), &c->done);
-    if (int e = ws_correct(s, c, (s.ws_parts & 1) ? s.ws_net->out : nullptr)) return e;
+    if (int e = ws_correct(s, c, (s.ws_parts & 1) ? s.ws_net->out : nullptr, s.ws_parts & 2)) return e;
     if (int e2 = launch_residual_ws(s, dt, c, s.done_host_dev, 0, -1.0, 1 << 30)) return e2;
     hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
                        backup, s.prb, 2 * s.Np);

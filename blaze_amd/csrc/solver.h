@@ -74,8 +74,22 @@ struct SolveCtl {
     int ws_kept, ws_rejected; // learned corrections kept / rejected in this solve
     double res_prev;          // residual of the check before the last one (-1: none)
     int skip_spare;           // 1: the predicted judge kept the correction (no uncorrected sweeps)
+    int ntrace;               // entries written to trace
+    blz_trace_entry* trace;   // residual history of this solve (nullptr: not recorded)
+    int trace_cap;            // capacity of trace
     int pad_;
 };
+
+// one residual evaluation of a solve, appended to its history (when enabled)
+__device__ __forceinline__ void trace_push(SolveCtl* c, int kind, double res) {
+    if (c->trace && c->ntrace < c->trace_cap) {
+        blz_trace_entry& t = c->trace[c->ntrace];
+        t.iters = c->iters;
+        t.kind = kind;
+        t.res = res;
+        c->ntrace += 1;
+    }
+}
 constexpr int WS_GMAX = 4096;      // cells of the warm start's global-correction grid (16^3)
 constexpr int MAX_SOLVES = 64;     // pressure solves per step (bounds max_substeps)
 constexpr int SOLVE_LOOKAHEAD = 2; // sweep batches the host may enqueue past the last one it has seen finish
@@ -130,6 +144,9 @@ struct Solver {
     double* red_host = nullptr; // pinned mirror
     SolveCtl* ctl = nullptr;      // MAX_SOLVES control blocks (device)
     SolveCtl* ctl_host = nullptr; // pinned copy, filled at the end of a step
+    blz_trace_entry* trace = nullptr; // residual histories: MAX_SOLVES x trace_cap entries (device)
+    int trace_cap = 0;                // 0: histories off
+    int last_nsolves = 0;             // solves of the last finalized step (their ctl_host entries are valid)
     int* zero_flag = nullptr;     // device int that stays 0: stop flag of fixed-count solves
     volatile int* done_host = nullptr; // mapped pinned int: sequence number of the last converged solve
     int* done_host_dev = nullptr;      // its device address

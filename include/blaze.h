@@ -203,6 +203,25 @@ int blz_set_rbgs_variant(blz_solver* h, int variant);
    exist). */
 int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep);
 
+/* Residual history of residual-driven solves (no reference counterpart; the
+   reference solve runs a fixed sweep count and evaluates no residual,
+   src/FluidSolver.cu:517-599). With max_entries > 0 every FP64 residual
+   evaluation of a solve is recorded (at most max_entries per solve, the
+   rest dropped); 0 turns recording off. blz_get_residual_trace copies the
+   history of solve `solve` (0-based, in enqueue order: one per substep) of
+   the last blz_step or blz_run_stage into out (at most max entries) and
+   sets *n to the number recorded. */
+#define BLZ_TRACE_CHECK 0     /* a residual check after a batch of sweeps */
+#define BLZ_TRACE_KEPT 1      /* a warm-start correction kept by the judge (residual after its batch) */
+#define BLZ_TRACE_REJECTED 2  /* a correction rejected (residual of the uncorrected path after the batch) */
+typedef struct blz_trace_entry {
+    int iters;                /* sweeps counted when evaluated */
+    int kind;                 /* BLZ_TRACE_* */
+    double res;               /* ||div u||_inf proxy (FP64) */
+} blz_trace_entry;
+int blz_set_residual_trace(blz_solver* h, int max_entries);
+int blz_get_residual_trace(blz_solver* h, int solve, blz_trace_entry* out, int max, int* n);
+
 /* ---- renderer: the solver's consumer (RenderEngine::render,
    src/RenderEngine.cu:312-365, render_kernel :149-225; Camera, Light,
    Shader, ColorRamp in src/Camera.h, src/Light.h, src/Shader.h,
@@ -281,8 +300,13 @@ int blz_net_train_step(blz_net* net, const float* in2, const float* target, cons
 int blz_set_warm_start(blz_solver* h, blz_net* net);
 /* Which corrections a warm start applies (no reference counterpart):
    bit 0 the network's, bit 1 the global one (an exact Galerkin solve on a
-   grid of at most 16^3 blocks of the residual the first leaves, which sets
-   the smoothest error modes); default 3. Both are judged together. */
+   grid of at most 16^3 blocks, which sets the smoothest error modes),
+   bit 2 an exact solve of the network grid's coarse problem in place of
+   the network (an ablation: what a perfect prediction would gain; needs
+   whole blocks and at most 32 coarse cells per axis), bit 3 the network's
+   correction before the global one (default: the global one first, then
+   the network's correction of the residual it leaves); default 3. The
+   corrections of one application are judged together. */
 int blz_set_warm_start_parts(blz_solver* h, int parts);
 /* The network input for the current pressure and rhs (for generating
    training data): in2 = 2 x net_dims values; dims reported in (W, H, D). */
