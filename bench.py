@@ -56,7 +56,11 @@ def parse():
     ap.add_argument("--warm-every", type=int, default=40,
                     help="warm_start setting of that leg (1: before the sweeps, N >= 2: also every N sweeps)")
     ap.add_argument("--warm-parts", type=int, default=3,
-                    help="warm-start corrections: 1 network, 2 global, 3 both")
+                    help="warm-start corrections of the headline-settings leg: 1 network, 2 global, 3 both")
+    ap.add_argument("--tol-steps", type=int, default=3,
+                    help="timed frames of the at-tolerance legs (plain, global, network + global, exact coarse "
+                         "+ global, all to the scene's tolerance with the sweep cap lifted)")
+    ap.add_argument("--tol-cap", type=int, default=20000, help="sweep cap of the at-tolerance legs")
     ap.add_argument("--no-render", action="store_true", help="skip the render of the plain leg's final state")
     ap.add_argument("--render-reps", type=int, default=3)
     ap.add_argument("--render-cpu-rows", type=int, default=8,
@@ -210,14 +214,81 @@ def main():
 
         if os.path.exists(WS.WEIGHTS):
             net = WS.load_net()
-            warm = {"scene_tol": run_leg(a, s, parts, net)}
-            # equal residual: both solvers run to the tolerance the plain
-            # solve reached (median over the timed frames; same cap)
-            teq = float(np.median(plain["residuals"]))
-            warm["tol_eq"] = teq
-            warm["plain_eq"] = run_leg(a, s, parts, None, tol=teq)
-            warm["warm_eq"] = run_leg(a, s, parts, net, tol=teq)
+            warm = {"scene_tol": run_leg(a, s, parts, net),
+                    "at_tol": {name: tolerance_leg(a, s, parts, net if bits else None, bits)
+                               for name, bits in TOL_LEGS}}
     report(a, s, gen, parts, plain, timed, warm)
+
+
+# the at-tolerance legs: name, warm-start parts (0: plain RBGS; bit 0
+# network, bit 1 global correction, bit 2 exact coarse solve in place of the
+# network -- the upper bound of what a perfect network would give)
+TOL_LEGS = (("plain", 0), ("global", 2), ("network+global", 3), ("exact_coarse+global", 6))
+
+
+def tolerance_leg(a, s, parts, net, bits):
+    """The solve to the scene's tolerance ||div u||_inf <= tol with the sweep
+    cap lifted (a.tol_cap): the same plain warmup frames as every other leg
+    (scene settings, no warm start, so all legs start the timed frames from
+    the same state bit for bit), then a.tol_steps timed frames with the
+    limits lifted and, when bits != 0, the warm start with those parts.
+    Returns voxel-steps/s, ms per frame, counted and executed sweeps (the
+    latter includes the judge's batches on the path not kept), kept and
+    rejected corrections, final residuals and the residual history of the
+    first timed solve (sweeps, residual, kind)."""
+    import numpy as np
+
+    import blaze_amd as B
+
+    W, H, D = s.grid_res
+    s2 = s.copy()
+    s2.warm_start = a.warm_every if bits else 0
+    g = B.Solver(s2)
+    g.set_residual_trace(4096)
+    t = np.float32(0.0)
+    frame = s.source_range[0]
+
+    def one_frame():
+        nonlocal t, frame
+        if frame <= s.source_range[1]:
+            g.add_source(parts if frame == s.source_range[0] else None)
+        g.step(float(t))
+        t = np.float32(t + np.float32(s.dt))
+        frame += 1
+        return g.stats()
+
+    for _ in range(a.warmup):
+        one_frame()
+    g.set_solve_limits(float(s.residual_tol), a.tol_cap, int(s.residual_check_every))
+    if bits:
+        g.set_warm_start(net, parts=bits)
+    g.sync()
+    rej0 = g.stats().warm_start_fallbacks
+    sub = iters = executed = kept = 0
+    res, curve = [], None
+    t0 = time.perf_counter()
+    for k in range(a.tol_steps):
+        st = one_frame()
+        sub += st.substeps
+        iters += st.rbgs_iters
+        executed += st.rbgs_sweeps_executed
+        kept += st.warm_start_used
+        res.append(st.residual)
+        if k == 0:
+            curve = g.residual_trace(0)
+        log(f"[tol {a.tol_cap} parts {bits}] frame {frame - 1}: substeps {st.substeps} sweeps {st.rbgs_iters} "
+            f"executed {st.rbgs_sweeps_executed} residual {st.residual:.3e} step {st.step_ms:.1f} ms", a.quiet)
+    g.sync()
+    el = time.perf_counter() - t0
+    rej = g.stats().warm_start_fallbacks - rej0
+    g.close()
+    step = max(1, len(curve) // 40)
+    return {"value": W * H * D * sub / el, "ms_per_step": el * 1e3 / a.tol_steps, "substeps": sub,
+            "rbgs_sweeps": iters, "rbgs_sweeps_executed": executed, "corrections_kept": kept,
+            "corrections_rejected": rej, "max_final_residual": max(res), "final_residuals": res,
+            "all_reached_tolerance": bool(max(res) <= float(s.residual_tol)),
+            "first_solve_residual_curve": [[e[0], e[2], "ckr"[e[1]]] for e in curve[::step]] + (
+                [[curve[-1][0], curve[-1][2], "ckr"[curve[-1][1]]]] if curve and (len(curve) - 1) % step else [])}
 
 
 def run_leg(a, s, parts, net, tol=None, timing=False, render=False):
@@ -240,7 +311,7 @@ def run_leg(a, s, parts, net, tol=None, timing=False, render=False):
     g.set_timing(timing)
     g.set_rbgs_variant(a.rbgs_variant)
     if net is not None:
-        g.set_warm_start(net, network=bool(a.warm_parts & 1), global_correction=bool(a.warm_parts & 2))
+        g.set_warm_start(net, parts=a.warm_parts)
     t = np.float32(0.0)
     frame = s.source_range[0]
     sr_end = s.source_range[1]
@@ -266,7 +337,7 @@ def run_leg(a, s, parts, net, tol=None, timing=False, render=False):
     kept = rejected0 = 0
     rejected0 = g.stats().warm_start_fallbacks
 
-    tot_sub = tot_iters = tot_checks = tot_launch = tot_timed_sweeps = 0
+    tot_sub = tot_iters = tot_checks = tot_launch = tot_timed_sweeps = tot_executed = 0
     kernel_kind = 0
     tot_rbgs_ms = 0.0
     stage_ms = [0.0] * B.NUM_STAGES
@@ -277,6 +348,7 @@ def run_leg(a, s, parts, net, tol=None, timing=False, render=False):
         st = one_frame()
         tot_sub += st.substeps
         tot_iters += st.rbgs_iters
+        tot_executed += st.rbgs_sweeps_executed
         tot_checks += st.residual_checks
         tot_launch += st.rbgs_launches
         tot_timed_sweeps += st.rbgs_sweeps_timed
@@ -291,7 +363,8 @@ def run_leg(a, s, parts, net, tol=None, timing=False, render=False):
     g.sync()
     elapsed = time.perf_counter() - t0
     rend = render_leg(a, g, s) if render else None
-    return dict(elapsed=elapsed, tot_sub=tot_sub, tot_iters=tot_iters, tot_checks=tot_checks, tot_launch=tot_launch,
+    return dict(elapsed=elapsed, tot_sub=tot_sub, tot_iters=tot_iters, tot_executed=tot_executed,
+                tot_checks=tot_checks, tot_launch=tot_launch,
                 tot_timed_sweeps=tot_timed_sweeps, tot_rbgs_ms=tot_rbgs_ms, kernel_kind=kernel_kind,
                 stage_ms=stage_ms, residuals=residuals, kept=kept,
                 rejected=g.stats().warm_start_fallbacks - rejected0, variant=g.stats().rbgs_kernel, render=rend)
@@ -430,24 +503,27 @@ def report(a, s, gen, parts, L, LT, warm):
     if warm is not None:
         def leg(L):
             return {"value": N3 * L["tot_sub"] / L["elapsed"], "ms_per_step": L["elapsed"] * 1e3 / a.steps,
-                    "rbgs_sweeps": L["tot_iters"], "substeps": L["tot_sub"],
+                    "rbgs_sweeps": L["tot_iters"], "rbgs_sweeps_executed": L["tot_executed"],
+                    "substeps": L["tot_sub"],
                     "mean_final_residual": sum(L["residuals"]) / max(1, len(L["residuals"])),
                     "corrections_kept": L["kept"], "corrections_rejected": L["rejected"]}
-        ws, pe, we = leg(warm["scene_tol"]), leg(warm["plain_eq"]), leg(warm["warm_eq"])
+        ws = leg(warm["scene_tol"])
+        T = warm["at_tol"]
+        pl = T["plain"]
         out["warm_start"] = {
-            "setting": f"warm_start={a.warm_every}, corrections: "
-                       + {1: "network", 2: "global", 3: "network + global"}[a.warm_parts]
-                       + ", weights blaze_amd/warm_start_weights.npy",
-            "at_scene_tolerance": dict(ws, ratio_vs_plain=ws["value"] / value,
-                                       plain_mean_final_residual=sum(residuals) / max(1, len(residuals))),
-            "at_equal_residual": {
-                "tolerance": warm["tol_eq"],
-                "note": "both solvers to the median final residual of the plain run (same cap)",
-                "plain": pe,
-                "warm": we,
-                "speedup": we["value"] / pe["value"],
-                "sweeps_ratio": pe["rbgs_sweeps"] / max(1, we["rbgs_sweeps"]),
+            "setting": f"warm_start={a.warm_every} (corrections before the sweeps and every {a.warm_every} sweeps), "
+                       "weights blaze_amd/warm_start_weights.npy",
+            "at_tolerance": {
+                "tolerance": float(s.residual_tol),
+                "cap": a.tol_cap,
+                "frames": f"{a.warmup + 1}..{a.warmup + a.tol_steps} (after {a.warmup} identical plain warmup frames)",
+                "legs": T,
+                "speedup_vs_plain": {k: v["value"] / pl["value"] for k, v in T.items() if k != "plain"},
+                "time_ratio_vs_plain": {k: pl["ms_per_step"] / v["ms_per_step"] for k, v in T.items() if k != "plain"},
+                "network_gain_over_global": T["global"]["ms_per_step"] / T["network+global"]["ms_per_step"],
             },
+            "at_headline_settings": dict(ws, parts="network + global", ratio_vs_plain=ws["value"] / value,
+                                         plain_mean_final_residual=sum(residuals) / max(1, len(residuals))),
         }
     print(json.dumps(out), flush=True)
 

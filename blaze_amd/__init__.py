@@ -67,7 +67,7 @@ class Stats(C.Structure):
         ("warm_start_used", C.c_int), ("warm_start_fallbacks", C.c_int),
         ("stage_ms", C.c_double * NUM_STAGES), ("source_ms", C.c_double), ("step_ms", C.c_double),
         ("rbgs_kernel_ms", C.c_double), ("rbgs_launches", C.c_int), ("residual_checks", C.c_int),
-        ("rbgs_sweeps_timed", C.c_int), ("rbgs_kernel", C.c_int),
+        ("rbgs_sweeps_timed", C.c_int), ("rbgs_kernel", C.c_int), ("rbgs_sweeps_executed", C.c_int),
     ]
 
 
@@ -154,6 +154,7 @@ def _load():
         "blz_set_timing": ([vp, C.c_int], C.c_int),
         "blz_set_rbgs_variant": ([vp, C.c_int], C.c_int),
         "blz_set_residual_trace": ([vp, C.c_int], C.c_int),
+        "blz_set_solve_limits": ([vp, C.c_float, C.c_int, C.c_int], C.c_int),
         "blz_get_residual_trace": ([vp, C.c_int, P(TraceEntry), C.c_int, P(C.c_int)], C.c_int),
         "blz_selftest_division": ([C.c_int, P(C.c_ulonglong)], C.c_int),
         "blz_last_error": ([], C.c_char_p),
@@ -344,6 +345,12 @@ class Solver:
     def set_rbgs_variant(self, v: int):
         _check(_lib.blz_set_rbgs_variant(self._h, v), "blz_set_rbgs_variant")
 
+    def set_solve_limits(self, residual_tol: float, max_iter_solve: int, residual_check_every: int = 10):
+        """Change the solve's tolerance, sweep cap and check interval for the
+        following steps."""
+        _check(_lib.blz_set_solve_limits(self._h, C.c_float(residual_tol), int(max_iter_solve),
+                                         int(residual_check_every)), "blz_set_solve_limits")
+
     def set_residual_trace(self, max_entries: int):
         """Record up to max_entries residual evaluations per solve (0: off)."""
         _check(_lib.blz_set_residual_trace(self._h, int(max_entries)), "blz_set_residual_trace")
@@ -384,15 +391,17 @@ class Solver:
         _check(_lib.blz_render(self._h, C.byref(settings), out.ctypes.data_as(C.POINTER(C.c_float))), "blz_render")
         return out
 
-    def set_warm_start(self, net, network: bool = True, global_correction: bool = True):
+    def set_warm_start(self, net, network: bool = True, global_correction: bool = True, parts: int = 0):
         """Attach a Net (None detaches); used when settings.warm_start != 0
         and residual_tol > 0. network / global_correction select the parts
-        of the correction (at least one)."""
+        of the correction (at least one); parts != 0 gives the bits of
+        blz_set_warm_start_parts directly (bit 2: exact coarse solve in place
+        of the network, bit 3: network before the global correction)."""
         self._net = net
         _check(_lib.blz_set_warm_start(self._h, net._h if net is not None else None), "blz_set_warm_start")
         if net is not None:
-            _check(_lib.blz_set_warm_start_parts(self._h, int(network) | (2 * int(global_correction))),
-                   "blz_set_warm_start_parts")
+            bits = parts or (int(network) | (2 * int(global_correction)))
+            _check(_lib.blz_set_warm_start_parts(self._h, bits), "blz_set_warm_start_parts")
 
     def warm_start_dims(self):
         nd, cd = (C.c_int * 3)(), (C.c_int * 3)()

@@ -242,7 +242,9 @@ int Solver::finalize() {
         double r = -1.0, r0 = -1.0;
         if (it < 0) {
             it = -it - 1;
+            st.rbgs_sweeps_executed += it;
         } else {
+            st.rbgs_sweeps_executed += ctl_host[q].swept;
             it = ctl_host[q].iters;
             r = ctl_host[q].res;
             r0 = ctl_host[q].res0;
@@ -282,6 +284,7 @@ void clear_step_stats(blz_stats& st) {
     st.rbgs_launches = 0;
     st.rbgs_sweeps_timed = 0;
     st.rbgs_iters = 0;
+    st.rbgs_sweeps_executed = 0;
     st.residual_checks = 0;
 }
 
@@ -739,6 +742,19 @@ int blz_get_stats(blz_solver* h, blz_stats* out) {
 int blz_set_timing(blz_solver* h, int on) {
     CHECK_HANDLE(h);
     h->s.timing = on != 0;
+    return BLZ_OK;
+}
+
+int blz_set_solve_limits(blz_solver* h, float residual_tol, int max_iter_solve, int residual_check_every) {
+    CHECK_HANDLE(h);
+    if (!(residual_tol >= 0.0f) || max_iter_solve < 0 || residual_check_every <= 0) {
+        blz::set_error("blz_set_solve_limits: need residual_tol >= 0, max_iter_solve >= 0, residual_check_every > 0");
+        return BLZ_E_INVALID;
+    }
+    BLZ_HIP(hipStreamSynchronize(h->s.stream));
+    h->s.s.residual_tol = residual_tol;
+    h->s.s.max_iter_solve = max_iter_solve;
+    h->s.s.residual_check_every = residual_check_every;
     return BLZ_OK;
 }
 

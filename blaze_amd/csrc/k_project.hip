@@ -1079,6 +1079,7 @@ __device__ __forceinline__ void res_final_block(const double* __restrict__ parti
             c->iters = 0;
         } else {
             c->iters += add;
+            c->swept += add;
         }
         c->res_prev = first ? -1.0 : c->res;
         c->res = r;
@@ -1109,6 +1110,7 @@ __global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap) 
     c->trace = trace;
     c->trace_cap = trace_cap;
     c->ntrace = 0;
+    c->swept = 0;
     c->done = 0;
     c->iters = 0;
     c->checks = 0;

@@ -500,7 +500,8 @@ __global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __re
 }
 
 // alpha from the per-block partial sums (one workgroup, fixed order: every
-// run gives the same alpha); 0 when undefined
+// run gives the same alpha); 0 when undefined. alpha[2] receives the mean of
+// the direction (two slots after alpha)
 __global__ void __launch_bounds__(BS) k_ws_alpha(const double* __restrict__ partial, int nb, double n,
                                                  double* __restrict__ alpha, const SolveCtl* __restrict__ c) {
     if (c->done) return;
@@ -519,11 +520,17 @@ __global__ void __launch_bounds__(BS) k_ws_alpha(const double* __restrict__ part
         for (int k = 0; k < 4; ++k)
             for (int w = 0; w < (int)(BS / 64); ++w) a[k] += sv[k][w];
         const double al = a[1] > 0.0 ? (a[0] - a[2] * a[3] / n) / a[1] : 0.0;
-        *alpha = (al == al && fabs(al) < 1e30) ? al : 0.0;
+        const bool ok = al == al && fabs(al) < 1e30;
+        *alpha = ok ? al : 0.0;
+        // the direction's mean: the correction added is alpha (e - mean e),
+        // since a constant changes no residual and would only raise |p| (and
+        // with it the FP32 rounding floor of the residual)
+        alpha[2] = ok ? a[3] / n : 0.0;
     }
 }
 
-// p += alpha1 e1 + alpha2 e2 (either may be absent); thread per (x pair,
+// p += alpha1 (e1 - mean e1) + alpha2 (e2 - mean e2) (either may be
+// absent; scales and means from k_ws_alpha); thread per (x pair,
 // row, k group) of the split layout: float4 read-modify-write of both
 // colours, natural-order reads of e (8 contiguous bytes per thread and k)
 __global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int D, const float* __restrict__ e1,
@@ -536,6 +543,7 @@ __global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int 
     if (t >= Wh * H * L.Dg) return;
     const int m = t % Wh, j = (t / Wh) % H, g = t / (Wh * H);
     const float a1 = e1 ? (float)*alpha1 : 0.0f, a2 = e2 ? (float)*alpha2 : 0.0f;
+    const float m1 = e1 ? (float)alpha1[2] : 0.0f, m2 = e2 ? (float)alpha2[2] : 0.0f; // means of e1, e2
     const int c0 = j & 1;
     f4* P0 = (f4*)p.c[c0];
     f4* P1 = (f4*)p.c[c0 ^ 1];
@@ -547,12 +555,12 @@ __global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int 
         if (k >= D) break;
         const size_t o = ((size_t)k * H + j) * W + 2 * m;
         if (e1) {
-            x0[sl] = x0[sl] + a1 * e1[o];
-            x1[sl] = x1[sl] + a1 * e1[o + 1];
+            x0[sl] = x0[sl] + a1 * (e1[o] - m1);
+            x1[sl] = x1[sl] + a1 * (e1[o + 1] - m1);
         }
         if (e2) {
-            x0[sl] = x0[sl] + a2 * e2[o];
-            x1[sl] = x1[sl] + a2 * e2[o + 1];
+            x0[sl] = x0[sl] + a2 * (e2[o] - m2);
+            x1[sl] = x1[sl] + a2 * (e2[o + 1] - m2);
         }
     }
     P0[e] = x0;
@@ -582,6 +590,7 @@ __global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, 
     const double rb = block_max(pb, nb_);
     if (threadIdx.x == 0) {
         c->iters += add;
+        c->swept += 2 * add; // both paths ran the batch
         c->checks += 2;
         c->res_ws = ra;
         c->res_prev = c->res;
@@ -619,6 +628,7 @@ __global__ void __launch_bounds__(BS) k_ws_judge_pred(const double* __restrict__
         const double rho = (c->res_prev > 0.0 && c->res >= 0.0) ? fmin(1.0, c->res / c->res_prev) : 1.0;
         const double pred = c->res * rho;
         c->checks += 1;
+        c->swept += add; // the corrected path's batch
         c->res_ws = ra;
         if (ra < pred) {
             c->res_prev = c->res;
@@ -650,6 +660,7 @@ __global__ void __launch_bounds__(BS) k_ws_judge_fin(const double* __restrict__ 
         c->res_prev = c->res;
         c->res = rb;
         c->iters += add;
+        c->swept += add; // the uncorrected path's batch after a rejection
         c->checks += 1;
         trace_push(c, BLZ_TRACE_REJECTED, rb);
         if (!(c->res > tol) || c->iters >= cap) {

@@ -77,7 +77,7 @@ struct SolveCtl {
     int ntrace;               // entries written to trace
     blz_trace_entry* trace;   // residual history of this solve (nullptr: not recorded)
     int trace_cap;            // capacity of trace
-    int pad_;
+    int swept;                // sweeps executed, including the judge's batches on the path not kept
 };
 
 // one residual evaluation of a solve, appended to its history (when enabled)

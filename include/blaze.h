@@ -128,6 +128,8 @@ typedef struct blz_stats {
     int residual_checks;      /* FP64 residual evaluations in the last step */
     int rbgs_sweeps_timed;    /* RBGS sweeps behind rbgs_kernel_ms */
     int rbgs_kernel;          /* kernel of the last sweeps: 1 two colour passes, 2 fused sweep, 3 row-pair fused sweep */
+    int rbgs_sweeps_executed; /* sweeps run in the last step, including the warm start judge's batches on the
+                                 path not kept (rbgs_iters counts each judged batch once) */
 } blz_stats;
 
 /* synthetic particle set: uniform in a ball, deterministic for a seed */
@@ -202,6 +204,12 @@ int blz_set_rbgs_variant(blz_solver* h, int variant);
    1203, 1204, 1403, 1404 (fused ones skipped when the spare planes do not
    exist). */
 int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep);
+
+/* Change the pressure solve's limits of a live handle (no reference
+   counterpart; the reference reads maxIterSolve once, src/FluidSolver.cu:
+   723-724): the residual_tol, max_iter_solve and residual_check_every
+   fields of blz_settings for the following steps. */
+int blz_set_solve_limits(blz_solver* h, float residual_tol, int max_iter_solve, int residual_check_every);
 
 /* Residual history of residual-driven solves (no reference counterpart; the
    reference solve runs a fixed sweep count and evaluates no residual,

@@ -8,7 +8,9 @@ blaze_amd/csrc/k_warm.hip does on the GPU after the network's correction:
   A_G x = b, A_G = P^T A P for piecewise-constant P: the graph Laplacian of
          the block grid with edge weight = fine faces shared by two blocks
   e  = trilinear prolongation of x (block centres), clamped at the ends
-  p += alpha e, alpha = <r, e - mean e> / <e, A e>   (least error A-norm)
+  p += alpha (e - mean e), alpha = <r, e - mean e> / <e, A e>
+                                            (least error A-norm; the constant
+                                            part changes no residual)
 
 There is no reference counterpart (the reference starts each solve from the
 previous pressure). The oracle is pinned by its own tests: the Galerkin
@@ -142,7 +144,7 @@ def global_correction(p, rhs, scale, F, iters=400, rtol2=1e-8):
     x = coarse_solve(coarse_rhs(r, F), r.shape, F, iters, rtol2)
     e = prolong(x, r.shape, F)
     al = energy_alpha(r, e)
-    return np.asarray(p, np.float64) + al * e, al
+    return np.asarray(p, np.float64) + al * (e - e.mean()), al
 
 
 def block_size(shape, cap=16):
