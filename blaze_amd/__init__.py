@@ -172,6 +172,7 @@ def _load():
         "blz_render_settings_default": ([P(RenderSettings)], C.c_int),
         "blz_parse_render": ([C.c_char_p, P(RenderSettings)], C.c_int),
         "blz_render": ([vp, P(RenderSettings), P(C.c_float)], C.c_int),
+        "blz_render_noalloc": ([vp, P(RenderSettings), P(C.c_float)], C.c_int),
         "blz_stage_source": ([vp, P(C.c_float), C.c_int], C.c_int),
         "blz_add_source_staged": ([vp], C.c_int),
         "blz_write_image": ([C.c_char_p, P(C.c_float), C.c_int, C.c_int], C.c_int),
@@ -389,6 +390,13 @@ class Solver:
         """(height, width, 3) float32 image of the current temperature."""
         out = np.empty((settings.height, settings.width, 3), np.float32)
         _check(_lib.blz_render(self._h, C.byref(settings), out.ctypes.data_as(C.POINTER(C.c_float))), "blz_render")
+        return out
+
+    def render_noalloc(self, settings: RenderSettings) -> np.ndarray:
+        """render() through blz_render_noalloc (the library's static scratch)."""
+        out = np.empty((settings.height, settings.width, 3), np.float32)
+        _check(_lib.blz_render_noalloc(self._h, C.byref(settings), out.ctypes.data_as(C.POINTER(C.c_float))),
+               "blz_render_noalloc")
         return out
 
     def set_warm_start(self, net, network: bool = True, global_correction: bool = True, parts: int = 0):

@@ -99,6 +99,7 @@ int Solver::init(const blz_settings& st_) {
     D = s.grid_res[2];
     dx = s.dx;
     smoke = s.smoke_density > 0.0f;
+    BLZ_HIP(hipGetDevice(&device));
     BLZ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // create_grids_kernel origins (src/FluidSolver.cu:36-43)
     const int dims[F_N][3] = {{W, H, D}, {W + 1, H, D}, {W, H + 1, D}, {W, H, D + 1}, {W, H, D}};
@@ -844,6 +845,15 @@ int blz_render(blz_solver* h, const blz_render_settings* r, float* rgb) {
         return BLZ_E_INVALID;
     }
     return blz::render_frame(h->s, *r, rgb);
+}
+
+int blz_render_noalloc(blz_solver* h, const blz_render_settings* r, float* rgb) {
+    CHECK_HANDLE(h);
+    if (!r || !rgb) {
+        blz::set_error("blz_render_noalloc: null argument");
+        return BLZ_E_INVALID;
+    }
+    return blz::render_frame_noalloc(h->s, *r, rgb);
 }
 
 int blz_set_warm_start_parts(blz_solver* h, int parts) {

@@ -21,6 +21,9 @@
 // bit). A shadow ray that misses the box (only possible for degenerate
 // directions; the reference would march it forever) leaves full
 // transmittance, and both marches stop after 65536 samples.
+#include <mutex>
+#include <string>
+
 #include "solver.h"
 
 namespace blz {
@@ -270,6 +273,20 @@ __global__ void __launch_bounds__(RT * RT) k_render(const float* __restrict__ T,
 
 inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// zero fill (a kernel, not a memset node, so a captured render replays it)
+__global__ void __launch_bounds__(256) k_rnd_zero(float* __restrict__ d, size_t n) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) d[t] = 0.0f;
+}
+
+// Scratch of blz_render_noalloc: the two multi-scatter volumes and the
+// device frame buffer, in place of the handle's grow-only allocation.
+// 32 MiB per device (held until the process ends): a 256^3 grid's volumes
+// (2 x 64^3 floats) plus a 1920x1080 frame (6.2M floats) fit. Every call
+// on a device shares its copy; calls that overlap on one device race.
+constexpr size_t RND_SCRATCH_FLOATS = (size_t)8 << 20;
+static __device__ __align__(256) float scratch[RND_SCRATCH_FLOATS];
+
 V3 h3(const float* a) { return V3{a[0], a[1], a[2]}; }
 V3 hadd(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 V3 hsub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -278,7 +295,10 @@ V3 hmulf(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
 
 } // namespace
 
-int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb) {
+// get_buf(n, &p): n floats of device scratch for the scatter volumes and the
+// frame, or an error
+template <class GetBuf>
+static int render_with(Solver& s, const blz_render_settings& r, float* host_rgb, GetBuf get_buf) {
     if (r.width <= 0 || r.height <= 0 || r.width % RT || r.height % RT || r.width > 16384 || r.height > 16384) {
         set_error("render size must be positive multiples of 8 (at most 16384)");
         return BLZ_E_INVALID;
@@ -347,19 +367,10 @@ int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb) {
     // buffers: two scatter volumes (at least one cell each) and the frame
     const size_t ns = (size_t)std::max(1, sc.Ws) * std::max(1, sc.Hs) * std::max(1, sc.Ds);
     const size_t nf = (size_t)r.width * r.height * 3;
-    const size_t bytes = (2 * ns + nf) * sizeof(float);
-    if (bytes > s.rnd_bytes) {
-        if (s.rnd_buf) BLZ_HIP(hipFree(s.rnd_buf));
-        s.rnd_buf = nullptr;
-        s.rnd_bytes = 0;
-        if (hipMalloc(&s.rnd_buf, bytes) != hipSuccess) {
-            set_error("hipMalloc failed for render buffers");
-            return BLZ_E_NOMEM;
-        }
-        s.rnd_bytes = bytes;
-    }
-    float* scat[2] = {s.rnd_buf, s.rnd_buf + ns};
-    float* fb = s.rnd_buf + 2 * ns;
+    float* buf = nullptr;
+    if (int e = get_buf(2 * ns + nf, &buf)) return e;
+    float* scat[2] = {buf, buf + ns};
+    float* fb = buf + 2 * ns;
     const float* T = s.f[F_T].front();
     const bool have_scatter = sc.Ws > 0 && sc.Hs > 0 && sc.Ds > 0;
     if (!have_scatter) {
@@ -380,7 +391,7 @@ int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb) {
             std::swap(scat[0], scat[1]);
         }
     } else {
-        BLZ_HIP(hipMemsetAsync(scat[0], 0, ns * sizeof(float), s.stream));
+        hipLaunchKernelGGL(k_rnd_zero, dim3(nblk(ns)), dim3(256), 0, s.stream, scat[0], ns);
     }
     const dim3 grid((unsigned)(r.width / RT), (unsigned)(r.height / RT)), block(RT, RT);
     float inv = 0.0f;
@@ -392,6 +403,69 @@ int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb) {
     BLZ_HIP(hipMemcpyAsync(host_rgb, fb, nf * sizeof(float), hipMemcpyDeviceToHost, s.stream));
     BLZ_HIP(hipStreamSynchronize(s.stream));
     return BLZ_OK;
+}
+
+int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb) {
+    return render_with(s, r, host_rgb, [&s](size_t n, float** out) -> int {
+        const size_t bytes = n * sizeof(float);
+        if (bytes > s.rnd_bytes) {
+            if (s.rnd_buf) BLZ_HIP(hipFree(s.rnd_buf));
+            s.rnd_buf = nullptr;
+            s.rnd_bytes = 0;
+            if (hipMalloc(&s.rnd_buf, bytes) != hipSuccess) {
+                set_error("hipMalloc failed for render buffers");
+                return BLZ_E_NOMEM;
+            }
+            s.rnd_bytes = bytes;
+        }
+        *out = s.rnd_buf;
+        return BLZ_OK;
+    });
+}
+
+// Device address of `scratch` on each device, looked up once per device
+// (with that device current) by hipGetSymbolAddress; host code uses these
+// addresses, never the array's name (a host-side stand-in).
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[64] = {};
+
+static int scratch_on(int device, float** out) {
+    if (device < 0 || device >= 64) {
+        set_error("blz_render_noalloc: device index out of range");
+        return BLZ_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        int cur = -1;
+        BLZ_HIP(hipGetDevice(&cur));
+        if (cur != device) {
+            set_error("blz_render_noalloc: the solver's device must be current");
+            return BLZ_E_INVALID;
+        }
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            return BLZ_E_HIP;
+        }
+        g_scratch_dev[device] = (float*)p;
+    }
+    *out = g_scratch_dev[device];
+    return BLZ_OK;
+}
+
+int render_frame_noalloc(Solver& s, const blz_render_settings& r, float* host_rgb) {
+    float* base = nullptr;
+    if (int e = scratch_on(s.device, &base)) return e;
+    return render_with(s, r, host_rgb, [base](size_t n, float** out) -> int {
+        if (n > RND_SCRATCH_FLOATS) {
+            set_error("blz_render_noalloc: scatter volumes plus frame need " + std::to_string(n) +
+                      " floats, above the scratch limit of " + std::to_string(RND_SCRATCH_FLOATS));
+            return BLZ_E_INVALID;
+        }
+        *out = base;
+        return BLZ_OK;
+    });
 }
 
 } // namespace blz

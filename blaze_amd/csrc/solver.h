@@ -123,6 +123,7 @@ struct ResScene {
 
 struct Solver {
     blz_settings s{};
+    int device = 0;           // HIP device current at creation (owns the stream and all buffers)
     int W = 0, H = 0, D = 0;
     float dx = 0;
     bool smoke = false;
@@ -238,6 +239,8 @@ int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int 
 int launch_ctl_reset(Solver& s, SolveCtl* c);
 int launch_residual_ws(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
 int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb);
+// the same with the library's static per-device scratch (no allocation)
+int render_frame_noalloc(Solver& s, const blz_render_settings& r, float* host_rgb);
 int warm_start_set_parts(Solver& s, int parts);
 int launch_residual_ws_final(Solver& s, SolveCtl* c, int* host_done, int seq, double tol, int cap, int nb_partial);
 int launch_rbgs_spare(Solver& s, float dt, int sweeps, const int* stop);

@@ -271,6 +271,15 @@ int blz_parse_render(const char* path, blz_render_settings* out);
    receives width*height*3 floats, pixel (i, j) at 3*(i + width*j). Width
    and height must be multiples of 8 (as the reference's 8x8 launch). */
 int blz_render(blz_solver* h, const blz_render_settings* r, float* rgb);
+/* blz_render without any allocation: no counterpart in the reference; for
+   callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc). The
+   scatter volumes and the device frame buffer live in the library's static
+   per-device scratch of 8M floats (32 MiB, held until the process ends):
+   2 * (W/4)(H/4)(D/4) + width*height*3 floats must fit, else BLZ_E_INVALID.
+   Same image as blz_render. Every call on a device shares that device's
+   scratch: calls that overlap on one device race. */
+int blz_render_noalloc(blz_solver* h, const blz_render_settings* r, float* rgb);
 
 /* Frame writer (RenderEngine::writeToDisk, src/RenderEngine.cu:367-476): by
    extension .ppm (ASCII P3, int(255.99 v)), .png (8-bit RGBA, (int)(255 v)
