@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--scene", default=os.path.join(ROOT, "scenes", "explosion256.blz"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=6)
+    ap.add_argument("--no-cpu-1thread", action="store_true", help="skip the one-thread CPU baseline")
     ap.add_argument("--rbgs-variant", type=int, default=0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a device copy of 256 MiB (reference kernel for HBM counter scaling)")
@@ -116,20 +117,39 @@ def traffic_probe(a):
     log(f"[probe] {a.probe_sweeps} sweeps on {W}x{H}x{D} done", a.quiet)
 
 
-def cpu_baseline(s, parts, substeps, iters_per_substep, checks_per_substep, n_sweeps, quiet):
-    """Time the oracle (all host threads; bitwise equal to serial) on one
-    substep of the same 256^3 workload, stage by stage, plus n_sweeps RBGS
-    sweeps, and scale to the GPU run's average sweeps and residual checks per
-    substep. Returns the CPU voxel-steps/s and a description."""
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and its logical CPUs."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 0
+
+
+def cpu_baseline(s, parts, substeps, iters_per_substep, checks_per_substep, n_sweeps, quiet, threads=0):
+    """Time the oracle (threads = 0: all host threads, bitwise equal to
+    serial; else that many) on one substep of the same 256^3 workload, stage
+    by stage, plus n_sweeps RBGS sweeps, and scale to the GPU run's average
+    sweeps and residual checks per substep. Returns the CPU voxel-steps/s
+    and a description."""
     import oracle as O
 
     s2 = s.copy()
     s2.residual_tol = 0.0
     s2.max_iter_solve = n_sweeps
+    all_threads = O.max_threads()
+    if threads:
+        O.set_threads(threads)
     o = O.Oracle(s2)
     cores = O.max_threads()
     t0 = time.perf_counter()
-    o.add_source(parts)
+    if parts is not None:   # the stages' cost does not depend on the values
+        o.add_source(parts)
     t_src = time.perf_counter() - t0
     dt = float(s.dt) / max(1, substeps)
     stages = []
@@ -164,6 +184,8 @@ def cpu_baseline(s, parts, substeps, iters_per_substep, checks_per_substep, n_sw
         o.run_stage(st, 0.0, dt)
         per[st] = time.perf_counter() - t0
         t_fixed += per[st]
+    if threads:
+        O.set_threads(all_threads)
     t_sweep = t_sweeps / n_sweeps
     t_substep = t_fixed + iters_per_substep * t_sweep + checks_per_substep * t_res
     W, H, D = s.grid_res
@@ -174,7 +196,9 @@ def cpu_baseline(s, parts, substeps, iters_per_substep, checks_per_substep, n_sw
               f"{t_fixed:.2f}s), {n_sweeps} RBGS sweeps ({t_sweep * 1e3:.1f} ms/sweep) and one residual "
               f"check ({t_res * 1e3:.1f} ms), combined with the GPU run's averages of "
               f"{iters_per_substep:.1f} sweeps and {checks_per_substep:.1f} checks per substep")
-    return {"value": value, "unit": "voxel-steps/s", "cores": cores, "kind": "port", "sample": sample}
+    model, logical = cpu_model()
+    return {"value": value, "unit": "voxel-steps/s", "cores": cores, "kind": "port", "sample": sample,
+            "cpu_model": model, "host_logical_cpus": logical}
 
 
 def main():
@@ -451,10 +475,14 @@ def report(a, s, gen, parts, L, LT, warm):
         if tuple(tj.get("grid", ())) == (W, H, D):
             traffic = tj.get("bytes_per_launch")
 
-    cpu = None
+    cpu = cpu1 = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(s, parts, max(1, round(tot_sub / a.steps)), tot_iters / max(1, tot_sub),
                            tot_checks / max(1, tot_sub), a.cpu_sweeps, a.quiet)
+        if not a.no_cpu_1thread:
+            # the parity mode: one thread (the same bits as every thread count)
+            cpu1 = cpu_baseline(s, None, max(1, round(tot_sub / a.steps)), tot_iters / max(1, tot_sub),
+                                tot_checks / max(1, tot_sub), 2, a.quiet, threads=1)
 
     out = {
         "metric": "voxel-steps/s at 256\u00b3, plain RBGS vs warm-start at equal \u2016\u2207\u00b7u\u2016\u221e; HBM GB/s",
@@ -497,6 +525,7 @@ def report(a, s, gen, parts, L, LT, warm):
             "timing_pass_ms_per_step": LT["elapsed"] * 1e3 / a.steps,
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
     }
     if L.get("render") is not None:
         out["render"] = render_report(a, L["render"])
@@ -619,36 +648,7 @@ def batched(a):
     import blaze_amd as B
     import blaze_amd.warm_start as WS
 
-    rng = np.random.default_rng(5)
-    sets, parts = [], []
-    for q in range(a.scenes):
-        n = int(np.clip(round(rng.uniform(32, 128) / 8) * 8, 32, 128))
-        s = B.default_settings()
-        for ax in range(3):
-            s.grid_res[ax] = n
-            s.domain_min[ax] = 0.0
-            s.domain_max[ax] = 1.0
-        s.dx = 1.0 / n
-        s.dt = 1.0 / 24.0
-        s.density = 1.0
-        s.gravity = -9.8
-        s.buoyancy = float(rng.uniform(0.5, 2.0))
-        for ax, v in enumerate((1, 1, 1, 0, 1, 1)):
-            s.closed_bounds[ax] = v
-        s.residual_tol = 1e-4
-        s.max_iter_solve = 100
-        s.residual_check_every = 10
-        s.cfl_number = 0.0
-        s.warm_start = 0 if a.no_warm_start else 1
-        g = B.ParticleGen()
-        g.count = int(20000 * (n / 64) ** 3)
-        g.seed = 1 + q
-        g.center[0], g.center[1], g.center[2] = 0.5, 0.1, 0.5
-        g.radius = float(rng.uniform(0.03, 0.1))
-        g.pscale = 2.0 / n
-        g.temp = 1.0
-        sets.append(s)
-        parts.append(B.generate_particles(g))
+    sets, parts = batched_scenes(a.scenes, warm_start=not a.no_warm_start)
     b = B.Batch(sets)
     net = None
     if not a.no_warm_start:
@@ -690,6 +690,48 @@ def batched(a):
                   "max_final_residual": max(res), "mean_final_residual": sum(res) / len(res)},
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_scenes(n=64, warm_start=True):
+    """The batched workload: n scenes, grids N^3 with N uniform in [32, 128]
+    rounded to multiples of 8 (seed 5), source radius U[0.03, 0.1], buoyancy
+    U[0.5, 2.0], solid walls with an open top, RBGS to 1e-4 with a cap of
+    100 sweeps; returns the settings and the particles of each scene."""
+    import numpy as np
+
+    import blaze_amd as B
+
+    rng = np.random.default_rng(5)
+    sets, parts = [], []
+    for q in range(n):
+        n = int(np.clip(round(rng.uniform(32, 128) / 8) * 8, 32, 128))
+        s = B.default_settings()
+        for ax in range(3):
+            s.grid_res[ax] = n
+            s.domain_min[ax] = 0.0
+            s.domain_max[ax] = 1.0
+        s.dx = 1.0 / n
+        s.dt = 1.0 / 24.0
+        s.density = 1.0
+        s.gravity = -9.8
+        s.buoyancy = float(rng.uniform(0.5, 2.0))
+        for ax, v in enumerate((1, 1, 1, 0, 1, 1)):
+            s.closed_bounds[ax] = v
+        s.residual_tol = 1e-4
+        s.max_iter_solve = 100
+        s.residual_check_every = 10
+        s.cfl_number = 0.0
+        s.warm_start = 1 if warm_start else 0
+        g = B.ParticleGen()
+        g.count = int(20000 * (n / 64) ** 3)
+        g.seed = 1 + q
+        g.center[0], g.center[1], g.center[2] = 0.5, 0.1, 0.5
+        g.radius = float(rng.uniform(0.03, 0.1))
+        g.pscale = 2.0 / n
+        g.temp = 1.0
+        sets.append(s)
+        parts.append(B.generate_particles(g))
+    return sets, parts
 
 
 if __name__ == "__main__":

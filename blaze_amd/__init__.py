@@ -51,7 +51,7 @@ class Settings(C.Structure):
         ("source_max_particle_count", C.c_int),
         ("noise_seed", C.c_int), ("residual_tol", C.c_float), ("residual_check_every", C.c_int),
         ("cfl_number", C.c_float), ("max_substeps", C.c_int), ("smoke_density", C.c_float),
-        ("warm_start", C.c_int),
+        ("warm_start", C.c_int), ("residual_check_adaptive", C.c_int),
     ]
 
     def copy(self) -> "Settings":

@@ -95,6 +95,7 @@ void Parser::sceneParse(SceneSettings* scene, const char* sceneFile) {
     scene->maxSubsteps = s.max_substeps;
     scene->smokeDensity = s.smoke_density;
     scene->warmStart = s.warm_start;
+    scene->residualCheckAdaptive = s.residual_check_adaptive;
     scene->hasParticleGen = hasGen != 0;
     if (hasGen) scene->particleGen = gen;
 }

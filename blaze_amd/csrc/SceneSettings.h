@@ -65,6 +65,7 @@ struct SceneSettings {
     int maxSubsteps;
     float smokeDensity;
     int warmStart;
+    int residualCheckAdaptive;
     bool hasParticleGen;
     blz_particle_gen particleGen;
     // renderer keys (src/SceneSettings.h:73-107), parsed when present
@@ -93,6 +94,7 @@ struct SceneSettings {
         s.noise_seed = noiseSeed; s.residual_tol = residualTol; s.residual_check_every = residualCheckEvery;
         s.cfl_number = cflNumber; s.max_substeps = maxSubsteps; s.smoke_density = smokeDensity;
         s.warm_start = warmStart;
+        s.residual_check_adaptive = residualCheckAdaptive;
         return s;
     }
 };

@@ -382,7 +382,11 @@ int Solver::project_solve(float dt_) {
     const int every = s.residual_check_every;
     const int seq = ++solve_seq;
     SolveCtl* c = ctl + q;
-    if (int e = launch_ctl_reset(*this, c)) return e;
+    // adaptive check spacing for plain solves (a warm start's judge reads
+    // every batch's check); the fixed-point stop for all
+    const bool warm = ws_net && s.warm_start && cap > 0;
+    if (int e = launch_ctl_reset(*this, c, s.residual_check_adaptive && !warm, s.residual_check_adaptive != 0))
+        return e;
     if (int e = launch_residual_check(*this, dt_, c, done_host_dev, seq, tol, cap, 0, 1)) return e;
     int it0 = 0;
     if (ws_net && s.warm_start && cap > 0) {

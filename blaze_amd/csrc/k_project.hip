@@ -1042,6 +1042,20 @@ __global__ void __launch_bounds__(BS) k_residual(RB p, RB rhs, RBL L, int W, int
     residual_block(p, rhs, L, W, H, D, c, cbA, cbB, partial, blockIdx.x, gridDim.x);
 }
 
+// a batch's check that an adaptive solve skips (not due yet)
+__device__ __forceinline__ bool check_skipped(const SolveCtl* c, int add, int cap, int first) {
+    return !first && c->adaptive && c->iters + add < c->next_check && c->iters + add < cap;
+}
+
+// the residual check of a solve: skipped when its solve is done or (adaptive
+// spacing) the check is not due
+__global__ void __launch_bounds__(BS) k_residual_chk(RB p, RB rhs, RBL L, int W, int H, int D, double c, int4 cbA,
+                                                     int2 cbB, double* __restrict__ partial,
+                                                     const SolveCtl* __restrict__ ctl, int add, int cap, int first) {
+    if (ctl->done || check_skipped(ctl, add, cap, first)) return;
+    residual_block(p, rhs, L, W, H, D, c, cbA, cbB, partial, blockIdx.x, gridDim.x);
+}
+
 // the residual of several grids in one launch (table as for the fused sweep)
 __global__ void __launch_bounds__(BS) k_residual_multi(const ResScene* __restrict__ sc, int n) {
     const int b = blockIdx.x;
@@ -1064,6 +1078,14 @@ __global__ void __launch_bounds__(BS) k_residual_multi(const ResScene* __restric
 __device__ __forceinline__ void res_final_block(const double* __restrict__ partial, int n, SolveCtl* c,
                                                 int* host_done, int seq, double tol, int cap, int add, int first) {
     if (c->done) return;
+    if (check_skipped(c, add, cap, first)) { // not due: count the batch's sweeps only
+        __syncthreads();                       // every thread has read c
+        if (threadIdx.x == 0) {
+            c->iters += add;
+            c->swept += add;
+        }
+        return;
+    }
     double mx = 0.0;
     for (int q = threadIdx.x; q < n; q += BS) mx = fmax(mx, partial[q]);
     __shared__ double sm[BS / 64];
@@ -1074,20 +1096,34 @@ __device__ __forceinline__ void res_final_block(const double* __restrict__ parti
     if (threadIdx.x == 0) {
         double r = sm[0];
         for (int q = 1; q < (int)(BS / 64); ++q) r = fmax(r, sm[q]);
+        const double prev = c->res;
         if (first) {
             c->res0 = r;
             c->iters = 0;
+            c->gap = c->every;
+            c->stall_it = 0;
         } else {
             c->iters += add;
             c->swept += add;
         }
-        c->res_prev = first ? -1.0 : c->res;
+        c->res_prev = first ? -1.0 : prev;
         c->res = r;
         c->checks += 1;
         trace_push(c, BLZ_TRACE_CHECK, r);
+        // an FP32 fixed point: the same residual at every check over
+        // STALL_SWEEPS sweeps
+        bool stalled = false;
+        if (c->stop_stall && !first) {
+            if (r == prev) stalled = c->iters - c->stall_it >= STALL_SWEEPS;
+            else c->stall_it = c->iters;
+        }
+        if (c->adaptive) {
+            if (!first) c->gap = next_check_gap(c->gap, c->every, r, prev, tol);
+            c->next_check = c->iters + c->gap;
+        }
         // a non-finite state (residual +inf) ends the solve at once: the
         // host reports it after the step instead of sweeping to the cap
-        if (!(r > tol) || c->iters >= cap || r == __longlong_as_double(0x7ff0000000000000LL)) {
+        if (!(r > tol) || c->iters >= cap || stalled || r == __longlong_as_double(0x7ff0000000000000LL)) {
             c->done = 1;
             __hip_atomic_store(host_done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __threadfence_system();
@@ -1106,7 +1142,14 @@ __global__ void __launch_bounds__(BS) k_res_final_multi(const ResScene* __restri
     res_final_block(e.partial, e.nblocks, e.ctl, e.host_done, e.seq, e.tol, e.cap, add, first);
 }
 
-__global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap) {
+__global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap, int adaptive, int stop_stall,
+                            int every) {
+    c->adaptive = adaptive;
+    c->stop_stall = stop_stall;
+    c->every = every;
+    c->gap = every;
+    c->next_check = 0;
+    c->stall_it = 0;
     c->trace = trace;
     c->trace_cap = trace_cap;
     c->ntrace = 0;
@@ -1473,11 +1516,12 @@ int launch_residual_multi(hipStream_t st, const ResScene* tab, int n, int blocks
     return BLZ_OK;
 }
 
-int launch_ctl_reset(Solver& s, SolveCtl* c) {
+int launch_ctl_reset(Solver& s, SolveCtl* c, bool adaptive, bool stop_stall) {
     // the residual history slot of this solve (control block q of s.ctl)
     const ptrdiff_t q = c - s.ctl;
     blz_trace_entry* tr = (s.trace && q >= 0 && q < MAX_SOLVES) ? s.trace + (size_t)q * s.trace_cap : nullptr;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s.stream, c, tr, tr ? s.trace_cap : 0);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s.stream, c, tr, tr ? s.trace_cap : 0, adaptive ? 1 : 0,
+                       stop_stall ? 1 : 0, std::max(1, s.s.residual_check_every));
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
@@ -1494,8 +1538,8 @@ int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int 
     const size_t items = (size_t)(s.W / 2) * s.H * ((s.L.Dg + RES_SEG - 1) / RES_SEG);
     const int nb = (int)std::min<size_t>(blocks_for(items, BS), RESIDUAL_BLOCKS);
     double* partial = s.red + 8;
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, cc, cb_a(s),
-                       cb_b(s), partial, &c->done);
+    hipLaunchKernelGGL(k_residual_chk, dim3(nb), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D, cc, cb_a(s),
+                       cb_b(s), partial, c, add, cap, first);
     BLZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_res_final, dim3(1), dim3(BS), 0, s.stream, partial, nb, c, host_done, seq, tol, cap, add,
                        first);

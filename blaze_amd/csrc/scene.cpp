@@ -51,6 +51,7 @@ extern "C" int blz_settings_default(blz_settings* s) {
     s->max_substeps = 8;
     s->smoke_density = 0.0f;
     s->warm_start = 0;
+    s->residual_check_adaptive = 0;
     return BLZ_OK;
 }
 
@@ -173,6 +174,7 @@ extern "C" int blz_parse_scene(const char* path, blz_settings* out, char* scene_
     r.integer("maxSubsteps", &s.max_substeps, false);
     r.num("smokeDensity", &s.smoke_density, false);
     r.integer("warmStart", &s.warm_start, false);
+    r.integer("residualCheckAdaptive", &s.residual_check_adaptive, false);
     if (!r.ok) {
         blz::set_error(std::string("missing or invalid .blz key: ") + r.missing);
         return BLZ_E_PARSE;

@@ -78,7 +78,26 @@ struct SolveCtl {
     blz_trace_entry* trace;   // residual history of this solve (nullptr: not recorded)
     int trace_cap;            // capacity of trace
     int swept;                // sweeps executed, including the judge's batches on the path not kept
+    // adaptive check spacing (residual_check_adaptive; the oracle's rbgs()
+    // follows the same rule): a batch's check is evaluated only when the
+    // sweep count reaches next_check (or the cap); `gap` is the interval in
+    // use, a multiple of `every`
+    int adaptive, every, gap, next_check;
+    int stop_stall;           // end the solve at an FP32 fixed point
+    int stall_it;             // sweep count at which the current residual value was first seen
 };
+constexpr int STALL_SWEEPS = 60;   // the same residual at every check over this many sweeps: fixed point
+constexpr int GAP_MAX_CHECKS = 64; // longest check interval, in residual_check_every units
+
+// interval to the next check of an adaptive solve from the contraction over
+// the last one (restated in oracle/blaze_oracle.hpp, Solver::next_gap)
+__host__ __device__ inline int next_check_gap(int gap, int every, double r, double prev, double tol) {
+    const double rho = r / prev;
+    if (rho < 1.0 && r * rho <= tol) return max(every, gap / 4 / every * every);
+    if (rho < 1.0 && r * rho * rho <= tol) return max(every, gap / 2 / every * every);
+    if (rho < 1.0 && r * rho * rho * rho * rho <= tol) return gap;
+    return min(2 * gap, GAP_MAX_CHECKS * every);
+}
 
 // one residual evaluation of a solve, appended to its history (when enabled)
 __device__ __forceinline__ void trace_push(SolveCtl* c, int kind, double res) {
@@ -236,7 +255,8 @@ bool rbgs_variant_valid(int v);
 void clear_step_stats(blz_stats& st);
 int launch_residual_check(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int add,
                           int first);
-int launch_ctl_reset(Solver& s, SolveCtl* c);
+// adaptive: check spacing from the contraction; stop_stall: end at an FP32 fixed point
+int launch_ctl_reset(Solver& s, SolveCtl* c, bool adaptive = false, bool stop_stall = false);
 int launch_residual_ws(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap);
 int render_frame(Solver& s, const blz_render_settings& r, float* host_rgb);
 // the same with the library's static per-device scratch (no allocation)

@@ -109,6 +109,12 @@ typedef struct blz_settings {
     float smoke_density;      /* >0: enable smoke density grid, splat value */
     int warm_start;           /* 0 plain RBGS; 1 learned correction before the sweeps; N >= 2 also
                                  after every N sweeps (blz_set_warm_start attaches the network) */
+    int residual_check_adaptive; /* 1: residual-driven solves space their checks from the measured
+                                    contraction (doubling the interval while the tolerance is far,
+                                    down to residual_check_every near it; solves with a warm start keep
+                                    every interval) and stop at an FP32 fixed point (the same FP64
+                                    residual at three checks in a row); 0: a check every
+                                    residual_check_every sweeps until tol or the cap */
 } blz_settings;
 
 /* per-step statistics of the last blz_step / blz_run_stage call */

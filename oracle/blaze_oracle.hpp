@@ -682,18 +682,44 @@ template <class R> struct Solver {
         }
         const int every = s.residual_check_every > 0 ? s.residual_check_every : 10;
         const double tol = (double)s.residual_tol;
-        int it = 0;
+        const bool adaptive = s.residual_check_adaptive != 0;
+        int it = 0, gap = every, stall_it = 0;
         double r = residual(dt);
         res0 = r;
         while (r > tol && it < cap) {
-            const int n = std::min(every, cap - it);
+            const int n = std::min(gap, cap - it);
             for (int l = 0; l < n; ++l) rbgs_sweep(scale);
             it += n;
+            const double prev = r;
             r = residual(dt);
+            if (!adaptive) continue;
+            /* an FP32 fixed point: the same residual at every check over
+               STALL_SWEEPS sweeps ends the solve */
+            if (r == prev) {
+                if (it - stall_it >= STALL_SWEEPS) break;
+            } else {
+                stall_it = it;
+            }
+            gap = next_gap(gap, every, r, prev, tol);
         }
         iters_last = it;
         res = r;
     }
+
+    /* Interval to the next residual check of an adaptive solve, from the
+       contraction over the last interval, rho = r / prev: a quarter of it
+       when the tolerance is predicted within one more interval, half within
+       two, the same within four, else double (at most GAP_MAX_CHECKS
+       checks' worth); always a multiple of `every`. */
+    static int next_gap(int gap, int every, double r, double prev, double tol) {
+        const double rho = r / prev;
+        if (rho < 1.0 && r * rho <= tol) return std::max(every, gap / 4 / every * every);
+        if (rho < 1.0 && r * rho * rho <= tol) return std::max(every, gap / 2 / every * every);
+        if (rho < 1.0 && r * rho * rho * rho * rho <= tol) return gap;
+        return std::min(2 * gap, GAP_MAX_CHECKS * every);
+    }
+    static constexpr int STALL_SWEEPS = 60;
+    static constexpr int GAP_MAX_CHECKS = 64;
 
     void pressure_update(R dt) { /* :601-669 */
         const R sc = dt / ((R)s.density * dx);

@@ -179,3 +179,70 @@ def test_batch_with_warm_start_and_handle_steps(variant):
         b.scenes[i].step(float(t))
         singles[i].step(float(t))
         compare(f"handle step scene {i}", b.scenes[i].field(B.FIELD_U), singles[i].field(B.FIELD_U))
+
+
+def test_batch_scenes_equal_the_oracle():
+    """The multi-grid sweep and residual kernels against the CPU oracle
+    directly: three uneven scenes (wind noise, vorticity, open top,
+    residual-driven RBGS) batched for two steps equal the oracle stepping
+    each scene on its own, field by field, bit for bit, with the same sweep
+    counts and final residuals."""
+    O = pytest.importorskip("oracle")
+    print("[batch-oracle] 3 scenes", flush=True)
+    cases = [scene(r, 30 + i) for i, r in enumerate([(32, 24, 16), (40, 16, 20), (24, 24, 28)])]
+    b = B.Batch([c[0] for c in cases])
+    oracles = [O.Oracle(c[0]) for c in cases]
+    for i, (s, part) in enumerate(cases):
+        b.scenes[i].add_source(part)
+        oracles[i].add_source(part)
+    t = np.float32(0.0)
+    for step in range(2):
+        b.step(float(t))
+        for o in oracles:
+            o.step(float(t))
+        t = np.float32(t + np.float32(cases[0][0].dt))
+        for i in range(len(cases)):
+            gs, os_ = b.scenes[i].stats(), oracles[i].stats()
+            assert gs.rbgs_iters == os_["rbgs_iters"], (step, i)
+            assert gs.residual == os_["residual"], (step, i)
+    for i in range(len(cases)):
+        for w in (B.FIELD_T, B.FIELD_U, B.FIELD_V, B.FIELD_W, B.FIELD_P):
+            compare(f"batch-oracle scene {i} field {w}", b.scenes[i].field(w), oracles[i].field(w))
+
+
+def test_batched_workload_64_scenes_equal_single_scenes():
+    """The benchmark's batched workload itself (64 scenes of 32^3..128^3,
+    seed 5, the shipped warm-start network inside the captured graph,
+    autotuned tiles): after two steps every scene equals the same scene
+    stepped alone with the same network, bit for bit."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    W = pytest.importorskip("blaze_amd.warm_start")
+    print("[batch-64] building 64 scenes", flush=True)
+    sets, parts = bench.batched_scenes(64, warm_start=True)
+    net = W.load_net()
+    b = B.Batch(sets)
+    b.set_warm_start(net)
+    for q in range(64):
+        b.scenes[q].add_source(parts[q])
+    v = b.autotune(10)
+    print(f"[batch-64] {b.stats().voxels} voxels, variant {v}", flush=True)
+    t = np.float32(0.0)
+    for _ in range(2):
+        b.step(float(t))
+        t = np.float32(t + np.float32(1.0 / 24.0))
+    for q in range(64):
+        g = B.Solver(sets[q])
+        g.set_warm_start(net)
+        g.set_rbgs_variant(v)
+        g.add_source(parts[q])
+        t = np.float32(0.0)
+        for _ in range(2):
+            g.step(float(t))
+            t = np.float32(t + np.float32(1.0 / 24.0))
+        gs, bs = g.stats(), b.scenes[q].stats()
+        assert bs.rbgs_iters == gs.rbgs_iters and bs.residual == gs.residual, q
+        for w in (B.FIELD_T, B.FIELD_U, B.FIELD_V, B.FIELD_W, B.FIELD_P):
+            compare(f"batch-64 scene {q} field {w}", b.scenes[q].field(w), g.field(w))
+        g.close()
+    print("[batch-64] all 64 scenes equal", flush=True)

@@ -201,6 +201,38 @@ def test_residual_matches_oracle():
         assert gs.residual == os_["residual"], cb
 
 
+@pytest.mark.parametrize("case", ["tolerance", "fixed-point"])
+def test_adaptive_checks_bitwise(case):
+    """residual_check_adaptive on the GPU (checks skipped on the device until
+    the interval the contraction sets, the FP32 fixed-point stop) against
+    the oracle's rule: same sweep count, same final residual, same pressure
+    bit for bit. "tolerance": a 64x48x40 solve to 1e-4; "fixed-point": an
+    unreachable tolerance on 8^3, which ends at the fixed point."""
+    res, tol, cap = ((64, 48, 40), 1e-4, 6000) if case == "tolerance" else ((8, 8, 8), 1e-30, 200000)
+    print(f"[adaptive] {case} {res}", flush=True)
+    s = settings(B, res, closed_bounds=(1, 1, 1, 1, 1, 1), max_iter_solve=cap, residual_tol=tol,
+                 residual_check_every=10, residual_check_adaptive=1)
+    W, H, D = res
+    rng = np.random.default_rng(21)
+    rhs = rng.standard_normal((D, H, W)).astype(np.float32)
+    rhs -= np.float32(rhs.mean())
+    g, o = B.Solver(s), O.Oracle(s)
+    for x in (g, o):
+        x.set_field(B.FIELD_RHS, rhs)
+        x.set_field(B.FIELD_P, np.zeros_like(rhs))
+    g.set_residual_trace(4096)
+    g.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    o.run_stage(B.STAGE_PRESSURE, 0.0, 1.0 / 24.0)
+    gs, os_ = g.stats(), o.stats()
+    checks = len(g.residual_trace(0))
+    print(f"[adaptive] sweeps {gs.rbgs_iters_last} (oracle {os_['rbgs_iters_last']}), {checks} checks, "
+          f"residual {gs.residual:.3e}", flush=True)
+    assert gs.rbgs_iters_last == os_["rbgs_iters_last"] < cap
+    assert gs.residual == os_["residual"]
+    assert checks < gs.rbgs_iters_last // 10
+    compare(f"adaptive {case} P", g.field(B.FIELD_P), o.field(B.FIELD_P))
+
+
 def random_particles(n, seed, lo=-0.1, hi=1.1):
     rng = np.random.default_rng(seed)
     p = np.zeros((n, 8), np.float32)
@@ -299,6 +331,18 @@ def test_frames_plume128_features_bitwise():
     gen.count = 20000
     print("[frames] plume128 features @48^3", flush=True)
     run_frames(s, gen, 8, 8, check_every=2)
+
+
+def test_frames_plume128_full_size_bitwise():
+    """The 128^3 plume at its full size with every force (vorticity 0.3,
+    curl-noise turbulence 0.5 at frequency 4 with seed 7, wind 0.2 with
+    noise on the x boundary, open x sides and top), 100k particles
+    re-splatted every frame, smoke density, CFL substeps and 200 fixed
+    sweeps: two frames bit-exact against the oracle."""
+    s, gen = scene_settings("plume128.blz")
+    assert tuple(s.grid_res) == (128, 128, 128) and gen.count == 100000 and s.max_iter_solve == 200
+    print("[frames] plume128 at full size, 2 frames", flush=True)
+    run_frames(s, gen, 2, 2, check_every=1)
 
 
 def test_frames_explosion_tol_bitwise():

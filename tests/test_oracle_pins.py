@@ -519,3 +519,58 @@ def test_cfl_substep_count():
         o.set_field(1, U)
         o.step(0.0)
         assert o.stats()["substeps"] == want, (umax, o.stats())
+
+
+def _poisson_case(res, seed, **kw):
+    s = settings(res, closed_bounds=(1, 1, 1, 1, 1, 1), residual_check_every=10, **kw)
+    W, H, D = res
+    rng = np.random.default_rng(seed)
+    rhs = rng.standard_normal((D, H, W)).astype(np.float32)
+    rhs -= np.float32(rhs.mean())
+    return s, rhs
+
+
+def test_adaptive_check_spacing_reaches_the_tolerance():
+    """residual_check_adaptive: the solve still ends at the first evaluated
+    residual <= tol, with few residual evaluations, and no earlier than the
+    every-10-sweeps solve (the checks it skips cannot end it sooner)."""
+    s, rhs = _poisson_case((24, 20, 16), 3, max_iter_solve=20000, residual_tol=1e-3)
+    runs = {}
+    for adaptive in (0, 1):
+        s.residual_check_adaptive = adaptive
+        o = O.Oracle(s)
+        o.set_field(6, rhs)
+        o.set_field(5, np.zeros_like(rhs))
+        o.run_stage(7, 0.0, 1 / 24)
+        runs[adaptive] = o.stats()
+    fixed, adap = runs[0], runs[1]
+    print(f"[adaptive] fixed {fixed['rbgs_iters_last']} sweeps, adaptive {adap['rbgs_iters_last']}", flush=True)
+    assert fixed["residual"] <= 1e-3 and adap["residual"] <= 1e-3
+    assert fixed["rbgs_iters_last"] <= adap["rbgs_iters_last"] <= 2 * fixed["rbgs_iters_last"]
+    assert adap["rbgs_iters_last"] % 10 == 0
+
+
+def test_fixed_point_stop_is_a_fixed_point():
+    """With an unreachable tolerance the adaptive solve ends well before its
+    cap once the residual has not changed over 60 sweeps, and there the FP32
+    iteration is indeed stationary: more sweeps leave the pressure bit for
+    bit unchanged."""
+    s, rhs = _poisson_case((8, 8, 8), 5, max_iter_solve=200000, residual_tol=1e-30)
+    s.residual_check_adaptive = 1
+    o = O.Oracle(s)
+    o.set_field(6, rhs)
+    o.set_field(5, np.zeros_like(rhs))
+    o.run_stage(7, 0.0, 1 / 24)
+    st = o.stats()
+    print(f"[stall] stopped after {st['rbgs_iters_last']} sweeps at residual {st['residual']:.3e}", flush=True)
+    assert st["rbgs_iters_last"] < 200000 and st["residual"] > 1e-30
+    p = o.field(5)
+    s2 = s.copy()
+    s2.residual_check_adaptive = 0
+    s2.residual_tol = 0.0
+    s2.max_iter_solve = 100
+    o2 = O.Oracle(s2)
+    o2.set_field(6, rhs)
+    o2.set_field(5, p)
+    o2.run_stage(7, 0.0, 1 / 24)
+    compare("fixed point", o2.field(5), p)
