@@ -179,6 +179,7 @@ def _load():
         "blz_warm_start_input": ([vp, C.c_float, P(C.c_float), P(C.c_int), P(C.c_int)], C.c_int),
         "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
         "blz_warm_start_apply": ([vp, C.c_float, P(C.c_float), P(C.c_double), P(C.c_double), P(C.c_int)], C.c_int),
+        "blz_warm_start_global": ([vp, C.c_float], C.c_int),
         "blz_batch_create": ([P(Settings), C.c_int, P(vp)], C.c_int),
         "blz_batch_destroy": ([vp], C.c_int),
         "blz_batch_solver": ([vp, C.c_int, P(vp)], C.c_int),
@@ -424,6 +425,10 @@ class Solver:
         _check(_lib.blz_warm_start_apply(self._h, C.c_float(dt), a.ctypes.data_as(C.POINTER(C.c_float)),
                                          C.byref(b), C.byref(f), C.byref(k)), "blz_warm_start_apply")
         return b.value, f.value, bool(k.value)
+
+    def warm_start_global(self, dt: float):
+        """Apply the global correction alone, unjudged (training data)."""
+        _check(_lib.blz_warm_start_global(self._h, C.c_float(dt)), "blz_warm_start_global")
 
     def warm_start_input(self, dt: float) -> np.ndarray:
         """(2, Dn, Hn, Wn) float32: normalised coarse residual and solid mask."""

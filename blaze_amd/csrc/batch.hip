@@ -37,6 +37,16 @@ struct Batch {
     float* off_dev = nullptr;
     FusedScene* fs = nullptr;  // 2n entries: home -> spare, then spare -> home
     ResScene* rs = nullptr;    // n entries
+    // the warm start's first judge of every warm-started scene in multi-grid
+    // launches: sweeps of the corrected (home) and uncorrected (judge
+    // planes) paths, their residual partials, the decisions and restores
+    FusedScene* fsw = nullptr; // 2n: home path pairs of the warm scenes
+    FusedScene* fsj = nullptr; // 2n: judge-plane path pairs (planes 6, 7 <-> spare 4, 5)
+    ResScene* rsw = nullptr;   // n: home planes, partials at red + 8
+    ResScene* rsj = nullptr;   // n: judge planes, partials at red + 8 + RESIDUAL_BLOCKS
+    JudgeScene* js = nullptr;  // n
+    std::vector<int> warm_q;   // scenes of these tables (empty: per-scene judges)
+    int w_fs_blocks = 0, w_rs_blocks = 0, w_restore_blocks = 0;
     std::vector<FusedScene> fs_h;
     std::vector<ResScene> rs_h;
     int fs_blocks = 0, rs_blocks = 0;
@@ -50,6 +60,7 @@ struct Batch {
     ~Batch();
     int init(const blz_settings* st, int n);
     int build_tables(int v);
+    int build_judge_tables();
     int capture(int which);
     int enqueue_step(hipStream_t st);
     int step(float time);
@@ -72,6 +83,8 @@ Batch::~Batch() {
     if (off_dev) (void)hipFree(off_dev);
     if (fs) (void)hipFree(fs);
     if (rs) (void)hipFree(rs);
+    for (void* q : {(void*)fsw, (void*)fsj, (void*)rsw, (void*)rsj, (void*)js})
+        if (q) (void)hipFree(q);
     for (blz_solver* h : sc) delete h;
     if (root) (void)hipStreamDestroy(root);
 }
@@ -96,7 +109,9 @@ int Batch::init(const blz_settings* st, int n) {
     BLZ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     BLZ_HIP(hipHostMalloc((void**)&off_host, 2 * n * sizeof(float), hipHostMallocDefault));
     if (hipMalloc(&off_dev, 2 * n * sizeof(float)) != hipSuccess || hipMalloc(&fs, 2 * n * sizeof(FusedScene)) != hipSuccess ||
-        hipMalloc(&rs, n * sizeof(ResScene)) != hipSuccess) {
+        hipMalloc(&rs, n * sizeof(ResScene)) != hipSuccess || hipMalloc(&fsw, 2 * n * sizeof(FusedScene)) != hipSuccess ||
+        hipMalloc(&fsj, 2 * n * sizeof(FusedScene)) != hipSuccess || hipMalloc(&rsw, n * sizeof(ResScene)) != hipSuccess ||
+        hipMalloc(&rsj, n * sizeof(ResScene)) != hipSuccess || hipMalloc(&js, n * sizeof(JudgeScene)) != hipSuccess) {
         set_error("hipMalloc failed for batch tables");
         return BLZ_E_NOMEM;
     }
@@ -182,6 +197,83 @@ int Batch::build_tables(int v) {
     return BLZ_OK;
 }
 
+// Tables of the first warm-start judge of each step for the scenes that
+// apply one (network attached, warm_start != 0, a sweep cap), in multi-grid
+// launches: the same sweeps, residuals and decisions as warm_start_judge
+// runs per scene. Needs judge planes of their own and a fused variant
+// (else warm_q stays empty and each scene judges on its own stream).
+int Batch::build_judge_tables() {
+    const int n = (int)sc.size();
+    warm_q.clear();
+    bool ok = variant >= 100;
+    for (int q = 0; q < n; ++q) {
+        const Solver& s = sc[q]->s;
+        if (s.ws_net && s.s.warm_start && s.s.max_iter_solve > 0) {
+            warm_q.push_back(q);
+            ok = ok && s.judge_set == 6;
+        }
+    }
+    if (!ok) warm_q.clear();
+    const int nw = (int)warm_q.size();
+    if (!nw) return BLZ_OK;
+    const int wpb = wpb_of(variant);
+    std::vector<FusedScene> hw(2 * nw), hj(2 * nw);
+    std::vector<ResScene> rw(nw), rj(nw);
+    std::vector<JudgeScene> jh(nw);
+    int fb = 0, rb = 0, cb = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int q = warm_q[w];
+        Solver& s = sc[q]->s;
+        const unsigned pb = (unsigned)(s.Np * 4);
+        for (int h = 0; h < 2; ++h) {
+            FusedScene e = fs_h[h * n + q];
+            e.first_block = fb;
+            hw[h * nw + w] = e;
+            // judge planes (6, 7) -> spare (4, 5), then back
+            FusedScene j = e;
+            const unsigned src = h == 0 ? 6u * pb : 4u * pb, dst = h == 0 ? 4u * pb : 6u * pb;
+            j.srcR = src;
+            j.srcB = src + pb;
+            j.dstR = dst;
+            j.dstB = dst + pb;
+            hj[h * nw + w] = j;
+        }
+        fb += (fs_h[q].nwaves + wpb - 1) / wpb;
+        ResScene a = rs_h[q];
+        a.first_block = rb;
+        rw[w] = a;
+        ResScene b = a;
+        b.p.c[0] = s.prb + 6 * s.Np;
+        b.p.c[1] = s.prb + 7 * s.Np;
+        b.partial = s.red + 8 + RESIDUAL_BLOCKS;
+        rj[w] = b;
+        rb += a.nblocks;
+        JudgeScene& J = jh[w];
+        J.pa = s.red + 8;
+        J.pb = s.red + 8 + RESIDUAL_BLOCKS;
+        J.na = J.nb = a.nblocks;
+        J.ctl = s.ctl;
+        J.host_done = s.done_host_dev;
+        J.tol = (double)s.s.residual_tol;
+        J.cap = s.s.max_iter_solve;
+        J.backup = s.prb + 6 * s.Np;
+        J.p = s.prb;
+        J.n = 2 * s.Np;
+        J.first_block = cb;
+        J.nblocks = (int)std::min<size_t>((J.n + 255) / 256, 1024);
+        cb += J.nblocks;
+    }
+    w_fs_blocks = fb;
+    w_rs_blocks = rb;
+    w_restore_blocks = cb;
+    BLZ_HIP(hipMemcpy(fsw, hw.data(), 2 * nw * sizeof(FusedScene), hipMemcpyHostToDevice));
+    BLZ_HIP(hipMemcpy(fsj, hj.data(), 2 * nw * sizeof(FusedScene), hipMemcpyHostToDevice));
+    BLZ_HIP(hipMemcpy(rsw, rw.data(), nw * sizeof(ResScene), hipMemcpyHostToDevice));
+    BLZ_HIP(hipMemcpy(rsj, rj.data(), nw * sizeof(ResScene), hipMemcpyHostToDevice));
+    BLZ_HIP(hipMemcpy(js, jh.data(), nw * sizeof(JudgeScene), hipMemcpyHostToDevice));
+    return BLZ_OK;
+}
+
 // one step of every scene: per-scene stages on the scenes' streams, the
 // solves of all scenes on `st`
 int Batch::enqueue_step(hipStream_t st) {
@@ -211,10 +303,32 @@ int Batch::enqueue_step(hipStream_t st) {
         const int cap = s.s.max_iter_solve;
         if (int e = launch_ctl_reset(s, c)) return e;
         if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, tol, cap, 0, 1)) return e;
-        if (s.ws_net && s.s.warm_start && cap > 0)
-            if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap, every)) return e;
+        if (s.ws_net && s.s.warm_start && cap > 0) {
+            if (!warm_q.empty()) {
+                if (int e = warm_start_prepare(s, dt, c)) return e; // judged below, with the other scenes
+            } else if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap, every)) {
+                return e;
+            }
+        }
         BLZ_HIP(hipEventRecord(join[q], s.stream));
         BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+    }
+    if (!warm_q.empty()) {
+        // the first judge of every warm-started scene: `every` sweeps on the
+        // corrected pressure (home planes), the same on the uncorrected one
+        // (judge planes), both residuals, the decisions and restores
+        const int nw = (int)warm_q.size();
+        for (int l = 0; l < every / 2; ++l) {
+            if (int e = launch_fused_multi(st, variant, fsw, nw, w_fs_blocks)) return e;
+            if (int e = launch_fused_multi(st, variant, fsw + nw, nw, w_fs_blocks)) return e;
+        }
+        for (int l = 0; l < every / 2; ++l) {
+            if (int e = launch_fused_multi(st, variant, fsj, nw, w_fs_blocks)) return e;
+            if (int e = launch_fused_multi(st, variant, fsj + nw, nw, w_fs_blocks)) return e;
+        }
+        if (int e = launch_residual_partials_multi(st, rsw, nw, w_rs_blocks)) return e;
+        if (int e = launch_residual_partials_multi(st, rsj, nw, w_rs_blocks)) return e;
+        if (int e = launch_ws_judge_multi(st, js, nw, w_restore_blocks, every)) return e;
     }
     // every scene's sweeps and checks; a converged scene's blocks return at
     // entry. it[q] follows scene q's sweep count as Solver::project_solve
@@ -287,6 +401,7 @@ int Batch::capture(int which) {
         for (int g = 0; g < F_N; ++g) field_cur[which][q][g] = sc[q]->s.f[g].cur;
     }
     auto t0 = std::chrono::steady_clock::now();
+    if (int e = build_judge_tables()) return e;
     hipGraph_t graph = nullptr;
     // captured noise kernels read their offsets from device memory; outside
     // the graph (blz_step on a scene handle) they take the time argument

@@ -899,6 +899,19 @@ int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_swe
     return BLZ_OK;
 }
 
+int blz_warm_start_global(blz_solver* h, float dt) {
+    CHECK_HANDLE(h);
+    if (!(dt > 0.0f)) {
+        blz::set_error("dt must be positive");
+        return BLZ_E_INVALID;
+    }
+    if (!h->s.ws_net) {
+        blz::set_error("no warm-start network attached (blz_set_warm_start)");
+        return BLZ_E_INVALID;
+    }
+    return blz::warm_start_global(h->s, dt);
+}
+
 int blz_warm_start_apply(blz_solver* h, float dt, const float* pred, double* before, double* after, int* kept) {
     CHECK_HANDLE(h);
     if (!pred || !before || !after || !kept || !(dt > 0.0f)) {

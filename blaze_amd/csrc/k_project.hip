@@ -1508,6 +1508,12 @@ int launch_fused_multi(hipStream_t st, int variant, const FusedScene* tab, int n
     }
 }
 
+int launch_residual_partials_multi(hipStream_t st, const ResScene* tab, int n, int blocks) {
+    hipLaunchKernelGGL(k_residual_multi, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
 int launch_residual_multi(hipStream_t st, const ResScene* tab, int n, int blocks, int add, int first) {
     hipLaunchKernelGGL(k_residual_multi, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
     BLZ_HIP(hipGetLastError());

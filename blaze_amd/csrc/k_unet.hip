@@ -17,6 +17,8 @@ namespace blz {
 namespace {
 
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f4m __attribute__((ext_vector_type(4)));
 
 // BF16 <-> FP32: round to nearest even, NaN kept quiet
@@ -40,109 +42,62 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
     b = (int)(t / g.D);
 }
 
-// 3x3x3 convolution, zero padding: Y[v][co] = sum_tap sum_ci X[v+tap][ci] W[tap][co][ci].
-// A wave computes 16 consecutive voxels x all NT*16 output channels with
-// v_mfma_f32_16x16x16_bf16: A = 16 voxels x 16 input channels (lane l:
-// voxel l%16, channels 4*(l/16)..+3), B = 16 input x 16 output channels
-// (lane l: output channel l%16, input channels 4*(l/16)..+3), D = 16
-// voxels x 16 output channels (lane l: output channel l%16, voxels
-// 4*(l/16)..+3). Optional epilogue: leaky ReLU, or the leaky ReLU's
-// derivative at the activation M (1 where M > 0, NET_LEAK elsewhere).
-template <int NT>
-__global__ void __launch_bounds__(256) k_conv3_fwd(const bf16* __restrict__ X, int Cx, int xoff, int Cin,
-                                                   const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y, int Cy,
-                                                   int yoff, const bf16* __restrict__ Mk, int Cm, int moff, int relu,
-                                                   Geo g, long long nvox) {
-    // this workgroup's output channels: tiles blockIdx.y*NT .. +NT-1 of 16
-    const int c0 = blockIdx.y * NT * 16;
-    const int lane = threadIdx.x & 63;
-    const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-    if (v0 >= nvox) return;
-    const int li = lane & 15, kb = lane >> 4;
-    const long long va = v0 + li;
-    int b, z, y, x;
-    vox_coords(va < nvox ? va : nvox - 1, g, b, z, y, x);
-    const bool vok = va < nvox;
-    f4m acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f4m{0.f, 0.f, 0.f, 0.f};
-    const int nchunk = Cin >> 4;
-    const long long HW = (long long)g.H * g.W;
-    for (int tap = 0; tap < 27; ++tap) {
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
-        const bool inb = vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
-        const long long vv = va + dz * HW + dy * g.W + dx;
-        const bf16* xp = X + (inb ? vv : 0) * Cx + xoff + kb * 4;
-        const bf16* wp = Wt + ((size_t)tap * Cout + c0 + li) * Cin + kb * 4;
-        for (int c = 0; c < nchunk; ++c) {
-            s4v a = {0, 0, 0, 0};
-            if (inb) a = *(const s4v*)(xp + c * 16);
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const s4v bw = *(const s4v*)(wp + (size_t)t * 16 * Cin + c * 16);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[t], 0, 0, 0);
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const long long vo = v0 + kb * 4 + q;
-            if (vo >= nvox) continue;
-            float r = acc[t][q];
-            const int co = c0 + t * 16 + li;
-            if (relu) r = r > 0.0f ? r : NET_LEAK * r;
-            if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : NET_LEAK * r;
-            Y[vo * Cy + yoff + co] = f2bf(r);
-        }
-    }
-}
-
-// The same convolution for one 16-channel output tile per wave with the
-// input chunk count CH = Cin/16 fixed at compile time: the 27 taps are
-// unrolled and every input load is unconditional (an out-of-grid tap reads
-// voxel 0 and is replaced by zeros), so the whole tap sequence is one block
-// whose loads the scheduler issues ahead of the MFMAs instead of waiting
-// out a load latency per tap. Same MFMA sequence per output as above.
-template <int CH>
-__global__ void __launch_bounds__(256) k_conv3_fwd1(const bf16* __restrict__ X, int Cx, int xoff,
-                                                    const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
-                                                    int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
-                                                    int relu, Geo g, long long nvox) {
-    constexpr int Cin = CH * 16;
+// 3x3x3 convolution, zero padding: Y[v][co] = sum_tap sum_ci X[v+tap][ci] W[tap][co][ci],
+// as an implicit GEMM on gfx950's v_mfma_f32_16x16x32_bf16 (A 16 x 32,
+// B 32 x 16, D 16 x 16 in FP32). The reduction runs over K = 27 * CIN
+// flattened as k = tap * CIN + ci, in chunks of 32: in a chunk, lane group
+// kg = lane / 16 supplies k = 32 c + 8 kg .. +7, i.e. eight consecutive
+// input channels of one tap (CIN is a multiple of 16, so a group never
+// straddles two taps), for its voxel (A, row lane % 16) and for its output
+// channel (B, column lane % 16); both operands take the same k for the same
+// (lane group, element), so the product is the convolution whatever order
+// the unit sums them in. A padding group past the last tap (27 * CIN not a
+// multiple of 32) supplies zeros. One wave: 16 consecutive voxels x one
+// 16-channel output tile (blockIdx.y); every load is 16 bytes and
+// unconditional (an out-of-grid tap reads voxel 0 and is replaced by
+// zeros), and the chunk loop is unrolled, so the loads are issued ahead of
+// the MFMAs. D: lane l holds output channel l % 16 of voxels 4 (l / 16)
+// .. +3. Optional epilogue: leaky ReLU, or the leaky ReLU's derivative at
+// the activation M (1 where M > 0, NET_LEAK elsewhere).
+template <int CIN>
+__global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, int Cx, int xoff,
+                                                   const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
+                                                   int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
+                                                   int relu, Geo g, long long nvox) {
+    constexpr int NCH = (27 * CIN + 31) / 32;
     const int c0 = blockIdx.y * 16;
     const int lane = threadIdx.x & 63;
     const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
     if (v0 >= nvox) return;
-    const int li = lane & 15, kb = lane >> 4;
+    const int li = lane & 15, kg = lane >> 4;
     const long long va = v0 + li;
     int b, z, y, x;
     vox_coords(va < nvox ? va : nvox - 1, g, b, z, y, x);
     const bool vok = va < nvox;
     f4m acc = f4m{0.f, 0.f, 0.f, 0.f};
     const long long HW = (long long)g.H * g.W;
-    const s4v zero = {0, 0, 0, 0};
+    const s8v zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    const bf16* wrow = Wt + (size_t)(c0 + li) * CIN; // tap 0 of this lane's output channel
 #pragma unroll
-    for (int tap = 0; tap < 27; ++tap) {
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    for (int c = 0; c < NCH; ++c) {
+        const int k0 = 32 * c + 8 * kg;
+        const int tap = k0 / CIN, ci = k0 - tap * CIN;
+        const bool real = tap < 27;
+        const int tp = real ? tap : 26;
+        const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
         const int zz = z + dz, yy = y + dy, xx = x + dx;
-        const bool inb = vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+        const bool inb = real && vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
         const long long vv = va + dz * HW + dy * g.W + dx;
-        const bf16* xp = X + (inb ? vv : 0) * Cx + xoff + kb * 4;
-        const bf16* wp = Wt + ((size_t)tap * Cout + c0 + li) * Cin + kb * 4;
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const s4v ld = *(const s4v*)(xp + c * 16);
-            const s4v a = inb ? ld : zero;
-            const s4v bw = *(const s4v*)(wp + c * 16);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
-        }
+        const s8v ld = *(const s8v*)(X + (inb ? vv : 0) * Cx + xoff + (real ? ci : 0));
+        const s8v wl = *(const s8v*)(wrow + (size_t)tp * Cout * CIN + (real ? ci : 0));
+        const s8v a = inb ? ld : zero;
+        const s8v bw = real ? wl : zero;
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, bw), acc,
+                                                       0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const long long vo = v0 + kb * 4 + q;
+        const long long vo = v0 + kg * 4 + q;
         if (vo >= nvox) continue;
         float r = acc[q];
         const int co = c0 + li;
@@ -419,40 +374,20 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     const bf16* w = (transposed ? n.Wt : n.Wb) + n.off[layer];
     const Geo g{d.B, d.D, d.H, d.W};
     const long long nv = d.n();
-    const unsigned grid = nblk((nv + 15) / 16, 4);
-    // small grids: one 16-channel tile per wave (grid.y tiles) for more
-    // waves in flight; every output keeps its accumulation order
-    const bool split = (nv + 15) / 16 < 2048 && cout > 16;
-    const int nt = split ? 1 : cout / 16;
-    const dim3 gr(grid, split ? (unsigned)(cout / 16) : 1u);
-    if (nt == 1 && (cin == 16 || cin == 32 || cin == 48 || cin == 64 || cin == 96)) {
-#define CONV1_CASE(CH)                                                                                             \
-    case CH:                                                                                                       \
-        hipLaunchKernelGGL(k_conv3_fwd1<CH>, gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, y.p, y.C, y.off, \
+    // one 16-channel output tile per wave (grid.y tiles), 4 waves of 16
+    // voxels per workgroup
+    const dim3 gr(nblk((nv + 15) / 16, 4), (unsigned)(cout / 16));
+#define CONV_CASE(CIN)                                                                                             \
+    case CIN:                                                                                                      \
+        hipLaunchKernelGGL(k_conv3_k32<CIN>, gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, y.p, y.C, y.off, \
                            mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv);                                         \
         break;
-        switch (cin / 16) {
-            CONV1_CASE(1)
-            CONV1_CASE(2)
-            CONV1_CASE(3)
-            CONV1_CASE(4)
-            CONV1_CASE(6)
-        }
-#undef CONV1_CASE
-        NET_HIP(hipGetLastError());
-        return BLZ_OK;
-    }
-#define CONV_CASE(NT)                                                                                              \
-    case NT:                                                                                                       \
-        hipLaunchKernelGGL(k_conv3_fwd<NT>, gr, dim3(256), 0, n.stream, x.p, x.C, x.off, cin, w, cout, y.p, y.C,   \
-                           y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv);                                  \
-        break;
-    switch (nt) {
-        CONV_CASE(1)
-        CONV_CASE(2)
-        CONV_CASE(3)
-        CONV_CASE(4)
-        CONV_CASE(6)
+    switch (cin) {
+        CONV_CASE(16)
+        CONV_CASE(32)
+        CONV_CASE(48)
+        CONV_CASE(64)
+        CONV_CASE(96)
     default: set_error("unsupported channel count"); return BLZ_E_INVALID;
     }
 #undef CONV_CASE

@@ -582,9 +582,9 @@ __device__ __forceinline__ double block_max(const double* __restrict__ partial, 
 
 // the judge: residuals of the corrected (A) and uncorrected (B) pressure
 // after the same sweeps; the lower one continues (B wins ties)
-__global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, int na, const double* __restrict__ pb,
-                                                 int nb_, SolveCtl* c, int* host_done, int seq, double tol, int cap,
-                                                 int add) {
+__device__ __forceinline__ void judge_block(const double* __restrict__ pa, int na, const double* __restrict__ pb,
+                                            int nb_, SolveCtl* c, int* host_done, int seq, double tol, int cap,
+                                            int add) {
     if (c->done) return;
     const double ra = block_max(pa, na);
     const double rb = block_max(pb, nb_);
@@ -610,6 +610,34 @@ __global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, 
             __threadfence_system();
         }
     }
+}
+
+__global__ void __launch_bounds__(BS) k_ws_judge(const double* __restrict__ pa, int na, const double* __restrict__ pb,
+                                                 int nb_, SolveCtl* c, int* host_done, int seq, double tol, int cap,
+                                                 int add) {
+    judge_block(pa, na, pb, nb_, c, host_done, seq, tol, cap, add);
+}
+
+// the judge of every scene of a batch in one launch (block q: scene q)
+__global__ void __launch_bounds__(BS) k_ws_judge_multi(const JudgeScene* __restrict__ js, int add) {
+    const JudgeScene& e = js[blockIdx.x];
+    judge_block(e.pa, e.na, e.pb, e.nb, e.ctl, e.host_done, 0, e.tol, e.cap, add);
+}
+
+// k_ws_restore over every scene of a batch: blocks map to scenes through
+// first_block (binary search); a scene whose correction was kept returns
+__global__ void __launch_bounds__(BS) k_ws_restore_multi(const JudgeScene* __restrict__ js, int n) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (js[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const JudgeScene& e = js[lo];
+    if (e.ctl->ws != 2) return;
+    for (size_t t = (size_t)(b - e.first_block) * BS + threadIdx.x; t < e.n; t += (size_t)e.nblocks * BS)
+        e.p[t] = e.backup[t];
 }
 
 // The predicted judge (re-applications within a solve): the corrected
@@ -1166,6 +1194,13 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred, bool glob) {
 int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
                      bool predicted) {
     if (!s.ws_net) return BLZ_OK;
+    if (int e = warm_start_prepare(s, dt, c)) return e;
+    return warm_start_judge(s, dt, c, host_done, seq, tol, cap, judge, predicted);
+}
+
+// the corrections of one application (the uncorrected pressure saved to the
+// judge planes first), without the judge's sweeps
+int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
     const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
     float* res = s.ws_buf;
     float* mask = s.ws_buf + nn;
@@ -1213,6 +1248,13 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
         }
         if (int e = ws_correct(s, c, coarse ? s.ws_net->out : nullptr, glob)) return e;
     }
+    return BLZ_OK;
+}
+
+// the judge of the application warm_start_prepare enqueued
+int warm_start_judge(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
+                     bool predicted) {
+    float* spare = s.prb + (size_t)s.judge_set * s.Np;
     // the judge: the same sweeps on both, the lower residual continues
     int launches = 0, kind = 0;
     const int keep = s.rbgs_variant;
@@ -1278,6 +1320,20 @@ int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* 
     return BLZ_OK;
 }
 
+// the global correction alone, applied without a judge: the state the
+// network's correction meets in a warm-started solve (for generating its
+// training data); synchronous
+int warm_start_global(Solver& s, float dt) {
+    SolveCtl* c = s.ctl;
+    if (int e = launch_ctl_reset(s, c)) return e;
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
+    hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
+                       scale, ws_r(s), &c->done);
+    if (int e = ws_correct(s, c, nullptr, true)) return e;
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    return BLZ_OK;
+}
+
 int warm_start_input(Solver& s, float dt, float* host_in2) {
     const size_t nn = (size_t)s.Wn * s.Hn * s.Dn;
     if (int e = ws_restrict(s, dt, s.zero_flag)) return e;
@@ -1287,6 +1343,14 @@ int warm_start_input(Solver& s, float dt, float* host_in2) {
     BLZ_HIP(hipGetLastError());
     BLZ_HIP(hipMemcpyAsync(host_in2, out2, 2 * nn * sizeof(float), hipMemcpyDeviceToHost, s.stream));
     BLZ_HIP(hipStreamSynchronize(s.stream));
+    return BLZ_OK;
+}
+
+int launch_ws_judge_multi(hipStream_t st, const JudgeScene* js, int n, int restore_blocks, int add) {
+    hipLaunchKernelGGL(k_ws_judge_multi, dim3((unsigned)n), dim3(BS), 0, st, js, add);
+    BLZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ws_restore_multi, dim3((unsigned)restore_blocks), dim3(BS), 0, st, js, n);
+    BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
 

@@ -140,6 +140,24 @@ struct ResScene {
     int* host_done;
 };
 
+// one scene of a batch's warm-start judge launch: the FP64 residual partials
+// of the corrected (pa) and uncorrected (pb) paths, the solve's control
+// block, and the restore copy (judge planes -> home planes, n floats) with
+// its blocks of the restore launch
+struct JudgeScene {
+    const double* pa;
+    const double* pb;
+    int na, nb;
+    SolveCtl* ctl;
+    int* host_done;
+    double tol;
+    int cap;
+    const float* backup;
+    float* p;
+    size_t n;
+    int first_block, nblocks;
+};
+
 struct Solver {
     blz_settings s{};
     int device = 0;           // HIP device current at creation (owns the stream and all buffers)
@@ -271,6 +289,16 @@ int warm_start_attach(Solver& s, const Net* net);
 int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
                      bool predicted = false);
 int warm_start_input(Solver& s, float dt, float* host_in2);
+// warm_start_apply in two parts: the corrections, then the judge's sweeps,
+// residuals and decision (a batch runs the second part for all scenes at once)
+int warm_start_prepare(Solver& s, float dt, SolveCtl* c);
+int warm_start_judge(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
+                     bool predicted);
+// a batch's judges: decisions (one block per scene) and restores
+int launch_ws_judge_multi(hipStream_t st, const JudgeScene* js, int n, int restore_blocks, int add);
+// residual partials of every grid of a table (no control-block update)
+int launch_residual_partials_multi(hipStream_t st, const ResScene* tab, int n, int blocks);
+int warm_start_global(Solver& s, float dt); // the global correction alone, unjudged (training data)
 int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* before, double* after, int* kept);
 int launch_pressure_update(Solver& s, float dt);
 int launch_advect(Solver& s, float dt);

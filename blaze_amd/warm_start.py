@@ -1,11 +1,14 @@
 """Training of the learned warm start on data the solver generates.
 
 No reference counterpart. A pair is taken at a pressure solve of a running
-scene: the network input is the solver's own coarse input (the residual the
-previous pressure leaves, restricted 4x and normalised, and the solid
-mask: ``Solver.warm_start_input``); the target is the coarse Poisson
-correction for that input, n E - sum_nbr E = r (the same Gauss-Seidel
-solver on the coarse grid, run for many sweeps from zero, mean removed).
+scene: the network input is the solver's own coarse input (the residual
+the pressure leaves after the global correction, restricted and
+normalised, and the solid mask: ``Solver.warm_start_global`` then
+``Solver.warm_start_input``), as a warm-started solve meets it; the target
+is the coarse Poisson correction for that input, n E - sum_nbr E = r on the
+coarse grid with Neumann walls (solved exactly by the orthonormal DCT-II,
+which diagonalises that operator, or, where the blocks are clipped, by the
+library's own Gauss-Seidel on the coarse grid), mean removed, unit RMS.
 Training runs on the GPU through the library's own forward, backward and
 Adam kernels (``Net.train_step``).
 """
@@ -67,12 +70,41 @@ def coarse_target(x, coarse_dims, sweeps):
     return out
 
 
-def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, log=None, points=1, between=50):
+def coarse_target_dct(x, coarse_dims):
+    """The exact solution of n E - sum_nbr E = x[0] on the coarse grid
+    (Neumann graph Laplacian, diagonalised by the orthonormal DCT-II:
+    eigenvalues sum over axes of 2 - 2 cos(pi a / n)), the constant mode set
+    to 0, padded to the network grid and scaled to unit RMS."""
+    import scipy.fft as F
+
+    wc, hc, dc = coarse_dims
+    _, dn, hn, wn = x.shape
+    r = x[0, :dc, :hc, :wc].astype(np.float64)
+    lam = [2.0 - 2.0 * np.cos(np.pi * np.arange(n) / n) for n in (dc, hc, wc)]
+    L = lam[0][:, None, None] + lam[1][None, :, None] + lam[2][None, None, :]
+    c = F.dctn(r, type=2, norm="ortho")
+    L[0, 0, 0] = 1.0
+    c = c / L
+    c[0, 0, 0] = 0.0
+    e = F.idctn(c, type=2, norm="ortho")
+    rms = np.sqrt(np.mean(e * e))
+    if rms > 0:
+        e /= rms
+    out = np.zeros((dn, hn, wn), np.float32)
+    out[:dc, :hc, :wc] = e
+    return out
+
+
+def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, log=None, points=1, between=50,
+                after_global=False):
     """Run `scene` (a .blz path) for `frames` frames stage by stage; at each
     `every`-th frame record (input, target) at `points` moments of the
     pressure solve: before its sweeps and after each further `between`
     sweeps (the residuals a repeated correction meets). `res` overrides the
-    grid (downsampled runs keep the domain and scale dx)."""
+    grid (downsampled runs keep the domain and scale dx, and the particle
+    count scales with the cells). after_global: the global correction is
+    applied before each sample (the input the network meets after it) and
+    the target is the exact coarse solution."""
     s, _, _, gen = parse_scene(scene)
     if res is not None:
         f = s.grid_res[0] / res[0]
@@ -81,6 +113,8 @@ def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, lo
         s.dx = s.dx * f
         if gen is not None:
             gen.pscale = gen.pscale * f
+            if after_global:
+                gen.count = max(1, int(round(gen.count / f ** 3)))
     s.cfl_number = 0.0
     s.warm_start = 0
     if points > 1:
@@ -101,11 +135,15 @@ def scene_pairs(scene, frames, res=None, sweeps=2000, every=1, seed_net=None, lo
         for st in _force_stages(s):
             g.run_stage(st, float(t), dt)
         g.run_stage(STAGE_DIVERGENCE, float(t), dt)
+        whole = all(s.grid_res[a] % cdims[a] == 0 for a in range(3))   # whole blocks: the DCT applies
         for c in range(points):
             if fr % every == 0:
+                if after_global:
+                    g.warm_start_global(dt)
                 x = g.warm_start_input(dt)
                 if np.isfinite(x).all() and np.abs(x[0]).max() > 0:
-                    pairs.append((x, coarse_target(x, cdims, sweeps)))
+                    tgt = coarse_target_dct(x, cdims) if (after_global and whole) else coarse_target(x, cdims, sweeps)
+                    pairs.append((x, tgt))
             if c < points - 1:
                 g.run_stage(STAGE_PRESSURE, float(t), dt)
         g.run_stage(STAGE_PRESSURE, float(t), dt)
@@ -181,15 +219,17 @@ def main(argv=None):
 
     root = os.path.dirname(PKG)
     ap = argparse.ArgumentParser(description="train the warm-start U-Net on solver-generated pairs")
-    ap.add_argument("--frames", type=int, default=100)
-    ap.add_argument("--every", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--sweeps", type=int, default=2000, help="coarse Gauss-Seidel sweeps per target")
     ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--points", type=int, default=3, help="pairs per solve: before the sweeps and after each --between sweeps")
-    ap.add_argument("--between", type=int, default=50)
+    ap.add_argument("--points", type=int, default=6, help="pairs per solve: before the sweeps and after each --between sweeps")
+    ap.add_argument("--between", type=int, default=40)
+    ap.add_argument("--before-global", action="store_true",
+                    help="inputs without the global correction applied first (the network-first order)")
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -201,13 +241,15 @@ def main(argv=None):
     pairs = []
     for path, res in scenes:
         pairs += scene_pairs(path, a.frames, res=res, sweeps=a.sweeps, every=a.every, log=log, points=a.points,
-                             between=a.between)
+                             between=a.between, after_global=not a.before_global)
     log(f"[data] {len(pairs)} pairs in {time.time() - t0:.1f}s")
     rng = np.random.default_rng(a.seed)
     perm = rng.permutation(len(pairs))
     n_test = max(1, len(pairs) // 10)
     test = [pairs[i] for i in perm[:n_test]]
     tr = augment([pairs[i] for i in perm[n_test:]])
+    if os.path.exists(WEIGHTS):
+        log(f"[train] shipped weights' held-out relative error {relative_error(load_net(), test):.3f}")
     net = Net(seed=a.seed)
     log(f"[train] untrained held-out relative error {relative_error(net, test):.3f}")
     train(net, tr, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)

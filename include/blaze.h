@@ -334,6 +334,11 @@ int blz_set_warm_start_parts(blz_solver* h, int parts);
 /* The network input for the current pressure and rhs (for generating
    training data): in2 = 2 x net_dims values; dims reported in (W, H, D). */
 int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], int coarse_dims[3]);
+/* The global correction alone, applied to the current pressure without a
+   judge: the residual the network's correction meets in a warm-started
+   solve (blz_warm_start_input afterwards gives that input; for generating
+   training data). */
+int blz_warm_start_global(blz_solver* h, float dt);
 /* The correction step with a caller-given coarse prediction (network grid,
    x fastest) in place of the network's, scaled to the least error energy and
    followed by the global correction when enabled: FP64 residual before and

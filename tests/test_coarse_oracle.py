@@ -103,3 +103,23 @@ def test_block_size_rule():
     assert C.block_size((128, 64, 32)) == 8
     assert C.block_size((16, 9, 3)) == 1
     assert C.block_size((17, 4, 4)) == 2
+
+
+def test_dct_training_target_solves_the_coarse_problem():
+    """The network's training target (blaze_amd/warm_start.py
+    coarse_target_dct): on the coarse grid, n E - sum_nbr E equals the
+    input's residual channel minus its mean, up to the unit-RMS scale, for
+    uneven grid sizes and a padded network grid."""
+    W = pytest.importorskip("blaze_amd.warm_start")
+    rng = np.random.default_rng(4)
+    for (wc, hc, dc), (wn, hn, dn) in (((8, 6, 5), (8, 8, 8)), ((16, 16, 16), (16, 16, 16))):
+        x = np.zeros((2, dn, hn, wn), np.float32)
+        x[0, :dc, :hc, :wc] = rng.standard_normal((dc, hc, wc))
+        e = W.coarse_target_dct(x, (wc, hc, dc)).astype(np.float64)[:dc, :hc, :wc]
+        lap = C.laplacian(e)
+        r = x[0, :dc, :hc, :wc].astype(np.float64)
+        r = r - r.mean()
+        k = float((lap * r).sum() / (lap * lap).sum())
+        assert k > 0
+        assert np.abs(k * lap - r).max() < 1e-4 * np.abs(r).max()
+        assert abs(e.mean()) < 1e-6 and abs(np.sqrt((e * e).mean()) - 1.0) < 1e-5
