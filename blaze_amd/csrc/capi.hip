@@ -507,17 +507,26 @@ int Solver::add_source(const float* part8, int n) {
         if (n > 0)
             BLZ_HIP(hipMemcpyAsync(part, part8, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice, stream));
         npart = n;
+        p2g_bound = p2g_entry_bound(part8, n, dx, smoke ? 5 : 4);
     }
-    // particle-to-grid synchronises internally (it sizes its scratch from a
-    // device count), so this span is resolved on its own
+    // the call returns with the splat done (its span resolved on its own)
     std::vector<Span> keep;
     keep.swap(spans);
     const size_t used = ev_used;
     int e0 = -1;
     if (int e = span_begin(&e0)) return e;
+    p2g_overflow = nullptr;
     if (int e = launch_p2g(*this)) return e;
     if (int e = span_end(SPAN_SOURCE, e0, 0)) return e;
     BLZ_HIP(hipStreamSynchronize(stream));
+    if (p2g_overflow) {
+        unsigned ov = 0;
+        BLZ_HIP(hipMemcpy(&ov, p2g_overflow, sizeof(unsigned), hipMemcpyDeviceToHost));
+        if (ov) {
+            set_error("particle-to-grid: footprint entries exceeded their bound");
+            return BLZ_E_INVALID;
+        }
+    }
     st.source_ms = 0.0;
     if (int e = resolve_spans()) return e;
     spans.swap(keep);
@@ -553,6 +562,7 @@ int Solver::stage_source(const float* part8, int n) {
     }
     if (n > 0) {
         std::memcpy(stage_host[q], part8, (size_t)n * 8 * sizeof(float));
+        stage_bound[q] = p2g_entry_bound(part8, n, dx, smoke ? 5 : 4);
         BLZ_HIP(hipMemcpyAsync(stage_dev[q], stage_host[q], (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice,
                                copy_stream));
     }
@@ -585,6 +595,7 @@ int Solver::add_source_staged() {
     if (n > 0)
         BLZ_HIP(hipMemcpyAsync(part, stage_dev[q], (size_t)n * 8 * sizeof(float), hipMemcpyDeviceToDevice, stream));
     npart = n;
+    p2g_bound = n > 0 ? stage_bound[q] : 0;
     stage_ready = -1;
     return add_source(nullptr, 0);
 }

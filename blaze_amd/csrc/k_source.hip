@@ -7,14 +7,16 @@
 // |back| > 1e-5).
 //
 // Deterministic gather: every (particle, channel) emits (cell key, value)
-// entries in particle order; a stable radix sort by cell key keeps
-// ascending particle order inside each cell; one thread per cell then sums
-// its run sequentially from +0 and overwrites the front value when
-// |sum| > 1e-5. Cells that receive nothing keep their front value, exactly
+// entries in particle order (offsets from an exclusive scan of the counts);
+// a stable radix sort by cell key (k_sort.hip) keeps ascending particle
+// order inside each cell; one thread per cell then sums its run
+// sequentially from +0 and overwrites the front value when |sum| > 1e-5.
+// The entry buffers are sized from a host-side upper bound of the
+// footprints (p2g_entry_bound), so no count is read back mid-pipeline. Cells that receive nothing keep their front value, exactly
 // as the reference's copy-where-nonzero does. The reference's back buffers
 // after addSource are dead (every later reader overwrites them first), so
 // no full-grid clear or copy pass is needed.
-#include <hipcub/hipcub.hpp>
+#include <cmath>
 
 #include "solver.h"
 
@@ -96,7 +98,6 @@ __device__ __forceinline__ void footprint(const float* r, const PGrid& g, const 
 __global__ void __launch_bounds__(BS) k_p2g_count(const float* __restrict__ part, int n, P2G P,
                                                   unsigned* __restrict__ counts) {
     const int t = blockIdx.x * BS + threadIdx.x;
-    if (t == n * P.nch) counts[t] = 0; // terminator, so the exclusive scan yields the total
     if (t >= n * P.nch) return;
     const int q = t / P.nch, ch = t % P.nch;
     const float* r = part + (size_t)q * 8;
@@ -107,8 +108,8 @@ __global__ void __launch_bounds__(BS) k_p2g_count(const float* __restrict__ part
 }
 
 __global__ void __launch_bounds__(BS) k_p2g_emit(const float* __restrict__ part, int n, P2G P,
-                                                 const unsigned* __restrict__ offs, unsigned* __restrict__ keys,
-                                                 float* __restrict__ vals) {
+                                                 const unsigned* __restrict__ offs, unsigned cap,
+                                                 unsigned* __restrict__ keys, float* __restrict__ vals) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n * P.nch) return;
     const int q = t / P.nch, ch = t % P.nch;
@@ -118,15 +119,28 @@ __global__ void __launch_bounds__(BS) k_p2g_emit(const float* __restrict__ part,
     unsigned o = offs[t];
     const unsigned base = P.g[ch].base;
     footprint(r, P.g[ch], P, val, [&](int cell, float v) {
-        keys[o] = base + (unsigned)cell;
-        vals[o] = v;
+        if (o < cap) { // the host bound makes this always true (k_p2g_total flags the contrary)
+            keys[o] = base + (unsigned)cell;
+            vals[o] = v;
+        }
         ++o;
     });
 }
 
+// the entry count the sort and the reduction use: the total of the scan,
+// clamped to the buffers (a total above them, which the host bound rules
+// out, is flagged and the splat reported as failed)
+__global__ void k_p2g_total(const unsigned* __restrict__ total, unsigned cap, unsigned* __restrict__ n_out,
+                            unsigned* __restrict__ overflow) {
+    const unsigned t = *total;
+    *n_out = t < cap ? t : cap;
+    *overflow = t > cap ? 1u : 0u;
+}
+
 // one thread per run of equal keys: sequential sum in particle order
 __global__ void __launch_bounds__(BS) k_p2g_reduce(const unsigned* __restrict__ keys, const float* __restrict__ vals,
-                                                   unsigned total, P2G P) {
+                                                   const unsigned* __restrict__ total_dev, P2G P) {
+    const unsigned total = *total_dev;
     const unsigned t = blockIdx.x * BS + threadIdx.x;
     if (t >= total) return;
     const unsigned key = keys[t];
@@ -157,12 +171,19 @@ int ensure(Solver& s, void** p, size_t* cap, size_t need) {
 }
 
 struct Scratch {
-    void* counts = nullptr; size_t counts_cap = 0;
-    void* entries = nullptr; size_t entries_cap = 0;
-    void* temp = nullptr; size_t temp_cap = 0;
+    void* counts = nullptr; size_t counts_cap = 0;   // counts, offsets, scan scratch, totals
+    void* entries = nullptr; size_t entries_cap = 0; // keys, values and their sorted copies
+    void* sort = nullptr; size_t sort_cap = 0;       // radix sort scratch
 };
 
 } // namespace
+
+size_t scan_scratch_words(size_t n_max);
+int scan_exclusive(hipStream_t st, const unsigned* in, unsigned* out, const unsigned* n_dev, size_t n_max,
+                   unsigned* scratch);
+size_t radix_scratch_words(size_t n_max);
+int radix_sort_pairs(hipStream_t st, unsigned* keys, float* vals, unsigned* keys_alt, float* vals_alt,
+                     const unsigned* n_dev, size_t n_max, int end_bit, unsigned* scratch, bool* in_alt);
 
 // scratch buffers hang off the solver through p2g_buf
 static Scratch& scratch_of(Solver& s) {
@@ -175,9 +196,26 @@ void free_p2g_scratch(Solver& s) {
     Scratch* sc = (Scratch*)s.p2g_buf;
     if (sc->counts) (void)hipFree(sc->counts);
     if (sc->entries) (void)hipFree(sc->entries);
-    if (sc->temp) (void)hipFree(sc->temp);
+    if (sc->sort) (void)hipFree(sc->sort);
     delete sc;
     s.p2g_buf = nullptr;
+}
+
+// Upper bound of the (cell, value) entries of a particle set, on the host:
+// a particle that passes the pscale > dx/2 gate (:83) covers at most
+// floor(pscale/dx + 0.01) + 2 cells per axis of a channel's grid (the
+// footprint box of sphereToGrid, src/Grid.h:315-324, spans pscale/dx cells
+// up to float rounding, plus the truncation of both ends), in each of the
+// nch channels.
+size_t p2g_entry_bound(const float* part8, int n, float dx, int nch) {
+    size_t b = 0;
+    for (int q = 0; q < n; ++q) {
+        const float ps = part8[(size_t)q * 8 + 6];
+        if (!(ps > dx / 2.0f)) continue;
+        const double w = std::floor((double)ps / (double)dx + 0.01) + 2.0;
+        b += (size_t)(w * w * w) * (size_t)nch;
+    }
+    return b;
 }
 
 int launch_p2g(Solver& s) {
@@ -201,42 +239,45 @@ int launch_p2g(Solver& s) {
     const unsigned key_end = base;
     int end_bit = 1;
     while (end_bit < 32 && (1ull << end_bit) < (unsigned long long)key_end) ++end_bit;
-
-    const size_t m = (size_t)n * P.nch;
-    if (int e = ensure(s, &sc.counts, &sc.counts_cap, 2 * (m + 1) * sizeof(unsigned))) return e;
-    unsigned* counts = (unsigned*)sc.counts;
-    unsigned* offs = counts + (m + 1);
-    hipLaunchKernelGGL(k_p2g_count, dim3(blocks_for(m + 1, BS)), dim3(BS), 0, s.stream, s.part, n, P, counts);
-    BLZ_HIP(hipGetLastError());
-    size_t tb = 0;
-    BLZ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts, offs, (int)(m + 1), s.stream));
-    if (int e = ensure(s, &sc.temp, &sc.temp_cap, tb)) return e;
-    BLZ_HIP(hipcub::DeviceScan::ExclusiveSum(sc.temp, tb, counts, offs, (int)(m + 1), s.stream));
-    unsigned total = 0;
-    BLZ_HIP(hipMemcpyAsync(&total, offs + m, sizeof(unsigned), hipMemcpyDeviceToHost, s.stream));
-    BLZ_HIP(hipStreamSynchronize(s.stream));
-    if (total == 0) return BLZ_OK;
-    if (total > 0x7fffffffu) {
-        set_error("particle-to-grid footprint exceeds 2^31 entries");
+    const size_t bound = s.p2g_bound;
+    if (bound == 0) return BLZ_OK; // no particle passes the pscale gate
+    if (bound > 0x7fffffffu) {
+        set_error("particle-to-grid footprint bound exceeds 2^31 entries");
         return BLZ_E_INVALID;
     }
-    if (int e = ensure(s, &sc.entries, &sc.entries_cap, (size_t)total * 16)) return e;
+
+    // counts (m), offsets (m + 1, the total last), scan scratch, the clamped
+    // total and the overflow flag
+    const size_t m = (size_t)n * P.nch;
+    const size_t scw = scan_scratch_words(m);
+    if (int e = ensure(s, &sc.counts, &sc.counts_cap, (2 * m + 1 + scw + 2) * sizeof(unsigned))) return e;
+    unsigned* counts = (unsigned*)sc.counts;
+    unsigned* offs = counts + m;
+    unsigned* scan_tmp = offs + m + 1;
+    unsigned* n_eff = scan_tmp + scw;
+    unsigned* overflow = n_eff + 1;
+    if (int e = ensure(s, &sc.entries, &sc.entries_cap, bound * 16)) return e;
+    if (int e = ensure(s, &sc.sort, &sc.sort_cap, radix_scratch_words(bound) * sizeof(unsigned))) return e;
     unsigned* keys_in = (unsigned*)sc.entries;
-    unsigned* keys_out = keys_in + total;
-    float* vals_in = (float*)(keys_out + total);
-    float* vals_out = vals_in + total;
-    hipLaunchKernelGGL(k_p2g_emit, dim3(blocks_for(m, BS)), dim3(BS), 0, s.stream, s.part, n, P, offs, keys_in,
-                       vals_in);
+    unsigned* keys_out = keys_in + bound;
+    float* vals_in = (float*)(keys_out + bound);
+    float* vals_out = vals_in + bound;
+
+    hipLaunchKernelGGL(k_p2g_count, dim3(blocks_for(m, BS)), dim3(BS), 0, s.stream, s.part, n, P, counts);
     BLZ_HIP(hipGetLastError());
-    tb = 0;
-    BLZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys_in, keys_out, vals_in, vals_out, (int)total, 0,
-                                               end_bit, s.stream));
-    if (int e = ensure(s, &sc.temp, &sc.temp_cap, tb)) return e;
-    BLZ_HIP(hipcub::DeviceRadixSort::SortPairs(sc.temp, tb, keys_in, keys_out, vals_in, vals_out, (int)total, 0,
-                                               end_bit, s.stream));
-    hipLaunchKernelGGL(k_p2g_reduce, dim3(blocks_for(total, BS)), dim3(BS), 0, s.stream, keys_out, vals_out, total,
-                       P);
+    if (int e = scan_exclusive(s.stream, counts, offs, nullptr, m, scan_tmp)) return e;
+    hipLaunchKernelGGL(k_p2g_total, dim3(1), dim3(1), 0, s.stream, offs + m, (unsigned)bound, n_eff, overflow);
+    hipLaunchKernelGGL(k_p2g_emit, dim3(blocks_for(m, BS)), dim3(BS), 0, s.stream, s.part, n, P, offs,
+                       (unsigned)bound, keys_in, vals_in);
     BLZ_HIP(hipGetLastError());
+    bool alt = false;
+    if (int e = radix_sort_pairs(s.stream, keys_in, vals_in, keys_out, vals_out, n_eff, bound, end_bit,
+                                 (unsigned*)sc.sort, &alt))
+        return e;
+    hipLaunchKernelGGL(k_p2g_reduce, dim3(blocks_for(bound, BS)), dim3(BS), 0, s.stream, alt ? keys_out : keys_in,
+                       alt ? vals_out : vals_in, n_eff, P);
+    BLZ_HIP(hipGetLastError());
+    s.p2g_overflow = overflow;
     return BLZ_OK;
 }
 

@@ -207,6 +207,9 @@ struct Solver {
     // particle-to-grid scratch (grow-only)
     void* p2g_buf = nullptr;
     size_t p2g_bytes = 0;
+    size_t p2g_bound = 0;              // upper bound of the splat's entries (host, from the particle set)
+    size_t stage_bound[2] = {0, 0};    // the same for each staged slot
+    const unsigned* p2g_overflow = nullptr; // device flag of the last splat: its entries exceeded the bound
     // stats / timing: spans of device work bracketed by pooled events,
     // resolved after the stream has been synchronised (no extra waits)
     blz_stats st{};
@@ -303,6 +306,7 @@ int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* 
 int launch_pressure_update(Solver& s, float dt);
 int launch_advect(Solver& s, float dt);
 int launch_p2g(Solver& s);
+size_t p2g_entry_bound(const float* part8, int n, float dx, int nch);
 int launch_max_abs_velocity(Solver& s, double* dev_out);
 int launch_clear_doubles(Solver& s, double* dev, int n);
 int launch_p_to_natural(Solver& s, float* out, const RB& src);

@@ -323,6 +323,15 @@ def test_frames_plume64_bitwise():
     run_frames(s, gen, 12, 12, check_every=3)
 
 
+def test_frames_plume64_all_50_frames_bitwise():
+    """The 64^3 plume for its whole length: 50 frames (20k particles
+    re-splatted every frame, 100 sweeps per solve, CFL substeps), sweep and
+    substep counts equal every frame, every field bit-exact every 10th."""
+    s, gen = scene_settings("plume64.blz")
+    print("[frames] plume64, 50 frames", flush=True)
+    run_frames(s, gen, 50, 50, check_every=10)
+
+
 def test_frames_plume128_features_bitwise():
     """Every force of the 128^3 plume (vorticity, curl noise, wind, open x
     sides and top) on a 48^3 grid, 8 frames bit-exact."""
