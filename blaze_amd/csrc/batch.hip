@@ -45,7 +45,10 @@ struct Batch {
     ResScene* rsw = nullptr;   // n: home planes, partials at red + 8
     ResScene* rsj = nullptr;   // n: judge planes, partials at red + 8 + RESIDUAL_BLOCKS
     JudgeScene* js = nullptr;  // n
+    FusedScene* fsj2 = nullptr; // 2n: fsj gated on the predicted judge's rejection (skip flag)
+    ResScene* rsj2 = nullptr;   // n: rsj gated the same way
     std::vector<int> warm_q;   // scenes of these tables (empty: per-scene judges)
+    bool multi_reapply = false; // re-applications also as multi-grid launches (one common interval and cap)
     int w_fs_blocks = 0, w_rs_blocks = 0, w_restore_blocks = 0;
     std::vector<FusedScene> fs_h;
     std::vector<ResScene> rs_h;
@@ -83,7 +86,7 @@ Batch::~Batch() {
     if (off_dev) (void)hipFree(off_dev);
     if (fs) (void)hipFree(fs);
     if (rs) (void)hipFree(rs);
-    for (void* q : {(void*)fsw, (void*)fsj, (void*)rsw, (void*)rsj, (void*)js})
+    for (void* q : {(void*)fsw, (void*)fsj, (void*)rsw, (void*)rsj, (void*)js, (void*)fsj2, (void*)rsj2})
         if (q) (void)hipFree(q);
     for (blz_solver* h : sc) delete h;
     if (root) (void)hipStreamDestroy(root);
@@ -111,7 +114,8 @@ int Batch::init(const blz_settings* st, int n) {
     if (hipMalloc(&off_dev, 2 * n * sizeof(float)) != hipSuccess || hipMalloc(&fs, 2 * n * sizeof(FusedScene)) != hipSuccess ||
         hipMalloc(&rs, n * sizeof(ResScene)) != hipSuccess || hipMalloc(&fsw, 2 * n * sizeof(FusedScene)) != hipSuccess ||
         hipMalloc(&fsj, 2 * n * sizeof(FusedScene)) != hipSuccess || hipMalloc(&rsw, n * sizeof(ResScene)) != hipSuccess ||
-        hipMalloc(&rsj, n * sizeof(ResScene)) != hipSuccess || hipMalloc(&js, n * sizeof(JudgeScene)) != hipSuccess) {
+        hipMalloc(&rsj, n * sizeof(ResScene)) != hipSuccess || hipMalloc(&js, n * sizeof(JudgeScene)) != hipSuccess ||
+        hipMalloc(&fsj2, 2 * n * sizeof(FusedScene)) != hipSuccess || hipMalloc(&rsj2, n * sizeof(ResScene)) != hipSuccess) {
         set_error("hipMalloc failed for batch tables");
         return BLZ_E_NOMEM;
     }
@@ -184,6 +188,7 @@ int Batch::build_tables(int v) {
         r.cap = s.s.max_iter_solve;
         r.seq = 0;
         r.host_done = s.done_host_dev;
+        r.gate = &s.ctl[0].done;
         rb += r.nblocks;
     }
     fs_blocks = fb;
@@ -215,10 +220,18 @@ int Batch::build_judge_tables() {
     }
     if (!ok) warm_q.clear();
     const int nw = (int)warm_q.size();
+    multi_reapply = false;
     if (!nw) return BLZ_OK;
+    multi_reapply = true;
+    for (int q : warm_q) {
+        const Solver& s = sc[q]->s;
+        const Solver& s0 = sc[warm_q[0]]->s;
+        multi_reapply = multi_reapply && s.s.warm_start >= 2 && s.s.warm_start == s0.s.warm_start &&
+                        s.s.max_iter_solve == s0.s.max_iter_solve;
+    }
     const int wpb = wpb_of(variant);
-    std::vector<FusedScene> hw(2 * nw), hj(2 * nw);
-    std::vector<ResScene> rw(nw), rj(nw);
+    std::vector<FusedScene> hw(2 * nw), hj(2 * nw), hj2(2 * nw);
+    std::vector<ResScene> rw(nw), rj(nw), rj2(nw);
     std::vector<JudgeScene> jh(nw);
     int fb = 0, rb = 0, cb = 0;
     for (int w = 0; w < nw; ++w) {
@@ -237,6 +250,8 @@ int Batch::build_judge_tables() {
             j.dstR = dst;
             j.dstB = dst + pb;
             hj[h * nw + w] = j;
+            j.stop = &s.ctl[0].skip_spare;
+            hj2[h * nw + w] = j;
         }
         fb += (fs_h[q].nwaves + wpb - 1) / wpb;
         ResScene a = rs_h[q];
@@ -247,6 +262,8 @@ int Batch::build_judge_tables() {
         b.p.c[1] = s.prb + 7 * s.Np;
         b.partial = s.red + 8 + RESIDUAL_BLOCKS;
         rj[w] = b;
+        b.gate = &s.ctl[0].skip_spare;
+        rj2[w] = b;
         rb += a.nblocks;
         JudgeScene& J = jh[w];
         J.pa = s.red + 8;
@@ -271,6 +288,8 @@ int Batch::build_judge_tables() {
     BLZ_HIP(hipMemcpy(rsw, rw.data(), nw * sizeof(ResScene), hipMemcpyHostToDevice));
     BLZ_HIP(hipMemcpy(rsj, rj.data(), nw * sizeof(ResScene), hipMemcpyHostToDevice));
     BLZ_HIP(hipMemcpy(js, jh.data(), nw * sizeof(JudgeScene), hipMemcpyHostToDevice));
+    BLZ_HIP(hipMemcpy(fsj2, hj2.data(), 2 * nw * sizeof(FusedScene), hipMemcpyHostToDevice));
+    BLZ_HIP(hipMemcpy(rsj2, rj2.data(), nw * sizeof(ResScene), hipMemcpyHostToDevice));
     return BLZ_OK;
 }
 
@@ -361,6 +380,38 @@ int Batch::enqueue_step(hipStream_t st) {
             any |= again[q] != 0;
         }
         if (!any) continue;
+        bool together = multi_reapply;
+        for (int q = 0; q < n && together; ++q)
+            together = (again[q] != 0) == (std::find(warm_q.begin(), warm_q.end(), q) != warm_q.end());
+        if (together) {
+            // every warm-started scene re-applies here (one common interval
+            // and cap): the corrections per scene, then the predicted judges
+            // of all of them as multi-grid launches, as warm_start_judge does
+            // per scene
+            const int nw = (int)warm_q.size();
+            BLZ_HIP(hipEventRecord(fork, st));
+            for (int q : warm_q) {
+                Solver& s = sc[q]->s;
+                BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
+                if (int e = warm_start_prepare(s, s.s.dt, s.ctl)) return e;
+                BLZ_HIP(hipEventRecord(join[q], s.stream));
+                BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+            }
+            for (int l = 0; l < every / 2; ++l) {
+                if (int e = launch_fused_multi(st, variant, fsw, nw, w_fs_blocks)) return e;
+                if (int e = launch_fused_multi(st, variant, fsw + nw, nw, w_fs_blocks)) return e;
+            }
+            if (int e = launch_residual_partials_multi(st, rsw, nw, w_rs_blocks)) return e;
+            if (int e = launch_ws_judge_pred_multi(st, js, nw, every)) return e;
+            for (int l = 0; l < every / 2; ++l) {
+                if (int e = launch_fused_multi(st, variant, fsj2, nw, w_fs_blocks)) return e;
+                if (int e = launch_fused_multi(st, variant, fsj2 + nw, nw, w_fs_blocks)) return e;
+            }
+            if (int e = launch_residual_partials_multi(st, rsj2, nw, w_rs_blocks)) return e;
+            if (int e = launch_ws_judge_fin_multi(st, js, nw, w_restore_blocks, every)) return e;
+            for (int q : warm_q) it[q] += every;
+            continue;
+        }
         BLZ_HIP(hipEventRecord(fork, st));
         for (int q = 0; q < n; ++q) {
             Solver& s = sc[q]->s;

@@ -1066,7 +1066,7 @@ __global__ void __launch_bounds__(BS) k_residual_multi(const ResScene* __restric
         else hi = mid - 1;
     }
     const ResScene& e = sc[lo];
-    if (e.ctl->done) return;
+    if (*e.gate) return;
     residual_block(e.p, e.rhs, e.L, e.W, e.H, e.D, e.c, make_int4(e.cb[0], e.cb[1], e.cb[2], e.cb[3]),
                    make_int2(e.cb[4], e.cb[5]), e.partial, b - e.first_block, e.nblocks);
 }

@@ -648,8 +648,8 @@ __global__ void __launch_bounds__(BS) k_ws_restore_multi(const JudgeScene* __res
 // the uncorrected sweeps are skipped. Rejected: the uncorrected pressure
 // gets the same batch of sweeps (k_ws_judge_fin takes its residual), so the
 // solve continues exactly as without the correction.
-__global__ void __launch_bounds__(BS) k_ws_judge_pred(const double* __restrict__ pa, int na, SolveCtl* c,
-                                                      int* host_done, int seq, double tol, int cap, int add) {
+__device__ __forceinline__ void judge_pred_block(const double* __restrict__ pa, int na, SolveCtl* c, int* host_done,
+                                                 int seq, double tol, int cap, int add) {
     if (c->done) return;
     const double ra = block_max(pa, na);
     if (threadIdx.x == 0) {
@@ -679,9 +679,14 @@ __global__ void __launch_bounds__(BS) k_ws_judge_pred(const double* __restrict__
     }
 }
 
+__global__ void __launch_bounds__(BS) k_ws_judge_pred(const double* __restrict__ pa, int na, SolveCtl* c,
+                                                      int* host_done, int seq, double tol, int cap, int add) {
+    judge_pred_block(pa, na, c, host_done, seq, tol, cap, add);
+}
+
 // after a rejection by the predicted judge: the uncorrected path's residual
-__global__ void __launch_bounds__(BS) k_ws_judge_fin(const double* __restrict__ pb, int nb_, SolveCtl* c,
-                                                     int* host_done, int seq, double tol, int cap, int add) {
+__device__ __forceinline__ void judge_fin_block(const double* __restrict__ pb, int nb_, SolveCtl* c, int* host_done,
+                                                int seq, double tol, int cap, int add) {
     if (c->done || c->ws != 2) return;
     const double rb = block_max(pb, nb_);
     if (threadIdx.x == 0) {
@@ -697,6 +702,22 @@ __global__ void __launch_bounds__(BS) k_ws_judge_fin(const double* __restrict__ 
             __threadfence_system();
         }
     }
+}
+
+__global__ void __launch_bounds__(BS) k_ws_judge_fin(const double* __restrict__ pb, int nb_, SolveCtl* c,
+                                                     int* host_done, int seq, double tol, int cap, int add) {
+    judge_fin_block(pb, nb_, c, host_done, seq, tol, cap, add);
+}
+
+// the predicted judges of a batch's scenes (block q: scene q)
+__global__ void __launch_bounds__(BS) k_ws_judge_pred_multi(const JudgeScene* __restrict__ js, int add) {
+    const JudgeScene& e = js[blockIdx.x];
+    judge_pred_block(e.pa, e.na, e.ctl, e.host_done, 0, e.tol, e.cap, add);
+}
+
+__global__ void __launch_bounds__(BS) k_ws_judge_fin_multi(const JudgeScene* __restrict__ js, int add) {
+    const JudgeScene& e = js[blockIdx.x];
+    judge_fin_block(e.pb, e.nb, e.ctl, e.host_done, 0, e.tol, e.cap, add);
 }
 
 // the "given prediction" path keeps the immediate comparison (no sweeps)
@@ -1348,6 +1369,20 @@ int warm_start_input(Solver& s, float dt, float* host_in2) {
 
 int launch_ws_judge_multi(hipStream_t st, const JudgeScene* js, int n, int restore_blocks, int add) {
     hipLaunchKernelGGL(k_ws_judge_multi, dim3((unsigned)n), dim3(BS), 0, st, js, add);
+    BLZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ws_restore_multi, dim3((unsigned)restore_blocks), dim3(BS), 0, st, js, n);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int launch_ws_judge_pred_multi(hipStream_t st, const JudgeScene* js, int n, int add) {
+    hipLaunchKernelGGL(k_ws_judge_pred_multi, dim3((unsigned)n), dim3(BS), 0, st, js, add);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int launch_ws_judge_fin_multi(hipStream_t st, const JudgeScene* js, int n, int restore_blocks, int add) {
+    hipLaunchKernelGGL(k_ws_judge_fin_multi, dim3((unsigned)n), dim3(BS), 0, st, js, add);
     BLZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ws_restore_multi, dim3((unsigned)restore_blocks), dim3(BS), 0, st, js, n);
     BLZ_HIP(hipGetLastError());

@@ -138,6 +138,7 @@ struct ResScene {
     double tol;
     int cap, seq;
     int* host_done;
+    const int* gate; // the launch skips this grid while *gate != 0 (the solve's done flag, or a judge's skip flag)
 };
 
 // one scene of a batch's warm-start judge launch: the FP64 residual partials
@@ -299,6 +300,11 @@ int warm_start_judge(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
                      bool predicted);
 // a batch's judges: decisions (one block per scene) and restores
 int launch_ws_judge_multi(hipStream_t st, const JudgeScene* js, int n, int restore_blocks, int add);
+// a batch's predicted judges (re-applications), in three launches: the
+// decision on the corrected path, then (after the uncorrected path's
+// sweeps and residuals, gated on the decision) its residual, and restores
+int launch_ws_judge_pred_multi(hipStream_t st, const JudgeScene* js, int n, int add);
+int launch_ws_judge_fin_multi(hipStream_t st, const JudgeScene* js, int n, int restore_blocks, int add);
 // residual partials of every grid of a table (no control-block update)
 int launch_residual_partials_multi(hipStream_t st, const ResScene* tab, int n, int blocks);
 int warm_start_global(Solver& s, float dt); // the global correction alone, unjudged (training data)
