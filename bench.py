@@ -650,8 +650,8 @@ def batched(a):
 
     net = None if a.no_warm_start else WS.load_net()
 
-    def run(warm, cap, steps, closed=False):
-        sets, parts = batched_scenes(a.scenes, warm_start=warm, cap=cap, closed=closed, warm_every=a.warm_every)
+    def run(warm, cap, steps):
+        sets, parts = batched_scenes(a.scenes, warm_start=warm, cap=cap, warm_every=a.warm_every)
         b = B.Batch(sets)
         if warm:
             b.set_warm_start(net)
@@ -711,11 +711,11 @@ def batched(a):
     print(json.dumps(out), flush=True)
 
 
-def batched_scenes(n=64, warm_start=True, cap=100, closed=False, warm_every=1):
+def batched_scenes(n=64, warm_start=True, cap=100, warm_every=1):
     """The batched workload: n scenes, grids N^3 with N uniform in [32, 128]
     rounded to multiples of 8 (seed 5), source radius U[0.03, 0.1], buoyancy
-    U[0.5, 2.0], solid walls with an open top (closed: a closed top too),
-    RBGS to 1e-4 with a cap of `cap` sweeps, the warm start (when on) before
+    U[0.5, 2.0], solid walls with an open top, RBGS to 1e-4 with a cap of
+    `cap` sweeps, the warm start (when on) before
     the sweeps and every `warm_every` sweeps (1: before them only); returns
     the settings and the particles of each scene."""
     import numpy as np
@@ -736,7 +736,7 @@ def batched_scenes(n=64, warm_start=True, cap=100, closed=False, warm_every=1):
         s.density = 1.0
         s.gravity = -9.8
         s.buoyancy = float(rng.uniform(0.5, 2.0))
-        for ax, v in enumerate((1, 1, 1, 1 if closed else 0, 1, 1)):
+        for ax, v in enumerate((1, 1, 1, 0, 1, 1)):
             s.closed_bounds[ax] = v
         s.residual_tol = 1e-4
         s.max_iter_solve = cap
