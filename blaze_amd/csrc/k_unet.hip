@@ -59,42 +59,87 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
 // the MFMAs. D: lane l holds output channel l % 16 of voxels 4 (l / 16)
 // .. +3. Optional epilogue: leaky ReLU, or the leaky ReLU's derivative at
 // the activation M (1 where M > 0, NET_LEAK elsewhere).
-template <int CIN>
+//
+// Split reduction (KS = 2 or 4) for the small grids of the lower levels,
+// where one wave per tile leaves most SIMDs idle and each wave's chain of
+// dependent MFMAs and loads is long: the KS waves of a tile group take
+// consecutive ranges of the chunks, and the first adds the others' FP32
+// tiles from LDS in split order (fixed, so every run gives the same bits).
+// A workgroup holds 4 / KS tiles.
+template <int CIN, int KS>
 __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, int Cx, int xoff,
                                                    const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
                                                    int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
                                                    int relu, Geo g, long long nvox) {
     constexpr int NCH = (27 * CIN + 31) / 32;
+    constexpr int NPER = (NCH + KS - 1) / KS; // chunks per split
     const int c0 = blockIdx.y * 16;
     const int lane = threadIdx.x & 63;
-    const long long v0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-    if (v0 >= nvox) return;
+    // wave index and split as scalars: the chunk-range tests below are then
+    // scalar branches, so every MFMA runs with the whole wave active (an
+    // MFMA does not honour a partial or empty EXEC mask)
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), split = wv % KS;
+    const long long v0 = ((long long)blockIdx.x * (4 / KS) + wv / KS) * 16;
+    // no early return: the waves of a workgroup meet at the split reduction
+    // (a tile past the end computes zeros from clamped addresses and stores
+    // nothing)
     const int li = lane & 15, kg = lane >> 4;
     const long long va = v0 + li;
     int b, z, y, x;
     vox_coords(va < nvox ? va : nvox - 1, g, b, z, y, x);
     const bool vok = va < nvox;
     f4m acc = f4m{0.f, 0.f, 0.f, 0.f};
-    const long long HW = (long long)g.H * g.W;
+    // element offsets fit in 32 bits (checked by the launcher)
+    const int HW = g.H * g.W;
+    const int vai = (int)(va < nvox ? va : 0);
     const s8v zero = {0, 0, 0, 0, 0, 0, 0, 0};
     const bf16* wrow = Wt + (size_t)(c0 + li) * CIN; // tap 0 of this lane's output channel
+    // chunks in batches of CB: all loads of a batch are issued before its
+    // first MFMA, so CB chunks' loads are in flight together
+    constexpr int CB = 8;
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        const int k0 = 32 * c + 8 * kg;
-        const int tap = k0 / CIN, ci = k0 - tap * CIN;
-        const bool real = tap < 27;
-        const int tp = real ? tap : 26;
-        const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
-        const bool inb = real && vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
-        const long long vv = va + dz * HW + dy * g.W + dx;
-        const s8v ld = *(const s8v*)(X + (inb ? vv : 0) * Cx + xoff + (real ? ci : 0));
-        const s8v wl = *(const s8v*)(wrow + (size_t)tp * Cout * CIN + (real ? ci : 0));
-        const s8v a = inb ? ld : zero;
-        const s8v bw = real ? wl : zero;
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, bw), acc,
-                                                       0, 0, 0);
+    for (int cb = 0; cb < NPER; cb += CB) {
+        s8v av[CB], bv[CB];
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+            const int c = split * NPER + cb + u;
+            if (cb + u >= NPER || c >= NCH) continue; // wave-uniform
+            const int k0 = 32 * c + 8 * kg;
+            const int tap = k0 / CIN, ci = k0 - tap * CIN;
+            const bool real = tap < 27;
+            const int tp = real ? tap : 26;
+            const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+            const int zz = z + dz, yy = y + dy, xx = x + dx;
+            const bool inb = real && vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+            const int vv = vai + dz * HW + dy * g.W + dx;
+            const s8v ld = *(const s8v*)(X + (inb ? vv : 0) * Cx + xoff + (real ? ci : 0));
+            const s8v wl = *(const s8v*)(wrow + tp * Cout * CIN + (real ? ci : 0));
+            av[u] = inb ? ld : zero;
+            bv[u] = real ? wl : zero;
+        }
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+            const int c = split * NPER + cb + u;
+            if (cb + u >= NPER || c >= NCH) continue;
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, av[u]), __builtin_bit_cast(bf8v, bv[u]),
+                                                           acc, 0, 0, 0);
+        }
     }
+    if constexpr (KS > 1) {
+        __shared__ f4m part[4][64];
+        if (split != 0) {
+            // through VGPRs (v_accvgpr_read) rather than an LDS write straight
+            // from the accumulation registers the MFMAs just wrote
+            f4m t = acc;
+            asm volatile("" : "+v"(t));
+            part[wv][lane] = t;
+        }
+        __syncthreads();
+        if (split != 0) return;
+#pragma unroll
+        for (int q = 1; q < KS; ++q) acc += part[wv + q][lane];
+    }
+    if (v0 >= nvox) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const long long vo = v0 + kg * 4 + q;
@@ -374,13 +419,26 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     const bf16* w = (transposed ? n.Wt : n.Wb) + n.off[layer];
     const Geo g{d.B, d.D, d.H, d.W};
     const long long nv = d.n();
-    // one 16-channel output tile per wave (grid.y tiles), 4 waves of 16
-    // voxels per workgroup
-    const dim3 gr(nblk((nv + 15) / 16, 4), (unsigned)(cout / 16));
+    // one 16-channel output tile per wave group (grid.y tiles) of 16
+    // voxels; the reduction split over KS waves where the tiles alone would
+    // give few waves (the 16^3 and 8^3 levels of a 32^3 input)
+    if (nv * std::max(std::max(x.C, y.C), mask.p ? mask.C : 0) >= (1ll << 31)) {
+        set_error("network activations too large for 32-bit offsets");
+        return BLZ_E_INVALID;
+    }
+    const long long tiles = (nv + 15) / 16;
+    const long long waves = tiles * (cout / 16);
+    const int nch = (27 * cin + 31) / 32;
+    const int ks = (waves <= 1024 && nch >= 12) ? 4 : (waves <= 4096 && nch >= 8) ? 2 : 1;
+    const dim3 gr(nblk(tiles, 4 / ks), (unsigned)(cout / 16));
+#define CONV_KS(CIN, KS)                                                                                           \
+    hipLaunchKernelGGL((k_conv3_k32<CIN, KS>), gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, y.p, y.C,    \
+                       y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv)
 #define CONV_CASE(CIN)                                                                                             \
     case CIN:                                                                                                      \
-        hipLaunchKernelGGL(k_conv3_k32<CIN>, gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, y.p, y.C, y.off, \
-                           mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv);                                         \
+        if (ks == 4) CONV_KS(CIN, 4);                                                                              \
+        else if (ks == 2) CONV_KS(CIN, 2);                                                                         \
+        else CONV_KS(CIN, 1);                                                                                      \
         break;
     switch (cin) {
         CONV_CASE(16)
@@ -391,6 +449,7 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     default: set_error("unsupported channel count"); return BLZ_E_INVALID;
     }
 #undef CONV_CASE
+#undef CONV_KS
     NET_HIP(hipGetLastError());
     return BLZ_OK;
 }

@@ -105,15 +105,60 @@ __global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int
 typedef float f4w __attribute__((ext_vector_type(4)));
 constexpr int RW = 16; // waves per workgroup of the row-layout passes
 
-// one workgroup per network cell (I, J, K): r summed over its F^3 block
-// (per-thread FP32 partial sums over a fixed subset, then a fixed FP64
-// tree: deterministic) / F, and the mask (1 where the block touches a wall);
-// padding cells get 0 and mask 1
-__global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r, int W, int H, int D, int F, int lf,
-                                                    int Wc,
-                                                    int Hc, int Dc, int Wn, int Hn, float* __restrict__ res,
-                                                    float* __restrict__ mask, const int* __restrict__ stop) {
+// A x at natural cell t = (i, j, k): n x - sum of the in-domain neighbours
+__device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int i, int j, int k, int W, int H,
+                                     int D) {
+    const float xc = x[t];
+    float s = 0.0f, n = 0.0f;
+    const size_t WH = (size_t)W * H;
+    if (i > 0) { s += x[t - 1]; n += 1.0f; }
+    if (i < W - 1) { s += x[t + 1]; n += 1.0f; }
+    if (j > 0) { s += x[t - W]; n += 1.0f; }
+    if (j < H - 1) { s += x[t + W]; n += 1.0f; }
+    if (k > 0) { s += x[t - WH]; n += 1.0f; }
+    if (k < D - 1) { s += x[t + WH]; n += 1.0f; }
+    return n * xc - s;
+}
+
+// A x for the four cells (i0..i0+3, j, k), i0 a multiple of 4 and W a
+// multiple of 4: the same per-cell formula as lap(), from float4 rows
+__device__ __forceinline__ void lap4(const float* __restrict__ x, size_t t, int i0, int j, int k, int W, int H,
+                                     int D, const f4w& cx, float out[4]) {
+    const size_t WH = (size_t)W * H;
+    const f4w zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const f4w yn = j > 0 ? *(const f4w*)(x + t - W) : zero;
+    const f4w ys = j < H - 1 ? *(const f4w*)(x + t + W) : zero;
+    const f4w zd = k > 0 ? *(const f4w*)(x + t - WH) : zero;
+    const f4w zu = k < D - 1 ? *(const f4w*)(x + t + WH) : zero;
+    const float xl = i0 > 0 ? x[t - 1] : 0.0f;
+    const float xr = i0 + 4 < W ? x[t + 4] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = i0 + q;
+        float s = 0.0f, n = 0.0f;
+        if (i > 0) { s += q > 0 ? cx[q - 1] : xl; n += 1.0f; }
+        if (i < W - 1) { s += q < 3 ? cx[q + 1] : xr; n += 1.0f; }
+        if (j > 0) { s += yn[q]; n += 1.0f; }
+        if (j < H - 1) { s += ys[q]; n += 1.0f; }
+        if (k > 0) { s += zd[q]; n += 1.0f; }
+        if (k < D - 1) { s += zu[q]; n += 1.0f; }
+        out[q] = n * cx[q] - s;
+    }
+}
+
+
+// one workgroup per network cell (I, J, K): r' = r - alpha1 A e1 (e1 a
+// correction whose scale is chosen but which is not yet added to the
+// pressure, or none) summed over its F^3 block (per-thread FP32 partial
+// sums over a fixed subset, then a fixed FP64 tree: deterministic) / F, and
+// the mask (1 where the block touches a wall); padding cells get 0 and mask 1
+__global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r, const float* __restrict__ e1,
+                                                    const double* __restrict__ alpha1, int W, int H, int D, int F,
+                                                    int lf, int Wc, int Hc, int Dc, int Wn, int Hn,
+                                                    float* __restrict__ res, float* __restrict__ mask,
+                                                    const int* __restrict__ stop) {
     if (*stop) return;
+    const float a1 = e1 ? (float)*alpha1 : 0.0f;
     const int t = blockIdx.x;
     const int I = t % Wn, J = (t / Wn) % Hn, K = t / (Wn * Hn);
     if (I >= Wc || J >= Hc || K >= Dc) {
@@ -126,7 +171,12 @@ __global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r,
     float sum = 0.0f;
     for (int q = threadIdx.x; q < (1 << (3 * lf)); q += BS) {
         const int i = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
-        if (i < W && j < H && k < D) sum += r[((size_t)k * H + j) * W + i];
+        if (i < W && j < H && k < D) {
+            const size_t t = ((size_t)k * H + j) * W + i;
+            float v = r[t];
+            if (e1) v -= a1 * lap(e1, t, i, j, k, W, H, D);
+            sum += v;
+        }
     }
     double acc = (double)sum;
     for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
@@ -142,15 +192,18 @@ __global__ void __launch_bounds__(BS) k_ws_restrict(const float* __restrict__ r,
     }
 }
 
-// The same restriction with one workgroup of RW waves per row (J, K) of the
-// network grid: lane = float4 column (W <= 256, W a multiple of 4, F >= 4),
+// The same restriction (of r' as above) with one workgroup of RW waves per
+// row (J, K) of the network grid: lane = float4 column (W <= 256, W a multiple of 4, F >= 4),
 // waves over the F x F fine rows, so each wave reads whole rows; cell I's
 // F/4 lanes are reduced by shuffles, the waves through LDS in a fixed order.
-__global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __restrict__ r, int W, int H, int D, int F,
-                                                              int Wc, int Hc, int Dc, int Wn, int Hn,
+__global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __restrict__ r,
+                                                              const float* __restrict__ e1,
+                                                              const double* __restrict__ alpha1, int W, int H, int D,
+                                                              int F, int Wc, int Hc, int Dc, int Wn, int Hn,
                                                               float* __restrict__ res, float* __restrict__ mask,
                                                               const int* __restrict__ stop) {
     if (*stop) return;
+    const float a1 = e1 ? (float)*alpha1 : 0.0f;
     const int J = blockIdx.x % Hn, K = blockIdx.x / Hn;
     const size_t row0 = ((size_t)K * Hn + J) * Wn;
     if (J >= Hc || K >= Dc) {
@@ -168,7 +221,15 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
     if (i0 < W)
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
-            const f4w v = *(const f4w*)(r + ((size_t)k * H + j) * W + i0);
+            const size_t t = ((size_t)k * H + j) * W + i0;
+            f4w v = *(const f4w*)(r + t);
+            if (e1) {
+                const f4w c1 = *(const f4w*)(e1 + t);
+                float l1[4];
+                lap4(e1, t, i0, j, k, W, H, D, c1, l1);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] -= a1 * l1[u];
+            }
             sum += v[0];
             sum += v[1];
             sum += v[2];
@@ -197,10 +258,14 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
 }
 
 // input scale s = RMS of the coarse residual over the real coarse cells (one
-// block, fixed order: deterministic); stored as float bits in *amax
+// block, fixed order: deterministic); stored as float bits in *amax (0 once
+// the solve is done, which makes the network's input 0)
 __global__ void __launch_bounds__(1024) k_ws_rms(const float* __restrict__ res, int n, int nreal,
                                                  unsigned* __restrict__ amax, const int* __restrict__ stop) {
-    if (*stop) return;
+    if (*stop) {
+        if (threadIdx.x == 0) *amax = 0u;
+        return;
+    }
     double a = 0.0;
     for (int t = threadIdx.x; t < n; t += 1024) a += (double)res[t] * (double)res[t];
     for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
@@ -312,21 +377,6 @@ __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W
     *(f4w*)(e + ((size_t)k * H + j) * W + 4 * q) = out;
 }
 
-// A x at natural cell t = (i, j, k): n x - sum of the in-domain neighbours
-__device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int i, int j, int k, int W, int H,
-                                     int D) {
-    const float xc = x[t];
-    float s = 0.0f, n = 0.0f;
-    const size_t WH = (size_t)W * H;
-    if (i > 0) { s += x[t - 1]; n += 1.0f; }
-    if (i < W - 1) { s += x[t + 1]; n += 1.0f; }
-    if (j > 0) { s += x[t - W]; n += 1.0f; }
-    if (j < H - 1) { s += x[t + W]; n += 1.0f; }
-    if (k > 0) { s += x[t - WH]; n += 1.0f; }
-    if (k < D - 1) { s += x[t + WH]; n += 1.0f; }
-    return n * xc - s;
-}
-
 // One workgroup per block of the global grid (F^3 cells, clipped), all on
 // natural-order arrays: with the residual r' = r - alpha1 A e1 (e1 the
 // correction already chosen, or none) and the direction e2 (or none):
@@ -338,32 +388,6 @@ __device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int 
 // residual's norm, such a step cannot raise the A-norm of the error, i.e.
 // cannot add smooth error that the sweeps remove slowly. The block sums of
 // r' are the global correction's right-hand side. Fixed reduction order.
-
-// A x for the four cells (i0..i0+3, j, k), i0 a multiple of 4 and W a
-// multiple of 4: the same per-cell formula as lap(), from float4 rows
-__device__ __forceinline__ void lap4(const float* __restrict__ x, size_t t, int i0, int j, int k, int W, int H,
-                                     int D, const f4w& cx, float out[4]) {
-    const size_t WH = (size_t)W * H;
-    const f4w zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    const f4w yn = j > 0 ? *(const f4w*)(x + t - W) : zero;
-    const f4w ys = j < H - 1 ? *(const f4w*)(x + t + W) : zero;
-    const f4w zd = k > 0 ? *(const f4w*)(x + t - WH) : zero;
-    const f4w zu = k < D - 1 ? *(const f4w*)(x + t + WH) : zero;
-    const float xl = i0 > 0 ? x[t - 1] : 0.0f;
-    const float xr = i0 + 4 < W ? x[t + 4] : 0.0f;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = i0 + q;
-        float s = 0.0f, n = 0.0f;
-        if (i > 0) { s += q > 0 ? cx[q - 1] : xl; n += 1.0f; }
-        if (i < W - 1) { s += q < 3 ? cx[q + 1] : xr; n += 1.0f; }
-        if (j > 0) { s += yn[q]; n += 1.0f; }
-        if (j < H - 1) { s += ys[q]; n += 1.0f; }
-        if (k > 0) { s += zd[q]; n += 1.0f; }
-        if (k < D - 1) { s += zu[q]; n += 1.0f; }
-        out[q] = n * cx[q] - s;
-    }
-}
 
 // the block-pass sums of the four cells (i0..i0+3, j, k) into v
 __device__ __forceinline__ void blockdots_cell4(const float* __restrict__ r, const float* __restrict__ e1, float a1,
@@ -532,8 +556,10 @@ __global__ void __launch_bounds__(BS) k_ws_alpha(const double* __restrict__ part
 // p += alpha1 (e1 - mean e1) + alpha2 (e2 - mean e2) (either may be
 // absent; scales and means from k_ws_alpha); thread per (x pair,
 // row, k group) of the split layout: float4 read-modify-write of both
-// colours, natural-order reads of e (8 contiguous bytes per thread and k)
-__global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int D, const float* __restrict__ e1,
+// colours, natural-order reads of e (8 contiguous bytes per thread and k).
+// bk (when bk.c[0] is set): planes of the same layout that receive the
+// uncorrected values (the interior; their ghost shell already holds -0)
+__global__ void __launch_bounds__(BS) k_ws_apply(RB p, RB bk, RBL L, int W, int H, int D, const float* __restrict__ e1,
                                                  const double* __restrict__ alpha1, const float* __restrict__ e2,
                                                  const double* __restrict__ alpha2, const SolveCtl* __restrict__ c) {
     typedef float f4 __attribute__((ext_vector_type(4)));
@@ -549,6 +575,10 @@ __global__ void __launch_bounds__(BS) k_ws_apply(RB p, RBL L, int W, int H, int 
     f4* P1 = (f4*)p.c[c0 ^ 1];
     const int e = (j + 1) * L.Wp + (m + 1) + (g + 1) * L.Wp * L.Hp;
     f4 x0 = P0[e], x1 = P1[e];
+    if (bk.c[0]) {
+        ((f4*)bk.c[c0])[e] = x0;
+        ((f4*)bk.c[c0 ^ 1])[e] = x1;
+    }
 #pragma unroll
     for (int sl = 0; sl < 4; ++sl) {
         const int k = 4 * g + sl;
@@ -1023,8 +1053,6 @@ __global__ void __launch_bounds__(GT) k_ws_cdct(const float* __restrict__ res, i
     }
 }
 
-__global__ void k_ws_zero(unsigned* a) { *a = 0u; }
-
 // a new application: no decision yet (the restore only follows this one's)
 __global__ void k_ws_begin(SolveCtl* c) {
     c->ws = 0;
@@ -1135,23 +1163,26 @@ int warm_start_attach(Solver& s, const Net* net) {
     return BLZ_OK;
 }
 
-// r (natural order, into s.tmp) and the network input: r restricted to the
-// network grid, its RMS as the input scale
-static int ws_restrict(Solver& s, float dt, const int* stop) {
+// The network input: r (natural order, in s.tmp; computed here from the
+// home planes when `resid`) less alpha1 A e1 when e1 is given (a correction
+// whose scale is chosen but which is not yet added), restricted to the
+// network grid, with its RMS as the input scale
+static int ws_restrict(Solver& s, float dt, const int* stop, bool resid = true, const float* e1 = nullptr,
+                       const double* a1 = nullptr) {
     const size_t nn = ws_nn(s);
     float* res = s.ws_buf;
     float* mask = s.ws_buf + nn;
     unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
     const float scale = (s.s.density * s.dx * s.dx) / dt;
-    hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
-                       scale, ws_r(s), stop);
-    hipLaunchKernelGGL(k_ws_zero, dim3(1), dim3(1), 0, s.stream, amax);
+    if (resid)
+        hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
+                           s.D, scale, ws_r(s), stop);
     if (s.W % 4 == 0 && s.W <= 256 && s.ws_f >= 4 && s.Wn <= 64)
-        hipLaunchKernelGGL(k_ws_restrict_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream, ws_r(s), s.W,
-                           s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
+        hipLaunchKernelGGL(k_ws_restrict_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream, ws_r(s), e1,
+                           a1, s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
     else
-        hipLaunchKernelGGL(k_ws_restrict, dim3((unsigned)nn), dim3(BS), 0, s.stream, ws_r(s), s.W, s.H, s.D, s.ws_f,
-                           ilog2(s.ws_f), s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
+        hipLaunchKernelGGL(k_ws_restrict, dim3((unsigned)nn), dim3(BS), 0, s.stream, ws_r(s), e1, a1, s.W, s.H, s.D,
+                           s.ws_f, ilog2(s.ws_f), s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
     hipLaunchKernelGGL(k_ws_rms, dim3(1), dim3(1024), 0, s.stream, res, (int)nn, s.Wc * s.Hc * s.Dc, amax, stop);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
@@ -1172,15 +1203,16 @@ int warm_start_set_parts(Solver& s, int parts) {
     return BLZ_OK;
 }
 
-// The corrections on the home planes (the spare ones already hold the
-// uncorrected pressure), from r in s.tmp (computed from the home planes):
+// The corrections on the home planes, from r in s.tmp (computed from the
+// home planes); bk (when set) receives the uncorrected pressure in the same
+// pass:
 // first the network's coarse prediction (when pred != nullptr), prolongated
 // trilinearly, with its energy-optimal scale; then the global one -- the
 // Galerkin solve on the G grid of the residual the first leaves, which sets
 // the smoothest error modes (the ones Gauss-Seidel reduces slowest, and the
 // ones a network's prediction is least reliable on), prolongated and scaled
 // the same way against that residual; both added in one pass.
-static int ws_correct(Solver& s, SolveCtl* c, const float* pred, bool glob) {
+static int ws_correct(Solver& s, SolveCtl* c, const float* pred, bool glob, RB bk = RB{}) {
     const size_t N = ncell(s);
     const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
     double* a_net = ws_alpha(s);
@@ -1206,8 +1238,8 @@ static int ws_correct(Solver& s, SolveCtl* c, const float* pred, bool glob) {
         launch_blockdots(s, ws_r(s), enet, a_net, ws_eg(s), ws_part(s), nullptr, c);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
     }
-    hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.L, s.W, s.H, s.D, enet,
-                       a_net, glob ? ws_eg(s) : nullptr, a_g, c);
+    hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, bk, s.L, s.W, s.H, s.D,
+                       enet, a_net, glob ? ws_eg(s) : nullptr, a_g, c);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
@@ -1247,27 +1279,53 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
     // the uncorrected pressure goes to the judge planes (the spare ones
     // where no judge planes exist), the corrected one stays in the home planes
     float* spare = s.prb + (size_t)s.judge_set * s.Np;
-    BLZ_HIP(hipMemcpyAsync(spare, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
     if (coarse && glob && !net_first) {
         // global first (it sets the smoothest modes), then the network's
-        // correction of the residual the global one leaves (the middle band)
-        const float scale = (s.s.density * s.dx * s.dx) / dt;
+        // correction of the residual the global one leaves (the middle band).
+        // Neither is added before both scales are known: the residual the
+        // global correction leaves, r - alpha_g A e_g, is evaluated where it
+        // is used (the network's input and the network's scale), and one pass
+        // adds both corrections and saves the uncorrected pressure.
+        const size_t N = ncell(s);
+        const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
+        double* a_net = ws_alpha(s);
+        double* a_g = ws_alpha(s) + 1;
         hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
                            s.D, scale, ws_r(s), &c->done);
-        if (int e = ws_correct(s, c, nullptr, true)) return e;
-        if (int e = ws_restrict(s, dt, &c->done)) return e;
+        launch_blockdots(s, ws_r(s), nullptr, nullptr, nullptr, ws_part(s), ws_sg(s), c);
+        if (s.W % s.ws_g == 0 && s.H % s.ws_g == 0 && s.D % s.ws_g == 0)
+            hipLaunchKernelGGL(k_ws_gdct, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh,
+                               s.Gd, s.ws_g, s.W, s.H, s.D, c);
+        else
+            hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh,
+                               s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
+        launch_prolong(s, ws_eg(s), ws_xg(s), 1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
+        launch_blockdots(s, ws_r(s), nullptr, nullptr, ws_eg(s), ws_part(s), nullptr, c);
+        hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
+        if (int e = ws_restrict(s, dt, &c->done, false, ws_eg(s), a_g)) return e;
         if (int e = predict()) return e;
-        if (int e = ws_correct(s, c, s.ws_net->out, false)) return e;
+        launch_prolong(s, ws_enet(s), s.ws_net->out, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
+        launch_blockdots(s, ws_r(s), ws_eg(s), a_g, ws_enet(s), ws_part(s), nullptr, c);
+        hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
+        RB bk;
+        bk.c[0] = spare;
+        bk.c[1] = spare + s.Np;
+        hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, bk, s.L, s.W, s.H,
+                           s.D, ws_eg(s), a_g, ws_enet(s), a_net, c);
+        BLZ_HIP(hipGetLastError());
     } else {
         if (!coarse) {
-            const float scale = (s.s.density * s.dx * s.dx) / dt;
             hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W,
                                s.H, s.D, scale, ws_r(s), &c->done);
         } else {
             if (int e = ws_restrict(s, dt, &c->done)) return e;
             if (int e = predict()) return e;
         }
-        if (int e = ws_correct(s, c, coarse ? s.ws_net->out : nullptr, glob)) return e;
+        RB bk;
+        bk.c[0] = spare;
+        bk.c[1] = spare + s.Np;
+        if (int e = ws_correct(s, c, coarse ? s.ws_net->out : nullptr, glob, bk)) return e;
     }
     return BLZ_OK;
 }
