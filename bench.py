@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run RBGS sweeps (for hardware counter runs)")
     ap.add_argument("--probe-sweeps", type=int, default=20)
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="time plain streaming kernels (torch) over the sweep's working set: four pressure/rhs-sized "
+                         "planes read, two written")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--warm-every", type=int, default=40,
@@ -115,6 +118,41 @@ def traffic_probe(a):
     g.set_field(B.FIELD_RHS, rng.standard_normal((D, H, W)).astype(np.float32))
     g.run_stage(B.STAGE_PRESSURE, 0.0, float(s.dt))
     log(f"[probe] {a.probe_sweeps} sweeps on {W}x{H}x{D} done", a.quiet)
+
+
+def stream_probe(a):
+    """The bandwidth a plain streaming kernel reaches on the same working set
+    as one RBGS sweep (four padded planes of the scene's grid read, two
+    written; x = a + b and y = c + d, torch's elementwise kernels), as the
+    reference point for the sweep kernel's rate. Prints one JSON line."""
+    import torch
+
+    import blaze_amd as B
+
+    s, _, _, _ = B.parse_scene(a.scene)
+    W, H, D = s.grid_res
+    n = (W // 2 + 2) * (H + 2) * ((D + 3) // 4 + 2) * 4
+    dev = torch.device("cuda:0")
+    pl = [torch.rand(n, device=dev) for _ in range(6)]
+    reps = 20
+
+    def once():
+        torch.add(pl[0], pl[1], out=pl[4])
+        torch.add(pl[2], pl[3], out=pl[5])
+
+    for _ in range(3):
+        once()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        once()
+    e1.record()
+    e1.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    gbs = 6 * n * 4 / (us * 1e3)
+    print(json.dumps({"stream_probe": {"planes_floats": n, "bytes_per_pass": 6 * n * 4, "us_per_pass": us,
+                                       "GB_per_s": gbs, "kernels": "torch.add x2 (2 reads, 1 write each)"}}),
+          flush=True)
 
 
 def cpu_model():
@@ -214,6 +252,9 @@ def main():
 
     if a.calib_copy:
         calib_copy(a.quiet)
+    if a.stream_probe:
+        stream_probe(a)
+        return
     if a.traffic_probe:
         traffic_probe(a)
         return
