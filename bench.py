@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--stream-probe", action="store_true",
                     help="time plain streaming kernels (torch) over the sweep's working set: four pressure/rhs-sized "
                          "planes read, two written")
+    ap.add_argument("--sweep-scaling", action="store_true",
+                    help="time every sweep variant on batches of 1 and 2 grids of 256^3 and 8 and 16 of 128^3 "
+                         "(the batch autotuner's timings; one JSON line per batch)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--warm-every", type=int, default=40,
@@ -153,6 +156,37 @@ def stream_probe(a):
     print(json.dumps({"stream_probe": {"planes_floats": n, "bytes_per_pass": 6 * n * 4, "us_per_pass": us,
                                        "GB_per_s": gbs, "kernels": "torch.add x2 (2 reads, 1 write each)"}}),
           flush=True)
+
+
+def sweep_scaling(a):
+    """Sweep time per cell as the working set and the wave count grow: one
+    and two 256^3 grids (212 and 424 MB of sweep working set, against the
+    256 MB Infinity Cache), eight and sixteen 128^3 grids (the same cells
+    as one and two 256^3 grids in 2.2 times the waves). Each batch's
+    autotuner times every sweep variant over all its grids in one launch
+    per sweep. Prints one JSON line per batch."""
+    import blaze_amd as B
+
+    s, _, _, _ = B.parse_scene(a.scene)
+    s.residual_tol = 1e-4
+    s.max_iter_solve = 20
+    s.residual_check_every = 10
+    s.cfl_number = 0.0
+    for n, res in ((1, 256), (2, 256), (8, 128), (16, 128)):
+        t = s.copy()
+        for ax in range(3):
+            t.grid_res[ax] = res
+        t.dx = 1.0 / res
+        b = B.Batch([t] * n)
+        b.autotune(20)
+        st = b.stats()
+        cells = n * res ** 3
+        us = {int(st.tune_variant[c]): st.tune_ms[c] * 1e3 for c in range(len(st.tune_variant))}
+        print(json.dumps({"sweep_scaling": {"grids": n, "res": res, "cells": cells,
+                                            "us_per_sweep": us,
+                                            "us_per_sweep_per_million_cells": {v: u / cells * 1e6 for v, u in us.items()}}}),
+              flush=True)
+        del b
 
 
 def cpu_model():
@@ -254,6 +288,9 @@ def main():
         calib_copy(a.quiet)
     if a.stream_probe:
         stream_probe(a)
+        return
+    if a.sweep_scaling:
+        sweep_scaling(a)
         return
     if a.traffic_probe:
         traffic_probe(a)
