@@ -35,9 +35,10 @@ def sample(seed, shape):
     return x, t
 
 
-# (1, 32, 32, 32): the network grid of a 256^3 scene; its levels run the
-# convolution with the reduction split over 1, 2 and 4 waves
-@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 4, 8, 12), (1, 32, 32, 32)])
+# (1, 32, 32, 32): the network grid of a 256^3 scene, whose levels run the
+# convolution with the reduction split over 2 and 4 waves; (4, 32, 32, 32)
+# also without the split (the first level's 8192 tiles)
+@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 4, 8, 12), (1, 32, 32, 32), (4, 32, 32, 32)])
 def test_forward_matches_oracle(shape):
     print(f"[unet] forward {shape}", flush=True)
     net = B.Net(seed=7)
