@@ -54,9 +54,17 @@ __device__ __forceinline__ bf16 f2bf_w(float f) {
 // float4 groups of the split layout (cells 2m and 2m+1 of 4 k); a missing
 // neighbour is a -0 ghost, the identity of the sum, so all six slots are
 // summed in the order i-1, i+1, j-1, j+1, k-1, k+1.
+// begin (when set): the control block of a new application, whose judge
+// state this launch resets first (no decision yet; the restore only follows
+// this application's)
 __global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
-                                                 float* __restrict__ r, const int* __restrict__ stop) {
+                                                 float* __restrict__ r, const int* __restrict__ stop,
+                                                 SolveCtl* __restrict__ begin) {
     typedef float f4 __attribute__((ext_vector_type(4)));
+    if (begin && blockIdx.x == 0 && threadIdx.x == 0) {
+        begin->ws = 0;
+        begin->skip_spare = 1;
+    }
     if (*stop) return;
     const int Wh = W >> 1;
     const int t = blockIdx.x * BS + threadIdx.x;
@@ -254,6 +262,153 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
             const bool wall = c == 0 || J == 0 || K == 0 || F * (c + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
             mask[row0 + c] = wall ? 1.0f : 0.0f;
         }
+    }
+}
+
+// The global correction's scale and the network's input in one pass over
+// the fine grid (the shape of k_ws_restrict_rows: a workgroup of RW waves per
+// row (J, K) of the network grid, lane = float4 column): with e the global
+// correction's prolongation (not yet added) and r the residual it meets,
+//   S[2 c], S[2 c + 1] = sum of r and of A e over network cell c  (FP64)
+//   partial[4 b ..]     = <r, e>, <e, A e>, sum r, sum e over the row  (FP64)
+// k_ws_alpha_res turns them into alpha_G and the network's input
+// (sum r - alpha_G sum A e) / F, the restriction of r - alpha_G A e. Per
+// thread FP32 sums, then fixed-order FP64 trees (deterministic).
+__global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* __restrict__ r,
+                                                                   const float* __restrict__ e, int W, int H, int D,
+                                                                   int F, int Wc, int Hc, int Dc, int Wn, int Hn,
+                                                                   double* __restrict__ S,
+                                                                   double* __restrict__ partial,
+                                                                   const int* __restrict__ stop) {
+    if (*stop) return;
+    const int J = blockIdx.x % Hn, K = blockIdx.x / Hn;
+    const size_t row0 = ((size_t)K * Hn + J) * Wn;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 4 * lane;
+    const int y0 = F * J, z0 = F * K;
+    const bool real_row = J < Hc && K < Dc;
+    float sr = 0.0f, sa = 0.0f;
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (real_row && i0 < W) {
+        const int ny = min(F, H - y0), nz = min(F, D - z0);
+        for (int q = w; q < ny * nz; q += RW) {
+            const int j = y0 + q % ny, k = z0 + q / ny;
+            const size_t t = ((size_t)k * H + j) * W + i0;
+            const f4w rv = *(const f4w*)(r + t);
+            const f4w ev = *(const f4w*)(e + t);
+            float l[4];
+            lap4(e, t, i0, j, k, W, H, D, ev, l);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                sr += rv[u];
+                sa += l[u];
+                v[0] += rv[u] * ev[u];
+                v[1] += ev[u] * l[u];
+                v[2] += rv[u];
+                v[3] += ev[u];
+            }
+        }
+    }
+    const int gl = F >> 2; // lanes per network cell
+    double cr = (double)sr, ca = (double)sa;
+    for (int o = 1; o < gl; o <<= 1) {
+        cr += __shfl_xor(cr, o, 64);
+        ca += __shfl_xor(ca, o, 64);
+    }
+    double d[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        d[q] = (double)v[q];
+        for (int o = 32; o >= 1; o >>= 1) d[q] += __shfl_xor(d[q], o, 64);
+    }
+    __shared__ double sc[RW][64][2];
+    __shared__ double sd[RW][4];
+    const int I = lane / gl;
+    if ((lane & (gl - 1)) == 0) {
+        sc[w][I][0] = cr;
+        sc[w][I][1] = ca;
+    }
+    if (lane < 4) sd[w][lane] = d[lane];
+    __syncthreads();
+    if (threadIdx.x < Wn) {
+        const int c = threadIdx.x;
+        double tr = 0.0, ta = 0.0;
+        if (real_row && c < Wc)
+            for (int u = 0; u < RW; ++u) {
+                tr += sc[u][c][0];
+                ta += sc[u][c][1];
+            }
+        S[2 * (row0 + c)] = tr;
+        S[2 * (row0 + c) + 1] = ta;
+    }
+    if (threadIdx.x < 4) {
+        double a = 0.0;
+        for (int u = 0; u < RW; ++u) a += sd[u][threadIdx.x];
+        partial[4 * blockIdx.x + threadIdx.x] = a;
+    }
+}
+
+// alpha_G from the row partials of k_ws_restrict_dots_rows (as k_ws_alpha:
+// the energy-optimal scale and the direction's mean into alpha[0], alpha[2]),
+// then the network's input res = (S[2 c] - alpha_G S[2 c + 1]) / F with the
+// wall mask (padding cells: 0 and mask 1) and its RMS over the real cells
+// (float bits into *amax; 0 once the solve is done). One workgroup, fixed
+// order.
+__global__ void __launch_bounds__(1024) k_ws_alpha_res(const double* __restrict__ partial, int np, double n,
+                                                       double* __restrict__ alpha, const double* __restrict__ S,
+                                                       float* __restrict__ res, float* __restrict__ mask,
+                                                       unsigned* __restrict__ amax, int W, int H, int D, int F,
+                                                       int Wc, int Hc, int Dc, int Wn, int Hn, int nn,
+                                                       const int* __restrict__ stop) {
+    if (*stop) {
+        if (threadIdx.x == 0) *amax = 0u;
+        return;
+    }
+    __shared__ double sv[4][16];
+    __shared__ double sal;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < np; q += 1024)
+        for (int k = 0; k < 4; ++k) v[k] += partial[4 * q + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k) sv[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k)
+            for (int u = 0; u < 16; ++u) a[k] += sv[k][u];
+        const double al = a[1] > 0.0 ? (a[0] - a[2] * a[3] / n) / a[1] : 0.0;
+        const bool ok = al == al && fabs(al) < 1e30;
+        alpha[0] = ok ? al : 0.0;
+        alpha[2] = ok ? a[3] / n : 0.0;
+        sal = ok ? al : 0.0;
+    }
+    __syncthreads();
+    const double al = sal;
+    double sq = 0.0;
+    for (int t = threadIdx.x; t < nn; t += 1024) {
+        const int I = t % Wn, J = (t / Wn) % Hn, K = t / (Wn * Hn);
+        if (I >= Wc || J >= Hc || K >= Dc) {
+            res[t] = 0.0f;
+            mask[t] = 1.0f;
+            continue;
+        }
+        const float x = (float)((S[2 * t] - al * S[2 * t + 1]) / (double)F);
+        res[t] = x;
+        const bool wall = I == 0 || J == 0 || K == 0 || F * (I + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
+        mask[t] = wall ? 1.0f : 0.0f;
+        sq += (double)x * (double)x;
+    }
+    for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sv[0][threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int u = 0; u < 16; ++u) tot += sv[0][u];
+        *amax = __float_as_uint((float)sqrt(tot / (double)max(Wc * Hc * Dc, 1)));
     }
 }
 
@@ -1167,6 +1322,9 @@ int warm_start_attach(Solver& s, const Net* net) {
 // home planes when `resid`) less alpha1 A e1 when e1 is given (a correction
 // whose scale is chosen but which is not yet added), restricted to the
 // network grid, with its RMS as the input scale
+// the row-layout restriction applies (float4 rows, at most 64 network cells a row)
+static bool restrict_rows_ok(const Solver& s) { return s.W % 4 == 0 && s.W <= 256 && s.ws_f >= 4 && s.Wn <= 64; }
+
 static int ws_restrict(Solver& s, float dt, const int* stop, bool resid = true, const float* e1 = nullptr,
                        const double* a1 = nullptr) {
     const size_t nn = ws_nn(s);
@@ -1176,8 +1334,8 @@ static int ws_restrict(Solver& s, float dt, const int* stop, bool resid = true, 
     const float scale = (s.s.density * s.dx * s.dx) / dt;
     if (resid)
         hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
-                           s.D, scale, ws_r(s), stop);
-    if (s.W % 4 == 0 && s.W <= 256 && s.ws_f >= 4 && s.Wn <= 64)
+                           s.D, scale, ws_r(s), stop, nullptr);
+    if (restrict_rows_ok(s))
         hipLaunchKernelGGL(k_ws_restrict_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream, ws_r(s), e1,
                            a1, s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, res, mask, stop);
     else
@@ -1258,11 +1416,13 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
     float* res = s.ws_buf;
     float* mask = s.ws_buf + nn;
     unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
-    hipLaunchKernelGGL(k_ws_begin, dim3(1), dim3(1), 0, s.stream, c);
     const bool exact = s.ws_parts & 4;           // exact coarse solve in place of the network
     const bool coarse = (s.ws_parts & 1) || exact;
     const bool glob = s.ws_parts & 2;
     const bool net_first = s.ws_parts & 8;       // the network's correction before the global one
+    // a new application: no decision yet (the default order's residual
+    // launch does this itself)
+    if (!(coarse && glob && !net_first)) hipLaunchKernelGGL(k_ws_begin, dim3(1), dim3(1), 0, s.stream, c);
     // the network's coarse correction of the residual in ws_buf (restricted)
     auto predict = [&]() -> int {
         if (exact) {
@@ -1292,7 +1452,7 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
         double* a_net = ws_alpha(s);
         double* a_g = ws_alpha(s) + 1;
         hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
-                           s.D, scale, ws_r(s), &c->done);
+                           s.D, scale, ws_r(s), &c->done, c);
         launch_blockdots(s, ws_r(s), nullptr, nullptr, nullptr, ws_part(s), ws_sg(s), c);
         if (s.W % s.ws_g == 0 && s.H % s.ws_g == 0 && s.D % s.ws_g == 0)
             hipLaunchKernelGGL(k_ws_gdct, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh,
@@ -1301,9 +1461,22 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
             hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh,
                                s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
         launch_prolong(s, ws_eg(s), ws_xg(s), 1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
-        launch_blockdots(s, ws_r(s), nullptr, nullptr, ws_eg(s), ws_part(s), nullptr, c);
-        hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
-        if (int e = ws_restrict(s, dt, &c->done, false, ws_eg(s), a_g)) return e;
+        // S: 2 nn doubles after the three natural-order fields in s.tmp
+        if (restrict_rows_ok(s) && 3 * N + 4 * nn <= s.tmp_floats) {
+            // alpha_G and the network's input from one pass over the fine grid
+            double* S = (double*)(s.tmp + 3 * N);
+            hipLaunchKernelGGL(k_ws_restrict_dots_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream,
+                               ws_r(s), ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s),
+                               &c->done);
+            hipLaunchKernelGGL(k_ws_alpha_res, dim3(1), dim3(1024), 0, s.stream, ws_part(s), (int)(nn / s.Wn),
+                               (double)N, a_g, S, res, mask, amax, s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn,
+                               s.Hn, (int)nn, &c->done);
+            BLZ_HIP(hipGetLastError());
+        } else {
+            launch_blockdots(s, ws_r(s), nullptr, nullptr, ws_eg(s), ws_part(s), nullptr, c);
+            hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
+            if (int e = ws_restrict(s, dt, &c->done, false, ws_eg(s), a_g)) return e;
+        }
         if (int e = predict()) return e;
         launch_prolong(s, ws_enet(s), s.ws_net->out, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
         launch_blockdots(s, ws_r(s), ws_eg(s), a_g, ws_enet(s), ws_part(s), nullptr, c);
@@ -1317,7 +1490,7 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
     } else {
         if (!coarse) {
             hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W,
-                               s.H, s.D, scale, ws_r(s), &c->done);
+                               s.H, s.D, scale, ws_r(s), &c->done, nullptr);
         } else {
             if (int e = ws_restrict(s, dt, &c->done)) return e;
             if (int e = predict()) return e;
@@ -1386,7 +1559,7 @@ int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* 
     BLZ_HIP(hipMemcpyAsync(backup, s.prb, 2 * s.Np * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
     const float scale = (s.s.density * s.dx * s.dx) / dt;
     hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
-                       scale, ws_r(s), &c->done);
+                       scale, ws_r(s), &c->done, nullptr);
     if (int e = ws_correct(s, c, (s.ws_parts & 1) ? s.ws_net->out : nullptr, s.ws_parts & 2)) return e;
     if (int e2 = launch_residual_ws(s, dt, c, s.done_host_dev, 0, -1.0, 1 << 30)) return e2;
     hipLaunchKernelGGL(k_ws_restore, dim3((unsigned)std::min<size_t>(nb(2 * s.Np), 4096)), dim3(BS), 0, s.stream, c,
@@ -1407,7 +1580,7 @@ int warm_start_global(Solver& s, float dt) {
     if (int e = launch_ctl_reset(s, c)) return e;
     const float scale = (s.s.density * s.dx * s.dx) / dt;
     hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
-                       scale, ws_r(s), &c->done);
+                       scale, ws_r(s), &c->done, nullptr);
     if (int e = ws_correct(s, c, nullptr, true)) return e;
     BLZ_HIP(hipStreamSynchronize(s.stream));
     return BLZ_OK;
