@@ -133,13 +133,15 @@ __device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int 
 __device__ __forceinline__ void lap4(const float* __restrict__ x, size_t t, int i0, int j, int k, int W, int H,
                                      int D, const f4w& cx, float out[4]) {
     const size_t WH = (size_t)W * H;
-    const f4w zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    const f4w yn = j > 0 ? *(const f4w*)(x + t - W) : zero;
-    const f4w ys = j < H - 1 ? *(const f4w*)(x + t + W) : zero;
-    const f4w zd = k > 0 ? *(const f4w*)(x + t - WH) : zero;
-    const f4w zu = k < D - 1 ? *(const f4w*)(x + t + WH) : zero;
-    const float xl = i0 > 0 ? x[t - 1] : 0.0f;
-    const float xr = i0 + 4 < W ? x[t + 4] : 0.0f;
+    // unconditional loads (a missing neighbour reads the cell's own row,
+    // whose value the conditions below never use), so that the loads of
+    // consecutive rows can be in flight together
+    const f4w yn = *(const f4w*)(x + t - (j > 0 ? (size_t)W : 0));
+    const f4w ys = *(const f4w*)(x + t + (j < H - 1 ? (size_t)W : 0));
+    const f4w zd = *(const f4w*)(x + t - (k > 0 ? WH : 0));
+    const f4w zu = *(const f4w*)(x + t + (k < D - 1 ? WH : 0));
+    const float xl = x[t - (i0 > 0 ? 1 : 0)];
+    const float xr = x[t + (i0 + 4 < W ? 4 : 0)];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int i = i0 + q;
@@ -227,6 +229,8 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
     const int ny = min(F, H - y0), nz = min(F, D - z0);
     float sum = 0.0f;
     if (i0 < W)
+        // unrolled so that the next rows' loads issue before this row's sums
+#pragma unroll 4
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
             const size_t t = ((size_t)k * H + j) * W + i0;
@@ -291,6 +295,8 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* 
     float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (real_row && i0 < W) {
         const int ny = min(F, H - y0), nz = min(F, D - z0);
+        // unrolled so that the next rows' loads issue before this row's sums
+#pragma unroll 4
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
             const size_t t = ((size_t)k * H + j) * W + i0;
@@ -387,20 +393,35 @@ __global__ void __launch_bounds__(1024) k_ws_alpha_res(const double* __restrict_
     }
     __syncthreads();
     const double al = sal;
+    const double invF = 1.0 / (double)F; // F a power of two: exact
     double sq = 0.0;
-    for (int t = threadIdx.x; t < nn; t += 1024) {
-        const int I = t % Wn, J = (t / Wn) % Hn, K = t / (Wn * Hn);
-        if (I >= Wc || J >= Hc || K >= Dc) {
-            res[t] = 0.0f;
-            mask[t] = 1.0f;
-            continue;
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    // threads over (row, I): I fixed per thread, rows J + Hn K strided, so
+    // only a row index is divided; UB rows' loads in flight together
+    const int rpp = 1024 / Wn;          // rows per pass
+    const int I = threadIdx.x % Wn, r0 = threadIdx.x / Wn;
+    const int nrows = Hn * (nn / (Wn * Hn));
+    constexpr int UB = 8;
+    if (r0 < rpp)
+        for (int rb = r0; rb < nrows; rb += rpp * UB) {
+            d2v sv2[UB];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) sv2[u] = ((const d2v*)S)[min(rb + rpp * u, nrows - 1) * Wn + I];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int rr = rb + rpp * u;
+                if (rr >= nrows) break;
+                const int J = rr % Hn, K = rr / Hn;
+                const int t = rr * Wn + I;
+                const bool real = I < Wc && J < Hc && K < Dc;
+                const float x = real ? (float)((sv2[u][0] - al * sv2[u][1]) * invF) : 0.0f;
+                res[t] = x;
+                const bool wall =
+                    I == 0 || J == 0 || K == 0 || F * (I + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
+                mask[t] = (wall || !real) ? 1.0f : 0.0f;
+                sq += (double)x * (double)x;
+            }
         }
-        const float x = (float)((S[2 * t] - al * S[2 * t + 1]) / (double)F);
-        res[t] = x;
-        const bool wall = I == 0 || J == 0 || K == 0 || F * (I + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
-        mask[t] = wall ? 1.0f : 0.0f;
-        sq += (double)x * (double)x;
-    }
     for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o, 64);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sv[0][threadIdx.x >> 6] = sq;
@@ -494,12 +515,13 @@ __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W
                                                     const float* __restrict__ pred, float invF, int Wc, int Hc,
                                                     int Dc, int Wn, int Hn, const SolveCtl* __restrict__ c) {
     if (c->done) return;
-    const int W4 = W >> 2;
-    const size_t t = (size_t)blockIdx.x * BS + threadIdx.x;
-    if (t >= (size_t)W4 * H * D) return;
-    const int q = (int)(t % (size_t)W4);
-    const size_t row = t / (size_t)W4;
-    const int j = (int)(row % (size_t)H), k = (int)(row / (size_t)H);
+    const unsigned W4 = (unsigned)W >> 2;
+    // 32-bit index arithmetic: W H D / 4 < 2^32 (the launcher's grid)
+    const unsigned t = blockIdx.x * BS + threadIdx.x;
+    if (t >= W4 * (unsigned)H * (unsigned)D) return;
+    const int q = (int)(t % W4);
+    const unsigned row = t / W4;
+    const int j = (int)(row % (unsigned)H), k = (int)(row / (unsigned)H);
     auto axis = [invF](int x, int n, int& a0, int& a1, float& wt) {
         float f = ((float)x + 0.5f) * invF - 0.5f;
         f = fminf(fmaxf(f, 0.0f), (float)(n - 1));
@@ -515,16 +537,31 @@ __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W
     const float* p10 = pred + ((size_t)z0 * Hn + y1) * Wn;
     const float* p01 = pred + ((size_t)z1 * Hn + y0) * Wn;
     const float* p11 = pred + ((size_t)z1 * Hn + y1) * Wn;
+    // all 32 coarse values first (independent loads in flight together),
+    // then the interpolation
+    int xa[4], xb[4];
+    float wxs[4], g[4][8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) axis(4 * q + u, Wc, xa[u], xb[u], wxs[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        g[u][0] = p00[xa[u]];
+        g[u][1] = p00[xb[u]];
+        g[u][2] = p10[xa[u]];
+        g[u][3] = p10[xb[u]];
+        g[u][4] = p01[xa[u]];
+        g[u][5] = p01[xb[u]];
+        g[u][6] = p11[xa[u]];
+        g[u][7] = p11[xb[u]];
+    }
     f4w out;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        int x0, x1;
-        float wx;
-        axis(4 * q + u, Wc, x0, x1, wx);
-        const float c00 = p00[x0] * (1.0f - wx) + p00[x1] * wx;
-        const float c10 = p10[x0] * (1.0f - wx) + p10[x1] * wx;
-        const float c01 = p01[x0] * (1.0f - wx) + p01[x1] * wx;
-        const float c11 = p11[x0] * (1.0f - wx) + p11[x1] * wx;
+        const float wx = wxs[u];
+        const float c00 = g[u][0] * (1.0f - wx) + g[u][1] * wx;
+        const float c10 = g[u][2] * (1.0f - wx) + g[u][3] * wx;
+        const float c01 = g[u][4] * (1.0f - wx) + g[u][5] * wx;
+        const float c11 = g[u][6] * (1.0f - wx) + g[u][7] * wx;
         const float c0 = c00 * (1.0f - wy) + c10 * wy, c1 = c01 * (1.0f - wy) + c11 * wy;
         const float val = c0 * (1.0f - wz) + c1 * wz;
         out[u] = (val == val && fabsf(val) <= 3.4e38f) ? val : 0.0f;
@@ -544,13 +581,37 @@ __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W
 // cannot add smooth error that the sweeps remove slowly. The block sums of
 // r' are the global correction's right-hand side. Fixed reduction order.
 
-// the block-pass sums of the four cells (i0..i0+3, j, k) into v
+// the block-pass sums of the four cells (i0..i0+3, j, k) into v; E1, E2:
+// e1, e2 given (compile-time, so a row's loads carry no branches). SYM
+// (with E1 and E2): only the scale alpha2 is wanted, which needs the sums
+// over the whole grid, so <r - a1 A e1, e2> is summed as <r, e2> - a1 <e1,
+// A e2> (A symmetric) and sum (r - a1 A e1) as sum r (A e1 sums to zero
+// over the grid: each cell's value enters its own term n times and its n
+// neighbours' terms once each): one A product per cell instead of two
+// (the per-block partials then differ; the grid totals are the same).
+template <bool E1, bool E2, bool SYM = false>
 __device__ __forceinline__ void blockdots_cell4(const float* __restrict__ r, const float* __restrict__ e1, float a1,
                                                 const float* __restrict__ e2, size_t t, int i0, int j, int k, int W,
                                                 int H, int D, float (&v)[6]) {
+    static_assert(!SYM || (E1 && E2), "SYM needs both directions");
     const f4w r4 = *(const f4w*)(r + t);
     float rr[4] = {r4[0], r4[1], r4[2], r4[3]};
-    if (e1) {
+    if constexpr (SYM) {
+        const f4w c1 = *(const f4w*)(e1 + t);
+        const f4w c2 = *(const f4w*)(e2 + t);
+        float l2[4];
+        lap4(e2, t, i0, j, k, W, H, D, c2, l2);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[2] += rr[u];
+            v[0] += rr[u] * c2[u] - a1 * (c1[u] * l2[u]);
+            v[1] += c2[u] * l2[u];
+            v[3] += c2[u];
+            v[4] += l2[u];
+        }
+        return;
+    }
+    if constexpr (E1) {
         const f4w c1 = *(const f4w*)(e1 + t);
         float l1[4];
         lap4(e1, t, i0, j, k, W, H, D, c1, l1);
@@ -559,7 +620,7 @@ __device__ __forceinline__ void blockdots_cell4(const float* __restrict__ r, con
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) v[2] += rr[u];
-    if (e2) {
+    if constexpr (E2) {
         const f4w c2 = *(const f4w*)(e2 + t);
         float l2[4];
         lap4(e2, t, i0, j, k, W, H, D, c2, l2);
@@ -589,7 +650,11 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
         for (int q = 4 * threadIdx.x; q < (1 << (3 * lf)); q += 4 * BS) {
             const int i0 = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
             if (i0 >= W || j >= H || k >= D) continue;
-            blockdots_cell4(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
+            const size_t t = ((size_t)k * H + j) * W + i0;
+            if (e1 && e2) blockdots_cell4<true, true>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
+            else if (e1) blockdots_cell4<true, false>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
+            else if (e2) blockdots_cell4<false, true>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
+            else blockdots_cell4<false, false>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
         }
     } else {
         // F = 2^lf: the block's F^3 cells by shifts, cells past the far faces skipped
@@ -635,6 +700,7 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
 // reads whole 1 KB rows instead of one block's 4F-byte pieces; waves take
 // the F x F rows in turn. Block I = lane / (F/4): its lanes are reduced by
 // shuffles, the waves through LDS in a fixed order.
+template <bool E1, bool E2, bool SYM = false>
 __global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __restrict__ r,
                                                                const float* __restrict__ e1,
                                                                const double* __restrict__ alpha1,
@@ -651,9 +717,11 @@ __global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __re
     const int y0 = F * J, z0 = F * K;
     const int ny = min(F, H - y0), nz = min(F, D - z0);
     if (i0 < W)
+        // unrolled so that the next rows' loads issue before this row's sums
+#pragma unroll 4
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
-            blockdots_cell4(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
+            blockdots_cell4<E1, E2, SYM>(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
         }
     const int gl = F >> 2; // lanes per block
     double d[5];
@@ -1243,11 +1311,26 @@ inline int ilog2(int f) {
 // the block pass: a workgroup per row of blocks where a row is at most 64
 // float4s, else a workgroup per block (float4 steps where rows split into
 // whole float4s)
+// sym: only the scale of e2 is wanted (see blockdots_cell4's SYM)
 void launch_blockdots(const Solver& s, const float* r, const float* e1, const double* a1, const float* e2,
-                      double* partial, double* sg, const SolveCtl* c) {
+                      double* partial, double* sg, const SolveCtl* c, bool sym = false) {
     if (s.W % 4 == 0 && s.W <= 256 && s.ws_g >= 4 && s.Gw <= 16) {
-        hipLaunchKernelGGL(k_ws_blockdots_rows, dim3((unsigned)(s.Gh * s.Gd)), dim3(64 * RW), 0, s.stream, r, e1, a1,
-                           e2, s.W, s.H, s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c);
+        const dim3 g((unsigned)(s.Gh * s.Gd)), b(64 * RW);
+        if (e1 && e2 && sym && !sg)
+            hipLaunchKernelGGL((k_ws_blockdots_rows<true, true, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H,
+                               s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c);
+        else if (e1 && e2)
+            hipLaunchKernelGGL((k_ws_blockdots_rows<true, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
+        else if (e1)
+            hipLaunchKernelGGL((k_ws_blockdots_rows<true, false>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
+        else if (e2)
+            hipLaunchKernelGGL((k_ws_blockdots_rows<false, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
+        else
+            hipLaunchKernelGGL((k_ws_blockdots_rows<false, false>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
         return;
     }
     const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
@@ -1479,7 +1562,7 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
         }
         if (int e = predict()) return e;
         launch_prolong(s, ws_enet(s), s.ws_net->out, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
-        launch_blockdots(s, ws_r(s), ws_eg(s), a_g, ws_enet(s), ws_part(s), nullptr, c);
+        launch_blockdots(s, ws_r(s), ws_eg(s), a_g, ws_enet(s), ws_part(s), nullptr, c, true);
         hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
         RB bk;
         bk.c[0] = spare;
