@@ -123,3 +123,27 @@ def test_dct_training_target_solves_the_coarse_problem():
         assert k > 0
         assert np.abs(k * lap - r).max() < 1e-4 * np.abs(r).max()
         assert abs(e.mean()) < 1e-6 and abs(np.sqrt((e * e).mean()) - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(10, 8, 6), (7, 5, 9)])
+def test_scale_after_a_first_correction_by_symmetry(shape):
+    """The network's scale is taken against r' = r - a1 A e1 without forming
+    A e1 per cell: <r', e2> = <r, e2> - a1 <e1, A e2> (A symmetric) and
+    sum r' = sum r (A e1 sums to zero over the grid). Both identities hold
+    for the Neumann graph Laplacian; a sign or transpose slip fails here."""
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal(shape)
+    e1 = rng.standard_normal(shape)
+    e2 = rng.standard_normal(shape)
+    a1 = 0.37
+    rp = r - a1 * C.laplacian(e1)
+    direct = float((rp * e2).sum())
+    via_sym = float((r * e2).sum()) - a1 * float((e1 * C.laplacian(e2)).sum())
+    assert abs(direct - via_sym) <= 1e-10 * (abs(direct) + 1.0)
+    assert abs(float(C.laplacian(e1).sum())) <= 1e-10 * float(np.abs(e1).sum())
+    assert abs(float(rp.sum()) - float(r.sum())) <= 1e-10 * float(np.abs(r).sum())
+    # and the resulting energy scale equals the one taken against r' itself
+    den = float((e2 * C.laplacian(e2)).sum())
+    n = e2.size
+    al_sym = (via_sym - float(r.sum()) * float(e2.sum()) / n) / den
+    assert abs(al_sym - C.energy_alpha(rp, e2)) <= 1e-10 * (abs(al_sym) + 1.0)
