@@ -66,11 +66,16 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
 // consecutive ranges of the chunks, and the first adds the others' FP32
 // tiles from LDS in split order (fixed, so every run gives the same bits).
 // A workgroup holds 4 / KS tiles.
+//
+// U (optional): a view of the grid twice as fine in each axis that also
+// receives every output value at the eight children of its voxel (the
+// nearest 2x upsampling of the decoder, fused into the producing layer).
 template <int CIN, int KS>
 __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, int Cx, int xoff,
                                                    const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
                                                    int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
-                                                   int relu, Geo g, long long nvox) {
+                                                   int relu, Geo g, long long nvox, bf16* __restrict__ U, int Cu,
+                                                   int uoff) {
     constexpr int NCH = (27 * CIN + 31) / 32;
     constexpr int NPER = (NCH + KS - 1) / KS; // chunks per split
     const int c0 = blockIdx.y * 16;
@@ -148,7 +153,19 @@ __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, i
         const int co = c0 + li;
         if (relu) r = r > 0.0f ? r : NET_LEAK * r;
         if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : NET_LEAK * r;
-        Y[vo * Cy + yoff + co] = f2bf(r);
+        const bf16 out = f2bf(r);
+        Y[vo * Cy + yoff + co] = out;
+        if (U) {
+            int ub, uz, uy, ux;
+            vox_coords(vo, g, ub, uz, uy, ux);
+            const long long FH = 2 * (long long)g.H, FW = 2 * (long long)g.W;
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                const long long vf =
+                    (((long long)ub * 2 * g.D + 2 * uz + (d >> 2)) * FH + 2 * uy + ((d >> 1) & 1)) * FW + 2 * ux + (d & 1);
+                U[vf * Cu + uoff + co] = out;
+            }
+        }
     }
 }
 
@@ -255,18 +272,6 @@ __global__ void k_pool_bwd(const bf16* __restrict__ gP, int Cp, int poff, bf16* 
     const long long vp = parent_of(v, fine, b, z, y, x);
     const float r = bf2f(gS[v * Cgs + gsoff + c]) + bf2f(gP[vp * Cp + poff + c]) * 0.125f;
     gS[v * Cgs + gsoff + c] = f2bf(bf2f(S[v * Cs + soff + c]) > 0.0f ? r : NET_LEAK * r);
-}
-
-// nearest 2x upsampling of a coarse view into a fine view
-__global__ void k_up_fwd(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd, int doff, int C,
-                         Geo fine, long long nfine) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nfine * C) return;
-    const int c = (int)(t % C);
-    const long long v = t / C;
-    int b, z, y, x;
-    const long long vp = parent_of(v, fine, b, z, y, x);
-    Dd[v * Cd + doff + c] = S[vp * Cs + soff + c];
 }
 
 // upsampling backward: gB[parent] = sum of the 8 children's gradients, then
@@ -413,7 +418,9 @@ inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs)
         }                                                            \
     } while (0)
 
-static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const NetDims& d, bool transposed) {
+// up (optional): the upsampling target of the output (see k_conv3_k32's U)
+static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const NetDims& d, bool transposed,
+                    TV up = TV{nullptr, 0, 0}) {
     const int cin = transposed ? kConv[layer].cout : kConv[layer].cin;
     const int cout = transposed ? kConv[layer].cin : kConv[layer].cout;
     const bf16* w = (transposed ? n.Wt : n.Wb) + n.off[layer];
@@ -433,7 +440,7 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     const dim3 gr(nblk(tiles, 4 / ks), (unsigned)(cout / 16));
 #define CONV_KS(CIN, KS)                                                                                           \
     hipLaunchKernelGGL((k_conv3_k32<CIN, KS>), gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, y.p, y.C,    \
-                       y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv)
+                       y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv, up.p, up.C, up.off)
 #define CONV_CASE(CIN)                                                                                             \
     case CIN:                                                                                                      \
         if (ks == 4) CONV_KS(CIN, 4);                                                                              \
@@ -605,11 +612,10 @@ int Net::forward() {
     if ((e = conv_fwd(*this, L_E2B, TV{a2, 32, 0}, TV{cat2, 96, 64}, none, true, d2, false))) return e;
     hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(n3 * 32, 256)), dim3(256), 0, stream, cat2, 96, 64, p2, 32, 0, 32, g2, n3);
     if ((e = conv_fwd(*this, L_E3A, TV{p2, 32, 0}, TV{a3, 64, 0}, none, true, d3, false))) return e;
-    if ((e = conv_fwd(*this, L_E3B, TV{a3, 64, 0}, TV{b3, 64, 0}, none, true, d3, false))) return e;
-    hipLaunchKernelGGL(k_up_fwd, dim3(nblk(n2 * 64, 256)), dim3(256), 0, stream, b3, 64, 0, cat2, 96, 0, 64, g2, n2);
+    // b3 and b4 also written upsampled into the skip concatenations
+    if ((e = conv_fwd(*this, L_E3B, TV{a3, 64, 0}, TV{b3, 64, 0}, none, true, d3, false, TV{cat2, 96, 0}))) return e;
     if ((e = conv_fwd(*this, L_D2A, TV{cat2, 96, 0}, TV{a4, 32, 0}, none, true, d2, false))) return e;
-    if ((e = conv_fwd(*this, L_D2B, TV{a4, 32, 0}, TV{b4, 32, 0}, none, true, d2, false))) return e;
-    hipLaunchKernelGGL(k_up_fwd, dim3(nblk(n1 * 32, 256)), dim3(256), 0, stream, b4, 32, 0, cat1, 48, 0, 32, g1, n1);
+    if ((e = conv_fwd(*this, L_D2B, TV{a4, 32, 0}, TV{b4, 32, 0}, none, true, d2, false, TV{cat1, 48, 0}))) return e;
     if ((e = conv_fwd(*this, L_D1A, TV{cat1, 48, 0}, TV{a5, 16, 0}, none, true, d1, false))) return e;
     if ((e = conv_fwd(*this, L_D1B, TV{a5, 16, 0}, TV{b5, 16, 0}, none, true, d1, false))) return e;
     hipLaunchKernelGGL(k_head_fwd, dim3(nblk(n1, 256)), dim3(256), 0, stream, b5, P + off[L_NCONV], out, n1);
