@@ -180,6 +180,7 @@ def _load():
         "blz_autotune_rbgs": ([vp, C.c_int, P(C.c_int), P(C.c_float)], C.c_int),
         "blz_warm_start_apply": ([vp, C.c_float, P(C.c_float), P(C.c_double), P(C.c_double), P(C.c_int)], C.c_int),
         "blz_warm_start_global": ([vp, C.c_float], C.c_int),
+        "blz_warm_start_input_global": ([vp, C.c_float, P(C.c_float)], C.c_int),
         "blz_batch_create": ([P(Settings), C.c_int, P(vp)], C.c_int),
         "blz_batch_destroy": ([vp], C.c_int),
         "blz_batch_solver": ([vp, C.c_int, P(vp)], C.c_int),
@@ -430,12 +431,18 @@ class Solver:
         """Apply the global correction alone, unjudged (training data)."""
         _check(_lib.blz_warm_start_global(self._h, C.c_float(dt)), "blz_warm_start_global")
 
-    def warm_start_input(self, dt: float) -> np.ndarray:
-        """(2, Dn, Hn, Wn) float32: normalised coarse residual and solid mask."""
+    def warm_start_input(self, dt: float, after_global: bool = False) -> np.ndarray:
+        """(2, Dn, Hn, Wn) float32: normalised coarse residual and solid mask.
+        after_global: the input a correction of the default order feeds the
+        network (the residual the global correction would leave; the
+        pressure is not changed)."""
         (wn, hn, dn), _ = self.warm_start_dims()
         out = np.empty((2, dn, hn, wn), np.float32)
-        _check(_lib.blz_warm_start_input(self._h, C.c_float(dt), out.ctypes.data_as(C.POINTER(C.c_float)),
-                                         None, None), "blz_warm_start_input")
+        ptr = out.ctypes.data_as(C.POINTER(C.c_float))
+        if after_global:
+            _check(_lib.blz_warm_start_input_global(self._h, C.c_float(dt), ptr), "blz_warm_start_input_global")
+        else:
+            _check(_lib.blz_warm_start_input(self._h, C.c_float(dt), ptr, None, None), "blz_warm_start_input")
         return out
 
 

@@ -901,6 +901,20 @@ int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], i
     return blz::warm_start_input(s, dt, in2);
 }
 
+int blz_warm_start_input_global(blz_solver* h, float dt, float* in2) {
+    CHECK_HANDLE(h);
+    Solver& s = h->s;
+    if (!s.ws_net) {
+        blz::set_error("no warm-start network attached (blz_set_warm_start)");
+        return BLZ_E_INVALID;
+    }
+    if (!in2 || !(dt > 0.0f)) {
+        blz::set_error("blz_warm_start_input_global: need an output array and dt > 0");
+        return BLZ_E_INVALID;
+    }
+    return blz::warm_start_input_global(s, dt, in2);
+}
+
 int blz_autotune_rbgs(blz_solver* h, int sweeps, int* variant, float* ms_per_sweep) {
     CHECK_HANDLE(h);
     int best = 0;

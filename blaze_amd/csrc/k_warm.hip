@@ -1492,6 +1492,48 @@ int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, 
     return warm_start_judge(s, dt, c, host_done, seq, tol, cap, judge, predicted);
 }
 
+// The default order's first half, from the home planes: r, the global
+// correction (block sums, coarse solve, prolongation e_G) with its scale
+// alpha_G, and the network's input restricted from r - alpha_G A e_G, the
+// residual the global correction leaves; nothing is added to the pressure
+// yet. begin: the control block whose judge state the residual launch
+// resets (a new application), or none.
+static int ws_global_and_input(Solver& s, float dt, SolveCtl* c, SolveCtl* begin) {
+    const size_t nn = ws_nn(s);
+    float* res = s.ws_buf;
+    float* mask = s.ws_buf + nn;
+    unsigned* amax = (unsigned*)(s.ws_buf + 2 * nn);
+    const float scale = (s.s.density * s.dx * s.dx) / dt;
+    const size_t N = ncell(s);
+    const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
+    double* a_g = ws_alpha(s) + 1;
+    hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
+                       scale, ws_r(s), &c->done, begin);
+    launch_blockdots(s, ws_r(s), nullptr, nullptr, nullptr, ws_part(s), ws_sg(s), c);
+    if (s.W % s.ws_g == 0 && s.H % s.ws_g == 0 && s.D % s.ws_g == 0)
+        hipLaunchKernelGGL(k_ws_gdct, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh, s.Gd,
+                           s.ws_g, s.W, s.H, s.D, c);
+    else
+        hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh, s.Gd,
+                           s.ws_g, s.W, s.H, s.D, 400, c);
+    launch_prolong(s, ws_eg(s), ws_xg(s), 1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
+    // S: 2 nn doubles after the three natural-order fields in s.tmp
+    if (restrict_rows_ok(s) && 3 * N + 4 * nn <= s.tmp_floats) {
+        // alpha_G and the network's input from one pass over the fine grid
+        double* S = (double*)(s.tmp + 3 * N);
+        hipLaunchKernelGGL(k_ws_restrict_dots_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream, ws_r(s),
+                           ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s), &c->done);
+        hipLaunchKernelGGL(k_ws_alpha_res, dim3(1), dim3(1024), 0, s.stream, ws_part(s), (int)(nn / s.Wn), (double)N,
+                           a_g, S, res, mask, amax, s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, (int)nn,
+                           &c->done);
+        BLZ_HIP(hipGetLastError());
+        return BLZ_OK;
+    }
+    launch_blockdots(s, ws_r(s), nullptr, nullptr, ws_eg(s), ws_part(s), nullptr, c);
+    hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
+    return ws_restrict(s, dt, &c->done, false, ws_eg(s), a_g);
+}
+
 // the corrections of one application (the uncorrected pressure saved to the
 // judge planes first), without the judge's sweeps
 int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
@@ -1534,32 +1576,7 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
         const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
         double* a_net = ws_alpha(s);
         double* a_g = ws_alpha(s) + 1;
-        hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H,
-                           s.D, scale, ws_r(s), &c->done, c);
-        launch_blockdots(s, ws_r(s), nullptr, nullptr, nullptr, ws_part(s), ws_sg(s), c);
-        if (s.W % s.ws_g == 0 && s.H % s.ws_g == 0 && s.D % s.ws_g == 0)
-            hipLaunchKernelGGL(k_ws_gdct, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh,
-                               s.Gd, s.ws_g, s.W, s.H, s.D, c);
-        else
-            hipLaunchKernelGGL(k_ws_gsolve, dim3(1), dim3(GT), 0, s.stream, ws_sg(s), nullptr, ws_xg(s), s.Gw, s.Gh,
-                               s.Gd, s.ws_g, s.W, s.H, s.D, 400, c);
-        launch_prolong(s, ws_eg(s), ws_xg(s), 1.0f / (float)s.ws_g, s.Gw, s.Gh, s.Gd, s.Gw, s.Gh, c);
-        // S: 2 nn doubles after the three natural-order fields in s.tmp
-        if (restrict_rows_ok(s) && 3 * N + 4 * nn <= s.tmp_floats) {
-            // alpha_G and the network's input from one pass over the fine grid
-            double* S = (double*)(s.tmp + 3 * N);
-            hipLaunchKernelGGL(k_ws_restrict_dots_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream,
-                               ws_r(s), ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s),
-                               &c->done);
-            hipLaunchKernelGGL(k_ws_alpha_res, dim3(1), dim3(1024), 0, s.stream, ws_part(s), (int)(nn / s.Wn),
-                               (double)N, a_g, S, res, mask, amax, s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn,
-                               s.Hn, (int)nn, &c->done);
-            BLZ_HIP(hipGetLastError());
-        } else {
-            launch_blockdots(s, ws_r(s), nullptr, nullptr, ws_eg(s), ws_part(s), nullptr, c);
-            hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_g, c);
-            if (int e = ws_restrict(s, dt, &c->done, false, ws_eg(s), a_g)) return e;
-        }
+        if (int e = ws_global_and_input(s, dt, c, c)) return e;
         if (int e = predict()) return e;
         launch_prolong(s, ws_enet(s), s.ws_net->out, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
         launch_blockdots(s, ws_r(s), ws_eg(s), a_g, ws_enet(s), ws_part(s), nullptr, c, true);
@@ -1665,6 +1682,23 @@ int warm_start_global(Solver& s, float dt) {
     hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W, s.H, s.D,
                        scale, ws_r(s), &c->done, nullptr);
     if (int e = ws_correct(s, c, nullptr, true)) return e;
+    BLZ_HIP(hipStreamSynchronize(s.stream));
+    return BLZ_OK;
+}
+
+// The network's input as a correction of the default order meets it:
+// the global correction's scale chosen and the residual it would leave
+// restricted, the pressure unchanged; synchronous
+int warm_start_input_global(Solver& s, float dt, float* host_in2) {
+    const size_t nn = ws_nn(s);
+    SolveCtl* c = s.ctl;
+    if (int e = launch_ctl_reset(s, c)) return e;
+    if (int e = ws_global_and_input(s, dt, c, nullptr)) return e;
+    float* out2 = (float*)s.tmp;
+    hipLaunchKernelGGL(k_ws_export, dim3(nb(nn)), dim3(BS), 0, s.stream, s.ws_buf, s.ws_buf + nn,
+                       (const unsigned*)(s.ws_buf + 2 * nn), out2, (int)nn);
+    BLZ_HIP(hipGetLastError());
+    BLZ_HIP(hipMemcpyAsync(host_in2, out2, 2 * nn * sizeof(float), hipMemcpyDeviceToHost, s.stream));
     BLZ_HIP(hipStreamSynchronize(s.stream));
     return BLZ_OK;
 }

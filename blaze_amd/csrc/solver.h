@@ -293,6 +293,7 @@ int warm_start_attach(Solver& s, const Net* net);
 int warm_start_apply(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
                      bool predicted = false);
 int warm_start_input(Solver& s, float dt, float* host_in2);
+int warm_start_input_global(Solver& s, float dt, float* host_in2);
 // warm_start_apply in two parts: the corrections, then the judge's sweeps,
 // residuals and decision (a batch runs the second part for all scenes at once)
 int warm_start_prepare(Solver& s, float dt, SolveCtl* c);

@@ -339,6 +339,13 @@ int blz_warm_start_input(blz_solver* h, float dt, float* in2, int net_dims[3], i
    solve (blz_warm_start_input afterwards gives that input; for generating
    training data). */
 int blz_warm_start_global(blz_solver* h, float dt);
+/* No reference counterpart. The network's input as a correction of the
+   default order meets it: the global correction's scale chosen and the
+   residual it would leave, r - alpha_G A e_G, restricted and normalised,
+   with the pressure left unchanged (in2 as for blz_warm_start_input). For
+   checking the input the solver computes without adding the global
+   correction first. */
+int blz_warm_start_input_global(blz_solver* h, float dt, float* in2);
 /* The correction step with a caller-given coarse prediction (network grid,
    x fastest) in place of the network's, scaled to the least error energy and
    followed by the global correction when enabled: FP64 residual before and

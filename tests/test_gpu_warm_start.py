@@ -168,6 +168,54 @@ def test_network_input_is_the_restricted_residual(res):
     assert not got[0][pad].any() and (got[1][pad] == 1.0).all()
 
 
+@pytest.mark.parametrize("res", [(64, 48, 40), (66, 20, 36)])
+def test_network_input_after_global_correction(res):
+    """The input the network gets in a correction of the default order:
+    the residual the global correction leaves, r - alpha_G A e_G, restricted
+    and normalised, computed without adding the correction
+    (blz_warm_start_input_global; width 64 takes the one-pass restriction
+    with the global scale's dots, 66 the per-cell passes). Checked against
+    the same input after the correction is added (blz_warm_start_global,
+    then blz_warm_start_input: the residual recomputed from the corrected
+    pressure) and against the oracle's global correction
+    (oracle/coarse_oracle.py, FP64)."""
+    C = pytest.importorskip("oracle.coarse_oracle")
+    print(f"[ws] network input after the global correction {res}", flush=True)
+    W, H, D = res
+    s = settings(B, res, max_iter_solve=10, residual_tol=1e-7, residual_check_every=10, warm_start=1)
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    rng = np.random.default_rng(23)
+    # a smooth right-hand side (the part the global correction removes) plus noise
+    rhs = (np.sin(np.pi * (x + 0.5) / W) * np.cos(np.pi * (y + 0.5) / H) * np.sin(2 * np.pi * (z + 0.5) / D)
+           + 0.05 * rng.standard_normal((D, H, W))).astype(np.float32)
+    p0 = (0.01 * rng.standard_normal((D, H, W))).astype(np.float32)
+    g = B.Solver(s)
+    g.set_field(B.FIELD_RHS, rhs)
+    g.set_field(B.FIELD_P, p0)
+    g.set_warm_start(B.Net(seed=0))
+    (wn, hn, dn), (wc, hc, dc) = g.warm_start_dims()
+    dt = 1.0 / 24.0
+    fused = g.warm_start_input(dt, after_global=True)
+    assert np.array_equal(g.field(B.FIELD_P), p0)  # the pressure is unchanged
+    g.warm_start_global(dt)
+    applied = g.warm_start_input(dt)
+    F = next(f for f in (1, 2, 4, 8, 16, 32, 64) if (-(-W // f), -(-H // f), -(-D // f)) == (wc, hc, dc))
+    scale = float(np.float32(np.float32(s.density * s.dx) * np.float32(s.dx)) / np.float32(dt))
+    pc, _ = C.global_correction(p0.astype(np.float64), rhs, scale, C.block_size((D, H, W)))
+    res_c = C.block_sums(C.residual(pc, rhs, scale), F) / F
+    want = res_c / np.sqrt(np.mean(res_c ** 2))
+    a = fused[0, :dc, :hc, :wc]
+    e_app = np.abs(a - applied[0, :dc, :hc, :wc]).max()
+    e_orc = np.abs(a - want).max()
+    print(f"[ws] F={F} max |fused - applied| {e_app:.2e}, max |fused - oracle| {e_orc:.2e}", flush=True)
+    # measured: 4e-5 / 9e-6 against the applied path (FP32 rounding of the
+    # corrected pressure), 7e-4 / 8e-6 against the oracle (the GPU's FP32
+    # prolongation and block passes against FP64), on normalised values
+    assert e_app <= 1e-3
+    assert e_orc <= 5e-3
+    assert np.array_equal(fused[1], applied[1])
+
+
 def run_frames(scene, res, frames, tol, cap, net, start=0, every_ws=1, network=True, global_correction=True):
     """Frames `start`..`start+frames-1` of `scene` at grid `res`, stage by
     stage; returns the sweeps of each frame's solve, the kept/rejected
