@@ -42,6 +42,13 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
     b = (int)(t / g.D);
 }
 
+// index in the grid twice as fine of the first child (2z, 2y, 2x) of voxel v
+__device__ __forceinline__ long long up_child0(long long v, const Geo& g) {
+    int b, z, y, x;
+    vox_coords(v, g, b, z, y, x);
+    return (((long long)b * 2 * g.D + 2 * z) * (2 * (long long)g.H) + 2 * y) * (2 * (long long)g.W) + 2 * x;
+}
+
 // 3x3x3 convolution, zero padding: Y[v][co] = sum_tap sum_ci X[v+tap][ci] W[tap][co][ci],
 // as an implicit GEMM on gfx950's v_mfma_f32_16x16x32_bf16 (A 16 x 32,
 // B 32 x 16, D 16 x 16 in FP32). The reduction runs over K = 27 * CIN
@@ -67,10 +74,11 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
 // tiles from LDS in split order (fixed, so every run gives the same bits).
 // A workgroup holds 4 / KS tiles.
 //
-// U (optional): a view of the grid twice as fine in each axis that also
+// UP: U is a view of the grid twice as fine in each axis that also
 // receives every output value at the eight children of its voxel (the
-// nearest 2x upsampling of the decoder, fused into the producing layer).
-template <int CIN, int KS>
+// nearest 2x upsampling of the decoder, fused into the producing layer); a
+// compile-time choice, so the other layers carry none of its code.
+template <int CIN, int KS, bool UP>
 __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, int Cx, int xoff,
                                                    const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
                                                    int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
@@ -155,14 +163,12 @@ __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, i
         if (Mk) r = bf2f(Mk[vo * Cm + moff + co]) > 0.0f ? r : NET_LEAK * r;
         const bf16 out = f2bf(r);
         Y[vo * Cy + yoff + co] = out;
-        if (U) {
-            int ub, uz, uy, ux;
-            vox_coords(vo, g, ub, uz, uy, ux);
-            const long long FH = 2 * (long long)g.H, FW = 2 * (long long)g.W;
+        if constexpr (UP) {
+            const long long f0 = up_child0(vo, g);
+            const long long FW = 2 * (long long)g.W, FHW = 4 * (long long)g.H * g.W;
 #pragma unroll
             for (int d = 0; d < 8; ++d) {
-                const long long vf =
-                    (((long long)ub * 2 * g.D + 2 * uz + (d >> 2)) * FH + 2 * uy + ((d >> 1) & 1)) * FW + 2 * ux + (d & 1);
+                const long long vf = f0 + (d >> 2) * FHW + ((d >> 1) & 1) * FW + (d & 1);
                 U[vf * Cu + uoff + co] = out;
             }
         }
@@ -439,8 +445,15 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     const int ks = (waves <= 1024 && nch >= 12) ? 4 : (waves <= 4096 && nch >= 8) ? 2 : 1;
     const dim3 gr(nblk(tiles, 4 / ks), (unsigned)(cout / 16));
 #define CONV_KS(CIN, KS)                                                                                           \
-    hipLaunchKernelGGL((k_conv3_k32<CIN, KS>), gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, y.p, y.C,    \
-                       y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv, up.p, up.C, up.off)
+    do {                                                                                                           \
+        if (up.p)                                                                                                  \
+            hipLaunchKernelGGL((k_conv3_k32<CIN, KS, true>), gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w, cout, \
+                               y.p, y.C, y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv, up.p, up.C, up.off); \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_conv3_k32<CIN, KS, false>), gr, dim3(256), 0, n.stream, x.p, x.C, x.off, w,     \
+                               cout, y.p, y.C, y.off, mask.p, mask.C, mask.off, relu ? 1 : 0, g, nv, up.p, up.C,   \
+                               up.off);                                                                            \
+    } while (0)
 #define CONV_CASE(CIN)                                                                                             \
     case CIN:                                                                                                      \
         if (ks == 4) CONV_KS(CIN, 4);                                                                              \
