@@ -49,6 +49,11 @@ struct Batch {
     ResScene* rsj2 = nullptr;   // n: rsj gated the same way
     std::vector<int> warm_q;   // scenes of these tables (empty: per-scene judges)
     bool multi_reapply = false; // re-applications also as multi-grid launches (one common interval and cap)
+    // the network forwards of the warm-started scenes as one launch sequence
+    // on the root stream, between the two halves of each scene's preparation
+    // (when every scene of warm_q runs the default order with one network)
+    NetMulti netm;
+    bool joint_fwd = false;
     int w_fs_blocks = 0, w_rs_blocks = 0, w_restore_blocks = 0;
     std::vector<FusedScene> fs_h;
     std::vector<ResScene> rs_h;
@@ -66,6 +71,7 @@ struct Batch {
     int build_judge_tables();
     int capture(int which);
     int enqueue_step(hipStream_t st);
+    int joint_forward_and_finish(hipStream_t st);
     int step(float time);
     int autotune(int sweeps, int* best);
 };
@@ -221,7 +227,21 @@ int Batch::build_judge_tables() {
     if (!ok) warm_q.clear();
     const int nw = (int)warm_q.size();
     multi_reapply = false;
+    joint_fwd = false;
     if (!nw) return BLZ_OK;
+    bool joint = true;
+    std::vector<Net*> views;
+    std::vector<const int*> stops;
+    for (int q : warm_q) {
+        const Solver& s = sc[q]->s;
+        joint = joint && warm_start_splits(s) && s.ws_src == sc[warm_q[0]]->s.ws_src;
+        views.push_back(s.ws_net);
+        stops.push_back(&s.ctl[0].done);
+    }
+    if (joint) {
+        if (int e = netm.build(views, stops)) return e;
+        joint_fwd = true;
+    }
     multi_reapply = true;
     for (int q : warm_q) {
         const Solver& s = sc[q]->s;
@@ -293,6 +313,22 @@ int Batch::build_judge_tables() {
     return BLZ_OK;
 }
 
+// the network forwards of all warm-started scenes on `st` (their inputs
+// packed on the scenes' streams, joined into `st`), then the rest of each
+// scene's corrections on its stream, joined back
+int Batch::joint_forward_and_finish(hipStream_t st) {
+    if (int e = netm.forward(st)) return e;
+    BLZ_HIP(hipEventRecord(fork, st));
+    for (int q : warm_q) {
+        Solver& s = sc[q]->s;
+        BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
+        if (int e = warm_start_prepare_finish(s, s.ctl)) return e;
+        BLZ_HIP(hipEventRecord(join[q], s.stream));
+        BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+    }
+    return BLZ_OK;
+}
+
 // one step of every scene: per-scene stages on the scenes' streams, the
 // solves of all scenes on `st`
 int Batch::enqueue_step(hipStream_t st) {
@@ -323,7 +359,10 @@ int Batch::enqueue_step(hipStream_t st) {
         if (int e = launch_ctl_reset(s, c)) return e;
         if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, tol, cap, 0, 1)) return e;
         if (s.ws_net && s.s.warm_start && cap > 0) {
-            if (!warm_q.empty()) {
+            if (joint_fwd) {
+                // network forward and the rest of the corrections below
+                if (int e = warm_start_prepare_input(s, dt, c)) return e;
+            } else if (!warm_q.empty()) {
                 if (int e = warm_start_prepare(s, dt, c)) return e; // judged below, with the other scenes
             } else if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap, every)) {
                 return e;
@@ -332,6 +371,8 @@ int Batch::enqueue_step(hipStream_t st) {
         BLZ_HIP(hipEventRecord(join[q], s.stream));
         BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
     }
+    if (joint_fwd)
+        if (int e = joint_forward_and_finish(st)) return e;
     if (!warm_q.empty()) {
         // the first judge of every warm-started scene: `every` sweeps on the
         // corrected pressure (home planes), the same on the uncorrected one
@@ -393,10 +434,16 @@ int Batch::enqueue_step(hipStream_t st) {
             for (int q : warm_q) {
                 Solver& s = sc[q]->s;
                 BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
-                if (int e = warm_start_prepare(s, s.s.dt, s.ctl)) return e;
+                if (joint_fwd) {
+                    if (int e = warm_start_prepare_input(s, s.s.dt, s.ctl)) return e;
+                } else if (int e = warm_start_prepare(s, s.s.dt, s.ctl)) {
+                    return e;
+                }
                 BLZ_HIP(hipEventRecord(join[q], s.stream));
                 BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
             }
+            if (joint_fwd)
+                if (int e = joint_forward_and_finish(st)) return e;
             for (int l = 0; l < every / 2; ++l) {
                 if (int e = launch_fused_multi(st, variant, fsw, nw, w_fs_blocks)) return e;
                 if (int e = launch_fused_multi(st, variant, fsw + nw, nw, w_fs_blocks)) return e;

@@ -78,12 +78,14 @@ __device__ __forceinline__ long long up_child0(long long v, const Geo& g) {
 // receives every output value at the eight children of its voxel (the
 // nearest 2x upsampling of the decoder, fused into the producing layer); a
 // compile-time choice, so the other layers carry none of its code.
+//
+// The body of one workgroup (tile group bx along the voxels, output tile
+// blockIdx.y), shared by the single-grid kernel and the multi-view one.
 template <int CIN, int KS, bool UP>
-__global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, int Cx, int xoff,
-                                                   const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
-                                                   int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
-                                                   int relu, Geo g, long long nvox, bf16* __restrict__ U, int Cu,
-                                                   int uoff) {
+__device__ __forceinline__ void conv3_tile(const bf16* __restrict__ X, int Cx, int xoff, const bf16* __restrict__ Wt,
+                                           int Cout, bf16* __restrict__ Y, int Cy, int yoff,
+                                           const bf16* __restrict__ Mk, int Cm, int moff, int relu, const Geo& g,
+                                           long long nvox, bf16* __restrict__ U, int Cu, int uoff, unsigned bx) {
     constexpr int NCH = (27 * CIN + 31) / 32;
     constexpr int NPER = (NCH + KS - 1) / KS; // chunks per split
     const int c0 = blockIdx.y * 16;
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, i
     // scalar branches, so every MFMA runs with the whole wave active (an
     // MFMA does not honour a partial or empty EXEC mask)
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), split = wv % KS;
-    const long long v0 = ((long long)blockIdx.x * (4 / KS) + wv / KS) * 16;
+    const long long v0 = ((long long)bx * (4 / KS) + wv / KS) * 16;
     // no early return: the waves of a workgroup meet at the split reduction
     // (a tile past the end computes zeros from clamped addresses and stores
     // nothing)
@@ -175,6 +177,46 @@ __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, i
     }
 }
 
+template <int CIN, int KS, bool UP>
+__global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, int Cx, int xoff,
+                                                   const bf16* __restrict__ Wt, int Cout, bf16* __restrict__ Y,
+                                                   int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
+                                                   int relu, Geo g, long long nvox, bf16* __restrict__ U, int Cu,
+                                                   int uoff) {
+    conv3_tile<CIN, KS, UP>(X, Cx, xoff, Wt, Cout, Y, Cy, yoff, Mk, Cm, moff, relu, g, nvox, U, Cu, uoff, blockIdx.x);
+}
+
+// One view's part of a multi-view launch: the forwards of several views of
+// one network (same weights, each on its own grid with its own activation
+// buffers) run a layer as one launch, blockIdx.z selecting the view. The
+// channel layout (counts and offsets) is the layer's and common to all.
+struct MView {
+    const bf16* X; // layer input (pooling: the fine activation)
+    bf16* Y;       // layer output (head: unused)
+    bf16* U;       // upsampled copy of the output (convolutions with UP)
+    float* out;    // head output
+    Geo g;         // grid of the input
+    long long n;   // voxels of the output (pooling: coarse voxels)
+    unsigned nblk; // workgroups of this view (the grid's x extent is the largest)
+    const int* stop; // nonzero: the view's output is not needed, its workgroups return
+};
+
+__device__ __forceinline__ bool mview_idle(const MView& m) {
+    return blockIdx.x >= m.nblk || (m.stop && *m.stop);
+}
+
+// the forward convolution (leaky ReLU epilogue) of every view in `tab`; all
+// views of one launch share the reduction split KS
+template <int CIN, int KS, bool UP>
+__global__ void __launch_bounds__(256) k_conv3_multi(const MView* __restrict__ tab, int Cx, int xoff,
+                                                     const bf16* __restrict__ Wt, int Cout, int Cy, int yoff, int Cu,
+                                                     int uoff) {
+    const MView m = tab[blockIdx.z];
+    if (mview_idle(m)) return; // the whole workgroup: no wave waits at the split reduction
+    conv3_tile<CIN, KS, UP>(m.X, Cx, xoff, Wt, Cout, m.Y, Cy, yoff, nullptr, 0, 0, 1, m.g, m.n, m.U, Cu, uoff,
+                            blockIdx.x);
+}
+
 // weight gradient: dW[tap][co][ci] += sum_v G[v][co] X[v+tap][ci]. A wave owns
 // one 16x16 (co, ci) tile and a run of voxels; per 16 voxels the G fragment
 // (A = 16 co x 16 voxels) is loaded once and used for all 27 taps (B = 16
@@ -248,9 +290,8 @@ __device__ __forceinline__ long long parent_of(long long v, const Geo& fine, int
 }
 
 // 2x2x2 average pooling (coarse voxel per thread-channel)
-__global__ void k_pool_fwd(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd, int doff,
-                           int C, Geo fine, long long ncoarse) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pool_one(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd,
+                                         int doff, int C, const Geo& fine, long long ncoarse, long long t) {
     if (t >= ncoarse * C) return;
     const int c = (int)(t % C);
     const long long vc = t / C;
@@ -264,6 +305,18 @@ __global__ void k_pool_fwd(const bf16* __restrict__ S, int Cs, int soff, bf16* _
         s += bf2f(S[vf * Cs + soff + c]);
     }
     Dd[vc * Cd + doff + c] = f2bf(s * 0.125f);
+}
+
+__global__ void k_pool_fwd(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd, int doff,
+                           int C, Geo fine, long long ncoarse) {
+    pool_one(S, Cs, soff, Dd, Cd, doff, C, fine, ncoarse, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// the pooling of every view in `tab` (X fine, Y coarse)
+__global__ void k_pool_multi(const MView* __restrict__ tab, int Cs, int soff, int Cd, int doff, int C) {
+    const MView m = tab[blockIdx.z];
+    if (mview_idle(m)) return;
+    pool_one(m.X, Cs, soff, m.Y, Cd, doff, C, m.g, m.n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // pooling backward added to the gradient of the pooled activation S, then
@@ -301,14 +354,25 @@ __global__ void k_up_bwd(const bf16* __restrict__ gU, int Cu, int uoff, bf16* __
 }
 
 // 1x1 head: out[v] = sum_c X[v][c] w[c] (FP32)
-__global__ void k_head_fwd(const bf16* __restrict__ X, const float* __restrict__ w, float* __restrict__ out,
-                           long long n) {
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void head_one(const bf16* __restrict__ X, const float* __restrict__ w,
+                                         float* __restrict__ out, long long n, long long v) {
     if (v >= n) return;
     float s = 0.0f;
 #pragma unroll
     for (int c = 0; c < NET_HEAD; ++c) s = fmaf(bf2f(X[v * NET_HEAD + c]), bf2f(f2bf(w[c])), s);
     out[v] = s;
+}
+
+__global__ void k_head_fwd(const bf16* __restrict__ X, const float* __restrict__ w, float* __restrict__ out,
+                           long long n) {
+    head_one(X, w, out, n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// the head of every view in `tab`
+__global__ void k_head_multi(const MView* __restrict__ tab, const float* __restrict__ w) {
+    const MView m = tab[blockIdx.z];
+    if (mview_idle(m)) return;
+    head_one(m.X, w, m.out, m.n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // loss = sum (out - target)^2 / n; gout = 2 (out - target) / n
@@ -424,6 +488,14 @@ inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs)
         }                                                            \
     } while (0)
 
+// reduction split of a forward convolution: over KS waves where the 16-voxel
+// tiles alone would give few waves (the 16^3 and 8^3 levels of a 32^3 input)
+static int conv_split(long long nv, int cin, int cout) {
+    const long long waves = (nv + 15) / 16 * (cout / 16);
+    const int nch = (27 * cin + 31) / 32;
+    return (waves <= 1024 && nch >= 12) ? 4 : (waves <= 4096 && nch >= 8) ? 2 : 1;
+}
+
 // up (optional): the upsampling target of the output (see k_conv3_k32's U)
 static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const NetDims& d, bool transposed,
                     TV up = TV{nullptr, 0, 0}) {
@@ -432,17 +504,13 @@ static int conv_fwd(Net& n, int layer, TV x, TV y, TV mask, bool relu, const Net
     const bf16* w = (transposed ? n.Wt : n.Wb) + n.off[layer];
     const Geo g{d.B, d.D, d.H, d.W};
     const long long nv = d.n();
-    // one 16-channel output tile per wave group (grid.y tiles) of 16
-    // voxels; the reduction split over KS waves where the tiles alone would
-    // give few waves (the 16^3 and 8^3 levels of a 32^3 input)
+    // one 16-channel output tile per wave group (grid.y tiles) of 16 voxels
     if (nv * std::max(std::max(x.C, y.C), mask.p ? mask.C : 0) >= (1ll << 31)) {
         set_error("network activations too large for 32-bit offsets");
         return BLZ_E_INVALID;
     }
     const long long tiles = (nv + 15) / 16;
-    const long long waves = tiles * (cout / 16);
-    const int nch = (27 * cin + 31) / 32;
-    const int ks = (waves <= 1024 && nch >= 12) ? 4 : (waves <= 4096 && nch >= 8) ? 2 : 1;
+    const int ks = conv_split(nv, cin, cout);
     const dim3 gr(nblk(tiles, 4 / ks), (unsigned)(cout / 16));
 #define CONV_KS(CIN, KS)                                                                                           \
     do {                                                                                                           \
@@ -612,26 +680,197 @@ int Net::ensure_dims(const NetDims& d) {
     return BLZ_OK;
 }
 
-int Net::forward() {
-    const NetDims d1 = dims, d2 = d1.half(), d3 = d2.half();
+// The forward pass as a list of steps over a view's buffers (level 0 is the
+// input grid, each level half the one before)
+struct FwdStep {
+    int kind;  // 0 convolution with leaky ReLU, 1 pooling (x fine -> y coarse), 2 head
+    int layer; // convolution layer
+    int level; // grid level of the input
+    int C;     // pooled channels
+    TV x, y, up;
+};
+constexpr int kFwdSteps = 13;
+
+static void forward_steps(const Net& n, FwdStep* st) {
     const TV none{nullptr, 0, 0};
-    const Geo g1{d1.B, d1.D, d1.H, d1.W}, g2{d2.B, d2.D, d2.H, d2.W};
-    const long long n1 = d1.n(), n2 = d2.n(), n3 = d3.n();
-    int e;
-    if ((e = conv_fwd(*this, L_E1A, TV{x0, 16, 0}, TV{a1, 16, 0}, none, true, d1, false))) return e;
-    if ((e = conv_fwd(*this, L_E1B, TV{a1, 16, 0}, TV{cat1, 48, 32}, none, true, d1, false))) return e;
-    hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(n2 * 16, 256)), dim3(256), 0, stream, cat1, 48, 32, p1, 16, 0, 16, g1, n2);
-    if ((e = conv_fwd(*this, L_E2A, TV{p1, 16, 0}, TV{a2, 32, 0}, none, true, d2, false))) return e;
-    if ((e = conv_fwd(*this, L_E2B, TV{a2, 32, 0}, TV{cat2, 96, 64}, none, true, d2, false))) return e;
-    hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(n3 * 32, 256)), dim3(256), 0, stream, cat2, 96, 64, p2, 32, 0, 32, g2, n3);
-    if ((e = conv_fwd(*this, L_E3A, TV{p2, 32, 0}, TV{a3, 64, 0}, none, true, d3, false))) return e;
+    int k = 0;
+    st[k++] = FwdStep{0, L_E1A, 0, 0, TV{n.x0, 16, 0}, TV{n.a1, 16, 0}, none};
+    st[k++] = FwdStep{0, L_E1B, 0, 0, TV{n.a1, 16, 0}, TV{n.cat1, 48, 32}, none};
+    st[k++] = FwdStep{1, -1, 0, 16, TV{n.cat1, 48, 32}, TV{n.p1, 16, 0}, none};
+    st[k++] = FwdStep{0, L_E2A, 1, 0, TV{n.p1, 16, 0}, TV{n.a2, 32, 0}, none};
+    st[k++] = FwdStep{0, L_E2B, 1, 0, TV{n.a2, 32, 0}, TV{n.cat2, 96, 64}, none};
+    st[k++] = FwdStep{1, -1, 1, 32, TV{n.cat2, 96, 64}, TV{n.p2, 32, 0}, none};
+    st[k++] = FwdStep{0, L_E3A, 2, 0, TV{n.p2, 32, 0}, TV{n.a3, 64, 0}, none};
     // b3 and b4 also written upsampled into the skip concatenations
-    if ((e = conv_fwd(*this, L_E3B, TV{a3, 64, 0}, TV{b3, 64, 0}, none, true, d3, false, TV{cat2, 96, 0}))) return e;
-    if ((e = conv_fwd(*this, L_D2A, TV{cat2, 96, 0}, TV{a4, 32, 0}, none, true, d2, false))) return e;
-    if ((e = conv_fwd(*this, L_D2B, TV{a4, 32, 0}, TV{b4, 32, 0}, none, true, d2, false, TV{cat1, 48, 0}))) return e;
-    if ((e = conv_fwd(*this, L_D1A, TV{cat1, 48, 0}, TV{a5, 16, 0}, none, true, d1, false))) return e;
-    if ((e = conv_fwd(*this, L_D1B, TV{a5, 16, 0}, TV{b5, 16, 0}, none, true, d1, false))) return e;
-    hipLaunchKernelGGL(k_head_fwd, dim3(nblk(n1, 256)), dim3(256), 0, stream, b5, P + off[L_NCONV], out, n1);
+    st[k++] = FwdStep{0, L_E3B, 2, 0, TV{n.a3, 64, 0}, TV{n.b3, 64, 0}, TV{n.cat2, 96, 0}};
+    st[k++] = FwdStep{0, L_D2A, 1, 0, TV{n.cat2, 96, 0}, TV{n.a4, 32, 0}, none};
+    st[k++] = FwdStep{0, L_D2B, 1, 0, TV{n.a4, 32, 0}, TV{n.b4, 32, 0}, TV{n.cat1, 48, 0}};
+    st[k++] = FwdStep{0, L_D1A, 0, 0, TV{n.cat1, 48, 0}, TV{n.a5, 16, 0}, none};
+    st[k++] = FwdStep{0, L_D1B, 0, 0, TV{n.a5, 16, 0}, TV{n.b5, 16, 0}, none};
+    st[k++] = FwdStep{2, -1, 0, 0, TV{n.b5, NET_HEAD, 0}, none, none};
+}
+
+static void level_dims(const NetDims& d, NetDims* lv) {
+    lv[0] = d;
+    lv[1] = d.half();
+    lv[2] = lv[1].half();
+    lv[3] = lv[2].half();
+}
+
+int Net::forward() {
+    FwdStep st[kFwdSteps];
+    forward_steps(*this, st);
+    NetDims lv[4];
+    level_dims(dims, lv);
+    const TV none{nullptr, 0, 0};
+    for (const FwdStep& f : st) {
+        const NetDims& d = lv[f.level];
+        if (f.kind == 0) {
+            if (int e = conv_fwd(*this, f.layer, f.x, f.y, none, true, d, false, f.up)) return e;
+        } else if (f.kind == 1) {
+            const long long nc = lv[f.level + 1].n();
+            hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(nc * f.C, 256)), dim3(256), 0, stream, f.x.p, f.x.C, f.x.off,
+                               f.y.p, f.y.C, f.y.off, f.C, Geo{d.B, d.D, d.H, d.W}, nc);
+        } else {
+            hipLaunchKernelGGL(k_head_fwd, dim3(nblk(d.n(), 256)), dim3(256), 0, stream, f.x.p, P + off[L_NCONV], out,
+                               d.n());
+        }
+    }
+    NET_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+NetMulti::~NetMulti() {
+    if (tab) (void)hipFree(tab);
+}
+
+int NetMulti::build(const std::vector<Net*>& views, const std::vector<const int*>& stop) {
+    const int nv = (int)views.size();
+    if (nv < 1 || nv > 65535 || (int)stop.size() != nv) {
+        set_error("multi-view forward: 1 to 65535 views, one stop flag each");
+        return BLZ_E_INVALID;
+    }
+    const Net& v0 = *views[0];
+    for (const Net* v : views)
+        if (v->Wb != v0.Wb || v->P != v0.P || !v->x0) {
+            set_error("multi-view forward: views of one network with activations");
+            return BLZ_E_INVALID;
+        }
+    std::vector<FwdStep> steps((size_t)nv * kFwdSteps);
+    std::vector<NetDims> lv((size_t)nv * 4);
+    for (int q = 0; q < nv; ++q) {
+        forward_steps(*views[q], &steps[(size_t)q * kFwdSteps]);
+        level_dims(views[q]->dims, &lv[(size_t)q * 4]);
+    }
+    std::vector<MView> t;
+    std::vector<Launch> pl;
+    for (int k = 0; k < kFwdSteps; ++k) {
+        const FwdStep& f0 = steps[k];
+        const int cin = f0.kind == 0 ? kConv[f0.layer].cin : 0, cout = f0.kind == 0 ? kConv[f0.layer].cout : 0;
+        // convolutions: one launch per reduction split in use
+        for (int ks : {4, 2, 1}) {
+            if (f0.kind != 0 && ks != 1) continue;
+            Launch L{f0.kind, f0.layer, ks, f0.C, f0.x, f0.y, f0.up, f0.kind == 0 ? v0.Wb + v0.off[f0.layer] : nullptr,
+                     (int)t.size(), 0, 0u};
+            for (int q = 0; q < nv; ++q) {
+                const FwdStep& f = steps[(size_t)q * kFwdSteps + k];
+                const NetDims& d = lv[(size_t)q * 4 + f.level];
+                MView m;
+                std::memset(&m, 0, sizeof m); // padding too: the table is compared bytewise
+                m.X = f.x.p;
+                m.Y = f.y.p;
+                m.U = f.up.p;
+                m.out = views[q]->out;
+                m.g = Geo{d.B, d.D, d.H, d.W};
+                m.n = d.n();
+                m.stop = stop[q];
+                if (f.kind == 0) {
+                    if (conv_split(d.n(), cin, cout) != ks) continue;
+                    if (d.n() * std::max(f.x.C, f.y.C) >= (1ll << 31)) {
+                        set_error("network activations too large for 32-bit offsets");
+                        return BLZ_E_INVALID;
+                    }
+                    m.nblk = nblk((d.n() + 15) / 16, 4 / ks);
+                } else if (f.kind == 1) {
+                    m.n = lv[(size_t)q * 4 + f.level + 1].n();
+                    m.nblk = nblk(m.n * f.C, 256);
+                } else {
+                    m.nblk = nblk(d.n(), 256);
+                }
+                t.push_back(m);
+                L.blocks = std::max(L.blocks, m.nblk);
+            }
+            L.count = (int)t.size() - L.first;
+            if (L.count) pl.push_back(L);
+        }
+    }
+    const size_t bytes = t.size() * sizeof(MView);
+    const unsigned char* raw = (const unsigned char*)t.data();
+    if (cur.size() != bytes || std::memcmp(cur.data(), raw, bytes) != 0) {
+        if (bytes > tab_bytes) {
+            if (tab) NET_HIP(hipFree(tab));
+            tab = nullptr;
+            tab_bytes = 0;
+            NET_HIP(hipMalloc(&tab, bytes));
+            tab_bytes = bytes;
+        }
+        NET_HIP(hipMemcpy(tab, raw, bytes, hipMemcpyHostToDevice));
+        cur.assign(raw, raw + bytes);
+    }
+    plan = pl;
+    head = v0.P + v0.off[L_NCONV];
+    return BLZ_OK;
+}
+
+template <int CIN>
+static void conv_multi(const NetMulti::Launch& L, const MView* tb, hipStream_t st) {
+    const int cout = kConv[L.layer].cout;
+    const dim3 gr(L.blocks, (unsigned)(cout / 16), (unsigned)L.count);
+#define CONV_MULTI(KS, UP)                                                                                         \
+    hipLaunchKernelGGL((k_conv3_multi<CIN, KS, UP>), gr, dim3(256), 0, st, tb, L.x.C, L.x.off, L.w, cout, L.y.C,   \
+                       L.y.off, L.up.C, L.up.off)
+    if (L.up.p) {
+        // the decoder's upsampled layers take 32 or 64 input channels
+        if constexpr (CIN == 32 || CIN == 64) {
+            if (L.ks == 4) CONV_MULTI(4, true);
+            else if (L.ks == 2) CONV_MULTI(2, true);
+            else CONV_MULTI(1, true);
+        }
+    } else if (L.ks == 4) {
+        CONV_MULTI(4, false);
+    } else if (L.ks == 2) {
+        CONV_MULTI(2, false);
+    } else {
+        CONV_MULTI(1, false);
+    }
+#undef CONV_MULTI
+}
+
+int NetMulti::forward(hipStream_t st) const {
+    const MView* t = (const MView*)tab;
+    for (const Launch& L : plan) {
+        const MView* tb = t + L.first;
+        if (L.kind == 0) {
+            const int cin = kConv[L.layer].cin;
+            if (L.up.p && cin != 32 && cin != 64) {
+                set_error("multi-view forward: no upsampled layer of this width");
+                return BLZ_E_INVALID;
+            }
+            switch (cin) {
+            case 16: conv_multi<16>(L, tb, st); break;
+            case 32: conv_multi<32>(L, tb, st); break;
+            case 48: conv_multi<48>(L, tb, st); break;
+            case 64: conv_multi<64>(L, tb, st); break;
+            case 96: conv_multi<96>(L, tb, st); break;
+            default: set_error("unsupported channel count"); return BLZ_E_INVALID;
+            }
+        } else if (L.kind == 1) {
+            hipLaunchKernelGGL(k_pool_multi, dim3(L.blocks, 1, (unsigned)L.count), dim3(256), 0, st, tb, L.x.C,
+                               L.x.off, L.y.C, L.y.off, L.C);
+        } else {
+            hipLaunchKernelGGL(k_head_multi, dim3(L.blocks, 1, (unsigned)L.count), dim3(256), 0, st, tb, head);
+        }
+    }
     NET_HIP(hipGetLastError());
     return BLZ_OK;
 }

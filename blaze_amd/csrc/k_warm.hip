@@ -1534,6 +1534,40 @@ static int ws_global_and_input(Solver& s, float dt, SolveCtl* c, SolveCtl* begin
     return ws_restrict(s, dt, &c->done, false, ws_eg(s), a_g);
 }
 
+// The default order (network and global correction, global first, no exact
+// coarse solve) split around the network's forward, for callers that run
+// the forwards of several scenes together (NetMulti): the first part ends
+// with the network's input packed into its view (s.ws_net->x0), the second
+// starts from its output (s.ws_net->out).
+bool warm_start_splits(const Solver& s) { return s.ws_net && (s.ws_parts & 15) == 3; }
+
+int warm_start_prepare_input(Solver& s, float dt, SolveCtl* c) {
+    const size_t nn = ws_nn(s);
+    if (int e = ws_global_and_input(s, dt, c, c)) return e;
+    hipLaunchKernelGGL(k_ws_pack, dim3(nb(nn * NET_IN_CH)), dim3(BS), 0, s.stream, s.ws_buf, s.ws_buf + nn,
+                       (const unsigned*)(s.ws_buf + 2 * nn), s.ws_net->x0, (int)nn);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int warm_start_prepare_finish(Solver& s, SolveCtl* c) {
+    const size_t N = ncell(s);
+    const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
+    double* a_net = ws_alpha(s);
+    double* a_g = ws_alpha(s) + 1;
+    float* spare = s.prb + (size_t)s.judge_set * s.Np;
+    launch_prolong(s, ws_enet(s), s.ws_net->out, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
+    launch_blockdots(s, ws_r(s), ws_eg(s), a_g, ws_enet(s), ws_part(s), nullptr, c, true);
+    hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
+    RB bk;
+    bk.c[0] = spare;
+    bk.c[1] = spare + s.Np;
+    hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, bk, s.L, s.W, s.H, s.D,
+                       ws_eg(s), a_g, ws_enet(s), a_net, c);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
 // the corrections of one application (the uncorrected pressure saved to the
 // judge planes first), without the judge's sweeps
 int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
@@ -1572,21 +1606,14 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c) {
         // global correction leaves, r - alpha_g A e_g, is evaluated where it
         // is used (the network's input and the network's scale), and one pass
         // adds both corrections and saves the uncorrected pressure.
-        const size_t N = ncell(s);
-        const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
-        double* a_net = ws_alpha(s);
-        double* a_g = ws_alpha(s) + 1;
-        if (int e = ws_global_and_input(s, dt, c, c)) return e;
-        if (int e = predict()) return e;
-        launch_prolong(s, ws_enet(s), s.ws_net->out, 1.0f / (float)s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, c);
-        launch_blockdots(s, ws_r(s), ws_eg(s), a_g, ws_enet(s), ws_part(s), nullptr, c, true);
-        hipLaunchKernelGGL(k_ws_alpha, dim3(1), dim3(BS), 0, s.stream, ws_part(s), (int)nbg, (double)N, a_net, c);
-        RB bk;
-        bk.c[0] = spare;
-        bk.c[1] = spare + s.Np;
-        hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, bk, s.L, s.W, s.H,
-                           s.D, ws_eg(s), a_g, ws_enet(s), a_net, c);
-        BLZ_HIP(hipGetLastError());
+        if (exact) {
+            if (int e = ws_global_and_input(s, dt, c, c)) return e;
+            if (int e = predict()) return e;
+        } else {
+            if (int e = warm_start_prepare_input(s, dt, c)) return e;
+            if (int e = s.ws_net->forward()) return e;
+        }
+        if (int e = warm_start_prepare_finish(s, c)) return e;
     } else {
         if (!coarse) {
             hipLaunchKernelGGL(k_ws_resid, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, s.rhs, s.L, s.W,

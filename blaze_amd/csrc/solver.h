@@ -297,6 +297,12 @@ int warm_start_input_global(Solver& s, float dt, float* host_in2);
 // warm_start_apply in two parts: the corrections, then the judge's sweeps,
 // residuals and decision (a batch runs the second part for all scenes at once)
 int warm_start_prepare(Solver& s, float dt, SolveCtl* c);
+// warm_start_prepare of the default order (warm_start_splits) in two parts
+// around the network's forward: up to its packed input, and from its output
+// (a batch runs the forwards of all scenes together in between)
+bool warm_start_splits(const Solver& s);
+int warm_start_prepare_input(Solver& s, float dt, SolveCtl* c);
+int warm_start_prepare_finish(Solver& s, SolveCtl* c);
 int warm_start_judge(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
                      bool predicted);
 // a batch's judges: decisions (one block per scene) and restores

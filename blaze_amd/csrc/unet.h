@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "common.h"
 
@@ -89,6 +90,34 @@ struct Net {
     int loss_and_grad();   // out, target -> gout, *loss (mean squared error)
     int backward();        // gout -> G (overwritten)
     int adam(float lr, float b1, float b2, float eps);
+};
+
+// The forwards of several views of one network (the same weights, each view
+// on its own grid with its own activations) as one sequence of launches: a
+// layer is one launch per reduction split in use, over a device table of
+// the views. Each view computes what its own forward() computes, bit for
+// bit (the same per-workgroup code and reduction split). A view whose stop
+// flag is set when a launch runs is skipped by it (its output is then
+// stale). Rebuilding with other views may move the table: graphs captured
+// from forward() hold its address.
+struct NetMulti {
+    struct Launch {
+        int kind;          // 0 convolution, 1 pooling, 2 head
+        int layer, ks, C;  // convolution layer and reduction split; pooled channels
+        TV x, y, up;       // channel layouts (the pointers are per view, in the table)
+        const bf16* w;     // convolution weights
+        int first, count;  // views [first, first + count) of the table
+        unsigned blocks;   // grid x extent (the largest view's)
+    };
+    std::vector<Launch> plan;
+    void* tab = nullptr; // device table
+    size_t tab_bytes = 0;
+    std::vector<unsigned char> cur; // its content
+    const float* head = nullptr;
+
+    ~NetMulti();
+    int build(const std::vector<Net*>& views, const std::vector<const int*>& stop);
+    int forward(hipStream_t st) const;
 };
 
 // launchers used by the warm start (stream-ordered)
