@@ -1,16 +1,20 @@
 // batch.hip — many independent scenes of uneven grids stepped together: one
 // step of every scene is one replay of a captured HIP graph, and the
-// pressure sweeps and residual checks of all scenes run as single launches
-// over a table of grids (k_rbgs_fused_multi, k_residual_multi), so small
-// grids share the GPU instead of being launch-bound one by one. The sweep
-// kernel's tile parameters (slab groups in flight, waves per block) come
-// from an autotuner that times each candidate on the batch itself.
+// pressure sweeps and residual checks, the pressure update, the advection
+// and the warm start's network of all scenes run as single launches over
+// tables of grids (k_rbgs_fused_multi, k_residual_multi, k_pupdate_multi,
+// k_advect_multi, NetMulti), so small grids share the GPU instead of being
+// launch-bound one by one. The sweep kernel's tile parameters (slab groups
+// in flight, waves per block) come from an autotuner that times each
+// candidate on the batch itself.
 //
 // No reference counterpart (the reference runs one scene per process,
 // src/main.cu:72-117). Each scene computes exactly what blz_step computes
-// for it alone with the same settings (the same kernels in the same order).
+// for it alone with the same settings (the same per-cell code in the same
+// stage order).
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <new>
 #include <vector>
 
@@ -54,6 +58,17 @@ struct Batch {
     // (when every scene of warm_q runs the default order with one network)
     NetMulti netm;
     bool joint_fwd = false;
+    // with joint_fwd, the corrections around the forward as multi-grid
+    // launches too, for the scenes warm_start_multi_ok admits (ws_multi[q])
+    WsMulti wsm;
+    std::vector<char> ws_multi;
+    // the pressure update and the advection of all scenes as multi-grid
+    // launches: job tables per captured graph (the fields' buffers
+    // alternate), filled on the host while capturing, uploaded after
+    char* post_dev[2] = {nullptr, nullptr};
+    std::vector<char> post_host[2];
+    size_t post_bytes = 0; // per graph
+    int cap_which = 0;     // graph being captured
     int w_fs_blocks = 0, w_rs_blocks = 0, w_restore_blocks = 0;
     std::vector<FusedScene> fs_h;
     std::vector<ResScene> rs_h;
@@ -72,6 +87,7 @@ struct Batch {
     int capture(int which);
     int enqueue_step(hipStream_t st);
     int joint_forward_and_finish(hipStream_t st);
+    int enqueue_update_and_advect(hipStream_t st);
     int step(float time);
     int autotune(int sweeps, int* best);
 };
@@ -92,7 +108,8 @@ Batch::~Batch() {
     if (off_dev) (void)hipFree(off_dev);
     if (fs) (void)hipFree(fs);
     if (rs) (void)hipFree(rs);
-    for (void* q : {(void*)fsw, (void*)fsj, (void*)rsw, (void*)rsj, (void*)js, (void*)fsj2, (void*)rsj2})
+    for (void* q : {(void*)fsw, (void*)fsj, (void*)rsw, (void*)rsj, (void*)js, (void*)fsj2, (void*)rsj2,
+                    (void*)post_dev[0], (void*)post_dev[1]})
         if (q) (void)hipFree(q);
     for (blz_solver* h : sc) delete h;
     if (root) (void)hipStreamDestroy(root);
@@ -125,6 +142,14 @@ int Batch::init(const blz_settings* st, int n) {
         set_error("hipMalloc failed for batch tables");
         return BLZ_E_NOMEM;
     }
+    // at most 3 pressure-update and 5 advection jobs per scene, in up to 9
+    // launch tables of 256-byte aligned starts
+    post_bytes = 3 * n * sizeof(PupJob) + 5 * n * sizeof(AdvJob) + 9 * 256;
+    for (auto& d : post_dev)
+        if (hipMalloc(&d, post_bytes) != hipSuccess) {
+            set_error("hipMalloc failed for batch tables");
+            return BLZ_E_NOMEM;
+        }
     for (int q = 0; q < n; ++q) {
         blz_solver* h = new (std::nothrow) blz_solver();
         if (!h) {
@@ -238,9 +263,20 @@ int Batch::build_judge_tables() {
         views.push_back(s.ws_net);
         stops.push_back(&s.ctl[0].done);
     }
+    ws_multi.assign(n, 0);
     if (joint) {
         if (int e = netm.build(views, stops)) return e;
         joint_fwd = true;
+        std::vector<Solver*> grids;
+        std::vector<float> dts;
+        for (int q : warm_q) {
+            Solver& s = sc[q]->s;
+            if (!warm_start_multi_ok(s)) continue;
+            ws_multi[q] = 1;
+            grids.push_back(&s);
+            dts.push_back(s.s.dt);
+        }
+        if (int e = wsm.build(grids, dts)) return e;
     }
     multi_reapply = true;
     for (int q : warm_q) {
@@ -313,13 +349,18 @@ int Batch::build_judge_tables() {
     return BLZ_OK;
 }
 
-// the network forwards of all warm-started scenes on `st` (their inputs
-// packed on the scenes' streams, joined into `st`), then the rest of each
-// scene's corrections on its stream, joined back
+// the corrections of all warm-started scenes from the point where the
+// scenes' streams joined into `st`: the first halves of the multi-grid
+// scenes (the others packed their inputs on their own streams), the network
+// forwards of all, the second halves of the multi-grid scenes, then those
+// of the others on their streams, joined back
 int Batch::joint_forward_and_finish(hipStream_t st) {
+    if (int e = wsm.input(st)) return e;
     if (int e = netm.forward(st)) return e;
+    if (int e = wsm.finish(st)) return e;
     BLZ_HIP(hipEventRecord(fork, st));
     for (int q : warm_q) {
+        if (ws_multi[q]) continue;
         Solver& s = sc[q]->s;
         BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
         if (int e = warm_start_prepare_finish(s, s.ctl)) return e;
@@ -360,8 +401,10 @@ int Batch::enqueue_step(hipStream_t st) {
         if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, tol, cap, 0, 1)) return e;
         if (s.ws_net && s.s.warm_start && cap > 0) {
             if (joint_fwd) {
-                // network forward and the rest of the corrections below
-                if (int e = warm_start_prepare_input(s, dt, c)) return e;
+                // the corrections below (this scene's own launches unless
+                // they run in the multi-grid ones)
+                if (!ws_multi[q])
+                    if (int e = warm_start_prepare_input(s, dt, c)) return e;
             } else if (!warm_q.empty()) {
                 if (int e = warm_start_prepare(s, dt, c)) return e; // judged below, with the other scenes
             } else if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap, every)) {
@@ -435,7 +478,8 @@ int Batch::enqueue_step(hipStream_t st) {
                 Solver& s = sc[q]->s;
                 BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
                 if (joint_fwd) {
-                    if (int e = warm_start_prepare_input(s, s.s.dt, s.ctl)) return e;
+                    if (!ws_multi[q])
+                        if (int e = warm_start_prepare_input(s, s.s.dt, s.ctl)) return e;
                 } else if (int e = warm_start_prepare(s, s.s.dt, s.ctl)) {
                     return e;
                 }
@@ -473,16 +517,67 @@ int Batch::enqueue_step(hipStream_t st) {
             BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
         }
     }
+    if (int e = enqueue_update_and_advect(st)) return e;
     BLZ_HIP(hipEventRecord(fork, st));
     for (int q = 0; q < n; ++q) {
         Solver& s = sc[q]->s;
         BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
-        const float dt = s.s.dt;
-        if (int e = s.run_stage(BLZ_STAGE_PROJECT_UPDATE, 0.0f, dt)) return e;
-        if (int e = s.run_stage(BLZ_STAGE_ADVECT, 0.0f, dt)) return e;
         BLZ_HIP(hipMemcpyAsync(s.ctl_host, s.ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s.stream));
         BLZ_HIP(hipEventRecord(join[q], s.stream));
         BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+    }
+    return BLZ_OK;
+}
+
+// The pressure update that ends project() (src/FluidSolver.cu:626-665) and
+// the advection (:940-950) of every scene: the update's three face axes,
+// then the advected fields,
+// each as one launch over a job table of all scenes (the tables of graph
+// cap_which, uploaded once the capture ends); the host's field swaps follow
+// each scene's own launchers
+int Batch::enqueue_update_and_advect(hipStream_t st) {
+    const int n = (int)sc.size();
+    std::vector<PupJob> pj[3];
+    std::vector<AdvJob> aj[6];
+    for (int q = 0; q < n; ++q) {
+        Solver& s = sc[q]->s;
+        pupdate_jobs(s, s.s.dt, pj);
+        advect_jobs(s, s.s.dt, aj);
+    }
+    std::vector<char>& h = post_host[cap_which];
+    h.assign(post_bytes, 0);
+    char* dev = post_dev[cap_which];
+    size_t off = 0;
+    // first blocks, placement in the table set; returns the launch's blocks
+    auto place = [&](auto& v, size_t* at) -> int {
+        int fb = 0;
+        for (auto& j : v) {
+            j.first_block = fb;
+            fb += (int)blocks_for((size_t)j.n, 256);
+        }
+        const size_t bytes = v.size() * sizeof(v[0]);
+        *at = off;
+        if (off + bytes > post_bytes) return -1;
+        if (bytes) std::memcpy(h.data() + off, v.data(), bytes);
+        off += (bytes + 255) / 256 * 256;
+        return fb;
+    };
+    auto overflow = [&]() {
+        set_error("batch job tables overflow");
+        return BLZ_E_INVALID;
+    };
+    for (int a = 0; a < 3; ++a) {
+        size_t at = 0;
+        const int blocks = place(pj[a], &at);
+        if (blocks < 0) return overflow();
+        if (int e = launch_pupdate_multi(st, a, (const PupJob*)(dev + at), (int)pj[a].size(), blocks)) return e;
+    }
+    for (int k = 0; k < 6; ++k) {
+        size_t at = 0;
+        const int blocks = place(aj[k], &at);
+        if (blocks < 0) return overflow();
+        if (int e = launch_advect_multi(st, k / 2, k & 1, (const AdvJob*)(dev + at), (int)aj[k].size(), blocks))
+            return e;
     }
     return BLZ_OK;
 }
@@ -504,6 +599,7 @@ int Batch::capture(int which) {
     // captured noise kernels read their offsets from device memory; outside
     // the graph (blz_step on a scene handle) they take the time argument
     for (int q = 0; q < n; ++q) sc[q]->s.dev_off = off_dev + 2 * q;
+    cap_which = which;
     BLZ_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
     int e = enqueue_step(root);
     hipError_t ce = hipStreamEndCapture(root, &graph);
@@ -513,6 +609,12 @@ int Batch::capture(int which) {
         return e;
     }
     BLZ_HIP(ce);
+    // the update and advection tables the captured launches read
+    if (hipMemcpy(post_dev[which], post_host[which].data(), post_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        set_error("batch job table upload failed");
+        return BLZ_E_HIP;
+    }
     size_t nn = 0;
     BLZ_HIP(hipGraphGetNodes(graph, nullptr, &nn));
     nodes = (int)nn;

@@ -1171,10 +1171,10 @@ __global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap, 
 // out-of-place: interior faces subtract s*(p_hi - p_lo); boundary faces are
 // 0 when closed, else the updated value of the adjacent interior face.
 template <int AXIS>
-__global__ void __launch_bounds__(BS) k_pupdate(const float* __restrict__ in, float* __restrict__ out, RB p, RBL L,
-                                                int W, int H, int D, float sc, int closedLo, int closedHi) {
+__device__ __forceinline__ void pupdate_face(const float* __restrict__ in, float* __restrict__ out, const RB& p,
+                                             const RBL& L, int W, int H, int D, float sc, int closedLo, int closedHi,
+                                             int t) {
     const int fx = W + (AXIS == 0), fy = H + (AXIS == 1), fz = D + (AXIS == 2);
-    const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= fx * fy * fz) return;
     const int i = t % fx, j = (t / fx) % fy, k = t / (fx * fy);
     const int f = AXIS == 0 ? i : (AXIS == 1 ? j : k);
@@ -1197,6 +1197,32 @@ __global__ void __launch_bounds__(BS) k_pupdate(const float* __restrict__ in, fl
     const float p_lo = AXIS == 0 ? p_at(p, L, fi - 1, fj, fk)
                                  : (AXIS == 1 ? p_at(p, L, fi, fj - 1, fk) : p_at(p, L, fi, fj, fk - 1));
     out[t] = in[g] - sc * (p_hi - p_lo);
+}
+
+template <int AXIS>
+__global__ void __launch_bounds__(BS) k_pupdate(const float* __restrict__ in, float* __restrict__ out, RB p, RBL L,
+                                                int W, int H, int D, float sc, int closedLo, int closedHi) {
+    pupdate_face<AXIS>(in, out, p, L, W, H, D, sc, closedLo, closedHi, (int)(blockIdx.x * BS + threadIdx.x));
+}
+
+// the job of block b in a table of first blocks (ascending)
+template <class J>
+__device__ __forceinline__ const J& job_of(const J* __restrict__ tab, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return tab[lo];
+}
+
+// the pressure update on one axis of every grid in a table
+template <int AXIS>
+__global__ void __launch_bounds__(BS) k_pupdate_multi(const PupJob* __restrict__ tab, int n) {
+    const PupJob& e = job_of(tab, n, (int)blockIdx.x);
+    pupdate_face<AXIS>(e.in, e.out, e.p, e.L, e.W, e.H, e.D, e.sc, e.closedLo, e.closedHi,
+                       (int)((blockIdx.x - e.first_block) * BS + threadIdx.x));
 }
 
 __global__ void __launch_bounds__(BS) k_rb_to_natural(RB p, RBL L, float* __restrict__ out, int W, int H, int D) {
@@ -1574,18 +1600,60 @@ int launch_max_abs_velocity(Solver& s, double* dev_out) {
     return BLZ_OK;
 }
 
+// closed_bounds slots of the low and high faces of each axis, as the
+// reference assigns them (src/FluidSolver.cu:626-665: V's j = 0 face takes
+// slot 3, its j = H face slot 2)
+constexpr int kClosedLo[3] = {0, 3, 4}, kClosedHi[3] = {1, 2, 5};
+
 int launch_pressure_update(Solver& s, float dt) {
     const float sc = dt / (s.s.density * s.dx);
     const int* cb = s.s.closed_bounds;
     Field& U = s.f[F_U];
     Field& V = s.f[F_V];
     Field& W = s.f[F_W];
-    LAUNCH(k_pupdate<0>, U.count(), U.front(), U.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[0], cb[1]);
-    LAUNCH(k_pupdate<1>, V.count(), V.front(), V.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[3], cb[2]);
-    LAUNCH(k_pupdate<2>, W.count(), W.front(), W.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[4], cb[5]);
+    LAUNCH(k_pupdate<0>, U.count(), U.front(), U.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[kClosedLo[0]],
+           cb[kClosedHi[0]]);
+    LAUNCH(k_pupdate<1>, V.count(), V.front(), V.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[kClosedLo[1]],
+           cb[kClosedHi[1]]);
+    LAUNCH(k_pupdate<2>, W.count(), W.front(), W.back(), s.p, s.L, s.W, s.H, s.D, sc, cb[kClosedLo[2]],
+           cb[kClosedHi[2]]);
     U.swap();
     V.swap();
     W.swap();
+    return BLZ_OK;
+}
+
+// the jobs of launch_pressure_update (one per axis, appended to jobs[axis],
+// first blocks left to the caller), with the same field swaps
+void pupdate_jobs(Solver& s, float dt, std::vector<PupJob>* jobs) {
+    const float sc = dt / (s.s.density * s.dx);
+    const int* cb = s.s.closed_bounds;
+    for (int a = 0; a < 3; ++a) {
+        Field& F = s.f[F_U + a];
+        PupJob j;
+        j.in = F.front();
+        j.out = F.back();
+        j.p = s.p;
+        j.L = s.L;
+        j.W = s.W;
+        j.H = s.H;
+        j.D = s.D;
+        j.sc = sc;
+        j.closedLo = cb[kClosedLo[a]];
+        j.closedHi = cb[kClosedHi[a]];
+        j.first_block = 0;
+        j.n = (int)F.count();
+        jobs[a].push_back(j);
+    }
+    for (int a = 0; a < 3; ++a) s.f[F_U + a].swap();
+}
+
+int launch_pupdate_multi(hipStream_t st, int axis, const PupJob* tab, int n, int blocks) {
+    if (n < 1 || blocks < 1) return BLZ_OK;
+    if (axis == 0) hipLaunchKernelGGL(k_pupdate_multi<0>, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    else if (axis == 1) hipLaunchKernelGGL(k_pupdate_multi<1>, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    else hipLaunchKernelGGL(k_pupdate_multi<2>, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
 

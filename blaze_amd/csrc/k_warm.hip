@@ -34,6 +34,8 @@
 //     prediction, having spent one batch of sweeps twice. The sweeps then
 //     run to the same tolerance as without it.
 // All decisions are taken on the device (no host round trip).
+#include <cstring>
+
 #include "solver.h"
 
 namespace blz {
@@ -57,17 +59,17 @@ __device__ __forceinline__ bf16 f2bf_w(float f) {
 // begin (when set): the control block of a new application, whose judge
 // state this launch resets first (no decision yet; the restore only follows
 // this application's)
-__global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
-                                                 float* __restrict__ r, const int* __restrict__ stop,
-                                                 SolveCtl* __restrict__ begin) {
+__device__ __forceinline__ void ws_resid_body(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
+                                              float* __restrict__ r, const int* __restrict__ stop,
+                                              SolveCtl* __restrict__ begin, unsigned bx) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    if (begin && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (begin && bx == 0 && threadIdx.x == 0) {
         begin->ws = 0;
         begin->skip_spare = 1;
     }
     if (*stop) return;
     const int Wh = W >> 1;
-    const int t = blockIdx.x * BS + threadIdx.x;
+    const int t = bx * BS + threadIdx.x;
     if (t >= Wh * H * L.Dg) return;
     const int m = t % Wh, j = (t / Wh) % H, g = t / (Wh * H);
     const int c0 = j & 1, c1 = c0 ^ 1;
@@ -108,6 +110,12 @@ __global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int
         r[o] = a - (nxy0 + nz) * cur0[sl];
         r[o + 1] = b - (nxy1 + nz) * cur1[sl];
     }
+}
+
+__global__ void __launch_bounds__(BS) k_ws_resid(RB p, RB rhs, RBL L, int W, int H, int D, float scale,
+                                                 float* __restrict__ r, const int* __restrict__ stop,
+                                                 SolveCtl* __restrict__ begin) {
+    ws_resid_body(p, rhs, L, W, H, D, scale, r, stop, begin, blockIdx.x);
 }
 
 typedef float f4w __attribute__((ext_vector_type(4)));
@@ -278,14 +286,12 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
 // k_ws_alpha_res turns them into alpha_G and the network's input
 // (sum r - alpha_G sum A e) / F, the restriction of r - alpha_G A e. Per
 // thread FP32 sums, then fixed-order FP64 trees (deterministic).
-__global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* __restrict__ r,
-                                                                   const float* __restrict__ e, int W, int H, int D,
-                                                                   int F, int Wc, int Hc, int Dc, int Wn, int Hn,
-                                                                   double* __restrict__ S,
-                                                                   double* __restrict__ partial,
-                                                                   const int* __restrict__ stop) {
+__device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restrict__ r, const float* __restrict__ e,
+                                                           int W, int H, int D, int F, int Wc, int Hc, int Dc, int Wn,
+                                                           int Hn, double* __restrict__ S, double* __restrict__ partial,
+                                                           const int* __restrict__ stop, unsigned bx) {
     if (*stop) return;
-    const int J = blockIdx.x % Hn, K = blockIdx.x / Hn;
+    const int J = bx % Hn, K = bx / Hn;
     const size_t row0 = ((size_t)K * Hn + J) * Wn;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i0 = 4 * lane;
@@ -350,8 +356,17 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* 
     if (threadIdx.x < 4) {
         double a = 0.0;
         for (int u = 0; u < RW; ++u) a += sd[u][threadIdx.x];
-        partial[4 * blockIdx.x + threadIdx.x] = a;
+        partial[4 * bx + threadIdx.x] = a;
     }
+}
+
+__global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* __restrict__ r,
+                                                                   const float* __restrict__ e, int W, int H, int D,
+                                                                   int F, int Wc, int Hc, int Dc, int Wn, int Hn,
+                                                                   double* __restrict__ S,
+                                                                   double* __restrict__ partial,
+                                                                   const int* __restrict__ stop) {
+    ws_restrict_dots_rows_body(r, e, W, H, D, F, Wc, Hc, Dc, Wn, Hn, S, partial, stop, blockIdx.x);
 }
 
 // alpha_G from the row partials of k_ws_restrict_dots_rows (as k_ws_alpha:
@@ -360,12 +375,12 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* 
 // wall mask (padding cells: 0 and mask 1) and its RMS over the real cells
 // (float bits into *amax; 0 once the solve is done). One workgroup, fixed
 // order.
-__global__ void __launch_bounds__(1024) k_ws_alpha_res(const double* __restrict__ partial, int np, double n,
-                                                       double* __restrict__ alpha, const double* __restrict__ S,
-                                                       float* __restrict__ res, float* __restrict__ mask,
-                                                       unsigned* __restrict__ amax, int W, int H, int D, int F,
-                                                       int Wc, int Hc, int Dc, int Wn, int Hn, int nn,
-                                                       const int* __restrict__ stop) {
+__device__ __forceinline__ void ws_alpha_res_body(const double* __restrict__ partial, int np, double n,
+                                                  double* __restrict__ alpha, const double* __restrict__ S,
+                                                  float* __restrict__ res, float* __restrict__ mask,
+                                                  unsigned* __restrict__ amax, int W, int H, int D, int F, int Wc,
+                                                  int Hc, int Dc, int Wn, int Hn, int nn,
+                                                  const int* __restrict__ stop) {
     if (*stop) {
         if (threadIdx.x == 0) *amax = 0u;
         return;
@@ -433,6 +448,15 @@ __global__ void __launch_bounds__(1024) k_ws_alpha_res(const double* __restrict_
     }
 }
 
+__global__ void __launch_bounds__(1024) k_ws_alpha_res(const double* __restrict__ partial, int np, double n,
+                                                       double* __restrict__ alpha, const double* __restrict__ S,
+                                                       float* __restrict__ res, float* __restrict__ mask,
+                                                       unsigned* __restrict__ amax, int W, int H, int D, int F,
+                                                       int Wc, int Hc, int Dc, int Wn, int Hn, int nn,
+                                                       const int* __restrict__ stop) {
+    ws_alpha_res_body(partial, np, n, alpha, S, res, mask, amax, W, H, D, F, Wc, Hc, Dc, Wn, Hn, nn, stop);
+}
+
 // input scale s = RMS of the coarse residual over the real coarse cells (one
 // block, fixed order: deterministic); stored as float bits in *amax (0 once
 // the solve is done, which makes the network's input 0)
@@ -457,9 +481,10 @@ __global__ void __launch_bounds__(1024) k_ws_rms(const float* __restrict__ res, 
 }
 
 // network input: (r / s, mask) in BF16, 16 channels (s = input scale)
-__global__ void k_ws_pack(const float* __restrict__ res, const float* __restrict__ mask,
-                          const unsigned* __restrict__ amax, bf16* __restrict__ x0, int n) {
-    const int t = blockIdx.x * BS + threadIdx.x;
+__device__ __forceinline__ void ws_pack_body(const float* __restrict__ res, const float* __restrict__ mask,
+                                             const unsigned* __restrict__ amax, bf16* __restrict__ x0, int n,
+                                             unsigned bx) {
+    const int t = bx * BS + threadIdx.x;
     if (t >= n * NET_IN_CH) return;
     const int c = t % NET_IN_CH, v = t / NET_IN_CH;
     const float s = __uint_as_float(*amax);
@@ -467,6 +492,11 @@ __global__ void k_ws_pack(const float* __restrict__ res, const float* __restrict
     if (c == 0) val = s > 0.0f ? res[v] / s : 0.0f;
     if (c == 1) val = mask[v];
     x0[t] = f2bf_w(val);
+}
+
+__global__ void k_ws_pack(const float* __restrict__ res, const float* __restrict__ mask,
+                          const unsigned* __restrict__ amax, bf16* __restrict__ x0, int n) {
+    ws_pack_body(res, mask, amax, x0, n, blockIdx.x);
 }
 
 // FP32 normalised input for data generation: (r / s, mask)
@@ -511,13 +541,13 @@ __global__ void __launch_bounds__(BS) k_ws_prolong(float* __restrict__ e, int W,
 
 // the same at four x-consecutive cells per thread (W a multiple of 4), one
 // float4 store, threads over (x/4, y, z)
-__global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W, int H, int D,
-                                                    const float* __restrict__ pred, float invF, int Wc, int Hc,
-                                                    int Dc, int Wn, int Hn, const SolveCtl* __restrict__ c) {
+__device__ __forceinline__ void ws_prolong4_body(float* __restrict__ e, int W, int H, int D,
+                                                 const float* __restrict__ pred, float invF, int Wc, int Hc, int Dc,
+                                                 int Wn, int Hn, const SolveCtl* __restrict__ c, unsigned bx) {
     if (c->done) return;
     const unsigned W4 = (unsigned)W >> 2;
     // 32-bit index arithmetic: W H D / 4 < 2^32 (the launcher's grid)
-    const unsigned t = blockIdx.x * BS + threadIdx.x;
+    const unsigned t = bx * BS + threadIdx.x;
     if (t >= W4 * (unsigned)H * (unsigned)D) return;
     const int q = (int)(t % W4);
     const unsigned row = t / W4;
@@ -567,6 +597,12 @@ __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W
         out[u] = (val == val && fabsf(val) <= 3.4e38f) ? val : 0.0f;
     }
     *(f4w*)(e + ((size_t)k * H + j) * W + 4 * q) = out;
+}
+
+__global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W, int H, int D,
+                                                    const float* __restrict__ pred, float invF, int Wc, int Hc,
+                                                    int Dc, int Wn, int Hn, const SolveCtl* __restrict__ c) {
+    ws_prolong4_body(e, W, H, D, pred, invF, Wc, Hc, Dc, Wn, Hn, c, blockIdx.x);
 }
 
 // One workgroup per block of the global grid (F^3 cells, clipped), all on
@@ -701,15 +737,13 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
 // the F x F rows in turn. Block I = lane / (F/4): its lanes are reduced by
 // shuffles, the waves through LDS in a fixed order.
 template <bool E1, bool E2, bool SYM = false>
-__global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __restrict__ r,
-                                                               const float* __restrict__ e1,
-                                                               const double* __restrict__ alpha1,
-                                                               const float* __restrict__ e2, int W, int H, int D,
-                                                               int F, int Gw, int Gh, double* __restrict__ partial,
-                                                               double* __restrict__ sg,
-                                                               const SolveCtl* __restrict__ c) {
+__device__ __forceinline__ void ws_blockdots_rows_body(const float* __restrict__ r, const float* __restrict__ e1,
+                                                       const double* __restrict__ alpha1, const float* __restrict__ e2,
+                                                       int W, int H, int D, int F, int Gw, int Gh,
+                                                       double* __restrict__ partial, double* __restrict__ sg,
+                                                       const SolveCtl* __restrict__ c, unsigned bx) {
     if (c->done) return;
-    const int J = blockIdx.x % Gh, K = blockIdx.x / Gh;
+    const int J = bx % Gh, K = bx / Gh;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i0 = 4 * lane;
     const float a1 = e1 ? (float)*alpha1 : 0.0f;
@@ -746,11 +780,22 @@ __global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __re
     }
 }
 
+template <bool E1, bool E2, bool SYM = false>
+__global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __restrict__ r,
+                                                               const float* __restrict__ e1,
+                                                               const double* __restrict__ alpha1,
+                                                               const float* __restrict__ e2, int W, int H, int D,
+                                                               int F, int Gw, int Gh, double* __restrict__ partial,
+                                                               double* __restrict__ sg,
+                                                               const SolveCtl* __restrict__ c) {
+    ws_blockdots_rows_body<E1, E2, SYM>(r, e1, alpha1, e2, W, H, D, F, Gw, Gh, partial, sg, c, blockIdx.x);
+}
+
 // alpha from the per-block partial sums (one workgroup, fixed order: every
 // run gives the same alpha); 0 when undefined. alpha[2] receives the mean of
 // the direction (two slots after alpha)
-__global__ void __launch_bounds__(BS) k_ws_alpha(const double* __restrict__ partial, int nb, double n,
-                                                 double* __restrict__ alpha, const SolveCtl* __restrict__ c) {
+__device__ __forceinline__ void ws_alpha_body(const double* __restrict__ partial, int nb, double n,
+                                              double* __restrict__ alpha, const SolveCtl* __restrict__ c) {
     if (c->done) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q = threadIdx.x; q < nb; q += BS)
@@ -776,19 +821,25 @@ __global__ void __launch_bounds__(BS) k_ws_alpha(const double* __restrict__ part
     }
 }
 
+__global__ void __launch_bounds__(BS) k_ws_alpha(const double* __restrict__ partial, int nb, double n,
+                                                 double* __restrict__ alpha, const SolveCtl* __restrict__ c) {
+    ws_alpha_body(partial, nb, n, alpha, c);
+}
+
 // p += alpha1 (e1 - mean e1) + alpha2 (e2 - mean e2) (either may be
 // absent; scales and means from k_ws_alpha); thread per (x pair,
 // row, k group) of the split layout: float4 read-modify-write of both
 // colours, natural-order reads of e (8 contiguous bytes per thread and k).
 // bk (when bk.c[0] is set): planes of the same layout that receive the
 // uncorrected values (the interior; their ghost shell already holds -0)
-__global__ void __launch_bounds__(BS) k_ws_apply(RB p, RB bk, RBL L, int W, int H, int D, const float* __restrict__ e1,
-                                                 const double* __restrict__ alpha1, const float* __restrict__ e2,
-                                                 const double* __restrict__ alpha2, const SolveCtl* __restrict__ c) {
+__device__ __forceinline__ void ws_apply_body(RB p, RB bk, RBL L, int W, int H, int D, const float* __restrict__ e1,
+                                              const double* __restrict__ alpha1, const float* __restrict__ e2,
+                                              const double* __restrict__ alpha2, const SolveCtl* __restrict__ c,
+                                              unsigned bx) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     if (c->done) return;
     const int Wh = W >> 1;
-    const int t = blockIdx.x * BS + threadIdx.x;
+    const int t = bx * BS + threadIdx.x;
     if (t >= Wh * H * L.Dg) return;
     const int m = t % Wh, j = (t / Wh) % H, g = t / (Wh * H);
     const float a1 = e1 ? (float)*alpha1 : 0.0f, a2 = e2 ? (float)*alpha2 : 0.0f;
@@ -818,6 +869,12 @@ __global__ void __launch_bounds__(BS) k_ws_apply(RB p, RB bk, RBL L, int W, int 
     }
     P0[e] = x0;
     P1[e] = x1;
+}
+
+__global__ void __launch_bounds__(BS) k_ws_apply(RB p, RB bk, RBL L, int W, int H, int D, const float* __restrict__ e1,
+                                                 const double* __restrict__ alpha1, const float* __restrict__ e2,
+                                                 const double* __restrict__ alpha2, const SolveCtl* __restrict__ c) {
+    ws_apply_body(p, bk, L, W, H, D, e1, alpha1, e2, alpha2, c, blockIdx.x);
 }
 
 __device__ __forceinline__ double block_max(const double* __restrict__ partial, int n) {
@@ -1117,9 +1174,9 @@ __global__ void __launch_bounds__(GT) k_ws_gsolve(const double* __restrict__ sg,
 // transform b along x, y, z, divide by F^2 (lx + ly + lz) (the constant
 // mode, absent from a consistent b, is set to 0), transform back. FP32 in
 // LDS with the coefficient tables computed in FP64.
-__global__ void __launch_bounds__(GT) k_ws_gdct(const double* __restrict__ sg, const double* __restrict__ alpha1,
-                                                float* __restrict__ x_out, int Gw, int Gh, int Gd, int F, int W,
-                                                int H, int D, const SolveCtl* __restrict__ c) {
+__device__ __forceinline__ void ws_gdct_body(const double* __restrict__ sg, const double* __restrict__ alpha1,
+                                             float* __restrict__ x_out, int Gw, int Gh, int Gd, int F, int W, int H,
+                                             int D, const SolveCtl* __restrict__ c) {
     if (c->done) return;
     __shared__ float A[WS_GMAX], Bf[WS_GMAX];
     __shared__ float cs[3][16][16]; // cs[axis][a][i]
@@ -1180,6 +1237,12 @@ __global__ void __launch_bounds__(GT) k_ws_gdct(const double* __restrict__ sg, c
     pass(A, Bf, 1, true);
     pass(Bf, A, 0, true);
     for (int g = threadIdx.x; g < n; g += GT) x_out[g] = A[g];
+}
+
+__global__ void __launch_bounds__(GT) k_ws_gdct(const double* __restrict__ sg, const double* __restrict__ alpha1,
+                                                float* __restrict__ x_out, int Gw, int Gh, int Gd, int F, int W,
+                                                int H, int D, const SolveCtl* __restrict__ c) {
+    ws_gdct_body(sg, alpha1, x_out, Gw, Gh, Gd, F, W, H, D, c);
 }
 
 // Exact solve of the network grid's coarse problem, the upper bound of what
@@ -1353,6 +1416,96 @@ void launch_prolong(const Solver& s, float* e, const float* pred, float invF, in
         hipLaunchKernelGGL(k_ws_prolong, prolong_grid(s), dim3(BS), 0, s.stream, e, s.W, s.H, s.D, pred, invF, Wc,
                            Hc, Dc, Wn, Hn, c);
 }
+
+// One grid's default-order correction in a batch's multi-grid launches
+// (WsMulti): the arguments warm_start_prepare_input and _finish give each
+// kernel for this grid, and the grid's first block in each launch
+enum {
+    WK_RESID, WK_SUMS, WK_GDCT, WK_PROLONG_G, WK_RESTRICT, WK_ALPHA_RES, WK_PACK, // input half
+    WK_PROLONG_N, WK_DOTS, WK_ALPHA, WK_APPLY,                                   // finish half
+    WK_N
+};
+static_assert(WK_N == WS_MULTI_LAUNCHES, "one first block per launch");
+struct WsJob {
+    RB p, rhs, bk; // home planes, right-hand side, judge planes (uncorrected copy)
+    RBL L;
+    int W, H, D;
+    float scale, invG, invF; // rhs scale; 1 / block size of the global and of the network grid
+    float *r, *eg, *enet, *xg, *res, *mask;
+    const float* out; // the network's output
+    double *part, *sg, *S, *alpha; // alpha: the network's scale, alpha + 1: the global one
+    unsigned* amax;
+    bf16* x0;
+    SolveCtl* c;
+    int ws_g, Gw, Gh, Gd, ws_f, Wc, Hc, Dc, Wn, Hn, nn;
+    double N;
+    int fb[WK_N];
+};
+
+__device__ __forceinline__ const WsJob& ws_job(const WsJob* __restrict__ tab, int n, int k, unsigned b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((unsigned)tab[mid].fb[k] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return tab[lo];
+}
+
+#define WS_JOB(K)                                             \
+    const WsJob& j = ws_job(tab, n, K, blockIdx.x);           \
+    const unsigned bx = blockIdx.x - (unsigned)j.fb[K];       \
+    (void)bx
+
+__global__ void __launch_bounds__(BS) k_wsm_resid(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_RESID);
+    ws_resid_body(j.p, j.rhs, j.L, j.W, j.H, j.D, j.scale, j.r, &j.c->done, j.c, bx);
+}
+__global__ void __launch_bounds__(64 * RW) k_wsm_sums(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_SUMS);
+    ws_blockdots_rows_body<false, false, false>(j.r, nullptr, nullptr, nullptr, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
+                                                j.part, j.sg, j.c, bx);
+}
+__global__ void __launch_bounds__(GT) k_wsm_gdct(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_GDCT);
+    ws_gdct_body(j.sg, nullptr, j.xg, j.Gw, j.Gh, j.Gd, j.ws_g, j.W, j.H, j.D, j.c);
+}
+__global__ void __launch_bounds__(BS) k_wsm_prolong_g(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_PROLONG_G);
+    ws_prolong4_body(j.eg, j.W, j.H, j.D, j.xg, j.invG, j.Gw, j.Gh, j.Gd, j.Gw, j.Gh, j.c, bx);
+}
+__global__ void __launch_bounds__(64 * RW) k_wsm_restrict(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_RESTRICT);
+    ws_restrict_dots_rows_body(j.r, j.eg, j.W, j.H, j.D, j.ws_f, j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.S, j.part,
+                               &j.c->done, bx);
+}
+__global__ void __launch_bounds__(1024) k_wsm_alpha_res(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_ALPHA_RES);
+    ws_alpha_res_body(j.part, j.nn / j.Wn, j.N, j.alpha + 1, j.S, j.res, j.mask, j.amax, j.W, j.H, j.D, j.ws_f,
+                      j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.nn, &j.c->done);
+}
+__global__ void __launch_bounds__(BS) k_wsm_pack(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_PACK);
+    ws_pack_body(j.res, j.mask, j.amax, j.x0, j.nn, bx);
+}
+__global__ void __launch_bounds__(BS) k_wsm_prolong_n(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_PROLONG_N);
+    ws_prolong4_body(j.enet, j.W, j.H, j.D, j.out, j.invF, j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.c, bx);
+}
+__global__ void __launch_bounds__(64 * RW) k_wsm_dots(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_DOTS);
+    ws_blockdots_rows_body<true, true, true>(j.r, j.eg, j.alpha + 1, j.enet, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
+                                             j.part, nullptr, j.c, bx);
+}
+__global__ void __launch_bounds__(BS) k_wsm_alpha(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_ALPHA);
+    ws_alpha_body(j.part, j.Gw * j.Gh * j.Gd, j.N, j.alpha, j.c);
+}
+__global__ void __launch_bounds__(BS) k_wsm_apply(const WsJob* __restrict__ tab, int n) {
+    WS_JOB(WK_APPLY);
+    ws_apply_body(j.p, j.bk, j.L, j.W, j.H, j.D, j.eg, j.alpha + 1, j.enet, j.alpha, j.c, bx);
+}
+#undef WS_JOB
 
 } // namespace
 
@@ -1564,6 +1717,133 @@ int warm_start_prepare_finish(Solver& s, SolveCtl* c) {
     bk.c[1] = spare + s.Np;
     hipLaunchKernelGGL(k_ws_apply, dim3(nb(split_threads(s))), dim3(BS), 0, s.stream, s.p, bk, s.L, s.W, s.H, s.D,
                        ws_eg(s), a_g, ws_enet(s), a_net, c);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+// the grids whose default-order corrections WsMulti runs: the row passes,
+// the DCT global solve, float4 prolongations and the one-pass restriction
+bool warm_start_multi_ok(const Solver& s) {
+    return warm_start_splits(s) && s.W % 4 == 0 && s.W <= 256 && s.ws_g >= 4 && s.Gw <= 16 &&
+           s.W % s.ws_g == 0 && s.H % s.ws_g == 0 && s.D % s.ws_g == 0 && restrict_rows_ok(s) &&
+           3 * ncell(s) + 4 * ws_nn(s) <= s.tmp_floats;
+}
+
+WsMulti::~WsMulti() {
+    if (tab) (void)hipFree(tab);
+}
+
+int WsMulti::build(const std::vector<Solver*>& grids, const std::vector<float>& dts) {
+    std::vector<WsJob> t;
+    int fb[WK_N] = {};
+    for (size_t q = 0; q < grids.size(); ++q) {
+        Solver& s = *grids[q];
+        if (!warm_start_multi_ok(s)) {
+            set_error("batched warm start: a grid outside the multi-grid correction path");
+            return BLZ_E_INVALID;
+        }
+        const size_t N = ncell(s), nn = ws_nn(s);
+        WsJob j;
+        std::memset(&j, 0, sizeof j); // padding too: the table is compared bytewise
+        j.p = s.p;
+        j.rhs = s.rhs;
+        float* spare = s.prb + (size_t)s.judge_set * s.Np;
+        j.bk.c[0] = spare;
+        j.bk.c[1] = spare + s.Np;
+        j.L = s.L;
+        j.W = s.W;
+        j.H = s.H;
+        j.D = s.D;
+        j.scale = (s.s.density * s.dx * s.dx) / dts[q];
+        j.invG = 1.0f / (float)s.ws_g;
+        j.invF = 1.0f / (float)s.ws_f;
+        j.r = ws_r(s);
+        j.eg = ws_eg(s);
+        j.enet = ws_enet(s);
+        j.xg = ws_xg(s);
+        j.res = s.ws_buf;
+        j.mask = s.ws_buf + nn;
+        j.out = s.ws_net->out;
+        j.part = ws_part(s);
+        j.sg = ws_sg(s);
+        j.S = (double*)(s.tmp + 3 * N);
+        j.alpha = ws_alpha(s);
+        j.amax = (unsigned*)(s.ws_buf + 2 * nn);
+        j.x0 = s.ws_net->x0;
+        j.c = s.ctl;
+        j.ws_g = s.ws_g;
+        j.Gw = s.Gw;
+        j.Gh = s.Gh;
+        j.Gd = s.Gd;
+        j.ws_f = s.ws_f;
+        j.Wc = s.Wc;
+        j.Hc = s.Hc;
+        j.Dc = s.Dc;
+        j.Wn = s.Wn;
+        j.Hn = s.Hn;
+        j.nn = (int)nn;
+        j.N = (double)N;
+        // the single-grid launches' grids
+        const unsigned nblk[WK_N] = {nb(split_threads(s)),
+                                     (unsigned)(s.Gh * s.Gd),
+                                     1u,
+                                     nb((size_t)(s.W / 4) * s.H * s.D),
+                                     (unsigned)(nn / s.Wn),
+                                     1u,
+                                     nb(nn * NET_IN_CH),
+                                     nb((size_t)(s.W / 4) * s.H * s.D),
+                                     (unsigned)(s.Gh * s.Gd),
+                                     1u,
+                                     nb(split_threads(s))};
+        for (int k = 0; k < WK_N; ++k) {
+            j.fb[k] = fb[k];
+            if ((long long)fb[k] + nblk[k] > 0x7fffffffll) {
+                set_error("batched warm start: too many workgroups");
+                return BLZ_E_INVALID;
+            }
+            fb[k] += (int)nblk[k];
+        }
+        t.push_back(j);
+    }
+    n = (int)t.size();
+    for (int k = 0; k < WK_N; ++k) blocks[k] = (unsigned)fb[k];
+    const size_t bytes_now = t.size() * sizeof(WsJob);
+    const unsigned char* raw = (const unsigned char*)t.data();
+    if (cur.size() != bytes_now || std::memcmp(cur.data(), raw, bytes_now) != 0) {
+        if (bytes_now > bytes) {
+            if (tab) BLZ_HIP(hipFree(tab));
+            tab = nullptr;
+            bytes = 0;
+            BLZ_HIP(hipMalloc(&tab, bytes_now));
+            bytes = bytes_now;
+        }
+        if (bytes_now) BLZ_HIP(hipMemcpy(tab, raw, bytes_now, hipMemcpyHostToDevice));
+        cur.assign(raw, raw + bytes_now);
+    }
+    return BLZ_OK;
+}
+
+int WsMulti::input(hipStream_t st) const {
+    if (!n) return BLZ_OK;
+    const WsJob* t = (const WsJob*)tab;
+    hipLaunchKernelGGL(k_wsm_resid, dim3(blocks[WK_RESID]), dim3(BS), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_sums, dim3(blocks[WK_SUMS]), dim3(64 * RW), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_gdct, dim3(blocks[WK_GDCT]), dim3(GT), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_prolong_g, dim3(blocks[WK_PROLONG_G]), dim3(BS), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_restrict, dim3(blocks[WK_RESTRICT]), dim3(64 * RW), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_alpha_res, dim3(blocks[WK_ALPHA_RES]), dim3(1024), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_pack, dim3(blocks[WK_PACK]), dim3(BS), 0, st, t, n);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+int WsMulti::finish(hipStream_t st) const {
+    if (!n) return BLZ_OK;
+    const WsJob* t = (const WsJob*)tab;
+    hipLaunchKernelGGL(k_wsm_prolong_n, dim3(blocks[WK_PROLONG_N]), dim3(BS), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_dots, dim3(blocks[WK_DOTS]), dim3(64 * RW), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_alpha, dim3(blocks[WK_ALPHA]), dim3(BS), 0, st, t, n);
+    hipLaunchKernelGGL(k_wsm_apply, dim3(blocks[WK_APPLY]), dim3(BS), 0, st, t, n);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }

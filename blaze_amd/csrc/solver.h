@@ -303,6 +303,23 @@ int warm_start_prepare(Solver& s, float dt, SolveCtl* c);
 bool warm_start_splits(const Solver& s);
 int warm_start_prepare_input(Solver& s, float dt, SolveCtl* c);
 int warm_start_prepare_finish(Solver& s, SolveCtl* c);
+// The same two halves for several grids at once (a batch), each kernel one
+// launch over a job table of the grids, for the grids warm_start_multi_ok
+// admits; every grid gets what its own launches give it. The tables hold
+// the grids' buffers: rebuild after a grid's warm-start state changes.
+constexpr int WS_MULTI_LAUNCHES = 11;
+bool warm_start_multi_ok(const Solver& s);
+struct WsMulti {
+    void* tab = nullptr;
+    size_t bytes = 0;
+    std::vector<unsigned char> cur; // the table's content
+    int n = 0;
+    unsigned blocks[WS_MULTI_LAUNCHES] = {};
+    ~WsMulti();
+    int build(const std::vector<Solver*>& grids, const std::vector<float>& dts);
+    int input(hipStream_t st) const;  // warm_start_prepare_input of every grid
+    int finish(hipStream_t st) const; // warm_start_prepare_finish of every grid
+};
 int warm_start_judge(Solver& s, float dt, SolveCtl* c, int* host_done, int seq, double tol, int cap, int judge,
                      bool predicted);
 // a batch's judges: decisions (one block per scene) and restores
@@ -318,6 +335,37 @@ int warm_start_global(Solver& s, float dt); // the global correction alone, unju
 int warm_start_apply_given(Solver& s, float dt, const float* host_pred, double* before, double* after, int* kept);
 int launch_pressure_update(Solver& s, float dt);
 int launch_advect(Solver& s, float dt);
+
+// The pressure update and the advection of many grids as multi-grid
+// launches (a batch's step): one job per (grid, face axis) or (grid,
+// advected field or pair), blocks mapped to jobs through ascending first
+// blocks. Each cell gets what the single-grid launch gives it.
+struct PupJob {
+    const float* in;
+    float* out;
+    RB p;
+    RBL L;
+    int W, H, D;
+    float sc;
+    int closedLo, closedHi;
+    int first_block, n; // n faces
+};
+enum { ADV_FREE = 0, ADV_CLAMPED = 1, ADV_PAIR = 2 }; // velocity component; T or S alone; T with S
+struct AdvJob {
+    Gv U, V, W; // front velocities
+    Gv G1, G2;  // advected field (G2: the pair's second field)
+    float* out1;
+    float* out2;
+    float dx, s; // cell size; the scale's parameter (dx, or 1/dx when dx is a power of two)
+    float a, b, dt;
+    int first_block, n; // n cells
+};
+// append grid s's jobs (pupdate: jobs[axis]; advection: jobs[2 kind + (1/dx
+// exact)]) and make the field swaps the single-grid launchers make
+void pupdate_jobs(Solver& s, float dt, std::vector<PupJob>* jobs);
+void advect_jobs(Solver& s, float dt, std::vector<AdvJob>* jobs);
+int launch_pupdate_multi(hipStream_t st, int axis, const PupJob* tab, int n, int blocks);
+int launch_advect_multi(hipStream_t st, int kind, bool mul, const AdvJob* tab, int n, int blocks);
 int launch_p2g(Solver& s);
 size_t p2g_entry_bound(const float* part8, int n, float dx, int nch);
 int launch_max_abs_velocity(Solver& s, double* dev_out);
