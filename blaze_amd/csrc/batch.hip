@@ -62,13 +62,37 @@ struct Batch {
     // launches too, for the scenes warm_start_multi_ok admits (ws_multi[q])
     WsMulti wsm;
     std::vector<char> ws_multi;
-    // the pressure update and the advection of all scenes as multi-grid
-    // launches: job tables per captured graph (the fields' buffers
-    // alternate), filled on the host while capturing, uploaded after
-    char* post_dev[2] = {nullptr, nullptr};
-    std::vector<char> post_host[2];
-    size_t post_bytes = 0; // per graph
+    // job tables of the stages run as multi-grid launches, per captured
+    // graph (the fields' buffers alternate): filled on the host while
+    // capturing, uploaded after
+    char* jobs_dev[2] = {nullptr, nullptr};
+    std::vector<char> jobs_host[2];
+    size_t jobs_bytes = 0; // per graph
+    size_t jobs_off = 0;   // fill level of the table set being captured
     int cap_which = 0;     // graph being captured
+    // a table's place in the set of graph cap_which (nullptr when full)
+    template <class J>
+    const J* put_jobs(const std::vector<J>& v) {
+        const size_t b = v.size() * sizeof(J);
+        if (jobs_off + b > jobs_bytes) {
+            set_error("batch job tables overflow");
+            return nullptr;
+        }
+        if (b) std::memcpy(jobs_host[cap_which].data() + jobs_off, v.data(), b);
+        const J* d = (const J*)(jobs_dev[cap_which] + jobs_off);
+        jobs_off += (b + 255) / 256 * 256;
+        return d;
+    }
+    // first blocks of a table of 256-thread jobs over n elements; the total
+    template <class J>
+    static int first_blocks(std::vector<J>& v) {
+        int fb = 0;
+        for (J& j : v) {
+            j.first_block = fb;
+            fb += (int)blocks_for((size_t)j.n, 256);
+        }
+        return fb;
+    }
     int w_fs_blocks = 0, w_rs_blocks = 0, w_restore_blocks = 0;
     std::vector<FusedScene> fs_h;
     std::vector<ResScene> rs_h;
@@ -109,7 +133,7 @@ Batch::~Batch() {
     if (fs) (void)hipFree(fs);
     if (rs) (void)hipFree(rs);
     for (void* q : {(void*)fsw, (void*)fsj, (void*)rsw, (void*)rsj, (void*)js, (void*)fsj2, (void*)rsj2,
-                    (void*)post_dev[0], (void*)post_dev[1]})
+                    (void*)jobs_dev[0], (void*)jobs_dev[1]})
         if (q) (void)hipFree(q);
     for (blz_solver* h : sc) delete h;
     if (root) (void)hipStreamDestroy(root);
@@ -142,11 +166,13 @@ int Batch::init(const blz_settings* st, int n) {
         set_error("hipMalloc failed for batch tables");
         return BLZ_E_NOMEM;
     }
-    // at most 3 pressure-update and 5 advection jobs per scene, in up to 9
-    // launch tables of 256-byte aligned starts
-    post_bytes = 3 * n * sizeof(PupJob) + 5 * n * sizeof(AdvJob) + 9 * 256;
-    for (auto& d : post_dev)
-        if (hipMalloc(&d, post_bytes) != hipSuccess) {
+    // per scene at most 2 stage jobs before the solve, 3 pressure-update and
+    // 5 advection jobs, and a control block reset, in up to 13 tables of
+    // 256-byte aligned starts
+    jobs_bytes = n * (sizeof(BuoyJob) + sizeof(DivJob) + sizeof(CtlJob) + 3 * sizeof(PupJob) + 5 * sizeof(AdvJob)) +
+                 13 * 256;
+    for (auto& d : jobs_dev)
+        if (hipMalloc(&d, jobs_bytes) != hipSuccess) {
             set_error("hipMalloc failed for batch tables");
             return BLZ_E_NOMEM;
         }
@@ -374,45 +400,93 @@ int Batch::joint_forward_and_finish(hipStream_t st) {
 // solves of all scenes on `st`
 int Batch::enqueue_step(hipStream_t st) {
     const int n = (int)sc.size();
+    jobs_host[cap_which].assign(jobs_bytes, 0);
+    jobs_off = 0;
     BLZ_HIP(hipMemcpyAsync(off_dev, off_host, 2 * n * sizeof(float), hipMemcpyHostToDevice, st));
-    BLZ_HIP(hipEventRecord(fork, st));
-    for (int q = 0; q < n; ++q) {
-        Solver& s = sc[q]->s;
-        BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
-        const float dt = s.s.dt;
-        // FluidSolver::step stages before the solve (src/FluidSolver.cu:868-925)
-        if (std::fabs(s.s.cooling_rate) > 0.0f)
-            if (int e = s.run_stage(BLZ_STAGE_COOLDOWN, 0.0f, dt)) return e;
-        if (s.s.drag > 0.0f)
-            if (int e = s.run_stage(BLZ_STAGE_DRAG, 0.0f, dt)) return e;
-        if (std::fabs(s.s.vorticity_conf) > 0.0f)
-            if (int e = s.run_stage(BLZ_STAGE_VORTICITY, 0.0f, dt)) return e;
-        if (std::fabs(s.s.buoyancy) > 0.0f)
-            if (int e = s.run_stage(BLZ_STAGE_BUOYANCY, 0.0f, dt)) return e;
-        if (s.s.wind_amp > 0.0f)
-            if (int e = s.run_stage(BLZ_STAGE_WIND, 0.0f, dt)) return e;
-        if (s.s.turbulence_amp > 0.0f)
-            if (int e = s.run_stage(BLZ_STAGE_TURBULENCE, 0.0f, dt)) return e;
-        if (int e = s.run_stage(BLZ_STAGE_DIVERGENCE, 0.0f, dt)) return e;
-        SolveCtl* c = s.ctl;
-        const double tol = (double)s.s.residual_tol;
-        const int cap = s.s.max_iter_solve;
-        if (int e = launch_ctl_reset(s, c)) return e;
-        if (int e = launch_residual_check(s, dt, c, s.done_host_dev, 0, tol, cap, 0, 1)) return e;
-        if (s.ws_net && s.s.warm_start && cap > 0) {
+    // FluidSolver::step's stages before the solve (src/FluidSolver.cu:868-925),
+    // each over the scenes that run it: buoyancy and divergence as
+    // multi-grid launches, the others on the scenes' streams
+    auto per_scene = [&](int stage, bool (*runs)(const blz_settings&)) -> int {
+        bool any = false;
+        for (int q = 0; q < n && !any; ++q) any = runs(sc[q]->s.s);
+        if (!any) return BLZ_OK;
+        BLZ_HIP(hipEventRecord(fork, st));
+        for (int q = 0; q < n; ++q) {
+            Solver& s = sc[q]->s;
+            if (!runs(s.s)) continue;
+            BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
+            if (int e = s.run_stage(stage, 0.0f, s.s.dt)) return e;
+            BLZ_HIP(hipEventRecord(join[q], s.stream));
+            BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
+        }
+        return BLZ_OK;
+    };
+    if (int e = per_scene(BLZ_STAGE_COOLDOWN, [](const blz_settings& b) { return std::fabs(b.cooling_rate) > 0.0f; }))
+        return e;
+    if (int e = per_scene(BLZ_STAGE_DRAG, [](const blz_settings& b) { return b.drag > 0.0f; })) return e;
+    if (int e = per_scene(BLZ_STAGE_VORTICITY, [](const blz_settings& b) { return std::fabs(b.vorticity_conf) > 0.0f; }))
+        return e;
+    {
+        std::vector<BuoyJob> bj[2];
+        for (int q = 0; q < n; ++q) {
+            Solver& s = sc[q]->s;
+            if (std::fabs(s.s.buoyancy) > 0.0f) buoyancy_jobs(s, s.s.dt, bj);
+        }
+        for (int m = 0; m < 2; ++m) {
+            const int blocks = first_blocks(bj[m]);
+            const BuoyJob* t = put_jobs(bj[m]);
+            if (!t) return BLZ_E_INVALID;
+            if (int e = launch_buoyancy_multi(st, m == 1, t, (int)bj[m].size(), blocks)) return e;
+        }
+    }
+    if (int e = per_scene(BLZ_STAGE_WIND, [](const blz_settings& b) { return b.wind_amp > 0.0f; })) return e;
+    if (int e = per_scene(BLZ_STAGE_TURBULENCE, [](const blz_settings& b) { return b.turbulence_amp > 0.0f; }))
+        return e;
+    {
+        std::vector<DivJob> dj;
+        std::vector<CtlJob> cj;
+        for (int q = 0; q < n; ++q) {
+            Solver& s = sc[q]->s;
+            divergence_jobs(s, dj);
+            ctl_reset_jobs(s, s.ctl, cj);
+        }
+        const int blocks = first_blocks(dj);
+        const DivJob* dt = put_jobs(dj);
+        const CtlJob* ct = put_jobs(cj);
+        if (!dt || !ct) return BLZ_E_INVALID;
+        if (int e = launch_divergence_multi(st, dt, n, blocks)) return e;
+        // the solve's control blocks, then its first residual check (as
+        // Solver::project_solve's launch_ctl_reset and launch_residual_check)
+        if (int e = launch_ctl_reset_multi(st, ct, n)) return e;
+        if (int e = launch_residual_multi(st, rs, n, rs_blocks, 0, 1)) return e;
+    }
+    // the warm start's corrections: on the scenes' streams where they do not
+    // run in multi-grid launches
+    bool any_ws = false;
+    for (int q = 0; q < n && !any_ws; ++q) {
+        const Solver& s = sc[q]->s;
+        any_ws = s.ws_net && s.s.warm_start && s.s.max_iter_solve > 0 && !(joint_fwd && ws_multi[q]);
+    }
+    if (any_ws) {
+        BLZ_HIP(hipEventRecord(fork, st));
+        for (int q = 0; q < n; ++q) {
+            Solver& s = sc[q]->s;
+            SolveCtl* c = s.ctl;
+            const double tol = (double)s.s.residual_tol;
+            const int cap = s.s.max_iter_solve;
+            if (!(s.ws_net && s.s.warm_start && cap > 0) || (joint_fwd && ws_multi[q])) continue;
+            BLZ_HIP(hipStreamWaitEvent(s.stream, fork, 0));
             if (joint_fwd) {
-                // the corrections below (this scene's own launches unless
-                // they run in the multi-grid ones)
-                if (!ws_multi[q])
-                    if (int e = warm_start_prepare_input(s, dt, c)) return e;
+                // network forward and the rest below
+                if (int e = warm_start_prepare_input(s, s.s.dt, c)) return e;
             } else if (!warm_q.empty()) {
-                if (int e = warm_start_prepare(s, dt, c)) return e; // judged below, with the other scenes
-            } else if (int e = warm_start_apply(s, dt, c, s.done_host_dev, 0, tol, cap, every)) {
+                if (int e = warm_start_prepare(s, s.s.dt, c)) return e; // judged below, with the other scenes
+            } else if (int e = warm_start_apply(s, s.s.dt, c, s.done_host_dev, 0, tol, cap, every)) {
                 return e;
             }
+            BLZ_HIP(hipEventRecord(join[q], s.stream));
+            BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
         }
-        BLZ_HIP(hipEventRecord(join[q], s.stream));
-        BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
     }
     if (joint_fwd)
         if (int e = joint_forward_and_finish(st)) return e;
@@ -544,40 +618,17 @@ int Batch::enqueue_update_and_advect(hipStream_t st) {
         pupdate_jobs(s, s.s.dt, pj);
         advect_jobs(s, s.s.dt, aj);
     }
-    std::vector<char>& h = post_host[cap_which];
-    h.assign(post_bytes, 0);
-    char* dev = post_dev[cap_which];
-    size_t off = 0;
-    // first blocks, placement in the table set; returns the launch's blocks
-    auto place = [&](auto& v, size_t* at) -> int {
-        int fb = 0;
-        for (auto& j : v) {
-            j.first_block = fb;
-            fb += (int)blocks_for((size_t)j.n, 256);
-        }
-        const size_t bytes = v.size() * sizeof(v[0]);
-        *at = off;
-        if (off + bytes > post_bytes) return -1;
-        if (bytes) std::memcpy(h.data() + off, v.data(), bytes);
-        off += (bytes + 255) / 256 * 256;
-        return fb;
-    };
-    auto overflow = [&]() {
-        set_error("batch job tables overflow");
-        return BLZ_E_INVALID;
-    };
     for (int a = 0; a < 3; ++a) {
-        size_t at = 0;
-        const int blocks = place(pj[a], &at);
-        if (blocks < 0) return overflow();
-        if (int e = launch_pupdate_multi(st, a, (const PupJob*)(dev + at), (int)pj[a].size(), blocks)) return e;
+        const int blocks = first_blocks(pj[a]);
+        const PupJob* t = put_jobs(pj[a]);
+        if (!t) return BLZ_E_INVALID;
+        if (int e = launch_pupdate_multi(st, a, t, (int)pj[a].size(), blocks)) return e;
     }
     for (int k = 0; k < 6; ++k) {
-        size_t at = 0;
-        const int blocks = place(aj[k], &at);
-        if (blocks < 0) return overflow();
-        if (int e = launch_advect_multi(st, k / 2, k & 1, (const AdvJob*)(dev + at), (int)aj[k].size(), blocks))
-            return e;
+        const int blocks = first_blocks(aj[k]);
+        const AdvJob* t = put_jobs(aj[k]);
+        if (!t) return BLZ_E_INVALID;
+        if (int e = launch_advect_multi(st, k / 2, k & 1, t, (int)aj[k].size(), blocks)) return e;
     }
     return BLZ_OK;
 }
@@ -609,8 +660,8 @@ int Batch::capture(int which) {
         return e;
     }
     BLZ_HIP(ce);
-    // the update and advection tables the captured launches read
-    if (hipMemcpy(post_dev[which], post_host[which].data(), post_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    // the job tables the captured launches read
+    if (hipMemcpy(jobs_dev[which], jobs_host[which].data(), jobs_bytes, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipGraphDestroy(graph);
         set_error("batch job table upload failed");
         return BLZ_E_HIP;

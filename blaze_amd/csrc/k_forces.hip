@@ -26,17 +26,38 @@ __global__ void __launch_bounds__(BS) k_scale(float* __restrict__ a, size_t n, f
 
 // V faces: V += dt * ((beta * T~) * g), T~ trilinear at the face position
 template <class S>
-__global__ void __launch_bounds__(BS) k_buoyancy(Gv T, float* __restrict__ V, int vnx, int vny, int vnz, float vox,
-                                                 float voy, float voz, S sc, float dx, float beta, float g,
-                                                 float dt) {
+__device__ __forceinline__ void buoyancy_cell(const Gv& T, float* __restrict__ V, int vnx, int vny, int vnz,
+                                              float vox, float voy, float voz, const S& sc, float dx, float beta,
+                                              float g, float dt, int t) {
     const int n = vnx * vny * vnz;
-    const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int i = t % vnx, j = (t / vnx) % vny, k = t / (vnx * vny);
     const f3 pos = mk3(((float)i + vox) * dx, ((float)j + voy) * dx, ((float)k + voz) * dx);
     const float temp = trilinear_s(T, sc, pos);
     const float force = beta * temp * g;
     V[t] += dt * force;
+}
+
+template <class S>
+__global__ void __launch_bounds__(BS) k_buoyancy(Gv T, float* __restrict__ V, int vnx, int vny, int vnz, float vox,
+                                                 float voy, float voz, S sc, float dx, float beta, float g,
+                                                 float dt) {
+    buoyancy_cell(T, V, vnx, vny, vnz, vox, voy, voz, sc, dx, beta, g, dt, (int)(blockIdx.x * BS + threadIdx.x));
+}
+
+// the buoyancy of every grid in a table (blocks mapped through first blocks)
+template <class S>
+__global__ void __launch_bounds__(BS) k_buoyancy_multi(const BuoyJob* __restrict__ tab, int n) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].first_block <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const BuoyJob& e = tab[lo];
+    const S sc{e.s};
+    buoyancy_cell(e.T, e.V, e.vnx, e.vny, e.vnz, e.vox, e.voy, e.voz, sc, e.dx, e.beta, e.g, e.dt,
+                  (int)((blockIdx.x - e.first_block) * BS + threadIdx.x));
 }
 
 // computeVorticity :174-201 at cell (i,j,k) of the T grid samples U, V and W
@@ -266,6 +287,37 @@ static int buoyancy(Solver& s, const S& sc, float dt) {
 int launch_buoyancy(Solver& s, float dt) {
     float inv = 0.0f;
     return dx_pow2(s.dx, &inv) ? buoyancy(s, MulDx{inv}, dt) : buoyancy(s, DivDx{s.dx}, dt);
+}
+
+void buoyancy_jobs(Solver& s, float dt, std::vector<BuoyJob>* jobs) {
+    float inv = 0.0f;
+    const bool mul = dx_pow2(s.dx, &inv);
+    const Field& V = s.f[F_V];
+    BuoyJob j;
+    j.T = s.f[F_T].view();
+    j.V = V.front();
+    j.vnx = V.nx;
+    j.vny = V.ny;
+    j.vnz = V.nz;
+    j.vox = V.ox;
+    j.voy = V.oy;
+    j.voz = V.oz;
+    j.s = mul ? inv : s.dx;
+    j.dx = s.dx;
+    j.beta = s.s.buoyancy;
+    j.g = s.s.gravity;
+    j.dt = dt;
+    j.first_block = 0;
+    j.n = (int)V.count();
+    jobs[mul ? 1 : 0].push_back(j);
+}
+
+int launch_buoyancy_multi(hipStream_t st, bool mul, const BuoyJob* tab, int n, int blocks) {
+    if (n < 1 || blocks < 1) return BLZ_OK;
+    if (mul) hipLaunchKernelGGL(k_buoyancy_multi<MulDx>, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    else hipLaunchKernelGGL(k_buoyancy_multi<DivDx>, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
 }
 
 static int gather3(Solver& s, bool range2) {

@@ -25,10 +25,9 @@ __device__ __forceinline__ float p_at(const RB& p, const RBL& L, int i, int j, i
 }
 
 // compute_divergence_kernel :475-515, written into the split layout
-__global__ void __launch_bounds__(BS) k_divergence(const float* __restrict__ U, const float* __restrict__ V,
-                                                   const float* __restrict__ Wf, RB rhs, RBL L, int W, int H, int D,
-                                                   float sc, int4 cbA, int2 cbB) {
-    const int t = blockIdx.x * BS + threadIdx.x;
+__device__ __forceinline__ void divergence_cell(const float* __restrict__ U, const float* __restrict__ V,
+                                                const float* __restrict__ Wf, const RB& rhs, const RBL& L, int W,
+                                                int H, int D, float sc, int4 cbA, int2 cbB, int t) {
     if (t >= W * H * D) return;
     const int i = t % W, j = (t / W) % H, k = t / (W * H);
     const int u0 = i + j * (W + 1) + k * (W + 1) * H;
@@ -55,6 +54,12 @@ __global__ void __launch_bounds__(BS) k_divergence(const float* __restrict__ U, 
         if (cbB.y) r += sc * w_hi;
     }
     rhs.c[(i + j) & 1][L.at(i, j, k)] = r;
+}
+
+__global__ void __launch_bounds__(BS) k_divergence(const float* __restrict__ U, const float* __restrict__ V,
+                                                   const float* __restrict__ Wf, RB rhs, RBL L, int W, int H, int D,
+                                                   float sc, int4 cbA, int2 cbB) {
+    divergence_cell(U, V, Wf, rhs, L, W, H, D, sc, cbA, cbB, (int)(blockIdx.x * BS + threadIdx.x));
 }
 
 // XCD-aware block order: the dispatcher hands consecutive workgroups to the
@@ -1142,8 +1147,8 @@ __global__ void __launch_bounds__(BS) k_res_final_multi(const ResScene* __restri
     res_final_block(e.partial, e.nblocks, e.ctl, e.host_done, e.seq, e.tol, e.cap, add, first);
 }
 
-__global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap, int adaptive, int stop_stall,
-                            int every) {
+__device__ __forceinline__ void ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap, int adaptive,
+                                          int stop_stall, int every) {
     c->adaptive = adaptive;
     c->stop_stall = stop_stall;
     c->every = every;
@@ -1165,6 +1170,19 @@ __global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap, 
     c->ws_rejected = 0;
     c->res_prev = -1.0;
     c->skip_spare = 1;
+}
+
+__global__ void k_ctl_reset(SolveCtl* c, blz_trace_entry* trace, int trace_cap, int adaptive, int stop_stall,
+                            int every) {
+    ctl_reset(c, trace, trace_cap, adaptive, stop_stall, every);
+}
+
+// the control blocks of a table, one thread each
+__global__ void __launch_bounds__(64) k_ctl_reset_multi(const CtlJob* __restrict__ tab, int n) {
+    const int t = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (t >= n) return;
+    const CtlJob& e = tab[t];
+    ctl_reset(e.c, e.trace, e.trace_cap, e.adaptive, e.stop_stall, e.every);
 }
 
 // pressure_gradient_update_kernel :601-669 for the faces along AXIS,
@@ -1215,6 +1233,13 @@ __device__ __forceinline__ const J& job_of(const J* __restrict__ tab, int n, int
         else hi = mid - 1;
     }
     return tab[lo];
+}
+
+// the divergence of every grid in a table
+__global__ void __launch_bounds__(BS) k_divergence_multi(const DivJob* __restrict__ tab, int n) {
+    const DivJob& e = job_of(tab, n, (int)blockIdx.x);
+    divergence_cell(e.U, e.V, e.Wf, e.rhs, e.L, e.W, e.H, e.D, e.sc, make_int4(e.cb[0], e.cb[1], e.cb[2], e.cb[3]),
+                    make_int2(e.cb[4], e.cb[5]), (int)((blockIdx.x - e.first_block) * BS + threadIdx.x));
 }
 
 // the pressure update on one axis of every grid in a table
@@ -1340,6 +1365,30 @@ int launch_divergence(Solver& s) {
     const float sc = 1.0f / s.dx;
     LAUNCH(k_divergence, (size_t)s.W * s.H * s.D, s.f[F_U].front(), s.f[F_V].front(), s.f[F_W].front(), s.rhs, s.L,
            s.W, s.H, s.D, sc, cb_a(s), cb_b(s));
+    return BLZ_OK;
+}
+
+void divergence_jobs(Solver& s, std::vector<DivJob>& jobs) {
+    DivJob j;
+    j.U = s.f[F_U].front();
+    j.V = s.f[F_V].front();
+    j.Wf = s.f[F_W].front();
+    j.rhs = s.rhs;
+    j.L = s.L;
+    j.W = s.W;
+    j.H = s.H;
+    j.D = s.D;
+    j.sc = 1.0f / s.dx;
+    for (int a = 0; a < 6; ++a) j.cb[a] = s.s.closed_bounds[a];
+    j.first_block = 0;
+    j.n = s.W * s.H * s.D;
+    jobs.push_back(j);
+}
+
+int launch_divergence_multi(hipStream_t st, const DivJob* tab, int n, int blocks) {
+    if (n < 1 || blocks < 1) return BLZ_OK;
+    hipLaunchKernelGGL(k_divergence_multi, dim3((unsigned)blocks), dim3(BS), 0, st, tab, n);
+    BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }
 
@@ -1548,12 +1597,30 @@ int launch_residual_multi(hipStream_t st, const ResScene* tab, int n, int blocks
     return BLZ_OK;
 }
 
-int launch_ctl_reset(Solver& s, SolveCtl* c, bool adaptive, bool stop_stall) {
+// the launch arguments of a control block's reset
+static CtlJob ctl_job(Solver& s, SolveCtl* c, bool adaptive, bool stop_stall) {
     // the residual history slot of this solve (control block q of s.ctl)
     const ptrdiff_t q = c - s.ctl;
     blz_trace_entry* tr = (s.trace && q >= 0 && q < MAX_SOLVES) ? s.trace + (size_t)q * s.trace_cap : nullptr;
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s.stream, c, tr, tr ? s.trace_cap : 0, adaptive ? 1 : 0,
-                       stop_stall ? 1 : 0, std::max(1, s.s.residual_check_every));
+    return CtlJob{c, tr, tr ? s.trace_cap : 0, adaptive ? 1 : 0, stop_stall ? 1 : 0,
+                  std::max(1, s.s.residual_check_every)};
+}
+
+int launch_ctl_reset(Solver& s, SolveCtl* c, bool adaptive, bool stop_stall) {
+    const CtlJob j = ctl_job(s, c, adaptive, stop_stall);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s.stream, j.c, j.trace, j.trace_cap, j.adaptive,
+                       j.stop_stall, j.every);
+    BLZ_HIP(hipGetLastError());
+    return BLZ_OK;
+}
+
+void ctl_reset_jobs(Solver& s, SolveCtl* c, std::vector<CtlJob>& jobs, bool adaptive, bool stop_stall) {
+    jobs.push_back(ctl_job(s, c, adaptive, stop_stall));
+}
+
+int launch_ctl_reset_multi(hipStream_t st, const CtlJob* tab, int n) {
+    if (n < 1) return BLZ_OK;
+    hipLaunchKernelGGL(k_ctl_reset_multi, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, tab, n);
     BLZ_HIP(hipGetLastError());
     return BLZ_OK;
 }

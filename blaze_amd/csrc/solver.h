@@ -366,6 +366,37 @@ void pupdate_jobs(Solver& s, float dt, std::vector<PupJob>* jobs);
 void advect_jobs(Solver& s, float dt, std::vector<AdvJob>* jobs);
 int launch_pupdate_multi(hipStream_t st, int axis, const PupJob* tab, int n, int blocks);
 int launch_advect_multi(hipStream_t st, int kind, bool mul, const AdvJob* tab, int n, int blocks);
+// the same for the stages before the solve: buoyancy (jobs[1/dx exact]),
+// divergence, and the control block's reset (one thread per job)
+struct BuoyJob {
+    Gv T;
+    float* V;
+    int vnx, vny, vnz;
+    float vox, voy, voz;
+    float s, dx, beta, g, dt; // s: the scale's parameter, as AdvJob's
+    int first_block, n;
+};
+struct DivJob {
+    const float *U, *V, *Wf;
+    RB rhs;
+    RBL L;
+    int W, H, D;
+    float sc;
+    int cb[6]; // closed_bounds
+    int first_block, n;
+};
+struct CtlJob {
+    SolveCtl* c;
+    blz_trace_entry* trace;
+    int trace_cap, adaptive, stop_stall, every;
+};
+void buoyancy_jobs(Solver& s, float dt, std::vector<BuoyJob>* jobs);
+void divergence_jobs(Solver& s, std::vector<DivJob>& jobs);
+void ctl_reset_jobs(Solver& s, SolveCtl* c, std::vector<CtlJob>& jobs, bool adaptive = false,
+                    bool stop_stall = false);
+int launch_buoyancy_multi(hipStream_t st, bool mul, const BuoyJob* tab, int n, int blocks);
+int launch_divergence_multi(hipStream_t st, const DivJob* tab, int n, int blocks);
+int launch_ctl_reset_multi(hipStream_t st, const CtlJob* tab, int n);
 int launch_p2g(Solver& s);
 size_t p2g_entry_bound(const float* part8, int n, float dx, int nch);
 int launch_max_abs_velocity(Solver& s, double* dev_out);
