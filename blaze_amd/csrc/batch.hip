@@ -448,7 +448,10 @@ int Batch::enqueue_step(hipStream_t st) {
         for (int q = 0; q < n; ++q) {
             Solver& s = sc[q]->s;
             divergence_jobs(s, dj);
-            ctl_reset_jobs(s, s.ctl, cj);
+            // adaptive check spacing and the fixed-point stop as
+            // Solver::project_solve sets them
+            const bool warm = s.ws_net && s.s.warm_start && s.s.max_iter_solve > 0;
+            ctl_reset_jobs(s, s.ctl, cj, s.s.residual_check_adaptive && !warm, s.s.residual_check_adaptive != 0);
         }
         const int blocks = first_blocks(dj);
         const DivJob* dt = put_jobs(dj);
