@@ -277,6 +277,29 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
     }
 }
 
+// Rows of at most 64 / P float4s (W <= 256 / P) leave lanes 64 / P .. 63 of
+// a wave idle. Such grids pack P virtual waves of 64 / P lanes into each
+// wave, and a workgroup then holds P blocks (virtual waves u = P w + lane /
+// (64 / P); block u / RW of the workgroup, wave u % RW of that block). A
+// virtual wave sums exactly what a full wave sums: its FP32 partials are
+// the same, and a full wave's butterfly over 64 lanes first adds the idle
+// lanes' +0 (x + 0, which the packed wave applies once) before the steps
+// inside 64 / P lanes. The LDS slots of the cells of a row: block b's cell
+// I at b (64 / P) + I, 64 in all as for one full block.
+__host__ __device__ __forceinline__ int rows_pack(int W) { return W <= 64 ? 4 : (W <= 128 ? 2 : 1); }
+
+// the 64-lane butterfly sum of a full wave, for a virtual wave of s = 64 / P
+// lanes whose other lanes would hold +0
+__device__ __forceinline__ double vwave_sum(double d, int P) {
+    if (P == 1) {
+        for (int o = 32; o >= 1; o >>= 1) d += __shfl_xor(d, o, 64);
+        return d;
+    }
+    d += 0.0;
+    for (int o = 32 / P; o >= 1; o >>= 1) d += __shfl_xor(d, o, 64);
+    return d;
+}
+
 // The global correction's scale and the network's input in one pass over
 // the fine grid (the shape of k_ws_restrict_rows: a workgroup of RW waves per
 // row (J, K) of the network grid, lane = float4 column): with e the global
@@ -289,14 +312,18 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
 __device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restrict__ r, const float* __restrict__ e,
                                                            int W, int H, int D, int F, int Wc, int Hc, int Dc, int Wn,
                                                            int Hn, double* __restrict__ S, double* __restrict__ partial,
-                                                           const int* __restrict__ stop, unsigned bx) {
+                                                           const int* __restrict__ stop, int P, unsigned nvb,
+                                                           unsigned bx) {
     if (*stop) return;
-    const int J = bx % Hn, K = bx / Hn;
-    const size_t row0 = ((size_t)K * Hn + J) * Wn;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lanes = 64 / P;
+    const int lane = (threadIdx.x & 63) % lanes, u = (int)(threadIdx.x >> 6) * P + (threadIdx.x & 63) / lanes;
+    const int vb = u / RW, w = u % RW; // block of the workgroup, its wave
+    const unsigned blk = bx * (unsigned)P + (unsigned)vb;
+    const bool live = blk < nvb;
+    const int J = (int)(blk % (unsigned)Hn), K = (int)(blk / (unsigned)Hn);
     const int i0 = 4 * lane;
     const int y0 = F * J, z0 = F * K;
-    const bool real_row = J < Hc && K < Dc;
+    const bool real_row = live && J < Hc && K < Dc;
     float sr = 0.0f, sa = 0.0f;
     float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (real_row && i0 < W) {
@@ -311,13 +338,13 @@ __device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restri
             float l[4];
             lap4(e, t, i0, j, k, W, H, D, ev, l);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                sr += rv[u];
-                sa += l[u];
-                v[0] += rv[u] * ev[u];
-                v[1] += ev[u] * l[u];
-                v[2] += rv[u];
-                v[3] += ev[u];
+            for (int uu = 0; uu < 4; ++uu) {
+                sr += rv[uu];
+                sa += l[uu];
+                v[0] += rv[uu] * ev[uu];
+                v[1] += ev[uu] * l[uu];
+                v[2] += rv[uu];
+                v[3] += ev[uu];
             }
         }
     }
@@ -329,34 +356,36 @@ __device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restri
     }
     double d[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        d[q] = (double)v[q];
-        for (int o = 32; o >= 1; o >>= 1) d[q] += __shfl_xor(d[q], o, 64);
-    }
+    for (int q = 0; q < 4; ++q) d[q] = vwave_sum((double)v[q], P);
     __shared__ double sc[RW][64][2];
-    __shared__ double sd[RW][4];
+    __shared__ double sd[4][RW][4];
     const int I = lane / gl;
     if ((lane & (gl - 1)) == 0) {
-        sc[w][I][0] = cr;
-        sc[w][I][1] = ca;
+        sc[w][vb * lanes + I][0] = cr;
+        sc[w][vb * lanes + I][1] = ca;
     }
-    if (lane < 4) sd[w][lane] = d[lane];
+    if (lane < 4) sd[vb][w][lane] = d[lane];
     __syncthreads();
-    if (threadIdx.x < Wn) {
-        const int c = threadIdx.x;
+    // thread groups of 256 finish the workgroup's blocks
+    const int ob = (int)(threadIdx.x >> 8), c = (int)(threadIdx.x & 255);
+    const unsigned oblk = bx * (unsigned)P + (unsigned)ob;
+    if (ob >= P || oblk >= nvb) return;
+    const int oJ = (int)(oblk % (unsigned)Hn), oK = (int)(oblk / (unsigned)Hn);
+    const size_t row0 = ((size_t)oK * Hn + oJ) * Wn;
+    if (c < Wn) {
         double tr = 0.0, ta = 0.0;
-        if (real_row && c < Wc)
-            for (int u = 0; u < RW; ++u) {
-                tr += sc[u][c][0];
-                ta += sc[u][c][1];
+        if (oJ < Hc && oK < Dc && c < Wc)
+            for (int uu = 0; uu < RW; ++uu) {
+                tr += sc[uu][ob * lanes + c][0];
+                ta += sc[uu][ob * lanes + c][1];
             }
         S[2 * (row0 + c)] = tr;
         S[2 * (row0 + c) + 1] = ta;
     }
-    if (threadIdx.x < 4) {
+    if (c < 4) {
         double a = 0.0;
-        for (int u = 0; u < RW; ++u) a += sd[u][threadIdx.x];
-        partial[4 * bx + threadIdx.x] = a;
+        for (int uu = 0; uu < RW; ++uu) a += sd[ob][uu][c];
+        partial[4 * oblk + c] = a;
     }
 }
 
@@ -365,8 +394,9 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* 
                                                                    int F, int Wc, int Hc, int Dc, int Wn, int Hn,
                                                                    double* __restrict__ S,
                                                                    double* __restrict__ partial,
-                                                                   const int* __restrict__ stop) {
-    ws_restrict_dots_rows_body(r, e, W, H, D, F, Wc, Hc, Dc, Wn, Hn, S, partial, stop, blockIdx.x);
+                                                                   const int* __restrict__ stop, int P,
+                                                                   unsigned nvb) {
+    ws_restrict_dots_rows_body(r, e, W, H, D, F, Wc, Hc, Dc, Wn, Hn, S, partial, stop, P, nvb, blockIdx.x);
 }
 
 // alpha_G from the row partials of k_ws_restrict_dots_rows (as k_ws_alpha:
@@ -741,16 +771,22 @@ __device__ __forceinline__ void ws_blockdots_rows_body(const float* __restrict__
                                                        const double* __restrict__ alpha1, const float* __restrict__ e2,
                                                        int W, int H, int D, int F, int Gw, int Gh,
                                                        double* __restrict__ partial, double* __restrict__ sg,
-                                                       const SolveCtl* __restrict__ c, unsigned bx) {
+                                                       const SolveCtl* __restrict__ c, int P, unsigned nvb,
+                                                       unsigned bx) {
     if (c->done) return;
-    const int J = bx % Gh, K = bx / Gh;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // P blocks per workgroup in packed virtual waves (rows_pack)
+    const int lanes = 64 / P;
+    const int lane = (threadIdx.x & 63) % lanes, u = (int)(threadIdx.x >> 6) * P + (threadIdx.x & 63) / lanes;
+    const int vb = u / RW, w = u % RW;
+    const unsigned blk = bx * (unsigned)P + (unsigned)vb;
+    const bool live = blk < nvb;
+    const int J = (int)(blk % (unsigned)Gh), K = (int)(blk / (unsigned)Gh);
     const int i0 = 4 * lane;
     const float a1 = e1 ? (float)*alpha1 : 0.0f;
     float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     const int y0 = F * J, z0 = F * K;
     const int ny = min(F, H - y0), nz = min(F, D - z0);
-    if (i0 < W)
+    if (live && i0 < W)
         // unrolled so that the next rows' loads issue before this row's sums
 #pragma unroll 4
         for (int q = w; q < ny * nz; q += RW) {
@@ -764,16 +800,20 @@ __device__ __forceinline__ void ws_blockdots_rows_body(const float* __restrict__
         d[q] = (double)v[q];
         for (int o = 1; o < gl; o <<= 1) d[q] += __shfl_xor(d[q], o, 64);
     }
-    __shared__ double sv[RW][16][5];
+    __shared__ double sv[RW][64][5]; // slot 16 vb + I
     const int I = lane / gl;
     if ((lane & (gl - 1)) == 0 && I < Gw)
-        for (int q = 0; q < 5; ++q) sv[w][I][q] = d[q];
+        for (int q = 0; q < 5; ++q) sv[w][16 * vb + I][q] = d[q];
     __syncthreads();
-    if (threadIdx.x < Gw * 5) {
-        const int bi = threadIdx.x / 5, q = threadIdx.x % 5;
+    const int ob = (int)(threadIdx.x >> 8), t = (int)(threadIdx.x & 255);
+    const unsigned oblk = bx * (unsigned)P + (unsigned)ob;
+    if (ob >= P || oblk >= nvb) return;
+    if (t < Gw * 5) {
+        const int oJ = (int)(oblk % (unsigned)Gh), oK = (int)(oblk / (unsigned)Gh);
+        const int bi = t / 5, q = t % 5;
         double a = 0.0;
-        for (int u = 0; u < RW; ++u) a += sv[u][bi][q];
-        const int b = (K * Gh + J) * Gw + bi;
+        for (int uu = 0; uu < RW; ++uu) a += sv[uu][16 * ob + bi][q];
+        const int b = (oK * Gh + oJ) * Gw + bi;
         if (q < 4) partial[4 * b + q] = a;
         if (sg && q == 2) sg[2 * b] = a;
         if (sg && q == 4) sg[2 * b + 1] = a;
@@ -787,8 +827,8 @@ __global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __re
                                                                const float* __restrict__ e2, int W, int H, int D,
                                                                int F, int Gw, int Gh, double* __restrict__ partial,
                                                                double* __restrict__ sg,
-                                                               const SolveCtl* __restrict__ c) {
-    ws_blockdots_rows_body<E1, E2, SYM>(r, e1, alpha1, e2, W, H, D, F, Gw, Gh, partial, sg, c, blockIdx.x);
+                                                               const SolveCtl* __restrict__ c, int P, unsigned nvb) {
+    ws_blockdots_rows_body<E1, E2, SYM>(r, e1, alpha1, e2, W, H, D, F, Gw, Gh, partial, sg, c, P, nvb, blockIdx.x);
 }
 
 // alpha from the per-block partial sums (one workgroup, fixed order: every
@@ -1378,22 +1418,24 @@ inline int ilog2(int f) {
 void launch_blockdots(const Solver& s, const float* r, const float* e1, const double* a1, const float* e2,
                       double* partial, double* sg, const SolveCtl* c, bool sym = false) {
     if (s.W % 4 == 0 && s.W <= 256 && s.ws_g >= 4 && s.Gw <= 16) {
-        const dim3 g((unsigned)(s.Gh * s.Gd)), b(64 * RW);
+        const int P = rows_pack(s.W);
+        const unsigned nvb = (unsigned)(s.Gh * s.Gd);
+        const dim3 g((nvb + P - 1) / P), b(64 * RW);
         if (e1 && e2 && sym && !sg)
             hipLaunchKernelGGL((k_ws_blockdots_rows<true, true, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H,
-                               s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c);
+                               s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
         else if (e1 && e2)
             hipLaunchKernelGGL((k_ws_blockdots_rows<true, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
         else if (e1)
             hipLaunchKernelGGL((k_ws_blockdots_rows<true, false>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
         else if (e2)
             hipLaunchKernelGGL((k_ws_blockdots_rows<false, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
         else
             hipLaunchKernelGGL((k_ws_blockdots_rows<false, false>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
         return;
     }
     const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
@@ -1438,6 +1480,7 @@ struct WsJob {
     bf16* x0;
     SolveCtl* c;
     int ws_g, Gw, Gh, Gd, ws_f, Wc, Hc, Dc, Wn, Hn, nn;
+    int P; // rows_pack(W)
     double N;
     int fb[WK_N];
 };
@@ -1464,7 +1507,7 @@ __global__ void __launch_bounds__(BS) k_wsm_resid(const WsJob* __restrict__ tab,
 __global__ void __launch_bounds__(64 * RW) k_wsm_sums(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_SUMS);
     ws_blockdots_rows_body<false, false, false>(j.r, nullptr, nullptr, nullptr, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
-                                                j.part, j.sg, j.c, bx);
+                                                j.part, j.sg, j.c, j.P, (unsigned)(j.Gh * j.Gd), bx);
 }
 __global__ void __launch_bounds__(GT) k_wsm_gdct(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_GDCT);
@@ -1477,7 +1520,7 @@ __global__ void __launch_bounds__(BS) k_wsm_prolong_g(const WsJob* __restrict__ 
 __global__ void __launch_bounds__(64 * RW) k_wsm_restrict(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_RESTRICT);
     ws_restrict_dots_rows_body(j.r, j.eg, j.W, j.H, j.D, j.ws_f, j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.S, j.part,
-                               &j.c->done, bx);
+                               &j.c->done, j.P, (unsigned)(j.nn / j.Wn), bx);
 }
 __global__ void __launch_bounds__(1024) k_wsm_alpha_res(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_ALPHA_RES);
@@ -1495,7 +1538,7 @@ __global__ void __launch_bounds__(BS) k_wsm_prolong_n(const WsJob* __restrict__ 
 __global__ void __launch_bounds__(64 * RW) k_wsm_dots(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_DOTS);
     ws_blockdots_rows_body<true, true, true>(j.r, j.eg, j.alpha + 1, j.enet, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
-                                             j.part, nullptr, j.c, bx);
+                                             j.part, nullptr, j.c, j.P, (unsigned)(j.Gh * j.Gd), bx);
 }
 __global__ void __launch_bounds__(BS) k_wsm_alpha(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_ALPHA);
@@ -1674,8 +1717,11 @@ static int ws_global_and_input(Solver& s, float dt, SolveCtl* c, SolveCtl* begin
     if (restrict_rows_ok(s) && 3 * N + 4 * nn <= s.tmp_floats) {
         // alpha_G and the network's input from one pass over the fine grid
         double* S = (double*)(s.tmp + 3 * N);
-        hipLaunchKernelGGL(k_ws_restrict_dots_rows, dim3((unsigned)(nn / s.Wn)), dim3(64 * RW), 0, s.stream, ws_r(s),
-                           ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s), &c->done);
+        const int P = rows_pack(s.W);
+        const unsigned nvb = (unsigned)(nn / s.Wn);
+        hipLaunchKernelGGL(k_ws_restrict_dots_rows, dim3((nvb + P - 1) / P), dim3(64 * RW), 0, s.stream, ws_r(s),
+                           ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s), &c->done, P,
+                           nvb);
         hipLaunchKernelGGL(k_ws_alpha_res, dim3(1), dim3(1024), 0, s.stream, ws_part(s), (int)(nn / s.Wn), (double)N,
                            a_g, S, res, mask, amax, s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, (int)nn,
                            &c->done);
@@ -1782,17 +1828,19 @@ int WsMulti::build(const std::vector<Solver*>& grids, const std::vector<float>& 
         j.Wn = s.Wn;
         j.Hn = s.Hn;
         j.nn = (int)nn;
+        j.P = rows_pack(s.W);
         j.N = (double)N;
         // the single-grid launches' grids
+        const unsigned sums = (unsigned)((s.Gh * s.Gd + j.P - 1) / j.P); // packed rows passes (rows_pack)
         const unsigned nblk[WK_N] = {nb(split_threads(s)),
-                                     (unsigned)(s.Gh * s.Gd),
+                                     sums,
                                      1u,
                                      nb((size_t)(s.W / 4) * s.H * s.D),
-                                     (unsigned)(nn / s.Wn),
+                                     (unsigned)((nn / s.Wn + j.P - 1) / j.P),
                                      1u,
                                      nb(nn * NET_IN_CH),
                                      nb((size_t)(s.W / 4) * s.H * s.D),
-                                     (unsigned)(s.Gh * s.Gd),
+                                     sums,
                                      1u,
                                      nb(split_threads(s))};
         for (int k = 0; k < WK_N; ++k) {
