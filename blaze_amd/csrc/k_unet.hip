@@ -42,10 +42,21 @@ __device__ __forceinline__ void vox_coords(long long v, const Geo& g, int& b, in
     b = (int)(t / g.D);
 }
 
+// the same for grids of fewer than 2^31 voxels (32-bit divisions)
+__device__ __forceinline__ void vox_coords32(unsigned v, const Geo& g, int& b, int& z, int& y, int& x) {
+    x = (int)(v % (unsigned)g.W);
+    unsigned t = v / (unsigned)g.W;
+    y = (int)(t % (unsigned)g.H);
+    t /= (unsigned)g.H;
+    z = (int)(t % (unsigned)g.D);
+    b = (int)(t / (unsigned)g.D);
+}
+
 // index in the grid twice as fine of the first child (2z, 2y, 2x) of voxel v
-__device__ __forceinline__ long long up_child0(long long v, const Geo& g) {
+// (v < 2^31)
+__device__ __forceinline__ long long up_child0(unsigned v, const Geo& g) {
     int b, z, y, x;
-    vox_coords(v, g, b, z, y, x);
+    vox_coords32(v, g, b, z, y, x);
     return (((long long)b * 2 * g.D + 2 * z) * (2 * (long long)g.H) + 2 * y) * (2 * (long long)g.W) + 2 * x;
 }
 
@@ -101,7 +112,8 @@ __device__ __forceinline__ void conv3_tile(const bf16* __restrict__ X, int Cx, i
     const int li = lane & 15, kg = lane >> 4;
     const long long va = v0 + li;
     int b, z, y, x;
-    vox_coords(va < nvox ? va : nvox - 1, g, b, z, y, x);
+    // voxel indices fit in 32 bits (checked by the launcher)
+    vox_coords32((unsigned)(va < nvox ? va : nvox - 1), g, b, z, y, x);
     const bool vok = va < nvox;
     f4m acc = f4m{0.f, 0.f, 0.f, 0.f};
     // element offsets fit in 32 bits (checked by the launcher)
@@ -166,7 +178,7 @@ __device__ __forceinline__ void conv3_tile(const bf16* __restrict__ X, int Cx, i
         const bf16 out = f2bf(r);
         Y[vo * Cy + yoff + co] = out;
         if constexpr (UP) {
-            const long long f0 = up_child0(vo, g);
+            const long long f0 = up_child0((unsigned)vo, g);
             const long long FW = 2 * (long long)g.W, FHW = 4 * (long long)g.H * g.W;
 #pragma unroll
             for (int d = 0; d < 8; ++d) {
@@ -293,11 +305,13 @@ __device__ __forceinline__ long long parent_of(long long v, const Geo& fine, int
 __device__ __forceinline__ void pool_one(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd,
                                          int doff, int C, const Geo& fine, long long ncoarse, long long t) {
     if (t >= ncoarse * C) return;
-    const int c = (int)(t % C);
-    const long long vc = t / C;
+    // ncoarse C < 2^31 (the pooled view's 32-bit offsets, checked where
+    // its convolution is launched): 32-bit divisions
+    const int c = (int)((unsigned)t % (unsigned)C);
+    const long long vc = (long long)((unsigned)t / (unsigned)C);
     const Geo cg{fine.B, fine.D / 2, fine.H / 2, fine.W / 2};
     int b, z, y, x;
-    vox_coords(vc, cg, b, z, y, x);
+    vox_coords32((unsigned)vc, cg, b, z, y, x);
     float s = 0.0f;
     for (int q = 0; q < 8; ++q) {
         const long long vf = (((long long)b * fine.D + 2 * z + (q >> 2)) * fine.H + 2 * y + ((q >> 1) & 1)) * fine.W +
@@ -339,11 +353,13 @@ __global__ void k_up_bwd(const bf16* __restrict__ gU, int Cu, int uoff, bf16* __
                          const bf16* __restrict__ Bm, int Cbm, int bmoff, int C, Geo fine, long long ncoarse) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncoarse * C) return;
-    const int c = (int)(t % C);
-    const long long vc = t / C;
+    // ncoarse C < 2^31 (the pooled view's 32-bit offsets, checked where
+    // its convolution is launched): 32-bit divisions
+    const int c = (int)((unsigned)t % (unsigned)C);
+    const long long vc = (long long)((unsigned)t / (unsigned)C);
     const Geo cg{fine.B, fine.D / 2, fine.H / 2, fine.W / 2};
     int b, z, y, x;
-    vox_coords(vc, cg, b, z, y, x);
+    vox_coords32((unsigned)vc, cg, b, z, y, x);
     float s = 0.0f;
     for (int q = 0; q < 8; ++q) {
         const long long vf = (((long long)b * fine.D + 2 * z + (q >> 2)) * fine.H + 2 * y + ((q >> 1) & 1)) * fine.W +

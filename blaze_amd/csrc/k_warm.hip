@@ -312,10 +312,93 @@ __device__ __forceinline__ double vwave_sum(double d, int P) {
 __device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restrict__ r, const float* __restrict__ e,
                                                            int W, int H, int D, int F, int Wc, int Hc, int Dc, int Wn,
                                                            int Hn, double* __restrict__ S, double* __restrict__ partial,
-                                                           const int* __restrict__ stop, int P, unsigned nvb,
-                                                           unsigned bx) {
+                                                           const int* __restrict__ stop, unsigned bx) {
     if (*stop) return;
-    const int lanes = 64 / P;
+    const int J = bx % Hn, K = bx / Hn;
+    const size_t row0 = ((size_t)K * Hn + J) * Wn;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 4 * lane;
+    const int y0 = F * J, z0 = F * K;
+    const bool real_row = J < Hc && K < Dc;
+    float sr = 0.0f, sa = 0.0f;
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (real_row && i0 < W) {
+        const int ny = min(F, H - y0), nz = min(F, D - z0);
+        // unrolled so that the next rows' loads issue before this row's sums
+#pragma unroll 4
+        for (int q = w; q < ny * nz; q += RW) {
+            const int j = y0 + q % ny, k = z0 + q / ny;
+            const size_t t = ((size_t)k * H + j) * W + i0;
+            const f4w rv = *(const f4w*)(r + t);
+            const f4w ev = *(const f4w*)(e + t);
+            float l[4];
+            lap4(e, t, i0, j, k, W, H, D, ev, l);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                sr += rv[u];
+                sa += l[u];
+                v[0] += rv[u] * ev[u];
+                v[1] += ev[u] * l[u];
+                v[2] += rv[u];
+                v[3] += ev[u];
+            }
+        }
+    }
+    const int gl = F >> 2; // lanes per network cell
+    double cr = (double)sr, ca = (double)sa;
+    for (int o = 1; o < gl; o <<= 1) {
+        cr += __shfl_xor(cr, o, 64);
+        ca += __shfl_xor(ca, o, 64);
+    }
+    double d[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        d[q] = (double)v[q];
+        for (int o = 32; o >= 1; o >>= 1) d[q] += __shfl_xor(d[q], o, 64);
+    }
+    __shared__ double sc[RW][64][2];
+    __shared__ double sd[RW][4];
+    const int I = lane / gl;
+    if ((lane & (gl - 1)) == 0) {
+        sc[w][I][0] = cr;
+        sc[w][I][1] = ca;
+    }
+    if (lane < 4) sd[w][lane] = d[lane];
+    __syncthreads();
+    if (threadIdx.x < Wn) {
+        const int c = threadIdx.x;
+        double tr = 0.0, ta = 0.0;
+        if (real_row && c < Wc)
+            for (int u = 0; u < RW; ++u) {
+                tr += sc[u][c][0];
+                ta += sc[u][c][1];
+            }
+        S[2 * (row0 + c)] = tr;
+        S[2 * (row0 + c) + 1] = ta;
+    }
+    if (threadIdx.x < 4) {
+        double a = 0.0;
+        for (int u = 0; u < RW; ++u) a += sd[u][threadIdx.x];
+        partial[4 * bx + threadIdx.x] = a;
+    }
+}
+
+__global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* __restrict__ r,
+                                                                   const float* __restrict__ e, int W, int H, int D,
+                                                                   int F, int Wc, int Hc, int Dc, int Wn, int Hn,
+                                                                   double* __restrict__ S,
+                                                                   double* __restrict__ partial,
+                                                                   const int* __restrict__ stop) {
+    ws_restrict_dots_rows_body(r, e, W, H, D, F, Wc, Hc, Dc, Wn, Hn, S, partial, stop, blockIdx.x);
+}
+
+template <int P>
+__device__ __forceinline__ void ws_restrict_dots_rows_p(const float* __restrict__ r, const float* __restrict__ e,
+                                                        int W, int H, int D, int F, int Wc, int Hc, int Dc, int Wn,
+                                                        int Hn, double* __restrict__ S, double* __restrict__ partial,
+                                                        const int* __restrict__ stop, unsigned nvb, unsigned bx) {
+    if (*stop) return;
+    constexpr int lanes = 64 / P;
     const int lane = (threadIdx.x & 63) % lanes, u = (int)(threadIdx.x >> 6) * P + (threadIdx.x & 63) / lanes;
     const int vb = u / RW, w = u % RW; // block of the workgroup, its wave
     const unsigned blk = bx * (unsigned)P + (unsigned)vb;
@@ -358,7 +441,7 @@ __device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restri
 #pragma unroll
     for (int q = 0; q < 4; ++q) d[q] = vwave_sum((double)v[q], P);
     __shared__ double sc[RW][64][2];
-    __shared__ double sd[4][RW][4];
+    __shared__ double sd[P][RW][4];
     const int I = lane / gl;
     if ((lane & (gl - 1)) == 0) {
         sc[w][vb * lanes + I][0] = cr;
@@ -389,14 +472,15 @@ __device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restri
     }
 }
 
-__global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows(const float* __restrict__ r,
-                                                                   const float* __restrict__ e, int W, int H, int D,
-                                                                   int F, int Wc, int Hc, int Dc, int Wn, int Hn,
-                                                                   double* __restrict__ S,
-                                                                   double* __restrict__ partial,
-                                                                   const int* __restrict__ stop, int P,
-                                                                   unsigned nvb) {
-    ws_restrict_dots_rows_body(r, e, W, H, D, F, Wc, Hc, Dc, Wn, Hn, S, partial, stop, P, nvb, blockIdx.x);
+template <int P>
+__global__ void __launch_bounds__(64 * RW) k_ws_restrict_dots_rows_packed(const float* __restrict__ r,
+                                                                          const float* __restrict__ e, int W, int H,
+                                                                          int D, int F, int Wc, int Hc, int Dc,
+                                                                          int Wn, int Hn, double* __restrict__ S,
+                                                                          double* __restrict__ partial,
+                                                                          const int* __restrict__ stop,
+                                                                          unsigned nvb) {
+    ws_restrict_dots_rows_p<P>(r, e, W, H, D, F, Wc, Hc, Dc, Wn, Hn, S, partial, stop, nvb, blockIdx.x);
 }
 
 // alpha_G from the row partials of k_ws_restrict_dots_rows (as k_ws_alpha:
@@ -771,11 +855,65 @@ __device__ __forceinline__ void ws_blockdots_rows_body(const float* __restrict__
                                                        const double* __restrict__ alpha1, const float* __restrict__ e2,
                                                        int W, int H, int D, int F, int Gw, int Gh,
                                                        double* __restrict__ partial, double* __restrict__ sg,
-                                                       const SolveCtl* __restrict__ c, int P, unsigned nvb,
-                                                       unsigned bx) {
+                                                       const SolveCtl* __restrict__ c, unsigned bx) {
+    if (c->done) return;
+    const int J = bx % Gh, K = bx / Gh;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 4 * lane;
+    const float a1 = e1 ? (float)*alpha1 : 0.0f;
+    float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const int y0 = F * J, z0 = F * K;
+    const int ny = min(F, H - y0), nz = min(F, D - z0);
+    if (i0 < W)
+        // unrolled so that the next rows' loads issue before this row's sums
+#pragma unroll 4
+        for (int q = w; q < ny * nz; q += RW) {
+            const int j = y0 + q % ny, k = z0 + q / ny;
+            blockdots_cell4<E1, E2, SYM>(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
+        }
+    const int gl = F >> 2; // lanes per block
+    double d[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        d[q] = (double)v[q];
+        for (int o = 1; o < gl; o <<= 1) d[q] += __shfl_xor(d[q], o, 64);
+    }
+    __shared__ double sv[RW][16][5];
+    const int I = lane / gl;
+    if ((lane & (gl - 1)) == 0 && I < Gw)
+        for (int q = 0; q < 5; ++q) sv[w][I][q] = d[q];
+    __syncthreads();
+    if (threadIdx.x < Gw * 5) {
+        const int bi = threadIdx.x / 5, q = threadIdx.x % 5;
+        double a = 0.0;
+        for (int u = 0; u < RW; ++u) a += sv[u][bi][q];
+        const int b = (K * Gh + J) * Gw + bi;
+        if (q < 4) partial[4 * b + q] = a;
+        if (sg && q == 2) sg[2 * b] = a;
+        if (sg && q == 4) sg[2 * b + 1] = a;
+    }
+}
+
+template <bool E1, bool E2, bool SYM = false>
+__global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __restrict__ r,
+                                                               const float* __restrict__ e1,
+                                                               const double* __restrict__ alpha1,
+                                                               const float* __restrict__ e2, int W, int H, int D,
+                                                               int F, int Gw, int Gh, double* __restrict__ partial,
+                                                               double* __restrict__ sg,
+                                                               const SolveCtl* __restrict__ c) {
+    ws_blockdots_rows_body<E1, E2, SYM>(r, e1, alpha1, e2, W, H, D, F, Gw, Gh, partial, sg, c, blockIdx.x);
+}
+
+template <bool E1, bool E2, bool SYM, int P>
+__device__ __forceinline__ void ws_blockdots_rows_p(const float* __restrict__ r, const float* __restrict__ e1,
+                                                    const double* __restrict__ alpha1, const float* __restrict__ e2,
+                                                    int W, int H, int D, int F, int Gw, int Gh,
+                                                    double* __restrict__ partial, double* __restrict__ sg,
+                                                    const SolveCtl* __restrict__ c, unsigned nvb, unsigned bx) {
     if (c->done) return;
     // P blocks per workgroup in packed virtual waves (rows_pack)
-    const int lanes = 64 / P;
+    constexpr int lanes = 64 / P;
     const int lane = (threadIdx.x & 63) % lanes, u = (int)(threadIdx.x >> 6) * P + (threadIdx.x & 63) / lanes;
     const int vb = u / RW, w = u % RW;
     const unsigned blk = bx * (unsigned)P + (unsigned)vb;
@@ -800,7 +938,7 @@ __device__ __forceinline__ void ws_blockdots_rows_body(const float* __restrict__
         d[q] = (double)v[q];
         for (int o = 1; o < gl; o <<= 1) d[q] += __shfl_xor(d[q], o, 64);
     }
-    __shared__ double sv[RW][64][5]; // slot 16 vb + I
+    __shared__ double sv[RW][16 * P][5]; // slot 16 vb + I
     const int I = lane / gl;
     if ((lane & (gl - 1)) == 0 && I < Gw)
         for (int q = 0; q < 5; ++q) sv[w][16 * vb + I][q] = d[q];
@@ -820,15 +958,16 @@ __device__ __forceinline__ void ws_blockdots_rows_body(const float* __restrict__
     }
 }
 
-template <bool E1, bool E2, bool SYM = false>
-__global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows(const float* __restrict__ r,
-                                                               const float* __restrict__ e1,
-                                                               const double* __restrict__ alpha1,
-                                                               const float* __restrict__ e2, int W, int H, int D,
-                                                               int F, int Gw, int Gh, double* __restrict__ partial,
-                                                               double* __restrict__ sg,
-                                                               const SolveCtl* __restrict__ c, int P, unsigned nvb) {
-    ws_blockdots_rows_body<E1, E2, SYM>(r, e1, alpha1, e2, W, H, D, F, Gw, Gh, partial, sg, c, P, nvb, blockIdx.x);
+template <bool E1, bool E2, bool SYM, int P>
+__global__ void __launch_bounds__(64 * RW) k_ws_blockdots_rows_packed(const float* __restrict__ r,
+                                                                      const float* __restrict__ e1,
+                                                                      const double* __restrict__ alpha1,
+                                                                      const float* __restrict__ e2, int W, int H,
+                                                                      int D, int F, int Gw, int Gh,
+                                                                      double* __restrict__ partial,
+                                                                      double* __restrict__ sg,
+                                                                      const SolveCtl* __restrict__ c, unsigned nvb) {
+    ws_blockdots_rows_p<E1, E2, SYM, P>(r, e1, alpha1, e2, W, H, D, F, Gw, Gh, partial, sg, c, nvb, blockIdx.x);
 }
 
 // alpha from the per-block partial sums (one workgroup, fixed order: every
@@ -1415,27 +1554,50 @@ inline int ilog2(int f) {
 // float4s, else a workgroup per block (float4 steps where rows split into
 // whole float4s)
 // sym: only the scale of e2 is wanted (see blockdots_cell4's SYM)
+template <int P>
+void launch_blockdots_packed(const Solver& s, const float* r, const float* e1, const double* a1, const float* e2,
+                             double* partial, double* sg, const SolveCtl* c, bool sym) {
+    const unsigned nvb = (unsigned)(s.Gh * s.Gd);
+    const dim3 g((nvb + P - 1) / P), b(64 * RW);
+    if (e1 && e2 && sym && !sg)
+        hipLaunchKernelGGL((k_ws_blockdots_rows_packed<true, true, true, P>), g, b, 0, s.stream, r, e1, a1, e2, s.W,
+                           s.H, s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c, nvb);
+    else if (e1 && e2)
+        hipLaunchKernelGGL((k_ws_blockdots_rows_packed<true, true, false, P>), g, b, 0, s.stream, r, e1, a1, e2, s.W,
+                           s.H, s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c, nvb);
+    else if (e1)
+        hipLaunchKernelGGL((k_ws_blockdots_rows_packed<true, false, false, P>), g, b, 0, s.stream, r, e1, a1, e2, s.W,
+                           s.H, s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c, nvb);
+    else if (e2)
+        hipLaunchKernelGGL((k_ws_blockdots_rows_packed<false, true, false, P>), g, b, 0, s.stream, r, e1, a1, e2, s.W,
+                           s.H, s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c, nvb);
+    else
+        hipLaunchKernelGGL((k_ws_blockdots_rows_packed<false, false, false, P>), g, b, 0, s.stream, r, e1, a1, e2,
+                           s.W, s.H, s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c, nvb);
+}
+
 void launch_blockdots(const Solver& s, const float* r, const float* e1, const double* a1, const float* e2,
                       double* partial, double* sg, const SolveCtl* c, bool sym = false) {
     if (s.W % 4 == 0 && s.W <= 256 && s.ws_g >= 4 && s.Gw <= 16) {
         const int P = rows_pack(s.W);
-        const unsigned nvb = (unsigned)(s.Gh * s.Gd);
-        const dim3 g((nvb + P - 1) / P), b(64 * RW);
+        if (P == 2) return launch_blockdots_packed<2>(s, r, e1, a1, e2, partial, sg, c, sym);
+        if (P == 4) return launch_blockdots_packed<4>(s, r, e1, a1, e2, partial, sg, c, sym);
+        const dim3 g((unsigned)(s.Gh * s.Gd)), b(64 * RW);
         if (e1 && e2 && sym && !sg)
             hipLaunchKernelGGL((k_ws_blockdots_rows<true, true, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H,
-                               s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
+                               s.D, s.ws_g, s.Gw, s.Gh, partial, sg, c);
         else if (e1 && e2)
             hipLaunchKernelGGL((k_ws_blockdots_rows<true, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
         else if (e1)
             hipLaunchKernelGGL((k_ws_blockdots_rows<true, false>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
         else if (e2)
             hipLaunchKernelGGL((k_ws_blockdots_rows<false, true>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
         else
             hipLaunchKernelGGL((k_ws_blockdots_rows<false, false>), g, b, 0, s.stream, r, e1, a1, e2, s.W, s.H, s.D,
-                               s.ws_g, s.Gw, s.Gh, partial, sg, c, P, nvb);
+                               s.ws_g, s.Gw, s.Gh, partial, sg, c);
         return;
     }
     const unsigned nbg = (unsigned)(s.Gw * s.Gh * s.Gd);
@@ -1506,8 +1668,16 @@ __global__ void __launch_bounds__(BS) k_wsm_resid(const WsJob* __restrict__ tab,
 }
 __global__ void __launch_bounds__(64 * RW) k_wsm_sums(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_SUMS);
-    ws_blockdots_rows_body<false, false, false>(j.r, nullptr, nullptr, nullptr, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
-                                                j.part, j.sg, j.c, j.P, (unsigned)(j.Gh * j.Gd), bx);
+    const unsigned nvb = (unsigned)(j.Gh * j.Gd);
+    if (j.P == 4)
+        ws_blockdots_rows_p<false, false, false, 4>(j.r, nullptr, nullptr, nullptr, j.W, j.H, j.D, j.ws_g, j.Gw,
+                                                    j.Gh, j.part, j.sg, j.c, nvb, bx);
+    else if (j.P == 2)
+        ws_blockdots_rows_p<false, false, false, 2>(j.r, nullptr, nullptr, nullptr, j.W, j.H, j.D, j.ws_g, j.Gw,
+                                                    j.Gh, j.part, j.sg, j.c, nvb, bx);
+    else
+        ws_blockdots_rows_body<false, false, false>(j.r, nullptr, nullptr, nullptr, j.W, j.H, j.D, j.ws_g, j.Gw,
+                                                    j.Gh, j.part, j.sg, j.c, bx);
 }
 __global__ void __launch_bounds__(GT) k_wsm_gdct(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_GDCT);
@@ -1519,8 +1689,16 @@ __global__ void __launch_bounds__(BS) k_wsm_prolong_g(const WsJob* __restrict__ 
 }
 __global__ void __launch_bounds__(64 * RW) k_wsm_restrict(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_RESTRICT);
-    ws_restrict_dots_rows_body(j.r, j.eg, j.W, j.H, j.D, j.ws_f, j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.S, j.part,
-                               &j.c->done, j.P, (unsigned)(j.nn / j.Wn), bx);
+    const unsigned nvb = (unsigned)(j.nn / j.Wn);
+    if (j.P == 4)
+        ws_restrict_dots_rows_p<4>(j.r, j.eg, j.W, j.H, j.D, j.ws_f, j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.S, j.part,
+                                   &j.c->done, nvb, bx);
+    else if (j.P == 2)
+        ws_restrict_dots_rows_p<2>(j.r, j.eg, j.W, j.H, j.D, j.ws_f, j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.S, j.part,
+                                   &j.c->done, nvb, bx);
+    else
+        ws_restrict_dots_rows_body(j.r, j.eg, j.W, j.H, j.D, j.ws_f, j.Wc, j.Hc, j.Dc, j.Wn, j.Hn, j.S, j.part,
+                                   &j.c->done, bx);
 }
 __global__ void __launch_bounds__(1024) k_wsm_alpha_res(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_ALPHA_RES);
@@ -1537,8 +1715,16 @@ __global__ void __launch_bounds__(BS) k_wsm_prolong_n(const WsJob* __restrict__ 
 }
 __global__ void __launch_bounds__(64 * RW) k_wsm_dots(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_DOTS);
-    ws_blockdots_rows_body<true, true, true>(j.r, j.eg, j.alpha + 1, j.enet, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
-                                             j.part, nullptr, j.c, j.P, (unsigned)(j.Gh * j.Gd), bx);
+    const unsigned nvb = (unsigned)(j.Gh * j.Gd);
+    if (j.P == 4)
+        ws_blockdots_rows_p<true, true, true, 4>(j.r, j.eg, j.alpha + 1, j.enet, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
+                                                 j.part, nullptr, j.c, nvb, bx);
+    else if (j.P == 2)
+        ws_blockdots_rows_p<true, true, true, 2>(j.r, j.eg, j.alpha + 1, j.enet, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
+                                                 j.part, nullptr, j.c, nvb, bx);
+    else
+        ws_blockdots_rows_body<true, true, true>(j.r, j.eg, j.alpha + 1, j.enet, j.W, j.H, j.D, j.ws_g, j.Gw, j.Gh,
+                                                 j.part, nullptr, j.c, bx);
 }
 __global__ void __launch_bounds__(BS) k_wsm_alpha(const WsJob* __restrict__ tab, int n) {
     WS_JOB(WK_ALPHA);
@@ -1719,9 +1905,17 @@ static int ws_global_and_input(Solver& s, float dt, SolveCtl* c, SolveCtl* begin
         double* S = (double*)(s.tmp + 3 * N);
         const int P = rows_pack(s.W);
         const unsigned nvb = (unsigned)(nn / s.Wn);
-        hipLaunchKernelGGL(k_ws_restrict_dots_rows, dim3((nvb + P - 1) / P), dim3(64 * RW), 0, s.stream, ws_r(s),
-                           ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s), &c->done, P,
-                           nvb);
+        if (P == 1)
+            hipLaunchKernelGGL(k_ws_restrict_dots_rows, dim3(nvb), dim3(64 * RW), 0, s.stream, ws_r(s), ws_eg(s), s.W,
+                               s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s), &c->done);
+        else if (P == 2)
+            hipLaunchKernelGGL(k_ws_restrict_dots_rows_packed<2>, dim3((nvb + 1) / 2), dim3(64 * RW), 0, s.stream,
+                               ws_r(s), ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s),
+                               &c->done, nvb);
+        else
+            hipLaunchKernelGGL(k_ws_restrict_dots_rows_packed<4>, dim3((nvb + 3) / 4), dim3(64 * RW), 0, s.stream,
+                               ws_r(s), ws_eg(s), s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, S, ws_part(s),
+                               &c->done, nvb);
         hipLaunchKernelGGL(k_ws_alpha_res, dim3(1), dim3(1024), 0, s.stream, ws_part(s), (int)(nn / s.Wn), (double)N,
                            a_g, S, res, mask, amax, s.W, s.H, s.D, s.ws_f, s.Wc, s.Hc, s.Dc, s.Wn, s.Hn, (int)nn,
                            &c->done);
