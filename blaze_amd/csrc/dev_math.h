@@ -36,6 +36,16 @@ __device__ __forceinline__ float ld(const Gv& g, int i, int j, int k) {
     return g.a[i + j * g.nx + k * (g.nx * g.ny)];
 }
 
+// A pointer known to address global (device) memory, typed so: loads and
+// stores through it are global instructions rather than flat ones (a pointer
+// read from memory, or picked from an array by a computed index, is
+// otherwise generic to the compiler). The address is the same.
+#define BLZ_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ BLZ_GLOBAL T* as_global(T* p) {
+    return (BLZ_GLOBAL T*)p;
+}
+
 // grid -> object position of cell (i,j,k): (i + o) * dx  (src/Grid.h:270-290)
 __device__ __forceinline__ f3 g2o(const Gv& g, float dx, int i, int j, int k) {
     return mk3(((float)i + g.ox) * dx, ((float)j + g.oy) * dx, ((float)k + g.oz) * dx);

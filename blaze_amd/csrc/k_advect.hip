@@ -44,7 +44,7 @@ __device__ __forceinline__ void advect_cell(const Gv& U, const Gv& V, const Gv& 
     if (t >= n) return;
     const int i = t % G.nx, j = (t / G.nx) % G.ny, k = t / (G.nx * G.ny);
     const f3 pos = backtrace(U, V, W, G, sc, dx, i, j, k, a, b, c1, c2, c3, dt);
-    out[t] = tricubic_s<CLAMP>(G, sc, pos);
+    as_global(out)[t] = tricubic_s<CLAMP>(G, sc, pos);
 }
 
 template <bool CLAMP, class S>
@@ -65,8 +65,8 @@ __device__ __forceinline__ void advect_pair_cell(const Gv& U, const Gv& V, const
     if (t >= n) return;
     const int i = t % G1.nx, j = (t / G1.nx) % G1.ny, k = t / (G1.nx * G1.ny);
     const f3 pos = backtrace(U, V, W, G1, sc, dx, i, j, k, a, b, c1, c2, c3, dt);
-    out1[t] = tricubic_s<true>(G1, sc, pos);
-    out2[t] = tricubic_s<true>(G2, sc, pos);
+    as_global(out1)[t] = tricubic_s<true>(G1, sc, pos);
+    as_global(out2)[t] = tricubic_s<true>(G2, sc, pos);
 }
 
 template <class S>

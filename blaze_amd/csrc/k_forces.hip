@@ -35,7 +35,7 @@ __device__ __forceinline__ void buoyancy_cell(const Gv& T, float* __restrict__ V
     const f3 pos = mk3(((float)i + vox) * dx, ((float)j + voy) * dx, ((float)k + voz) * dx);
     const float temp = trilinear_s(T, sc, pos);
     const float force = beta * temp * g;
-    V[t] += dt * force;
+    as_global(V)[t] += dt * force;
 }
 
 template <class S>

@@ -21,7 +21,7 @@ namespace {
 constexpr unsigned BS = 256;
 
 __device__ __forceinline__ float p_at(const RB& p, const RBL& L, int i, int j, int k) {
-    return p.c[(i + j) & 1][L.at(i, j, k)];
+    return as_global(((i + j) & 1) ? p.c[1] : p.c[0])[L.at(i, j, k)];
 }
 
 // compute_divergence_kernel :475-515, written into the split layout
@@ -33,9 +33,10 @@ __device__ __forceinline__ void divergence_cell(const float* __restrict__ U, con
     const int u0 = i + j * (W + 1) + k * (W + 1) * H;
     const int v0 = i + j * W + k * W * (H + 1);
     const int w0 = t;
-    const float u_lo = U[u0], u_hi = U[u0 + 1];
-    const float v_lo = V[v0], v_hi = V[v0 + W];
-    const float w_lo = Wf[w0], w_hi = Wf[w0 + W * H];
+    const BLZ_GLOBAL float *Ug = as_global(U), *Vg = as_global(V), *Wg = as_global(Wf);
+    const float u_lo = Ug[u0], u_hi = Ug[u0 + 1];
+    const float v_lo = Vg[v0], v_hi = Vg[v0 + W];
+    const float w_lo = Wg[w0], w_hi = Wg[w0 + W * H];
     float r = -sc * (u_hi - u_lo + v_hi - v_lo + w_hi - w_lo);
     // closedBounds: cbA = (xMin, xMax, yMin, yMax), cbB = (zMin, zMax)
     if (i == 0) {
@@ -53,7 +54,7 @@ __device__ __forceinline__ void divergence_cell(const float* __restrict__ U, con
     } else if (k == D - 1) {
         if (cbB.y) r += sc * w_hi;
     }
-    rhs.c[(i + j) & 1][L.at(i, j, k)] = r;
+    as_global(((i + j) & 1) ? rhs.c[1] : rhs.c[0])[L.at(i, j, k)] = r;
 }
 
 __global__ void __launch_bounds__(BS) k_divergence(const float* __restrict__ U, const float* __restrict__ V,
@@ -973,11 +974,11 @@ __device__ __forceinline__ void residual_block(const RB& p, const RB& rhs, const
     double mx = 0.0;
     for (int t = blk * BS + threadIdx.x; t < total; t += nblk * BS) {
         const int m = t % Wh, j = (t / Wh) % H, seg = t / (Wh * H);
-        const int c0 = j & 1, c1 = c0 ^ 1;   // colours of i0 = 2m and i1 = 2m+1
-        const f4* P0 = (const f4*)p.c[c0];
-        const f4* P1 = (const f4*)p.c[c1];
-        const f4* R0 = (const f4*)rhs.c[c0];
-        const f4* R1 = (const f4*)rhs.c[c1];
+        const bool odd = j & 1; // colours of i0 = 2m and i1 = 2m+1: (odd, !odd)
+        const BLZ_GLOBAL f4* P0 = as_global((const f4*)(odd ? p.c[1] : p.c[0]));
+        const BLZ_GLOBAL f4* P1 = as_global((const f4*)(odd ? p.c[0] : p.c[1]));
+        const BLZ_GLOBAL f4* R0 = as_global((const f4*)(odd ? rhs.c[1] : rhs.c[0]));
+        const BLZ_GLOBAL f4* R1 = as_global((const f4*)(odd ? rhs.c[0] : rhs.c[1]));
         const int e0 = (j + 1) * L.Wp + (m + 1) + gs; // element of group 0
         const int i0 = 2 * m, i1 = 2 * m + 1;
         const double nxy0 = (double)((int)(i0 > 0) + 1 + (int)(j > 0) + (int)(j < H - 1));
@@ -1037,7 +1038,7 @@ __device__ __forceinline__ void residual_block(const RB& p, const RB& rhs, const
     if (threadIdx.x == 0) {
         double w = sm[0];
         for (int q = 1; q < (int)(BS / 64); ++q) w = fmax(w, sm[q]);
-        partial[blk] = w;
+        as_global(partial)[blk] = w;
     }
 }
 
@@ -1200,11 +1201,11 @@ __device__ __forceinline__ void pupdate_face(const float* __restrict__ in, float
     const int sF = AXIS == 0 ? 1 : (AXIS == 1 ? fx : fx * fy); // face stride along the axis
     int g, fi = i, fj = j, fk = k; // face whose updated value is taken
     if (f == 0) {
-        if (closedLo) { out[t] = 0.0f; return; }
+        if (closedLo) { as_global(out)[t] = 0.0f; return; }
         g = t + sF;
         if (AXIS == 0) fi = 1; else if (AXIS == 1) fj = 1; else fk = 1;
     } else if (f == nA) {
-        if (closedHi) { out[t] = 0.0f; return; }
+        if (closedHi) { as_global(out)[t] = 0.0f; return; }
         g = t - sF;
         if (AXIS == 0) fi = nA - 1; else if (AXIS == 1) fj = nA - 1; else fk = nA - 1;
     } else {
@@ -1214,7 +1215,7 @@ __device__ __forceinline__ void pupdate_face(const float* __restrict__ in, float
     const float p_hi = p_at(p, L, fi, fj, fk);
     const float p_lo = AXIS == 0 ? p_at(p, L, fi - 1, fj, fk)
                                  : (AXIS == 1 ? p_at(p, L, fi, fj - 1, fk) : p_at(p, L, fi, fj, fk - 1));
-    out[t] = in[g] - sc * (p_hi - p_lo);
+    as_global(out)[t] = as_global(in)[g] - sc * (p_hi - p_lo);
 }
 
 template <int AXIS>

@@ -10,6 +10,7 @@
 #include <random>
 #include <vector>
 
+#include "dev_math.h"
 #include "unet.h"
 
 namespace blz {
@@ -93,10 +94,12 @@ __device__ __forceinline__ long long up_child0(unsigned v, const Geo& g) {
 // The body of one workgroup (tile group bx along the voxels, output tile
 // blockIdx.y), shared by the single-grid kernel and the multi-view one.
 template <int CIN, int KS, bool UP>
-__device__ __forceinline__ void conv3_tile(const bf16* __restrict__ X, int Cx, int xoff, const bf16* __restrict__ Wt,
-                                           int Cout, bf16* __restrict__ Y, int Cy, int yoff,
-                                           const bf16* __restrict__ Mk, int Cm, int moff, int relu, const Geo& g,
-                                           long long nvox, bf16* __restrict__ U, int Cu, int uoff, unsigned bx) {
+__device__ __forceinline__ void conv3_tile(const BLZ_GLOBAL bf16* __restrict__ X, int Cx, int xoff,
+                                           const BLZ_GLOBAL bf16* __restrict__ Wt, int Cout,
+                                           BLZ_GLOBAL bf16* __restrict__ Y, int Cy, int yoff,
+                                           const BLZ_GLOBAL bf16* __restrict__ Mk, int Cm, int moff, int relu,
+                                           const Geo& g, long long nvox, BLZ_GLOBAL bf16* __restrict__ U, int Cu,
+                                           int uoff, unsigned bx) {
     constexpr int NCH = (27 * CIN + 31) / 32;
     constexpr int NPER = (NCH + KS - 1) / KS; // chunks per split
     const int c0 = blockIdx.y * 16;
@@ -120,7 +123,7 @@ __device__ __forceinline__ void conv3_tile(const bf16* __restrict__ X, int Cx, i
     const int HW = g.H * g.W;
     const int vai = (int)(va < nvox ? va : 0);
     const s8v zero = {0, 0, 0, 0, 0, 0, 0, 0};
-    const bf16* wrow = Wt + (size_t)(c0 + li) * CIN; // tap 0 of this lane's output channel
+    const BLZ_GLOBAL bf16* wrow = Wt + (size_t)(c0 + li) * CIN; // tap 0 of this lane's output channel
     // chunks in batches of CB: all loads of a batch are issued before its
     // first MFMA, so CB chunks' loads are in flight together
     constexpr int CB = 8;
@@ -139,8 +142,8 @@ __device__ __forceinline__ void conv3_tile(const bf16* __restrict__ X, int Cx, i
             const int zz = z + dz, yy = y + dy, xx = x + dx;
             const bool inb = real && vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
             const int vv = vai + dz * HW + dy * g.W + dx;
-            const s8v ld = *(const s8v*)(X + (inb ? vv : 0) * Cx + xoff + (real ? ci : 0));
-            const s8v wl = *(const s8v*)(wrow + tp * Cout * CIN + (real ? ci : 0));
+            const s8v ld = *(const BLZ_GLOBAL s8v*)(X + (inb ? vv : 0) * Cx + xoff + (real ? ci : 0));
+            const s8v wl = *(const BLZ_GLOBAL s8v*)(wrow + tp * Cout * CIN + (real ? ci : 0));
             av[u] = inb ? ld : zero;
             bv[u] = real ? wl : zero;
         }
@@ -195,7 +198,8 @@ __global__ void __launch_bounds__(256) k_conv3_k32(const bf16* __restrict__ X, i
                                                    int Cy, int yoff, const bf16* __restrict__ Mk, int Cm, int moff,
                                                    int relu, Geo g, long long nvox, bf16* __restrict__ U, int Cu,
                                                    int uoff) {
-    conv3_tile<CIN, KS, UP>(X, Cx, xoff, Wt, Cout, Y, Cy, yoff, Mk, Cm, moff, relu, g, nvox, U, Cu, uoff, blockIdx.x);
+    conv3_tile<CIN, KS, UP>(as_global(X), Cx, xoff, as_global(Wt), Cout, as_global(Y), Cy, yoff, as_global(Mk), Cm,
+                            moff, relu, g, nvox, as_global(U), Cu, uoff, blockIdx.x);
 }
 
 // One view's part of a multi-view launch: the forwards of several views of
@@ -225,7 +229,8 @@ __global__ void __launch_bounds__(256) k_conv3_multi(const MView* __restrict__ t
                                                      int uoff) {
     const MView m = tab[blockIdx.z];
     if (mview_idle(m)) return; // the whole workgroup: no wave waits at the split reduction
-    conv3_tile<CIN, KS, UP>(m.X, Cx, xoff, Wt, Cout, m.Y, Cy, yoff, nullptr, 0, 0, 1, m.g, m.n, m.U, Cu, uoff,
+    conv3_tile<CIN, KS, UP>(as_global(m.X), Cx, xoff, as_global(Wt), Cout, as_global(m.Y), Cy, yoff, nullptr, 0, 0, 1,
+                            m.g, m.n, as_global(m.U), Cu, uoff,
                             blockIdx.x);
 }
 
@@ -302,7 +307,8 @@ __device__ __forceinline__ long long parent_of(long long v, const Geo& fine, int
 }
 
 // 2x2x2 average pooling (coarse voxel per thread-channel)
-__device__ __forceinline__ void pool_one(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd,
+__device__ __forceinline__ void pool_one(const BLZ_GLOBAL bf16* __restrict__ S, int Cs, int soff,
+                                         BLZ_GLOBAL bf16* __restrict__ Dd, int Cd,
                                          int doff, int C, const Geo& fine, long long ncoarse, long long t) {
     if (t >= ncoarse * C) return;
     // ncoarse C < 2^31 (the pooled view's 32-bit offsets, checked where
@@ -323,14 +329,16 @@ __device__ __forceinline__ void pool_one(const bf16* __restrict__ S, int Cs, int
 
 __global__ void k_pool_fwd(const bf16* __restrict__ S, int Cs, int soff, bf16* __restrict__ Dd, int Cd, int doff,
                            int C, Geo fine, long long ncoarse) {
-    pool_one(S, Cs, soff, Dd, Cd, doff, C, fine, ncoarse, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    pool_one(as_global(S), Cs, soff, as_global(Dd), Cd, doff, C, fine, ncoarse,
+             (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // the pooling of every view in `tab` (X fine, Y coarse)
 __global__ void k_pool_multi(const MView* __restrict__ tab, int Cs, int soff, int Cd, int doff, int C) {
     const MView m = tab[blockIdx.z];
     if (mview_idle(m)) return;
-    pool_one(m.X, Cs, soff, m.Y, Cd, doff, C, m.g, m.n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    pool_one(as_global(m.X), Cs, soff, as_global(m.Y), Cd, doff, C, m.g, m.n,
+             (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // pooling backward added to the gradient of the pooled activation S, then
@@ -370,8 +378,8 @@ __global__ void k_up_bwd(const bf16* __restrict__ gU, int Cu, int uoff, bf16* __
 }
 
 // 1x1 head: out[v] = sum_c X[v][c] w[c] (FP32)
-__device__ __forceinline__ void head_one(const bf16* __restrict__ X, const float* __restrict__ w,
-                                         float* __restrict__ out, long long n, long long v) {
+__device__ __forceinline__ void head_one(const BLZ_GLOBAL bf16* __restrict__ X, const float* __restrict__ w,
+                                         BLZ_GLOBAL float* __restrict__ out, long long n, long long v) {
     if (v >= n) return;
     float s = 0.0f;
 #pragma unroll
@@ -381,14 +389,14 @@ __device__ __forceinline__ void head_one(const bf16* __restrict__ X, const float
 
 __global__ void k_head_fwd(const bf16* __restrict__ X, const float* __restrict__ w, float* __restrict__ out,
                            long long n) {
-    head_one(X, w, out, n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    head_one(as_global(X), w, as_global(out), n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // the head of every view in `tab`
 __global__ void k_head_multi(const MView* __restrict__ tab, const float* __restrict__ w) {
     const MView m = tab[blockIdx.z];
     if (mview_idle(m)) return;
-    head_one(m.X, w, m.out, m.n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    head_one(as_global(m.X), w, as_global(m.out), m.n, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // loss = sum (out - target)^2 / n; gout = 2 (out - target) / n

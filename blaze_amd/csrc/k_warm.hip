@@ -72,11 +72,12 @@ __device__ __forceinline__ void ws_resid_body(RB p, RB rhs, RBL L, int W, int H,
     const int t = bx * BS + threadIdx.x;
     if (t >= Wh * H * L.Dg) return;
     const int m = t % Wh, j = (t / Wh) % H, g = t / (Wh * H);
-    const int c0 = j & 1, c1 = c0 ^ 1;
-    const f4* P0 = (const f4*)p.c[c0];
-    const f4* P1 = (const f4*)p.c[c1];
-    const f4* R0 = (const f4*)rhs.c[c0];
-    const f4* R1 = (const f4*)rhs.c[c1];
+    const bool odd = j & 1;
+    const BLZ_GLOBAL f4* P0 = as_global((const f4*)(odd ? p.c[1] : p.c[0]));
+    const BLZ_GLOBAL f4* P1 = as_global((const f4*)(odd ? p.c[0] : p.c[1]));
+    const BLZ_GLOBAL f4* R0 = as_global((const f4*)(odd ? rhs.c[1] : rhs.c[0]));
+    const BLZ_GLOBAL f4* R1 = as_global((const f4*)(odd ? rhs.c[0] : rhs.c[1]));
+    BLZ_GLOBAL float* rg = as_global(r);
     const int gs = L.Wp * L.Hp;
     const int e = (j + 1) * L.Wp + (m + 1) + (g + 1) * gs;
     const f4 cur0 = P0[e], cur1 = P1[e], prv0 = P0[e - gs], prv1 = P1[e - gs], nxt0 = P0[e + gs], nxt1 = P1[e + gs];
@@ -107,8 +108,8 @@ __device__ __forceinline__ void ws_resid_body(RB p, RB rhs, RBL L, int W, int H,
         b += dn1;
         b += up1;
         const size_t o = ((size_t)k * H + j) * W + i0;
-        r[o] = a - (nxy0 + nz) * cur0[sl];
-        r[o + 1] = b - (nxy1 + nz) * cur1[sl];
+        rg[o] = a - (nxy0 + nz) * cur0[sl];
+        rg[o + 1] = b - (nxy1 + nz) * cur1[sl];
     }
 }
 
@@ -138,16 +139,16 @@ __device__ __forceinline__ float lap(const float* __restrict__ x, size_t t, int 
 
 // A x for the four cells (i0..i0+3, j, k), i0 a multiple of 4 and W a
 // multiple of 4: the same per-cell formula as lap(), from float4 rows
-__device__ __forceinline__ void lap4(const float* __restrict__ x, size_t t, int i0, int j, int k, int W, int H,
+__device__ __forceinline__ void lap4(const BLZ_GLOBAL float* __restrict__ x, size_t t, int i0, int j, int k, int W, int H,
                                      int D, const f4w& cx, float out[4]) {
     const size_t WH = (size_t)W * H;
     // unconditional loads (a missing neighbour reads the cell's own row,
     // whose value the conditions below never use), so that the loads of
     // consecutive rows can be in flight together
-    const f4w yn = *(const f4w*)(x + t - (j > 0 ? (size_t)W : 0));
-    const f4w ys = *(const f4w*)(x + t + (j < H - 1 ? (size_t)W : 0));
-    const f4w zd = *(const f4w*)(x + t - (k > 0 ? WH : 0));
-    const f4w zu = *(const f4w*)(x + t + (k < D - 1 ? WH : 0));
+    const f4w yn = *(const BLZ_GLOBAL f4w*)(x + t - (j > 0 ? (size_t)W : 0));
+    const f4w ys = *(const BLZ_GLOBAL f4w*)(x + t + (j < H - 1 ? (size_t)W : 0));
+    const f4w zd = *(const BLZ_GLOBAL f4w*)(x + t - (k > 0 ? WH : 0));
+    const f4w zu = *(const BLZ_GLOBAL f4w*)(x + t + (k < D - 1 ? WH : 0));
     const float xl = x[t - (i0 > 0 ? 1 : 0)];
     const float xr = x[t + (i0 + 4 < W ? 4 : 0)];
 #pragma unroll
@@ -244,9 +245,9 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
             const size_t t = ((size_t)k * H + j) * W + i0;
             f4w v = *(const f4w*)(r + t);
             if (e1) {
-                const f4w c1 = *(const f4w*)(e1 + t);
+                const f4w c1 = *(const BLZ_GLOBAL f4w*)(as_global(e1) + t);
                 float l1[4];
-                lap4(e1, t, i0, j, k, W, H, D, c1, l1);
+                lap4(as_global(e1), t, i0, j, k, W, H, D, c1, l1);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) v[u] -= a1 * l1[u];
             }
@@ -329,10 +330,10 @@ __device__ __forceinline__ void ws_restrict_dots_rows_body(const float* __restri
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
             const size_t t = ((size_t)k * H + j) * W + i0;
-            const f4w rv = *(const f4w*)(r + t);
-            const f4w ev = *(const f4w*)(e + t);
+            const f4w rv = *(const BLZ_GLOBAL f4w*)(as_global(r) + t);
+            const f4w ev = *(const BLZ_GLOBAL f4w*)(as_global(e) + t);
             float l[4];
-            lap4(e, t, i0, j, k, W, H, D, ev, l);
+            lap4(as_global(e), t, i0, j, k, W, H, D, ev, l);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 sr += rv[u];
@@ -416,10 +417,10 @@ __device__ __forceinline__ void ws_restrict_dots_rows_p(const float* __restrict_
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
             const size_t t = ((size_t)k * H + j) * W + i0;
-            const f4w rv = *(const f4w*)(r + t);
-            const f4w ev = *(const f4w*)(e + t);
+            const f4w rv = *(const BLZ_GLOBAL f4w*)(as_global(r) + t);
+            const f4w ev = *(const BLZ_GLOBAL f4w*)(as_global(e) + t);
             float l[4];
-            lap4(e, t, i0, j, k, W, H, D, ev, l);
+            lap4(as_global(e), t, i0, j, k, W, H, D, ev, l);
 #pragma unroll
             for (int uu = 0; uu < 4; ++uu) {
                 sr += rv[uu];
@@ -503,7 +504,7 @@ __device__ __forceinline__ void ws_alpha_res_body(const double* __restrict__ par
     __shared__ double sal;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q = threadIdx.x; q < np; q += 1024)
-        for (int k = 0; k < 4; ++k) v[k] += partial[4 * q + k];
+        for (int k = 0; k < 4; ++k) v[k] += as_global(partial)[4 * q + k];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
@@ -535,7 +536,7 @@ __device__ __forceinline__ void ws_alpha_res_body(const double* __restrict__ par
         for (int rb = r0; rb < nrows; rb += rpp * UB) {
             d2v sv2[UB];
 #pragma unroll
-            for (int u = 0; u < UB; ++u) sv2[u] = ((const d2v*)S)[min(rb + rpp * u, nrows - 1) * Wn + I];
+            for (int u = 0; u < UB; ++u) sv2[u] = ((const BLZ_GLOBAL d2v*)as_global(S))[min(rb + rpp * u, nrows - 1) * Wn + I];
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
                 const int rr = rb + rpp * u;
@@ -544,10 +545,10 @@ __device__ __forceinline__ void ws_alpha_res_body(const double* __restrict__ par
                 const int t = rr * Wn + I;
                 const bool real = I < Wc && J < Hc && K < Dc;
                 const float x = real ? (float)((sv2[u][0] - al * sv2[u][1]) * invF) : 0.0f;
-                res[t] = x;
+                as_global(res)[t] = x;
                 const bool wall =
                     I == 0 || J == 0 || K == 0 || F * (I + 1) >= W || F * (J + 1) >= H || F * (K + 1) >= D;
-                mask[t] = (wall || !real) ? 1.0f : 0.0f;
+                as_global(mask)[t] = (wall || !real) ? 1.0f : 0.0f;
                 sq += (double)x * (double)x;
             }
         }
@@ -677,10 +678,11 @@ __device__ __forceinline__ void ws_prolong4_body(float* __restrict__ e, int W, i
     float wy, wz;
     axis(j, Hc, y0, y1, wy);
     axis(k, Dc, z0, z1, wz);
-    const float* p00 = pred + ((size_t)z0 * Hn + y0) * Wn;
-    const float* p10 = pred + ((size_t)z0 * Hn + y1) * Wn;
-    const float* p01 = pred + ((size_t)z1 * Hn + y0) * Wn;
-    const float* p11 = pred + ((size_t)z1 * Hn + y1) * Wn;
+    const BLZ_GLOBAL float* pg = as_global(pred);
+    const BLZ_GLOBAL float* p00 = pg + ((size_t)z0 * Hn + y0) * Wn;
+    const BLZ_GLOBAL float* p10 = pg + ((size_t)z0 * Hn + y1) * Wn;
+    const BLZ_GLOBAL float* p01 = pg + ((size_t)z1 * Hn + y0) * Wn;
+    const BLZ_GLOBAL float* p11 = pg + ((size_t)z1 * Hn + y1) * Wn;
     // all 32 coarse values first (independent loads in flight together),
     // then the interpolation
     int xa[4], xb[4];
@@ -710,7 +712,7 @@ __device__ __forceinline__ void ws_prolong4_body(float* __restrict__ e, int W, i
         const float val = c0 * (1.0f - wz) + c1 * wz;
         out[u] = (val == val && fabsf(val) <= 3.4e38f) ? val : 0.0f;
     }
-    *(f4w*)(e + ((size_t)k * H + j) * W + 4 * q) = out;
+    *(BLZ_GLOBAL f4w*)(as_global(e) + ((size_t)k * H + j) * W + 4 * q) = out;
 }
 
 __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W, int H, int D,
@@ -740,15 +742,16 @@ __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W
 // neighbours' terms once each): one A product per cell instead of two
 // (the per-block partials then differ; the grid totals are the same).
 template <bool E1, bool E2, bool SYM = false>
-__device__ __forceinline__ void blockdots_cell4(const float* __restrict__ r, const float* __restrict__ e1, float a1,
-                                                const float* __restrict__ e2, size_t t, int i0, int j, int k, int W,
-                                                int H, int D, float (&v)[6]) {
+__device__ __forceinline__ void blockdots_cell4(const BLZ_GLOBAL float* __restrict__ r,
+                                                const BLZ_GLOBAL float* __restrict__ e1, float a1,
+                                                const BLZ_GLOBAL float* __restrict__ e2, size_t t, int i0, int j, int k,
+                                                int W, int H, int D, float (&v)[6]) {
     static_assert(!SYM || (E1 && E2), "SYM needs both directions");
-    const f4w r4 = *(const f4w*)(r + t);
+    const f4w r4 = *(const BLZ_GLOBAL f4w*)(r + t);
     float rr[4] = {r4[0], r4[1], r4[2], r4[3]};
     if constexpr (SYM) {
-        const f4w c1 = *(const f4w*)(e1 + t);
-        const f4w c2 = *(const f4w*)(e2 + t);
+        const f4w c1 = *(const BLZ_GLOBAL f4w*)(e1 + t);
+        const f4w c2 = *(const BLZ_GLOBAL f4w*)(e2 + t);
         float l2[4];
         lap4(e2, t, i0, j, k, W, H, D, c2, l2);
 #pragma unroll
@@ -762,7 +765,7 @@ __device__ __forceinline__ void blockdots_cell4(const float* __restrict__ r, con
         return;
     }
     if constexpr (E1) {
-        const f4w c1 = *(const f4w*)(e1 + t);
+        const f4w c1 = *(const BLZ_GLOBAL f4w*)(e1 + t);
         float l1[4];
         lap4(e1, t, i0, j, k, W, H, D, c1, l1);
 #pragma unroll
@@ -771,7 +774,7 @@ __device__ __forceinline__ void blockdots_cell4(const float* __restrict__ r, con
 #pragma unroll
     for (int u = 0; u < 4; ++u) v[2] += rr[u];
     if constexpr (E2) {
-        const f4w c2 = *(const f4w*)(e2 + t);
+        const f4w c2 = *(const BLZ_GLOBAL f4w*)(e2 + t);
         float l2[4];
         lap4(e2, t, i0, j, k, W, H, D, c2, l2);
 #pragma unroll
@@ -801,10 +804,10 @@ __global__ void __launch_bounds__(BS) k_ws_blockdots(const float* __restrict__ r
             const int i0 = F * I + (q & (F - 1)), j = F * J + ((q >> lf) & (F - 1)), k = F * K + (q >> (2 * lf));
             if (i0 >= W || j >= H || k >= D) continue;
             const size_t t = ((size_t)k * H + j) * W + i0;
-            if (e1 && e2) blockdots_cell4<true, true>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
-            else if (e1) blockdots_cell4<true, false>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
-            else if (e2) blockdots_cell4<false, true>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
-            else blockdots_cell4<false, false>(r, e1, a1, e2, t, i0, j, k, W, H, D, v);
+            if (e1 && e2) blockdots_cell4<true, true>(as_global(r), as_global(e1), a1, as_global(e2), t, i0, j, k, W, H, D, v);
+            else if (e1) blockdots_cell4<true, false>(as_global(r), as_global(e1), a1, as_global(e2), t, i0, j, k, W, H, D, v);
+            else if (e2) blockdots_cell4<false, true>(as_global(r), as_global(e1), a1, as_global(e2), t, i0, j, k, W, H, D, v);
+            else blockdots_cell4<false, false>(as_global(r), as_global(e1), a1, as_global(e2), t, i0, j, k, W, H, D, v);
         }
     } else {
         // F = 2^lf: the block's F^3 cells by shifts, cells past the far faces skipped
@@ -869,7 +872,7 @@ __device__ __forceinline__ void ws_blockdots_rows_body(const float* __restrict__
 #pragma unroll 4
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
-            blockdots_cell4<E1, E2, SYM>(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
+            blockdots_cell4<E1, E2, SYM>(as_global(r), as_global(e1), a1, as_global(e2), ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
         }
     const int gl = F >> 2; // lanes per block
     double d[5];
@@ -929,7 +932,7 @@ __device__ __forceinline__ void ws_blockdots_rows_p(const float* __restrict__ r,
 #pragma unroll 4
         for (int q = w; q < ny * nz; q += RW) {
             const int j = y0 + q % ny, k = z0 + q / ny;
-            blockdots_cell4<E1, E2, SYM>(r, e1, a1, e2, ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
+            blockdots_cell4<E1, E2, SYM>(as_global(r), as_global(e1), a1, as_global(e2), ((size_t)k * H + j) * W + i0, i0, j, k, W, H, D, v);
         }
     const int gl = F >> 2; // lanes per block
     double d[5];
@@ -1023,27 +1026,29 @@ __device__ __forceinline__ void ws_apply_body(RB p, RB bk, RBL L, int W, int H, 
     const int m = t % Wh, j = (t / Wh) % H, g = t / (Wh * H);
     const float a1 = e1 ? (float)*alpha1 : 0.0f, a2 = e2 ? (float)*alpha2 : 0.0f;
     const float m1 = e1 ? (float)alpha1[2] : 0.0f, m2 = e2 ? (float)alpha2[2] : 0.0f; // means of e1, e2
-    const int c0 = j & 1;
-    f4* P0 = (f4*)p.c[c0];
-    f4* P1 = (f4*)p.c[c0 ^ 1];
+    const bool odd = j & 1;
+    BLZ_GLOBAL f4* P0 = as_global((f4*)(odd ? p.c[1] : p.c[0]));
+    BLZ_GLOBAL f4* P1 = as_global((f4*)(odd ? p.c[0] : p.c[1]));
     const int e = (j + 1) * L.Wp + (m + 1) + (g + 1) * L.Wp * L.Hp;
     f4 x0 = P0[e], x1 = P1[e];
     if (bk.c[0]) {
-        ((f4*)bk.c[c0])[e] = x0;
-        ((f4*)bk.c[c0 ^ 1])[e] = x1;
+        as_global((f4*)(odd ? bk.c[1] : bk.c[0]))[e] = x0;
+        as_global((f4*)(odd ? bk.c[0] : bk.c[1]))[e] = x1;
     }
+    const BLZ_GLOBAL float* g1 = as_global(e1);
+    const BLZ_GLOBAL float* g2 = as_global(e2);
 #pragma unroll
     for (int sl = 0; sl < 4; ++sl) {
         const int k = 4 * g + sl;
         if (k >= D) break;
         const size_t o = ((size_t)k * H + j) * W + 2 * m;
         if (e1) {
-            x0[sl] = x0[sl] + a1 * (e1[o] - m1);
-            x1[sl] = x1[sl] + a1 * (e1[o + 1] - m1);
+            x0[sl] = x0[sl] + a1 * (g1[o] - m1);
+            x1[sl] = x1[sl] + a1 * (g1[o + 1] - m1);
         }
         if (e2) {
-            x0[sl] = x0[sl] + a2 * (e2[o] - m2);
-            x1[sl] = x1[sl] + a2 * (e2[o + 1] - m2);
+            x0[sl] = x0[sl] + a2 * (g2[o] - m2);
+            x1[sl] = x1[sl] + a2 * (g2[o + 1] - m2);
         }
     }
     P0[e] = x0;
