@@ -93,7 +93,11 @@ __device__ __forceinline__ long long up_child0(unsigned v, const Geo& g) {
 //
 // The body of one workgroup (tile group bx along the voxels, output tile
 // blockIdx.y), shared by the single-grid kernel and the multi-view one.
-template <int CIN, int KS, bool UP>
+// ONE: each wave takes a whole tile and runs the KS splits' MFMA chains
+// itself, each from zero over the same chunk range, then adds them in split
+// order: the same sums as KS waves and the LDS reduction, for launches with
+// tiles enough to fill the GPU without splitting them.
+template <int CIN, int KS, bool UP, bool ONE = false>
 __device__ __forceinline__ void conv3_tile(const BLZ_GLOBAL bf16* __restrict__ X, int Cx, int xoff,
                                            const BLZ_GLOBAL bf16* __restrict__ Wt, int Cout,
                                            BLZ_GLOBAL bf16* __restrict__ Y, int Cy, int yoff,
@@ -107,8 +111,8 @@ __device__ __forceinline__ void conv3_tile(const BLZ_GLOBAL bf16* __restrict__ X
     // wave index and split as scalars: the chunk-range tests below are then
     // scalar branches, so every MFMA runs with the whole wave active (an
     // MFMA does not honour a partial or empty EXEC mask)
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), split = wv % KS;
-    const long long v0 = ((long long)bx * (4 / KS) + wv / KS) * 16;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), split = ONE ? 0 : wv % KS;
+    const long long v0 = ONE ? ((long long)bx * 4 + wv) * 16 : ((long long)bx * (4 / KS) + wv / KS) * 16;
     // no early return: the waves of a workgroup meet at the split reduction
     // (a tile past the end computes zeros from clamped addresses and stores
     // nothing)
@@ -118,7 +122,6 @@ __device__ __forceinline__ void conv3_tile(const BLZ_GLOBAL bf16* __restrict__ X
     // voxel indices fit in 32 bits (checked by the launcher)
     vox_coords32((unsigned)(va < nvox ? va : nvox - 1), g, b, z, y, x);
     const bool vok = va < nvox;
-    f4m acc = f4m{0.f, 0.f, 0.f, 0.f};
     // element offsets fit in 32 bits (checked by the launcher)
     const int HW = g.H * g.W;
     const int vai = (int)(va < nvox ? va : 0);
@@ -127,35 +130,47 @@ __device__ __forceinline__ void conv3_tile(const BLZ_GLOBAL bf16* __restrict__ X
     // chunks in batches of CB: all loads of a batch are issued before its
     // first MFMA, so CB chunks' loads are in flight together
     constexpr int CB = 8;
+    constexpr int NS = ONE ? KS : 1; // splits this wave runs
+    f4m accs[NS];
 #pragma unroll
-    for (int cb = 0; cb < NPER; cb += CB) {
-        s8v av[CB], bv[CB];
+    for (int sw = 0; sw < NS; ++sw) {
+        const int sp = ONE ? sw : split;
+        f4m acc = f4m{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int u = 0; u < CB; ++u) {
-            const int c = split * NPER + cb + u;
-            if (cb + u >= NPER || c >= NCH) continue; // wave-uniform
-            const int k0 = 32 * c + 8 * kg;
-            const int tap = k0 / CIN, ci = k0 - tap * CIN;
-            const bool real = tap < 27;
-            const int tp = real ? tap : 26;
-            const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
-            const int zz = z + dz, yy = y + dy, xx = x + dx;
-            const bool inb = real && vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
-            const int vv = vai + dz * HW + dy * g.W + dx;
-            const s8v ld = *(const BLZ_GLOBAL s8v*)(X + (inb ? vv : 0) * Cx + xoff + (real ? ci : 0));
-            const s8v wl = *(const BLZ_GLOBAL s8v*)(wrow + tp * Cout * CIN + (real ? ci : 0));
-            av[u] = inb ? ld : zero;
-            bv[u] = real ? wl : zero;
+        for (int cb = 0; cb < NPER; cb += CB) {
+            s8v av[CB], bv[CB];
+#pragma unroll
+            for (int u = 0; u < CB; ++u) {
+                const int c = sp * NPER + cb + u;
+                if (cb + u >= NPER || c >= NCH) continue; // wave-uniform
+                const int k0 = 32 * c + 8 * kg;
+                const int tap = k0 / CIN, ci = k0 - tap * CIN;
+                const bool real = tap < 27;
+                const int tp = real ? tap : 26;
+                const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+                const int zz = z + dz, yy = y + dy, xx = x + dx;
+                const bool inb = real && vok && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H && xx >= 0 && xx < g.W;
+                const int vv = vai + dz * HW + dy * g.W + dx;
+                const s8v ld = *(const BLZ_GLOBAL s8v*)(X + (inb ? vv : 0) * Cx + xoff + (real ? ci : 0));
+                const s8v wl = *(const BLZ_GLOBAL s8v*)(wrow + tp * Cout * CIN + (real ? ci : 0));
+                av[u] = inb ? ld : zero;
+                bv[u] = real ? wl : zero;
+            }
+#pragma unroll
+            for (int u = 0; u < CB; ++u) {
+                const int c = sp * NPER + cb + u;
+                if (cb + u >= NPER || c >= NCH) continue;
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, av[u]),
+                                                               __builtin_bit_cast(bf8v, bv[u]), acc, 0, 0, 0);
+            }
         }
-#pragma unroll
-        for (int u = 0; u < CB; ++u) {
-            const int c = split * NPER + cb + u;
-            if (cb + u >= NPER || c >= NCH) continue;
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, av[u]), __builtin_bit_cast(bf8v, bv[u]),
-                                                           acc, 0, 0, 0);
-        }
+        accs[sw] = acc;
     }
-    if constexpr (KS > 1) {
+    f4m acc = accs[0];
+    if constexpr (ONE) {
+#pragma unroll
+        for (int q = 1; q < KS; ++q) acc += accs[q];
+    } else if constexpr (KS > 1) {
         __shared__ f4m part[4][64];
         if (split != 0) {
             // through VGPRs (v_accvgpr_read) rather than an LDS write straight
@@ -222,14 +237,16 @@ __device__ __forceinline__ bool mview_idle(const MView& m) {
 }
 
 // the forward convolution (leaky ReLU epilogue) of every view in `tab`; all
-// views of one launch share the reduction split KS
-template <int CIN, int KS, bool UP>
+// views of one launch share the reduction split KS (ONE: each wave runs a
+// whole tile's splits, for launches whose views give tiles enough)
+template <int CIN, int KS, bool UP, bool ONE>
 __global__ void __launch_bounds__(256) k_conv3_multi(const MView* __restrict__ tab, int Cx, int xoff,
                                                      const bf16* __restrict__ Wt, int Cout, int Cy, int yoff, int Cu,
                                                      int uoff) {
     const MView m = tab[blockIdx.z];
     if (mview_idle(m)) return; // the whole workgroup: no wave waits at the split reduction
-    conv3_tile<CIN, KS, UP>(as_global(m.X), Cx, xoff, as_global(Wt), Cout, as_global(m.Y), Cy, yoff, nullptr, 0, 0, 1,
+    conv3_tile<CIN, KS, UP, ONE>(as_global(m.X), Cx, xoff, as_global(Wt), Cout, as_global(m.Y), Cy, yoff, nullptr, 0,
+                                 0, 1,
                             m.g, m.n, as_global(m.U), Cu, uoff,
                             blockIdx.x);
 }
@@ -794,8 +811,17 @@ int NetMulti::build(const std::vector<Net*>& views, const std::vector<const int*
         // convolutions: one launch per reduction split in use
         for (int ks : {4, 2, 1}) {
             if (f0.kind != 0 && ks != 1) continue;
-            Launch L{f0.kind, f0.layer, ks, f0.C, f0.x, f0.y, f0.up, f0.kind == 0 ? v0.Wb + v0.off[f0.layer] : nullptr,
-                     (int)t.size(), 0, 0u};
+            // one wave per tile (the splits run in it) where the launch's
+            // views give at least 16384 tile waves, 64 per CU
+            long long tile_waves = 0;
+            if (f0.kind == 0)
+                for (int q = 0; q < nv; ++q) {
+                    const long long n = lv[(size_t)q * 4 + steps[(size_t)q * kFwdSteps + k].level].n();
+                    if (conv_split(n, cin, cout) == ks) tile_waves += (n + 15) / 16 * (cout / 16);
+                }
+            const bool one = tile_waves >= 16384;
+            Launch L{f0.kind, f0.layer, ks, f0.C, one ? 1 : 0, f0.x, f0.y, f0.up,
+                     f0.kind == 0 ? v0.Wb + v0.off[f0.layer] : nullptr, (int)t.size(), 0, 0u};
             for (int q = 0; q < nv; ++q) {
                 const FwdStep& f = steps[(size_t)q * kFwdSteps + k];
                 const NetDims& d = lv[(size_t)q * 4 + f.level];
@@ -814,7 +840,7 @@ int NetMulti::build(const std::vector<Net*>& views, const std::vector<const int*
                         set_error("network activations too large for 32-bit offsets");
                         return BLZ_E_INVALID;
                     }
-                    m.nblk = nblk((d.n() + 15) / 16, 4 / ks);
+                    m.nblk = nblk((d.n() + 15) / 16, one ? 4 : 4 / ks);
                 } else if (f.kind == 1) {
                     m.n = lv[(size_t)q * 4 + f.level + 1].n();
                     m.nblk = nblk(m.n * f.C, 256);
@@ -850,9 +876,15 @@ template <int CIN>
 static void conv_multi(const NetMulti::Launch& L, const MView* tb, hipStream_t st) {
     const int cout = kConv[L.layer].cout;
     const dim3 gr(L.blocks, (unsigned)(cout / 16), (unsigned)L.count);
+#define CONV_MULTI_ONE(KS, UP, ONE)                                                                                \
+    hipLaunchKernelGGL((k_conv3_multi<CIN, KS, UP, ONE>), gr, dim3(256), 0, st, tb, L.x.C, L.x.off, L.w, cout,     \
+                       L.y.C, L.y.off, L.up.C, L.up.off)
+    // (with KS = 1 the two modes are the same code)
 #define CONV_MULTI(KS, UP)                                                                                         \
-    hipLaunchKernelGGL((k_conv3_multi<CIN, KS, UP>), gr, dim3(256), 0, st, tb, L.x.C, L.x.off, L.w, cout, L.y.C,   \
-                       L.y.off, L.up.C, L.up.off)
+    do {                                                                                                           \
+        if (KS > 1 && L.one) CONV_MULTI_ONE(KS, UP, true);                                                         \
+        else CONV_MULTI_ONE(KS, UP, false);                                                                        \
+    } while (0)
     if (L.up.p) {
         // the decoder's upsampled layers take 32 or 64 input channels
         if constexpr (CIN == 32 || CIN == 64) {
@@ -868,6 +900,7 @@ static void conv_multi(const NetMulti::Launch& L, const MView* tb, hipStream_t s
         CONV_MULTI(1, false);
     }
 #undef CONV_MULTI
+#undef CONV_MULTI_ONE
 }
 
 int NetMulti::forward(hipStream_t st) const {

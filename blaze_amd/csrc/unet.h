@@ -104,6 +104,7 @@ struct NetMulti {
     struct Launch {
         int kind;          // 0 convolution, 1 pooling, 2 head
         int layer, ks, C;  // convolution layer and reduction split; pooled channels
+        int one;           // each wave runs a whole tile's splits (enough tiles in the launch)
         TV x, y, up;       // channel layouts (the pointers are per view, in the table)
         const bf16* w;     // convolution weights
         int first, count;  // views [first, first + count) of the table
