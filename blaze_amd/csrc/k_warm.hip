@@ -286,7 +286,9 @@ __global__ void __launch_bounds__(64 * RW) k_ws_restrict_rows(const float* __res
 // the same, and a full wave's butterfly over 64 lanes first adds the idle
 // lanes' +0 (x + 0, which the packed wave applies once) before the steps
 // inside 64 / P lanes. The LDS slots of the cells of a row: block b's cell
-// I at b (64 / P) + I, 64 in all as for one full block.
+// I at b (64 / P) + I, 64 in all as for one full block. P = 1 keeps the
+// one-block kernels; the packed ones are separate kernels for P = 2 and 4
+// (one kernel for all three ran the full-width case slower).
 __host__ __device__ __forceinline__ int rows_pack(int W) { return W <= 64 ? 4 : (W <= 128 ? 2 : 1); }
 
 // the 64-lane butterfly sum of a full wave, for a virtual wave of s = 64 / P
