@@ -77,6 +77,13 @@ def parse():
     ap.add_argument("--batched", action="store_true",
                     help="instead: 64 scenes of 32^3..128^3 in one captured-graph step (warm start on)")
     ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--batched-tol-cap", type=int, default=0,
+                    help="with --batched: also run the scenes with closed tops, warm start on and off, to each "
+                         "tolerance of --batched-tol under this sweep cap (0: skip)")
+    ap.add_argument("--batched-tol", default="0.05",
+                    help="comma-separated tolerances of those runs (1e-4 lies below the FP32 floor of "
+                         "||div u||_inf that these scenes' solves reach with closed tops, 1e-3 to 2e-2)")
+    ap.add_argument("--batched-tol-steps", type=int, default=2, help="timed steps of those runs")
     return ap.parse_args()
 
 
@@ -728,8 +735,9 @@ def batched(a):
 
     net = None if a.no_warm_start else WS.load_net()
 
-    def run(warm, cap, steps):
-        sets, parts = batched_scenes(a.scenes, warm_start=warm, cap=cap, warm_every=a.warm_every)
+    def run(warm, cap, steps, closed_top=False, tol=1e-4):
+        sets, parts = batched_scenes(a.scenes, warm_start=warm, cap=cap, warm_every=a.warm_every,
+                                     closed_top=closed_top, tol=tol)
         b = B.Batch(sets)
         if warm:
             b.set_warm_start(net)
@@ -747,8 +755,9 @@ def batched(a):
         el = time.perf_counter() - t0
         res = [b.scenes[q].stats().residual for q in range(a.scenes)]
         sweeps = sum(b.scenes[q].stats().rbgs_sweeps_executed for q in range(a.scenes))
-        log(f"[batched warm={warm} cap={cap}] {el * 1e3 / steps:.1f} ms per step, residual max {max(res):.3e} "
-            f"mean {sum(res) / len(res):.3e}, sweeps executed in the last step {sweeps}", a.quiet)
+        log(f"[batched warm={warm} cap={cap} closed_top={closed_top} tol={tol:g}] {el * 1e3 / steps:.1f} ms per step, "
+            f"residual max {max(res):.3e} mean {sum(res) / len(res):.3e}, sweeps executed in the last step "
+            f"{sweeps}", a.quiet)
         return b, v, el, res, sweeps
 
     b, v, elapsed, res, _ = run(not a.no_warm_start, 100, a.steps)
@@ -786,14 +795,34 @@ def batched(a):
                                 "mean_residual_ratio": out["plain_same_cap"]["mean_final_residual"]
                                 / max(1e-300, out["batch"]["mean_final_residual"])}
         del bb
+    if a.batched_tol_cap > 0 and not a.no_warm_start:
+        # equal tolerance: every wall closed (no residual floor |mean(rhs)|
+        # from an open top), the cap lifted, warm start on and off; where
+        # every scene reaches the tolerance, the time per step is the time
+        # to it
+        legs = []
+        for t in (float(x) for x in a.batched_tol.split(",")):
+            tol = {"tolerance": t, "cap": a.batched_tol_cap, "steps": a.batched_tol_steps}
+            for warm in (True, False):
+                bt, _, el, rr, sw = run(warm, a.batched_tol_cap, a.batched_tol_steps, closed_top=True, tol=t)
+                tol["warm" if warm else "plain"] = {
+                    "ms_per_step": el * 1e3 / a.batched_tol_steps,
+                    "scenes_at_tolerance": sum(1 for r in rr if r <= t),
+                    "max_final_residual": max(rr), "mean_final_residual": sum(rr) / len(rr),
+                    "sweeps_executed_last_step": sw, "graph_nodes": bt.stats().graph_nodes,
+                    "graphs_run_last_step": bt.stats().graphs_run}
+                del bt
+            tol["time_ratio_plain_over_warm"] = tol["plain"]["ms_per_step"] / tol["warm"]["ms_per_step"]
+            legs.append(tol)
+        out["at_tolerance_closed_top"] = legs
     print(json.dumps(out), flush=True)
 
 
-def batched_scenes(n=64, warm_start=True, cap=100, warm_every=1):
+def batched_scenes(n=64, warm_start=True, cap=100, warm_every=1, closed_top=False, tol=1e-4):
     """The batched workload: n scenes, grids N^3 with N uniform in [32, 128]
     rounded to multiples of 8 (seed 5), source radius U[0.03, 0.1], buoyancy
-    U[0.5, 2.0], solid walls with an open top, RBGS to 1e-4 with a cap of
-    `cap` sweeps, the warm start (when on) before
+    U[0.5, 2.0], solid walls with an open top (closed_top: closed too), RBGS
+    to `tol` with a cap of `cap` sweeps, the warm start (when on) before
     the sweeps and every `warm_every` sweeps (1: before them only); returns
     the settings and the particles of each scene."""
     import numpy as np
@@ -814,9 +843,9 @@ def batched_scenes(n=64, warm_start=True, cap=100, warm_every=1):
         s.density = 1.0
         s.gravity = -9.8
         s.buoyancy = float(rng.uniform(0.5, 2.0))
-        for ax, v in enumerate((1, 1, 1, 0, 1, 1)):
+        for ax, v in enumerate((1, 1, 1, int(closed_top), 1, 1)):
             s.closed_bounds[ax] = v
-        s.residual_tol = 1e-4
+        s.residual_tol = tol
         s.max_iter_solve = cap
         s.residual_check_every = 10
         s.cfl_number = 0.0

@@ -81,7 +81,7 @@ class TraceEntry(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [
-        ("scenes", C.c_int), ("variant", C.c_int), ("graph_nodes", C.c_int), ("pad_", C.c_int),
+        ("scenes", C.c_int), ("variant", C.c_int), ("graph_nodes", C.c_int), ("graphs_run", C.c_int),
         ("voxels", C.c_longlong), ("capture_ms", C.c_double), ("step_ms", C.c_double),
         ("tune_variant", C.c_int * BATCH_TUNE), ("tune_ms", C.c_double * BATCH_TUNE),
     ]

@@ -32,8 +32,18 @@ int residual_blocks(const Solver& s);
 struct Batch {
     std::vector<blz_solver*> sc;
     hipStream_t root = nullptr;
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    int next = 0; // graph of the next replay
+    // the graphs of one step, per parity of steps, launched in order. When
+    // the solve loop runs more than CHUNK_SWEEPS sweeps, its later batches
+    // are cut into graphs of CHUNK_SWEEPS sweeps each and the pressure
+    // update and advection go to a last graph of their own: once every
+    // scene's solve has ended, the step skips the remaining sweep graphs, as
+    // a single solve stops enqueuing sweeps (their launches would all return
+    // at entry)
+    std::vector<hipGraphExec_t> exec[2];
+    std::vector<hipGraph_t> cap_graphs; // graphs of the capture in progress
+    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+    int graphs_run = 0; // graphs launched by the last step
+    int next = 0; // parity of the next replay
     int field_cur[2][1024][F_N] = {};
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
@@ -109,6 +119,8 @@ struct Batch {
     int build_tables(int v);
     int build_judge_tables();
     int capture(int which);
+    int cut(hipStream_t st);
+    void drop_graphs();
     int enqueue_step(hipStream_t st);
     int joint_forward_and_finish(hipStream_t st);
     int enqueue_update_and_advect(hipStream_t st);
@@ -120,12 +132,16 @@ struct Batch {
 // hundreds digit
 static const int kVariants[NTUNE] = {102, 103, 202, 203, 402, 403, 1103, 1104, 1203, 1204};
 static int wpb_of(int v) { return (v / 100) % 10; }
+// sweeps per graph of a step's later sweeps (about 30 ms of sweeps of the
+// 64-scene benchmark batch)
+constexpr int CHUNK_SWEEPS = 160;
 
 Batch::~Batch() {
     if (root) (void)hipStreamSynchronize(root);
-    for (auto& e : exec)
-        if (e) (void)hipGraphExecDestroy(e);
+    drop_graphs();
     if (fork) (void)hipEventDestroy(fork);
+    for (hipEvent_t e : chunk_ev)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : join)
         if (e) (void)hipEventDestroy(e);
     if (off_host) (void)hipHostFree(off_host);
@@ -157,6 +173,7 @@ int Batch::init(const blz_settings* st, int n) {
     }
     BLZ_HIP(hipStreamCreateWithFlags(&root, hipStreamNonBlocking));
     BLZ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    for (hipEvent_t& e : chunk_ev) BLZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     BLZ_HIP(hipHostMalloc((void**)&off_host, 2 * n * sizeof(float), hipHostMallocDefault));
     if (hipMalloc(&off_dev, 2 * n * sizeof(float)) != hipSuccess || hipMalloc(&fs, 2 * n * sizeof(FusedScene)) != hipSuccess ||
         hipMalloc(&rs, n * sizeof(ResScene)) != hipSuccess || hipMalloc(&fsw, 2 * n * sizeof(FusedScene)) != hipSuccess ||
@@ -524,7 +541,16 @@ int Batch::enqueue_step(hipStream_t st) {
             if (it[q] < sc[q]->s.s.max_iter_solve) return true;
         return false;
     };
+    // a new graph every chunk_iters batches (each whole: its forked streams
+    // have joined the root stream again)
+    const int chunk_iters = std::max(1, CHUNK_SWEEPS / every);
+    int batches = 0, cuts = 0;
     while (pending()) {
+        if (batches > 0 && batches % chunk_iters == 0) {
+            if (int e = cut(st)) return e;
+            ++cuts;
+        }
+        ++batches;
         for (int l = 0; l < every / 2; ++l) {
             if (int e = launch_fused_multi(st, variant, fs, n, fs_blocks)) return e;
             if (int e = launch_fused_multi(st, variant, fs + n, n, fs_blocks)) return e;
@@ -594,6 +620,8 @@ int Batch::enqueue_step(hipStream_t st) {
             BLZ_HIP(hipStreamWaitEvent(st, join[q], 0));
         }
     }
+    if (cuts)
+        if (int e = cut(st)) return e;
     if (int e = enqueue_update_and_advect(st)) return e;
     BLZ_HIP(hipEventRecord(fork, st));
     for (int q = 0; q < n; ++q) {
@@ -649,41 +677,77 @@ int Batch::capture(int which) {
     }
     auto t0 = std::chrono::steady_clock::now();
     if (int e = build_judge_tables()) return e;
-    hipGraph_t graph = nullptr;
     // captured noise kernels read their offsets from device memory; outside
     // the graph (blz_step on a scene handle) they take the time argument
     for (int q = 0; q < n; ++q) sc[q]->s.dev_off = off_dev + 2 * q;
     cap_which = which;
+    cap_graphs.clear();
+    auto drop_captured = [&]() {
+        for (hipGraph_t g : cap_graphs) (void)hipGraphDestroy(g);
+        cap_graphs.clear();
+    };
     BLZ_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
     int e = enqueue_step(root);
+    hipGraph_t graph = nullptr;
     hipError_t ce = hipStreamEndCapture(root, &graph);
+    if (graph) cap_graphs.push_back(graph);
     for (int q = 0; q < n; ++q) sc[q]->s.dev_off = nullptr;
+    if (!e && ce != hipSuccess) {
+        set_error(std::string("batch capture: ") + hipGetErrorString(ce));
+        e = BLZ_E_HIP;
+    }
+    // the job tables the captured launches read
+    if (!e && hipMemcpy(jobs_dev[which], jobs_host[which].data(), jobs_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("batch job table upload failed");
+        e = BLZ_E_HIP;
+    }
+    for (hipGraphExec_t x : exec[which]) (void)hipGraphExecDestroy(x);
+    exec[which].clear();
+    size_t total = 0;
+    for (size_t k = 0; k < cap_graphs.size() && !e; ++k) {
+        size_t nn = 0;
+        hipGraphExec_t x = nullptr;
+        hipError_t ge = hipGraphGetNodes(cap_graphs[k], nullptr, &nn);
+        if (ge == hipSuccess) ge = hipGraphInstantiate(&x, cap_graphs[k], nullptr, nullptr, 0);
+        if (ge != hipSuccess) {
+            set_error(std::string("batch graph instantiation: ") + hipGetErrorString(ge));
+            e = BLZ_E_HIP;
+            break;
+        }
+        exec[which].push_back(x);
+        total += nn;
+    }
+    drop_captured();
     if (e) {
-        if (graph) (void)hipGraphDestroy(graph);
+        for (hipGraphExec_t x : exec[which]) (void)hipGraphExecDestroy(x);
+        exec[which].clear();
         return e;
     }
-    BLZ_HIP(ce);
-    // the job tables the captured launches read
-    if (hipMemcpy(jobs_dev[which], jobs_host[which].data(), jobs_bytes, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        set_error("batch job table upload failed");
-        return BLZ_E_HIP;
-    }
-    size_t nn = 0;
-    BLZ_HIP(hipGraphGetNodes(graph, nullptr, &nn));
-    nodes = (int)nn;
-    if (exec[which]) BLZ_HIP(hipGraphExecDestroy(exec[which]));
-    exec[which] = nullptr;
-    hipError_t ie = hipGraphInstantiate(&exec[which], graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    BLZ_HIP(ie);
+    nodes = (int)total;
     capture_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return BLZ_OK;
 }
 
+// End the capture in progress on `st` as one graph of the step and start
+// capturing the next
+int Batch::cut(hipStream_t st) {
+    hipGraph_t g = nullptr;
+    BLZ_HIP(hipStreamEndCapture(st, &g));
+    cap_graphs.push_back(g);
+    BLZ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    return BLZ_OK;
+}
+
+void Batch::drop_graphs() {
+    for (auto& v : exec) {
+        for (hipGraphExec_t x : v) (void)hipGraphExecDestroy(x);
+        v.clear();
+    }
+}
+
 int Batch::step(float time) {
     const int n = (int)sc.size();
-    if (!exec[0]) {
+    if (exec[0].empty() || exec[1].empty()) {
         // capture both parities from the current state; the enqueue swapped
         // the host pointers once per captured step
         if (int e = capture(0)) return e;
@@ -698,9 +762,37 @@ int Batch::step(float time) {
         BLZ_HIP(hipStreamSynchronize(s.stream)); // sourcing or field access before the step
         clear_step_stats(s.st);
     }
+    // a batched solve reports its end through its mapped flag with sequence
+    // number 0
+    for (int q = 0; q < n; ++q) *sc[q]->s.done_host = -1;
+    auto all_done = [&]() {
+        for (int q = 0; q < n; ++q)
+            if (*sc[q]->s.done_host != 0) return false;
+        return true;
+    };
     auto t0 = std::chrono::steady_clock::now();
-    BLZ_HIP(hipGraphLaunch(exec[next], root));
+    const std::vector<hipGraphExec_t>& g = exec[next];
+    const int m = (int)g.size();
+    BLZ_HIP(hipGraphLaunch(g[0], root));
+    int run = 1;
+    if (m > 1) {
+        // graphs 1 .. m-2 hold the later sweeps, graph m-1 the update and
+        // advection; one sweep graph stays queued ahead of the host's check
+        BLZ_HIP(hipEventRecord(chunk_ev[0], root));
+        for (int i = 1; i < m - 1; ++i) {
+            if (i >= 2) {
+                BLZ_HIP(hipEventSynchronize(chunk_ev[i & 1])); // the end of graph i - 2
+                if (all_done()) break;
+            }
+            BLZ_HIP(hipGraphLaunch(g[i], root));
+            BLZ_HIP(hipEventRecord(chunk_ev[i & 1], root));
+            ++run;
+        }
+        BLZ_HIP(hipGraphLaunch(g[m - 1], root));
+        ++run;
+    }
     BLZ_HIP(hipStreamSynchronize(root));
+    graphs_run = run;
     step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     next ^= 1;
     for (int q = 0; q < n; ++q) {
@@ -770,11 +862,7 @@ int Batch::autotune(int sweeps, int* best) {
     BLZ_HIP(hipStreamSynchronize(root));
     if (rc) return rc;
     if (int e = build_tables(bestv)) return e;
-    for (auto& x : exec)
-        if (x) {
-            BLZ_HIP(hipGraphExecDestroy(x));
-            x = nullptr;
-        }
+    drop_graphs();
     if (best) *best = bestv;
     return BLZ_OK;
 }
@@ -831,11 +919,7 @@ int blz_batch_set_warm_start(blz_batch* b, blz_net* net) {
         BLZ_HIP(hipStreamSynchronize(h->s.stream));
         if (int e = blz::warm_start_attach(h->s, net ? &net->n : nullptr)) return e;
     }
-    for (auto& x : b->b.exec)
-        if (x) {
-            BLZ_HIP(hipGraphExecDestroy(x));
-            x = nullptr;
-        }
+    b->b.drop_graphs();
     return BLZ_OK;
 }
 
@@ -860,11 +944,7 @@ int blz_batch_set_variant(blz_batch* b, int variant) {
     for (blz_solver* h : b->b.sc) BLZ_HIP(hipStreamSynchronize(h->s.stream));
     BLZ_HIP(hipStreamSynchronize(b->b.root));
     if (int e = b->b.build_tables(variant)) return e;
-    for (auto& x : b->b.exec)
-        if (x) {
-            BLZ_HIP(hipGraphExecDestroy(x));
-            x = nullptr;
-        }
+    b->b.drop_graphs();
     return BLZ_OK;
 }
 
@@ -885,6 +965,7 @@ int blz_batch_get_stats(blz_batch* b, blz_batch_stats* out) {
     out->scenes = (int)B.sc.size();
     out->variant = B.variant;
     out->graph_nodes = B.nodes;
+    out->graphs_run = B.graphs_run;
     out->capture_ms = B.capture_ms;
     out->step_ms = B.step_ms;
     long long vox = 0;

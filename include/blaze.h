@@ -357,7 +357,10 @@ int blz_warm_start_apply(blz_solver* h, float dt, const float* pred, double* bef
    scene per process, src/main.cu:72-117) ----
    n independent scenes of any grid sizes; one blz_batch_step advances every
    scene by one step as a single replay of a captured HIP graph, with the
-   pressure sweeps and residual checks of all scenes in single launches.
+   pressure sweeps and residual checks of all scenes in single launches
+   (with a sweep cap above 160, a step is a short sequence of graphs: the
+   later sweeps in graphs of 160, skipped once every scene's solve has
+   ended, then the pressure update and advection).
    Each scene computes what blz_step computes for it alone. Scenes need
    residual_tol > 0, cfl_number 0 and one even residual_check_every that
    divides their max_iter_solve. Sourcing (blz_add_source on the scene
@@ -367,8 +370,8 @@ typedef struct blz_batch blz_batch;
 typedef struct blz_batch_stats {
     int scenes;
     int variant;              /* fused sweep variant in use (blz_set_rbgs_variant numbering) */
-    int graph_nodes;          /* nodes of one captured step */
-    int pad_;
+    int graph_nodes;          /* nodes of one captured step (all its graphs) */
+    int graphs_run;           /* graphs the last blz_batch_step launched */
     long long voxels;         /* cells over all scenes */
     double capture_ms;        /* host time spent capturing and instantiating */
     double step_ms;           /* wall time of the last blz_batch_step replay */

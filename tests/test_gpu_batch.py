@@ -223,6 +223,80 @@ def test_batch_with_warm_start_and_handle_steps(variant):
         compare(f"handle step scene {i}", b.scenes[i].field(B.FIELD_U), singles[i].field(B.FIELD_U))
 
 
+def graphs_launched(batch_end, graphs=5, per_graph=16):
+    """Graphs a batch step launches when the last solve ends in batch
+    `batch_end` of the solve loop (0: at the check before the sweeps): graph
+    0 holds the batches up to per_graph, each later sweep graph per_graph
+    more; the host launches graph 1 unchecked, then graph i only after
+    graph i - 2 ended with some solve still running, and the last graph."""
+    k = 0 if batch_end == 0 else (batch_end - 1) // per_graph
+    return min(k + 2, graphs - 1) + 1
+
+
+@pytest.mark.parametrize("warm", [False, True])
+def test_batch_chunked_graphs_equal_single_scenes(warm):
+    """A sweep cap of 600 (60 batches of 10 sweeps; with the learned
+    correction before the sweeps and every 20, a cap of 1200 for as many
+    batches) splits each captured step into graphs of 16 batches and a last
+    one with the update and advection. With a tight tolerance every solve
+    runs to the cap and the step launches all five graphs; with a loose one
+    the solves end early and the step skips the remaining sweep graphs.
+    Either way every scene equals the same scene stepped alone, bit for bit,
+    over both captured parities (warm: re-applications and their judges on
+    either side of the graph boundaries)."""
+    net = B.Net(seed=4) if warm else None
+    cap = 1200 if warm else 600
+    runs = {}
+    for tol in (1e-7, 2e-2):
+        print(f"[batch-chunks] warm={warm} tol={tol:g}", flush=True)
+        cases = [scene(r, 40 + i) for i, r in enumerate(SHAPES)]
+        for s, _ in cases:
+            s.max_iter_solve = cap
+            s.residual_tol = tol
+            s.closed_bounds[3] = 1
+            s.warm_start = 20 if warm else 0
+        b = B.Batch([c[0] for c in cases])
+        singles = [B.Solver(c[0]) for c in cases]
+        if warm:
+            b.set_warm_start(net)
+        for i, (s, part) in enumerate(cases):
+            if warm:
+                singles[i].set_warm_start(net)
+            b.scenes[i].add_source(part)
+            singles[i].add_source(part)
+        t = np.float32(0.0)
+        graphs, iters = [], []
+        for step in range(3):
+            b.step(float(t))
+            for g in singles:
+                g.step(float(t))
+            t = np.float32(t + np.float32(cases[0][0].dt))
+            graphs.append(b.stats().graphs_run)
+            iters.append([b.scenes[i].stats().rbgs_iters for i in range(len(cases))])
+            for i in range(len(cases)):
+                gs, ss = b.scenes[i].stats(), singles[i].stats()
+                assert gs.rbgs_iters == ss.rbgs_iters, (tol, step, i)
+                assert gs.residual == ss.residual, (tol, step, i)
+                assert gs.warm_start_fallbacks == ss.warm_start_fallbacks, (tol, step, i)
+        for i in range(len(cases)):
+            for w in (B.FIELD_T, B.FIELD_U, B.FIELD_V, B.FIELD_W, B.FIELD_P):
+                compare(f"batch-chunks tol {tol:g} scene {i} field {w}", b.scenes[i].field(w), singles[i].field(w))
+        print(f"[batch-chunks] graphs run per step {graphs}, sweeps per step {iters}", flush=True)
+        runs[tol] = (graphs, iters)
+    (g_tight, i_tight), (g_loose, i_loose) = runs[1e-7], runs[2e-2]
+    assert all(max(x) == cap for x in i_tight), i_tight
+    assert g_tight == [5, 5, 5], g_tight
+    for n, x in zip(g_loose, i_loose):
+        if warm:
+            # the judge's batch first, then 20 sweeps per batch of the loop
+            end = -(-(max(x) - 10) // 20)
+        else:
+            end = max(x) // 10
+        if not warm:
+            assert n == graphs_launched(end), (n, x)
+        assert n <= graphs_launched(end), (n, x)
+
+
 def test_batch_scenes_equal_the_oracle():
     """The multi-grid sweep and residual kernels against the CPU oracle
     directly: three uneven scenes (wind noise, vorticity, open top,
