@@ -680,11 +680,14 @@ __device__ __forceinline__ void ws_prolong4_body(float* __restrict__ e, int W, i
     float wy, wz;
     axis(j, Hc, y0, y1, wy);
     axis(k, Dc, z0, z1, wz);
-    const BLZ_GLOBAL float* pg = as_global(pred);
-    const BLZ_GLOBAL float* p00 = pg + ((size_t)z0 * Hn + y0) * Wn;
-    const BLZ_GLOBAL float* p10 = pg + ((size_t)z0 * Hn + y1) * Wn;
-    const BLZ_GLOBAL float* p01 = pg + ((size_t)z1 * Hn + y0) * Wn;
-    const BLZ_GLOBAL float* p11 = pg + ((size_t)z1 * Hn + y1) * Wn;
+    // the coarse grid is small: 32-bit byte offsets from its (uniform) base,
+    // so each load adds one offset instead of forming a 64-bit address
+    const BLZ_GLOBAL char* pg = (const BLZ_GLOBAL char*)as_global(pred);
+    auto ldc = [pg](unsigned row, int x) { return *(const BLZ_GLOBAL float*)(pg + (row + (unsigned)x) * 4u); };
+    const unsigned p00 = ((unsigned)z0 * (unsigned)Hn + (unsigned)y0) * (unsigned)Wn;
+    const unsigned p10 = ((unsigned)z0 * (unsigned)Hn + (unsigned)y1) * (unsigned)Wn;
+    const unsigned p01 = ((unsigned)z1 * (unsigned)Hn + (unsigned)y0) * (unsigned)Wn;
+    const unsigned p11 = ((unsigned)z1 * (unsigned)Hn + (unsigned)y1) * (unsigned)Wn;
     // all 32 coarse values first (independent loads in flight together),
     // then the interpolation
     int xa[4], xb[4];
@@ -693,14 +696,14 @@ __device__ __forceinline__ void ws_prolong4_body(float* __restrict__ e, int W, i
     for (int u = 0; u < 4; ++u) axis(4 * q + u, Wc, xa[u], xb[u], wxs[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        g[u][0] = p00[xa[u]];
-        g[u][1] = p00[xb[u]];
-        g[u][2] = p10[xa[u]];
-        g[u][3] = p10[xb[u]];
-        g[u][4] = p01[xa[u]];
-        g[u][5] = p01[xb[u]];
-        g[u][6] = p11[xa[u]];
-        g[u][7] = p11[xb[u]];
+        g[u][0] = ldc(p00, xa[u]);
+        g[u][1] = ldc(p00, xb[u]);
+        g[u][2] = ldc(p10, xa[u]);
+        g[u][3] = ldc(p10, xb[u]);
+        g[u][4] = ldc(p01, xa[u]);
+        g[u][5] = ldc(p01, xb[u]);
+        g[u][6] = ldc(p11, xa[u]);
+        g[u][7] = ldc(p11, xb[u]);
     }
     f4w out;
 #pragma unroll
@@ -714,7 +717,7 @@ __device__ __forceinline__ void ws_prolong4_body(float* __restrict__ e, int W, i
         const float val = c0 * (1.0f - wz) + c1 * wz;
         out[u] = (val == val && fabsf(val) <= 3.4e38f) ? val : 0.0f;
     }
-    *(BLZ_GLOBAL f4w*)(as_global(e) + ((size_t)k * H + j) * W + 4 * q) = out;
+    *(BLZ_GLOBAL f4w*)(as_global(e) + 4 * (size_t)t) = out; // cell (4q, j, k)
 }
 
 __global__ void __launch_bounds__(BS) k_ws_prolong4(float* __restrict__ e, int W, int H, int D,
@@ -1366,6 +1369,7 @@ __device__ __forceinline__ void ws_gdct_body(const double* __restrict__ sg, cons
     if (c->done) return;
     __shared__ float A[WS_GMAX], Bf[WS_GMAX];
     __shared__ float cs[3][16][16]; // cs[axis][a][i]
+    __shared__ float ct[3][16][16]; // ct[axis][i][a] = cs[axis][a][i]
     __shared__ float lam[3][16];
     __shared__ double red[GT / 64];
     const int n = Gw * Gh * Gd;
@@ -1373,11 +1377,14 @@ __device__ __forceinline__ void ws_gdct_body(const double* __restrict__ sg, cons
     for (int q = threadIdx.x; q < 3 * 256; q += GT) {
         const int ax = q >> 8, a = (q >> 4) & 15, i = q & 15;
         const int m = dims[ax];
+        float v = 0.0f; // zero outside the axis length
         if (a < m && i < m) {
             const double sa = a == 0 ? sqrt(1.0 / m) : sqrt(2.0 / m);
-            cs[ax][a][i] = (float)(sa * cos(M_PI * a * (i + 0.5) / m));
+            v = (float)(sa * cos(M_PI * a * (i + 0.5) / m));
             if (i == 0) lam[ax][a] = (float)(2.0 - 2.0 * cos(M_PI * a / m));
         }
+        cs[ax][a][i] = v;
+        ct[ax][i][a] = v;
     }
     // b = sum r' per block, minus its constant part (uniform blocks: the mean)
     const double a1 = alpha1 ? *alpha1 : 0.0;
@@ -1396,16 +1403,33 @@ __device__ __forceinline__ void ws_gdct_body(const double* __restrict__ sg, cons
     for (int g = threadIdx.x; g < n; g += GT) A[g] -= mean;
     __syncthreads();
     // one 1-D transform along axis ax (forward: sum_i c[a][i] v[i]; inverse:
-    // sum_a c[a][i] v[a]) from src into dst
+    // sum_a c[a][i] v[a]) from src into dst: four threads per line, each
+    // with the line's m <= 16 values in registers and the outputs co = part,
+    // part + 4, ... (four independent sums), each summed over q = 0..m-1 in
+    // order. Lines shorter than 16 are zero-padded: the padded terms add +0
+    // after the m real ones, which leaves the sum's bits as they are (a sum
+    // started at +0 is never -0).
     auto pass = [&](const float* src, float* dst, int ax, bool inv) {
         const int m = dims[ax];
         const int st = ax == 0 ? 1 : (ax == 1 ? Gw : Gw * Gh);
-        for (int g = threadIdx.x; g < n; g += GT) {
-            const int co = ax == 0 ? g % Gw : (ax == 1 ? (g / Gw) % Gh : g / (Gw * Gh));
-            const int base = g - co * st;
-            float acc = 0.0f;
-            for (int q = 0; q < m; ++q) acc += (inv ? cs[ax][q][co] : cs[ax][co][q]) * src[base + q * st];
-            dst[g] = acc;
+        const int nl = n / m;
+        const float* tab = inv ? &ct[ax][0][0] : &cs[ax][0][0]; // row co: the weights of output co
+        for (int t = threadIdx.x; t < 4 * nl; t += GT) {
+            const int l = t >> 2, part = t & 3;
+            // line l: (y, z) for ax 0, (x, z) for ax 1, (x, y) for ax 2
+            const int base = ax == 0 ? l * Gw : (ax == 1 ? l % Gw + (l / Gw) * (Gw * Gh) : l);
+            float v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = q < m ? src[base + q * st] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int co = part + 4 * u;
+                const float* row = tab + 16 * co;
+                float acc = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc += row[q] * v[q];
+                if (co < m) dst[base + co * st] = acc;
+            }
         }
         __syncthreads();
     };
