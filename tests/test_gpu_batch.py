@@ -240,14 +240,16 @@ def test_batch_chunked_graphs_equal_single_scenes(warm):
     batches) splits each captured step into graphs of 16 batches and a last
     one with the update and advection. With a tight tolerance every solve
     runs to the cap and the step launches all five graphs; with a loose one
-    the solves end early and the step skips the remaining sweep graphs.
+    the solves end early and the step launches only the graphs up to one
+    past the graph in which the last solve ended, then the last graph.
     Either way every scene equals the same scene stepped alone, bit for bit,
     over both captured parities (warm: re-applications and their judges on
     either side of the graph boundaries)."""
     net = B.Net(seed=4) if warm else None
     cap = 1200 if warm else 600
     runs = {}
-    for tol in (1e-7, 2e-2):
+    loose_tol = 2e-2 if warm else 0.3
+    for tol in (1e-7, loose_tol):
         print(f"[batch-chunks] warm={warm} tol={tol:g}", flush=True)
         cases = [scene(r, 40 + i) for i, r in enumerate(SHAPES)]
         for s, _ in cases:
@@ -283,7 +285,7 @@ def test_batch_chunked_graphs_equal_single_scenes(warm):
                 compare(f"batch-chunks tol {tol:g} scene {i} field {w}", b.scenes[i].field(w), singles[i].field(w))
         print(f"[batch-chunks] graphs run per step {graphs}, sweeps per step {iters}", flush=True)
         runs[tol] = (graphs, iters)
-    (g_tight, i_tight), (g_loose, i_loose) = runs[1e-7], runs[2e-2]
+    (g_tight, i_tight), (g_loose, i_loose) = runs[1e-7], runs[loose_tol]
     assert all(max(x) == cap for x in i_tight), i_tight
     assert g_tight == [5, 5, 5], g_tight
     for n, x in zip(g_loose, i_loose):
