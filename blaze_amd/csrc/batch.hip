@@ -267,7 +267,20 @@ int Batch::build_tables(int v) {
     }
     fs_blocks = fb;
     rs_blocks = rb;
-    BLZ_HIP(hipMemcpy(fs, fs_h.data(), 2 * n * sizeof(FusedScene), hipMemcpyHostToDevice));
+    // the device table takes the second sweep of a pair (spare -> home) in
+    // reverse scene order: the blocks of a launch start roughly in index
+    // order, so each sweep begins with the scenes the previous one ended
+    // with, whose planes are the likeliest to be still in the Infinity Cache
+    // when the batch's planes exceed it (the scenes are independent, so
+    // the order changes no value)
+    std::vector<FusedScene> dev(fs_h);
+    for (int r = 0, f1 = 0; r < n; ++r) {
+        FusedScene e = fs_h[n + (n - 1 - r)];
+        e.first_block = f1;
+        f1 += (e.nwaves + wpb - 1) / wpb;
+        dev[n + r] = e;
+    }
+    BLZ_HIP(hipMemcpy(fs, dev.data(), 2 * n * sizeof(FusedScene), hipMemcpyHostToDevice));
     BLZ_HIP(hipMemcpy(rs, rs_h.data(), n * sizeof(ResScene), hipMemcpyHostToDevice));
     // the scenes' own sweeps (the warm start's judge, on each scene's stream)
     // use the same variant as a single-grid launch: one launch per sweep
